@@ -1,0 +1,107 @@
+/*
+ * nnkifmm.h -- C ABI of the MI355X KIFMM library (libnnkifmm.so).
+ *
+ * The reference (Walkersb101/Stokes-NNKIFMM) is MATLAB; a MEX gateway (see
+ * INTEGRATION.md) binds these entry points so that the reference's classes
+ * and functions keep their signatures.  Each declaration names the reference
+ * interface it replaces (paths relative to the reference root).
+ *
+ * Conventions
+ *  - Every function returns 0 on success or a positive status code; the
+ *    message of the last failure on the calling thread is nnk_last_error().
+ *    No C++ exception crosses this boundary.
+ *  - Point arrays are n x 3 row-major doubles ([x1 y1 z1 x2 ...]), which is
+ *    MATLAB's (3,n) column-major layout, i.e. the reference's "format 1"
+ *    GMRES vector (Classes/KIFMM.m:84-85, 98-99).
+ *  - Node ids are 1-based as in Classes/OcTree.m.
+ *  - Host-pointer entry points copy in and out; *_device entry points take
+ *    device pointers and a hipStream_t (passed as void*, 0 = null stream).
+ *  - Handles are not re-entrant: one call at a time per handle.
+ */
+#ifndef NNKIFMM_H
+#define NNKIFMM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct nnk_tree nnk_tree;
+typedef struct nnk_fmm nnk_fmm;
+
+enum {
+  NNK_OK = 0,
+  NNK_EARG = 1,      /* invalid argument                                   */
+  NNK_EMAXDEPTH = 2, /* OcTree.m:168-170 'Max level reached'               */
+  NNK_EHIP = 3,      /* HIP runtime failure                                */
+  NNK_ENOMEM = 4,
+  NNK_ESTATE = 5     /* inconsistent state / unsupported configuration     */
+};
+
+enum { NNK_LIST_U = 0, NNK_LIST_V = 1, NNK_LIST_W = 2, NNK_LIST_X = 3 };
+enum { NNK_FP64 = 0 };
+
+/* Library version (major*10000 + minor*100 + patch). */
+int32_t nnk_version(void);
+
+/* Message of the last failed call on this thread ("" if none). */
+const char *nnk_last_error(void);
+
+/* Replaces: OcTree(potPoints, targetPoints, 'nodeCapacity', c, 'maxDepth', d)
+ *           Classes/OcTree.m:77-113 (unique/ismember, root cube, Divide,
+ *           geninteractionlists). */
+int32_t nnk_tree_create(const double *pot, int64_t npot, const double *tgt, int64_t ntgt,
+                        int32_t node_capacity, int32_t max_depth, nnk_tree **out);
+
+/* Replaces: OcTree(..., 'nearest', 1, 'finePoints', F, 'NN', NN)
+ *           Classes/OcTree.m:115-128.  NN is Q x npot in CSC form (MATLAB
+ *           sparse jc/ir/pr with 0-based row indices). */
+int32_t nnk_tree_set_nearest(nnk_tree *t, const double *fine, int64_t nfine,
+                             const int64_t *nn_colptr, const int64_t *nn_rowidx,
+                             const double *nn_val);
+
+/* Replaces: properties points/nodeCount/nodeCapacity (Classes/OcTree.m:59-74). */
+int32_t nnk_tree_sizes(const nnk_tree *t, int64_t *npts, int64_t *npot, int64_t *ntgt,
+                       int32_t *node_count, int32_t *node_capacity);
+
+/* Replaces: properties points, potentials, targets, pointIndex,
+ *           nodeCorners (count x 6 row-major), nodeLevel, nodeParents,
+ *           nodeChildren (count x 8 row-major)   Classes/OcTree.m:59-74. */
+int32_t nnk_tree_export(const nnk_tree *t, double *points, uint8_t *is_pot, uint8_t *is_tgt,
+                        int32_t *point_index, double *corners, int32_t *level,
+                        int32_t *parent, int32_t *children);
+
+/* Replaces: property interactions{node, kind}  Classes/OcTree.m:270-288.
+ * ptr has node_count+1 entries; the list of node n (1-based) is
+ * idx[ptr[n-1] .. ptr[n]). */
+int32_t nnk_tree_list_size(const nnk_tree *t, int32_t kind, int64_t *total);
+int32_t nnk_tree_list_export(const nnk_tree *t, int32_t kind, int64_t *ptr, int32_t *idx);
+
+void nnk_tree_destroy(nnk_tree *t);
+
+/* Replaces: KIFMM(tree, [eps mu], 'coronaRes', r, 'coronaShells', s)
+ *           Classes/KIFMM.m:46-61.  Builds the device-resident operator:
+ *           per-level check-to-equivalent factors and translation
+ *           operators, point data in leaf order.  The tree must stay alive
+ *           while the handle exists. */
+int32_t nnk_fmm_create(const nnk_tree *t, double eps, double mu, int32_t corona_res,
+                       int32_t corona_shells, int32_t precision, nnk_fmm **out);
+
+/* Replaces: KIFMM.computeVel  Classes/KIFMM.m:63-105 (upwardPass +
+ *           downwardPass).  f: npot x 3 host array in points(potentials)
+ *           order; u: ntgt x 3 host array in points(targets) order. */
+int32_t nnk_fmm_apply(nnk_fmm *h, const double *f, double *u);
+int32_t nnk_fmm_apply_device(nnk_fmm *h, const double *d_f, double *d_u, void *stream);
+
+/* Replaces: KIFMM.reducedComputeVel  Classes/KIFMM.m:157-198
+ *           (Tools/Passes/reducedPass.m). */
+int32_t nnk_fmm_reduced_apply(nnk_fmm *h, const double *f, double *u);
+int32_t nnk_fmm_reduced_apply_device(nnk_fmm *h, const double *d_f, double *d_u, void *stream);
+
+void nnk_fmm_destroy(nnk_fmm *h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* NNKIFMM_H */

@@ -1,0 +1,14 @@
+"""MI355X-native KIFMM matvec and mobility solve for regularized Stokeslets.
+
+Host mirror of the reference's MATLAB operator API (Classes/OcTree.m,
+Classes/KIFMM.m) over the C ABI of libnnkifmm (include/nnkifmm.h).
+Import as ``importlib.import_module("stokes-nnkifmm_amd")``.
+"""
+from ._lib import NNKError, lib  # noqa: F401
+from .kifmm import KIFMM  # noqa: F401
+from .octree import OcTree  # noqa: F401
+
+
+def library_path():
+    from ._lib import LIBPATH
+    return LIBPATH

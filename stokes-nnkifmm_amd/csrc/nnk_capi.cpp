@@ -1,0 +1,172 @@
+// extern "C" boundary: status codes and a thread-local error message; no
+// exception crosses it.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <memory>
+#include <string>
+
+#include "../../include/nnkifmm.h"
+#include "nnk_fmm.hpp"
+#include "nnk_handles.hpp"
+#include "nnk_tree.hpp"
+
+
+namespace {
+thread_local std::string g_err;
+
+int32_t fail(int32_t code, const std::string &m) {
+  g_err = m;
+  return code;
+}
+
+template <class F> int32_t guard(F &&fn) {
+  try {
+    g_err.clear();
+    return fn();
+  } catch (const nnk::Error &e) {
+    return fail(e.code, e.what());
+  } catch (const std::bad_alloc &) {
+    return fail(NNK_ENOMEM, "out of host memory");
+  } catch (const std::exception &e) {
+    return fail(NNK_ESTATE, e.what());
+  } catch (...) {
+    return fail(NNK_ESTATE, "unknown failure");
+  }
+}
+
+// host-pointer variant: stage through device buffers on the null stream
+int32_t apply_host(nnk_fmm *h, const double *f, double *u, bool reduced) {
+  nnk::DBuf df, du;
+  df.alloc(sizeof(double) * 3 * std::max<int64_t>(h->npot, 1));
+  du.alloc(sizeof(double) * 3 * std::max<int64_t>(h->ntgt, 1));
+  NNK_HIP(hipMemcpy(df.p, f, sizeof(double) * 3 * h->npot, hipMemcpyHostToDevice));
+  du.zero();
+  if (reduced) h->f->reduced_apply(df.as<double>(), du.as<double>(), 0);
+  else h->f->apply(df.as<double>(), du.as<double>(), 0);
+  NNK_HIP(hipMemcpy(u, du.p, sizeof(double) * 3 * h->ntgt, hipMemcpyDeviceToHost));
+  return NNK_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int32_t nnk_version(void) { return 100; }
+
+const char *nnk_last_error(void) { return g_err.c_str(); }
+
+int32_t nnk_tree_create(const double *pot, int64_t npot, const double *tgt, int64_t ntgt,
+                        int32_t node_capacity, int32_t max_depth, nnk_tree **out) {
+  return guard([&]() -> int32_t {
+    if (!out) return fail(NNK_EARG, "nnk_tree_create: null out");
+    *out = nullptr;
+    if ((npot && !pot) || (ntgt && !tgt)) return fail(NNK_EARG, "nnk_tree_create: null points");
+    if (node_capacity < 1) return fail(NNK_EARG, "nodeCapacity must be >= 1");
+    auto t = std::make_shared<nnk::Tree>();
+    std::string err;
+    int rc = nnk::build_tree(pot, npot, tgt, ntgt, node_capacity, max_depth, *t, err);
+    if (rc) return fail(rc == nnk::NNK_EMAXDEPTH ? NNK_EMAXDEPTH : NNK_EARG, err);
+    *out = new nnk_tree{t};
+    return NNK_OK;
+  });
+}
+
+int32_t nnk_tree_set_nearest(nnk_tree *t, const double *fine, int64_t nfine,
+                             const int64_t *colptr, const int64_t *row, const double *val) {
+  return guard([&]() -> int32_t {
+    if (!t || !colptr || (nfine && !fine)) return fail(NNK_EARG, "nnk_tree_set_nearest: null argument");
+    std::string err;
+    int rc = nnk::set_nearest(*t->t, fine, nfine, colptr, row, val, err);
+    if (rc) return fail(NNK_EARG, err);
+    return NNK_OK;
+  });
+}
+
+int32_t nnk_tree_sizes(const nnk_tree *t, int64_t *npts, int64_t *npot, int64_t *ntgt,
+                       int32_t *node_count, int32_t *cap) {
+  if (!t) return fail(NNK_EARG, "nnk_tree_sizes: null tree");
+  if (npts) *npts = t->t->npts;
+  if (npot) *npot = t->t->npot;
+  if (ntgt) *ntgt = t->t->ntgt;
+  if (node_count) *node_count = t->t->count;
+  if (cap) *cap = t->t->capacity;
+  return NNK_OK;
+}
+
+int32_t nnk_tree_export(const nnk_tree *h, double *points, uint8_t *is_pot, uint8_t *is_tgt,
+                        int32_t *point_index, double *corners, int32_t *level, int32_t *parent,
+                        int32_t *children) {
+  if (!h) return fail(NNK_EARG, "nnk_tree_export: null tree");
+  const nnk::Tree &t = *h->t;
+  if (points) std::memcpy(points, t.points.data(), sizeof(double) * 3 * t.npts);
+  if (is_pot) std::memcpy(is_pot, t.is_pot.data(), t.npts);
+  if (is_tgt) std::memcpy(is_tgt, t.is_tgt.data(), t.npts);
+  if (point_index) std::memcpy(point_index, t.point_index.data(), sizeof(int32_t) * t.npts);
+  if (corners) std::memcpy(corners, t.corners.data(), sizeof(double) * 6 * t.count);
+  if (level) std::memcpy(level, t.level.data(), sizeof(int32_t) * t.count);
+  if (parent) std::memcpy(parent, t.parent.data(), sizeof(int32_t) * t.count);
+  if (children) std::memcpy(children, t.children.data(), sizeof(int32_t) * 8 * t.count);
+  return NNK_OK;
+}
+
+int32_t nnk_tree_list_size(const nnk_tree *h, int32_t kind, int64_t *total) {
+  if (!h || kind < 0 || kind > 3 || !total) return fail(NNK_EARG, "nnk_tree_list_size: bad argument");
+  *total = (int64_t)h->t->lidx[kind].size();
+  return NNK_OK;
+}
+
+int32_t nnk_tree_list_export(const nnk_tree *h, int32_t kind, int64_t *ptr, int32_t *idx) {
+  if (!h || kind < 0 || kind > 3 || !ptr || !idx) return fail(NNK_EARG, "nnk_tree_list_export: bad argument");
+  const nnk::Tree &t = *h->t;
+  for (int n = 0; n <= t.count; n++) ptr[n] = t.lptr[kind][n + 1];
+  std::memcpy(idx, t.lidx[kind].data(), sizeof(int32_t) * t.lidx[kind].size());
+  return NNK_OK;
+}
+
+void nnk_tree_destroy(nnk_tree *t) { delete t; }
+
+int32_t nnk_fmm_create(const nnk_tree *t, double eps, double mu, int32_t res, int32_t shells,
+                       int32_t precision, nnk_fmm **out) {
+  return guard([&]() -> int32_t {
+    if (!t || !out) return fail(NNK_EARG, "nnk_fmm_create: null argument");
+    *out = nullptr;
+    if (precision != NNK_FP64) return fail(NNK_EARG, "nnk_fmm_create: unknown precision");
+    auto f = std::make_unique<nnk::Fmm>(t->t, eps, mu, res, shells);
+    *out = new nnk_fmm{std::move(f), t->t->npot, t->t->ntgt};
+    return NNK_OK;
+  });
+}
+
+int32_t nnk_fmm_apply(nnk_fmm *h, const double *f, double *u) {
+  return guard([&]() -> int32_t {
+    if (!h || (!f && h->npot) || (!u && h->ntgt)) return fail(NNK_EARG, "nnk_fmm_apply: null argument");
+    return apply_host(h, f, u, false);
+  });
+}
+
+int32_t nnk_fmm_apply_device(nnk_fmm *h, const double *d_f, double *d_u, void *stream) {
+  return guard([&]() -> int32_t {
+    if (!h || (!d_f && h->npot) || (!d_u && h->ntgt)) return fail(NNK_EARG, "nnk_fmm_apply_device: null argument");
+    h->f->apply(d_f, d_u, (hipStream_t)stream);
+    return NNK_OK;
+  });
+}
+
+int32_t nnk_fmm_reduced_apply(nnk_fmm *h, const double *f, double *u) {
+  return guard([&]() -> int32_t {
+    if (!h || (!f && h->npot) || (!u && h->ntgt)) return fail(NNK_EARG, "nnk_fmm_reduced_apply: null argument");
+    return apply_host(h, f, u, true);
+  });
+}
+
+int32_t nnk_fmm_reduced_apply_device(nnk_fmm *h, const double *d_f, double *d_u, void *stream) {
+  return guard([&]() -> int32_t {
+    if (!h || (!d_f && h->npot) || (!d_u && h->ntgt)) return fail(NNK_EARG, "nnk_fmm_reduced_apply_device: null argument");
+    h->f->reduced_apply(d_f, d_u, (hipStream_t)stream);
+    return NNK_OK;
+  });
+}
+
+void nnk_fmm_destroy(nnk_fmm *h) { delete h; }
+
+}  // extern "C"

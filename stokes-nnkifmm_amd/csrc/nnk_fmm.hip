@@ -1,0 +1,792 @@
+// KIFMM matvec on gfx950, FP64 form.
+//
+// Pass structure follows Tools/Passes/upwardPass.m and downwardPass.m:
+//   S2M   (Up.m:31-33)          check potential of a leaf's sources on its
+//                               down surface                  k_surf_eval
+//   M2M   (upwardPass.m:81-93)  children's up densities onto the parent's
+//                               down surface                  gemm (8 ops)
+//   up solve  K(up->down) \ chk per level                     gemm x2 (SVD)
+//   M2L   (DownNonleaf.m:42)    V list onto the up surface    gemm (189 ops)
+//   L2L   (DownNonleaf.m:44)    parent down density           gemm (1 op, +=)
+//   P2L   (DownNonleaf.m:43)    X-list sources                k_surf_eval
+//   down solve  K(down->up) \ chk per level                   gemm x2 (SVD)
+//   L2T+P2P+M2T (Downleaf.m:37-40) per target leaf            k_leaf
+// Per-level operators use the level's canonical box geometry (all boxes of a
+// level share their width; their absolute corners differ from the canonical
+// ones only by rounding).  The check-to-equivalent solves apply one-sided
+// Jacobi SVD factors, x = V (S^-1 (U^T b)), which is backward stable like
+// the reference's pivoted LU.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+
+#include "nnk_fmm.hpp"
+#include "nnk_gemm.hpp"
+#include "nnk_kernels.hpp"
+
+namespace nnk {
+
+int svd_jacobi(const double *A, int n, double *U, double *s, double *V);
+
+
+// ---------------------------------------------------------------------------
+// kernels
+
+__global__ void k_strength(int64_t nsrc, const int32_t *__restrict__ from,
+                           const int64_t *__restrict__ cptr, const int32_t *__restrict__ ccol,
+                           const double *__restrict__ cw, const double *__restrict__ f,
+                           double *__restrict__ q) {
+  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nsrc) return;
+  if (from) {
+    int64_t j = from[k];
+    q[3 * k] = f[3 * j];
+    q[3 * k + 1] = f[3 * j + 1];
+    q[3 * k + 2] = f[3 * j + 2];
+  } else {
+    double gx = 0, gy = 0, gz = 0;
+    for (int64_t e = cptr[k]; e < cptr[k + 1]; e++) {
+      int64_t j = ccol[e];
+      double w = cw[e];
+      gx += w * f[3 * j];
+      gy += w * f[3 * j + 1];
+      gz += w * f[3 * j + 2];
+    }
+    q[3 * k] = gx;
+    q[3 * k + 1] = gy;
+    q[3 * k + 2] = gz;
+  }
+}
+
+// At[(3q+b)*Mpad + 3p+a] = c * S_ab(t_p - s_q)    (dense kernel.m block)
+__global__ void k_form_At(const double *__restrict__ sp, int ns, const double *__restrict__ tp,
+                          int nt, double e2, double c, double *__restrict__ At, int Mpad) {
+  int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= ns * nt) return;
+  int p = idx % nt, q = idx / nt;
+  double rx = tp[3 * p] - sp[3 * q], ry = tp[3 * p + 1] - sp[3 * q + 1],
+         rz = tp[3 * p + 2] - sp[3 * q + 2];
+  double r2 = rx * rx + ry * ry + rz * rz;
+  double re = r2 + e2;
+  double inv = rsqrt(re);
+  double s = c * inv * inv * inv;
+  double iso = r2 + 2.0 * e2;
+  double r[3] = {rx, ry, rz};
+  for (int b = 0; b < 3; b++)
+    for (int a = 0; a < 3; a++)
+      At[(size_t)(3 * q + b) * Mpad + 3 * p + a] = (r[a] * r[b] + (a == b ? iso : 0.0)) * s;
+}
+
+struct SurfArgs {
+  const int32_t *nodes;       // work items (node ids)
+  const int64_t *lptr;        // per-node source-leaf lists (X); null = the node itself
+  const int32_t *lidx;
+  int down, accumulate;
+  const double *corners;
+  const int32_t *surf_idx;
+  int nq, n1, K;
+  const int32_t *src_ptr;     // per node id, [src_ptr[n], src_ptr[n+1])
+  const double *src_pos, *src_q;
+  double e2, c;
+  double *chk;
+};
+
+// Check potential on a node surface from the sources of a set of leaves
+// (S2M: the leaf itself onto its down surface; P2L: the X list onto the up
+// surface).  One block per node; sources staged in LDS.
+__global__ __launch_bounds__(256) void k_surf_eval(SurfArgs a) {
+  const int n = a.nodes[blockIdx.x];
+  __shared__ double sp[256 * 6];
+  const double *box = a.corners + 6 * (size_t)(n - 1);
+  const int tid = threadIdx.x;
+  int64_t lb = 0, le = 1;
+  if (a.lptr) { lb = a.lptr[n]; le = a.lptr[n + 1]; }
+  for (int p0 = 0; p0 < a.nq; p0 += 256) {
+    const int p = p0 + tid;
+    double tx = 0, ty = 0, tz = 0;
+    if (p < a.nq) surf_point(box, a.down, a.surf_idx[p], a.n1, tx, ty, tz);
+    double ux = 0, uy = 0, uz = 0;
+    for (int64_t e = lb; e < le; e++) {
+      const int leaf = a.lptr ? a.lidx[e] : n;
+      const int sb = a.src_ptr[leaf], se = a.src_ptr[leaf + 1];
+      for (int s0 = sb; s0 < se; s0 += 256) {
+        const int cnt = min(256, se - s0);
+        if (tid < cnt) {
+          const int s = s0 + tid;
+          for (int d = 0; d < 3; d++) {
+            sp[6 * tid + d] = a.src_pos[3 * (size_t)s + d];
+            sp[6 * tid + 3 + d] = a.src_q[3 * (size_t)s + d];
+          }
+        }
+        __syncthreads();
+        if (p < a.nq)
+          for (int j = 0; j < cnt; j++)
+            stokeslet_acc(tx, ty, tz, sp[6 * j], sp[6 * j + 1], sp[6 * j + 2], sp[6 * j + 3],
+                          sp[6 * j + 4], sp[6 * j + 5], a.e2, ux, uy, uz);
+        __syncthreads();
+      }
+    }
+    if (p < a.nq) {
+      double *o = a.chk + (size_t)(n - 1) * a.K + 3 * p;
+      if (a.accumulate) {
+        o[0] += a.c * ux; o[1] += a.c * uy; o[2] += a.c * uz;
+      } else {
+        o[0] = a.c * ux; o[1] = a.c * uy; o[2] = a.c * uz;
+      }
+    }
+  }
+}
+
+struct LeafArgs {
+  const int32_t *leaves;
+  const double *corners;
+  const int32_t *surf_idx;
+  int nq, n1, K;
+  const int32_t *tgt_ptr;
+  const double *tgt_pos;
+  const int32_t *tgt_out;
+  const int32_t *src_ptr;
+  const double *src_pos, *src_q;
+  const int64_t *uptr; const int32_t *uidx;
+  const int64_t *wptr; const int32_t *widx;
+  const double *uppot, *downpot;
+  double e2, c;
+  int far;          // 0: sources of the leaf itself only (reducedPass.m)
+  double *out;      // ntgt x 3
+};
+
+// Velocity at the targets of one leaf: L2T from its down density, P2P from
+// its U-list sources, M2T from its W-list up densities (Downleaf.m:37-40).
+// Each target is shared by nsub lanes that split the sources; partial sums
+// are combined in a fixed order.
+__global__ __launch_bounds__(256) void k_leaf(LeafArgs a) {
+  const int n = a.leaves[blockIdx.x];
+  __shared__ double sp[256 * 6];
+  __shared__ double red[256 * 3];
+  const int tid = threadIdx.x;
+  const int tb = a.tgt_ptr[n], nt = a.tgt_ptr[n + 1] - tb;
+  for (int t0 = 0; t0 < nt; t0 += 256) {
+    const int ntc = min(256, nt - t0);
+    const int nsub = 256 / ntc;
+    const bool act = tid < ntc * nsub;
+    const int ti = tid % ntc, sub = tid / ntc;
+    double tx = 0, ty = 0, tz = 0;
+    if (act) {
+      const double *pp = a.tgt_pos + 3 * (size_t)(tb + t0 + ti);
+      tx = pp[0]; ty = pp[1]; tz = pp[2];
+    }
+    double ux = 0, uy = 0, uz = 0;
+    auto consume = [&](int cnt) {
+      if (act)
+        for (int j = sub; j < cnt; j += nsub)
+          stokeslet_acc(tx, ty, tz, sp[6 * j], sp[6 * j + 1], sp[6 * j + 2], sp[6 * j + 3],
+                        sp[6 * j + 4], sp[6 * j + 5], a.e2, ux, uy, uz);
+    };
+    auto surface = [&](int node, int down, const double *dens) {
+      const double *box = a.corners + 6 * (size_t)(node - 1);
+      for (int p0 = 0; p0 < a.nq; p0 += 256) {
+        const int cnt = min(256, a.nq - p0);
+        if (tid < cnt) {
+          double x, y, z;
+          surf_point(box, down, a.surf_idx[p0 + tid], a.n1, x, y, z);
+          sp[6 * tid] = x; sp[6 * tid + 1] = y; sp[6 * tid + 2] = z;
+          const double *q = dens + (size_t)(node - 1) * a.K + 3 * (p0 + tid);
+          sp[6 * tid + 3] = q[0]; sp[6 * tid + 4] = q[1]; sp[6 * tid + 5] = q[2];
+        }
+        __syncthreads();
+        consume(cnt);
+        __syncthreads();
+      }
+    };
+    auto sources = [&](int leaf) {
+      const int sb = a.src_ptr[leaf], se = a.src_ptr[leaf + 1];
+      for (int s0 = sb; s0 < se; s0 += 256) {
+        const int cnt = min(256, se - s0);
+        if (tid < cnt) {
+          const int s = s0 + tid;
+          for (int d = 0; d < 3; d++) {
+            sp[6 * tid + d] = a.src_pos[3 * (size_t)s + d];
+            sp[6 * tid + 3 + d] = a.src_q[3 * (size_t)s + d];
+          }
+        }
+        __syncthreads();
+        consume(cnt);
+        __syncthreads();
+      }
+    };
+    if (a.far) {
+      surface(n, 1, a.downpot);
+      for (int64_t e = a.wptr[n]; e < a.wptr[n + 1]; e++) surface(a.widx[e], 0, a.uppot);
+      for (int64_t e = a.uptr[n]; e < a.uptr[n + 1]; e++) sources(a.uidx[e]);
+    } else {
+      sources(n);
+    }
+    red[3 * tid] = ux; red[3 * tid + 1] = uy; red[3 * tid + 2] = uz;
+    __syncthreads();
+    if (tid < ntc) {
+      double sx = 0, sy = 0, sz = 0;
+      for (int s = 0; s < nsub; s++) {
+        sx += red[3 * (s * ntc + tid)];
+        sy += red[3 * (s * ntc + tid) + 1];
+        sz += red[3 * (s * ntc + tid) + 2];
+      }
+      const int o = a.tgt_out[tb + t0 + tid];
+      a.out[3 * (size_t)o] = a.c * sx;
+      a.out[3 * (size_t)o + 1] = a.c * sy;
+      a.out[3 * (size_t)o + 2] = a.c * sz;
+    }
+    __syncthreads();
+  }
+}
+
+constexpr int GBM = 160, GBN = 64, GBK = 24;
+
+// ---------------------------------------------------------------------------
+// host side
+
+namespace {
+
+std::vector<int32_t> surface_codes(int res, int shells) {
+  std::vector<int32_t> c;
+  for (int iz = 0; iz < res; iz++)
+    for (int ix = 0; ix < res; ix++)
+      for (int iy = 0; iy < res; iy++) {
+        bool inside = iy >= shells && iy < res - shells && ix >= shells && ix < res - shells &&
+                      iz >= shells && iz < res - shells;
+        if (!inside) c.push_back(ix | (iy << 8) | (iz << 16));
+      }
+  return c;
+}
+
+double lin_h(double d1, double d2, int k, int n1) {
+  if (k == 0) return d1;
+  if (k == n1) return d2;
+  return d1 + ((double)k * (d2 - d1)) / (double)n1;
+}
+
+// surface points of a box (lo, width w per axis) as in genupsurf/gendownsurf
+std::vector<double> surface_pts(const std::vector<int32_t> &codes, int res, const double lo[3],
+                                double w, bool down) {
+  double l[3], h[3];
+  for (int d = 0; d < 3; d++) {
+    l[d] = lo[d];
+    h[d] = lo[d] + w;
+    if (down) {
+      double ww = h[d] - l[d];
+      l[d] -= ww;
+      h[d] += ww;
+    }
+  }
+  std::vector<double> p(3 * codes.size());
+  for (size_t i = 0; i < codes.size(); i++) {
+    p[3 * i] = lin_h(l[0], h[0], codes[i] & 255, res - 1);
+    p[3 * i + 1] = lin_h(l[1], h[1], (codes[i] >> 8) & 255, res - 1);
+    p[3 * i + 2] = lin_h(l[2], h[2], (codes[i] >> 16) & 255, res - 1);
+  }
+  return p;
+}
+
+inline int offset_slot(int ox, int oy, int oz) { return (ox + 3) * 49 + (oy + 3) * 7 + (oz + 3); }
+
+// the 189 V-list offsets available to a child of class a (a = 4gx+2gy+gz)
+std::vector<std::array<int, 3>> class_offsets(int a) {
+  int g[3] = {(a >> 2) & 1, (a >> 1) & 1, a & 1};
+  std::vector<std::array<int, 3>> out;
+  for (int ox = -2 - g[0]; ox <= 3 - g[0]; ox++)
+    for (int oy = -2 - g[1]; oy <= 3 - g[1]; oy++)
+      for (int oz = -2 - g[2]; oz <= 3 - g[2]; oz++)
+        if (std::max({std::abs(ox), std::abs(oy), std::abs(oz)}) >= 2) out.push_back({ox, oy, oz});
+  return out;
+}
+
+template <class T> DBuf make_dbuf(const std::vector<T> &v) {
+  DBuf d;
+  d.upload(v);
+  return d;
+}
+
+}  // namespace
+
+void Fmm::gemm(const GemmBatch &b, const double *X, int ldx, double *C, int ldc, int beta,
+               hipStream_t s) {
+  if (b.njobs == 0 || b.maxN == 0) return;
+  dim3 grid((b.maxN + GBN - 1) / GBN, Mpad_ / GBM, b.njobs);
+  hipLaunchKernelGGL((k_gemm_gather<GBM, GBN, GBK>), grid, dim3(256), 0, s, K_, K_, Mpad_, X, ldx,
+                     C, ldc, beta, b.jobs.as<GemmJob>());
+  NNK_HIP(hipGetLastError());
+}
+
+Fmm::Fmm(std::shared_ptr<const Tree> t, double eps, double mu, int res, int shells)
+    : tp_(std::move(t)), tree_(*tp_), eps_(eps), mu_(mu), res_(res), shells_(shells) {
+  const Tree &T = tree_;
+  if (res < 2 || res > 16 || shells < 1 || 2 * shells > res)
+    throw Error(NNK_EARG, "coronaRes must be in [2,16] and 1 <= coronaShells <= res/2");
+  if (!(eps > 0) || !(mu > 0)) throw Error(NNK_EARG, "kernel parameters eps and mu must be > 0");
+  auto codes = surface_codes(res, shells);
+  nq_ = (int)codes.size();
+  K_ = 3 * nq_;
+  Kpad_ = (K_ + GBK - 1) / GBK * GBK;
+  Mpad_ = (K_ + GBM - 1) / GBM * GBM;
+  npot_ = T.npot;
+  ntgt_ = T.ntgt;
+  maxlev_ = T.max_level;
+  w0_ = T.corners[3] - T.corners[0];
+  surf_idx_.upload(codes);
+  corners_.upload(T.corners);
+  const int nc = T.count;
+  // ---- sources in leaf order ----
+  std::vector<int32_t> src_ptr(nc + 2, 0);
+  std::vector<double> src_pos;
+  std::vector<int32_t> src_from;
+  std::vector<int64_t> ctr_ptr{0};
+  std::vector<int32_t> ctr_col;
+  std::vector<double> ctr_w;
+  if (!T.nearest) {
+    for (int n = 1; n <= nc; n++) {
+      src_ptr[n + 1] = src_ptr[n];
+      for (int64_t k = T.pt_ptr[n]; k < T.pt_ptr[n + 1]; k++) {
+        int64_t r = T.pt_rows[k];
+        if (!T.is_pot[r]) continue;
+        src_pos.insert(src_pos.end(), &T.points[3 * r], &T.points[3 * r + 3]);
+        src_from.push_back((int32_t)T.pot_ord[r]);
+        src_ptr[n + 1]++;
+      }
+    }
+  } else {
+    // fine points with a nonzero NN weight on one of the leaf's coarse
+    // points, each once per leaf, ascending fine index (upwardPass.m:58-69)
+    struct Tr { int64_t row; int32_t col; double w; };
+    std::vector<Tr> tr;
+    for (int n = 1; n <= nc; n++) {
+      src_ptr[n + 1] = src_ptr[n];
+      tr.clear();
+      for (int64_t k = T.pt_ptr[n]; k < T.pt_ptr[n + 1]; k++) {
+        int64_t r = T.pt_rows[k];
+        if (!T.is_pot[r]) continue;
+        int64_t c = T.pot_ord[r];
+        for (int64_t e = T.nn_colptr[c]; e < T.nn_colptr[c + 1]; e++)
+          if (T.nn_val[e] != 0.0) tr.push_back({T.nn_row[e], (int32_t)c, T.nn_val[e]});
+      }
+      std::sort(tr.begin(), tr.end(), [](const Tr &x, const Tr &y) {
+        return x.row != y.row ? x.row < y.row : x.col < y.col;
+      });
+      for (size_t e = 0; e < tr.size(); e++) {
+        if (e == 0 || tr[e].row != tr[e - 1].row) {
+          if (e) ctr_ptr.push_back((int64_t)ctr_col.size());
+          src_pos.insert(src_pos.end(), &T.fine[3 * tr[e].row], &T.fine[3 * tr[e].row + 3]);
+          src_ptr[n + 1]++;
+        }
+        ctr_col.push_back(tr[e].col);
+        ctr_w.push_back(tr[e].w);
+      }
+      if (!tr.empty()) ctr_ptr.push_back((int64_t)ctr_col.size());
+    }
+  }
+  nsrc_ = src_pos.size() / 3;
+  src_ptr_.upload(src_ptr);
+  src_pos_.upload(src_pos);
+  src_q_.alloc(sizeof(double) * 3 * std::max<int64_t>(nsrc_, 1));
+  if (!T.nearest) src_from_.upload(src_from);
+  else {
+    ctr_ptr_.upload(ctr_ptr);
+    ctr_col_.upload(ctr_col);
+    ctr_w_.upload(ctr_w);
+  }
+  // ---- targets in leaf order ----
+  std::vector<int32_t> tgt_ptr(nc + 2, 0), tgt_out;
+  std::vector<double> tgt_pos;
+  for (int n = 1; n <= nc; n++) {
+    tgt_ptr[n + 1] = tgt_ptr[n];
+    for (int64_t k = T.pt_ptr[n]; k < T.pt_ptr[n + 1]; k++) {
+      int64_t r = T.pt_rows[k];
+      if (!T.is_tgt[r]) continue;
+      tgt_pos.insert(tgt_pos.end(), &T.points[3 * r], &T.points[3 * r + 3]);
+      tgt_out.push_back((int32_t)T.tgt_ord[r]);
+      tgt_ptr[n + 1]++;
+    }
+  }
+  tgt_ptr_.upload(tgt_ptr);
+  tgt_pos_.upload(tgt_pos);
+  tgt_out_.upload(tgt_out);
+  uptr_.upload(T.lptr[0]);
+  uidx_.upload(T.lidx[0]);
+  wptr_.upload(T.lptr[2]);
+  widx_.upload(T.lidx[2]);
+  xptr_.upload(T.lptr[3]);
+  xidx_.upload(T.lidx[3]);
+  std::vector<int32_t> pl, tl;
+  for (int n = 1; n <= nc; n++) {
+    if (T.pot_leaf[n]) pl.push_back(n);
+    if (T.tgt_leaf[n]) tl.push_back(n);
+  }
+  n_pleaves_ = (int)pl.size();
+  n_tleaves_ = (int)tl.size();
+  pleaves_.upload(pl);
+  tleaves_.upload(tl);
+  chk_.alloc(sizeof(double) * (size_t)nc * K_);
+  uppot_.alloc(sizeof(double) * (size_t)nc * K_);
+  downpot_.alloc(sizeof(double) * (size_t)nc * K_);
+  chk_.zero();
+  uppot_.zero();
+  downpot_.zero();
+  ops_.resize(maxlev_ + 1);
+  for (int l = 2; l <= maxlev_; l++) build_ops(l);
+  plan();
+  NNK_HIP(hipDeviceSynchronize());
+}
+
+void Fmm::build_ops(int l) {
+  LevelOps &O = ops_[l];
+  const auto codes = surface_codes(res_, shells_);
+  const double w = std::ldexp(w0_, -l);
+  const double e2 = eps_ * eps_, c = 1.0 / (8.0 * kPi * mu_);
+  const size_t mat = (size_t)Kpad_ * Mpad_;
+  const double zero[3] = {0, 0, 0};
+  // ---- check-to-equivalent factors from K(up -> down) ----
+  {
+    auto up = surface_pts(codes, res_, zero, w, false);
+    auto dn = surface_pts(codes, res_, zero, w, true);
+    const int n = K_;
+    std::vector<double> A((size_t)n * n), U((size_t)n * n), V((size_t)n * n), S(n);
+    for (int p = 0; p < nq_; p++)      // row point (down surface, target)
+      for (int q = 0; q < nq_; q++) {  // column point (up surface, source)
+        double r[3] = {dn[3 * p] - up[3 * q], dn[3 * p + 1] - up[3 * q + 1], dn[3 * p + 2] - up[3 * q + 2]};
+        double r2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+        double invs = (1.0 / std::pow(r2 + e2, 1.5)) * c;
+        for (int a = 0; a < 3; a++)
+          for (int b = 0; b < 3; b++)
+            A[(size_t)(3 * p + a) * n + 3 * q + b] = (r[a] * r[b] + (a == b ? r2 + 2 * e2 : 0.0)) * invs;
+      }
+    svd_jacobi(A.data(), n, U.data(), S.data(), V.data());
+    // up:   x = V (S^-1 U^T b);  down (K(down->up) = A^T): x = U (S^-1 V^T b)
+    std::vector<double> Wup(mat, 0.0), Vup(mat, 0.0), Wdn(mat, 0.0), Udn(mat, 0.0);
+    for (int k = 0; k < n; k++)
+      for (int m = 0; m < n; m++) {
+        Wup[(size_t)k * Mpad_ + m] = U[(size_t)k * n + m] / S[m];  // W[m][k] = U[k][m]/s_m
+        Vup[(size_t)k * Mpad_ + m] = V[(size_t)m * n + k];         // V[m][k]
+        Wdn[(size_t)k * Mpad_ + m] = V[(size_t)k * n + m] / S[m];
+        Udn[(size_t)k * Mpad_ + m] = U[(size_t)m * n + k];
+      }
+    O.Wup.upload(Wup);
+    O.Vup.upload(Vup);
+    O.Wdn.upload(Wdn);
+    O.Udn.upload(Udn);
+  }
+  // ---- kernel-matrix operators formed on the device ----
+  auto form = [&](const std::vector<double> &src, const std::vector<double> &tgt, double *dst) {
+    DBuf ds, dt;
+    ds.upload(src);
+    dt.upload(tgt);
+    int tot = nq_ * nq_;
+    hipLaunchKernelGGL(k_form_At, dim3((tot + 255) / 256), dim3(256), 0, 0, ds.as<double>(), nq_,
+                       dt.as<double>(), nq_, e2, c, dst, Mpad_);
+    NNK_HIP(hipGetLastError());
+    NNK_HIP(hipDeviceSynchronize());
+  };
+  O.M2M.alloc(sizeof(double) * mat * 8);
+  O.L2L.alloc(sizeof(double) * mat * 8);
+  O.M2M.zero();
+  O.L2L.zero();
+  NNK_HIP(hipDeviceSynchronize());
+  {
+    const double plo[3] = {0, 0, 0};
+    auto pdn = surface_pts(codes, res_, plo, 2 * w, true);  // parent (width 2w) down surface
+    for (int k = 0; k < 8; k++) {
+      double lo[3] = {((k >> 2) & 1) * w, ((k >> 1) & 1) * w, (k & 1) * w};
+      auto cup = surface_pts(codes, res_, lo, w, false);
+      form(cup, pdn, O.M2M.as<double>() + k * mat);  // child up -> parent down
+      form(pdn, cup, O.L2L.as<double>() + k * mat);  // parent down -> child up
+    }
+  }
+  O.T.alloc(sizeof(double) * mat * 316);
+  O.T.zero();
+  NNK_HIP(hipDeviceSynchronize());
+  O.tslot.assign(343, nullptr);
+  {
+    auto tup = surface_pts(codes, res_, zero, w, false);
+    int k = 0;
+    for (int ox = -3; ox <= 3; ox++)
+      for (int oy = -3; oy <= 3; oy++)
+        for (int oz = -3; oz <= 3; oz++) {
+          if (std::max({std::abs(ox), std::abs(oy), std::abs(oz)}) < 2) continue;
+          double lo[3] = {ox * w, oy * w, oz * w};
+          auto sup = surface_pts(codes, res_, lo, w, false);
+          double *dst = O.T.as<double>() + (size_t)k * mat;
+          form(sup, tup, dst);
+          O.tslot[offset_slot(ox, oy, oz)] = dst;
+          k++;
+        }
+  }
+}
+
+void Fmm::plan() {
+  const Tree &T = tree_;
+  const int nc = T.count;
+  // which nodes carry a nonzero up density / need a down density
+  std::vector<uint8_t> pot_below(nc + 1, 0), tgt_below(nc + 1, 0);
+  {
+    std::vector<int> order(nc);
+    for (int i = 0; i < nc; i++) order[i] = i + 1;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int a, int b) { return T.level[a - 1] > T.level[b - 1]; });
+    for (int n : order) {
+      if (T.is_leaf(n)) {
+        pot_below[n] = T.pot_leaf[n];
+        tgt_below[n] = T.tgt_leaf[n];
+      } else {
+        for (int k = 0; k < 8; k++) {
+          int ch = T.children[8 * (n - 1) + k];
+          pot_below[n] |= pot_below[ch];
+          tgt_below[n] |= tgt_below[ch];
+        }
+      }
+    }
+  }
+  plans_.clear();
+  plans_.resize(maxlev_ + 1);
+  size_t maxn = 1;
+  std::vector<std::vector<int>> bylev(maxlev_ + 1);
+  for (int n = 1; n <= nc; n++) bylev[T.level[n - 1]].push_back(n);
+  auto add_job = [&](GemmBatch &B, std::vector<GemmJob> &jobs, const std::vector<const double *> &A,
+                     const std::vector<int32_t> *gidx, const std::vector<int32_t> *cidx, int N) {
+    GemmJob j{};
+    B.owned.emplace_back(make_dbuf(A));
+    j.A = B.owned.back().as<const double *>();
+    if (gidx) { B.owned.emplace_back(make_dbuf(*gidx)); j.gidx = B.owned.back().as<int32_t>(); }
+    if (cidx) { B.owned.emplace_back(make_dbuf(*cidx)); j.cidx = B.owned.back().as<int32_t>(); }
+    j.nb = (int)A.size();
+    j.N = N;
+    jobs.push_back(j);
+    B.maxN = std::max(B.maxN, N);
+  };
+  auto finish = [&](GemmBatch &B, std::vector<GemmJob> &jobs) {
+    B.njobs = (int)jobs.size();
+    B.jobs.upload(jobs);
+  };
+  for (int l = 2; l <= maxlev_; l++) {
+    LevelPlan &P = plans_[l];
+    LevelOps &O = ops_[l];
+    P.level = l;
+    // ---- upward ----
+    std::vector<int32_t> up, nl;
+    for (int n : bylev[l]) {
+      if (T.is_leaf(n)) {
+        if (T.pot_leaf[n]) up.push_back(n - 1);
+      } else if (pot_below[n]) {
+        nl.push_back(n - 1);
+      }
+    }
+    if (!nl.empty()) {
+      // M2M: the children (level l+1) carry class-k operators of level l+1
+      std::vector<const double *> A(8);
+      const size_t mat = (size_t)Kpad_ * Mpad_;
+      for (int k = 0; k < 8; k++) A[k] = ops_[l + 1].M2M.as<double>() + k * mat;
+      std::vector<int32_t> g(8 * nl.size());
+      for (size_t j = 0; j < nl.size(); j++)
+        for (int k = 0; k < 8; k++) {
+          int ch = T.children[8 * nl[j] + k];
+          g[k * nl.size() + j] = pot_below[ch] ? ch - 1 : -1;
+        }
+      std::vector<GemmJob> jobs;
+      add_job(P.m2m, jobs, A, &g, &nl, (int)nl.size());
+      finish(P.m2m, jobs);
+    }
+    std::vector<int32_t> upall = up;
+    upall.insert(upall.end(), nl.begin(), nl.end());
+    P.n_up = (int)upall.size();
+    maxn = std::max(maxn, upall.size());
+    if (P.n_up) {
+      std::vector<GemmJob> j1, j2;
+      add_job(P.up_solve1, j1, {O.Wup.as<double>()}, &upall, nullptr, P.n_up);
+      finish(P.up_solve1, j1);
+      add_job(P.up_solve2, j2, {O.Vup.as<double>()}, nullptr, &upall, P.n_up);
+      finish(P.up_solve2, j2);
+    }
+    P.up_nodes.upload(upall);
+    // ---- downward ----
+    std::vector<int32_t> dn;
+    std::vector<int32_t> cls[8];
+    std::vector<int> clsnode[8];
+    for (int n : bylev[l]) {
+      if (!tgt_below[n]) continue;
+      dn.push_back(n - 1);
+      int par = T.parent[n - 1];
+      int a = n - T.children[8 * (par - 1)];
+      cls[a].push_back(n - 1);
+      clsnode[a].push_back(n);
+    }
+    P.n_down = (int)dn.size();
+    maxn = std::max(maxn, dn.size());
+    if (P.n_down) {
+      std::vector<GemmJob> jobs, ljobs;
+      const size_t mat = (size_t)Kpad_ * Mpad_;
+      for (int a = 0; a < 8; a++) {
+        if (cls[a].empty()) continue;
+        auto offs = class_offsets(a);
+        std::map<int, int> pos;
+        std::vector<const double *> A;
+        for (size_t b = 0; b < offs.size(); b++) {
+          pos[offset_slot(offs[b][0], offs[b][1], offs[b][2])] = (int)b;
+          A.push_back(O.tslot[offset_slot(offs[b][0], offs[b][1], offs[b][2])]);
+        }
+        const int nb = (int)A.size(), N = (int)cls[a].size();
+        std::vector<int32_t> g((size_t)nb * N, -1);
+        for (int j = 0; j < N; j++) {
+          int n = clsnode[a][j];
+          const double *bn = T.box(n);
+          for (int64_t e = T.lptr[1][n]; e < T.lptr[1][n + 1]; e++) {
+            int v = T.lidx[1][e];
+            if (!pot_below[v]) continue;
+            const double *bv = T.box(v);
+            int o[3];
+            for (int d = 0; d < 3; d++) o[d] = (int)std::lround((bv[d] - bn[d]) / std::ldexp(w0_, -l));
+            auto it = pos.find(offset_slot(o[0], o[1], o[2]));
+            if (it == pos.end() || std::max({std::abs(o[0]), std::abs(o[1]), std::abs(o[2])}) > 3)
+              throw Error(NNK_ESTATE, "V-list node outside the class offset set");
+            g[(size_t)it->second * N + j] = v - 1;
+          }
+        }
+        add_job(P.m2l, jobs, A, &g, &cls[a], N);
+        if (l >= 3) {  // parent's down density onto the child's up surface
+          std::vector<int32_t> gp(N);
+          for (int j = 0; j < N; j++) gp[j] = T.parent[clsnode[a][j] - 1] - 1;
+          add_job(P.l2l, ljobs, {O.L2L.as<double>() + a * mat}, &gp, &cls[a], N);
+        }
+      }
+      finish(P.m2l, jobs);
+      finish(P.l2l, ljobs);
+      std::vector<GemmJob> j1, j2;
+      add_job(P.dn_solve1, j1, {O.Wdn.as<double>()}, &dn, nullptr, P.n_down);
+      finish(P.dn_solve1, j1);
+      add_job(P.dn_solve2, j2, {O.Udn.as<double>()}, nullptr, &dn, P.n_down);
+      finish(P.dn_solve2, j2);
+    }
+    P.down_nodes.upload(dn);
+    std::vector<int32_t> xs;
+    for (int n : bylev[l])
+      if (tgt_below[n] && T.lptr[3][n + 1] > T.lptr[3][n]) xs.push_back(n);
+    P.n_x = (int)xs.size();
+    P.x_nodes.upload(xs);
+  }
+  tmp_.alloc(sizeof(double) * maxn * K_);
+}
+
+void Fmm::strengths(const double *d_f, hipStream_t s) {
+  if (nsrc_ == 0) return;
+  hipLaunchKernelGGL(k_strength, dim3((nsrc_ + 255) / 256), dim3(256), 0, s, nsrc_,
+                     tree_.nearest ? nullptr : src_from_.as<int32_t>(), ctr_ptr_.as<int64_t>(),
+                     ctr_col_.as<int32_t>(), ctr_w_.as<double>(), d_f, src_q_.as<double>());
+  NNK_HIP(hipGetLastError());
+}
+
+void Fmm::apply(const double *d_f, double *d_u, hipStream_t s) {
+  const double e2 = eps_ * eps_, c = 1.0 / (8.0 * kPi * mu_);
+  strengths(d_f, s);
+  double *chk = chk_.as<double>(), *up = uppot_.as<double>(), *dn = downpot_.as<double>(),
+         *tmp = tmp_.as<double>();
+  // S2M for every potential leaf at level >= 2 (coarser up densities are unused)
+  if (maxlev_ >= 2 && n_pleaves_) {
+    SurfArgs a{};
+    a.nodes = pleaves_.as<int32_t>();
+    a.down = 1;
+    a.corners = corners_.as<double>();
+    a.surf_idx = surf_idx_.as<int32_t>();
+    a.nq = nq_; a.n1 = res_ - 1; a.K = K_;
+    a.src_ptr = src_ptr_.as<int32_t>();
+    a.src_pos = src_pos_.as<double>();
+    a.src_q = src_q_.as<double>();
+    a.e2 = e2; a.c = c;
+    a.chk = chk;
+    hipLaunchKernelGGL(k_surf_eval, dim3(n_pleaves_), dim3(256), 0, s, a);
+    NNK_HIP(hipGetLastError());
+  }
+  for (int l = maxlev_; l >= 2; l--) {
+    LevelPlan &P = plans_[l];
+    if (!P.n_up) continue;
+    gemm(P.m2m, up, K_, chk, K_, 0, s);
+    gemm(P.up_solve1, chk, K_, tmp, K_, 0, s);
+    gemm(P.up_solve2, tmp, K_, up, K_, 0, s);
+  }
+  for (int l = 2; l <= maxlev_; l++) {
+    LevelPlan &P = plans_[l];
+    if (!P.n_down) continue;
+    gemm(P.m2l, up, K_, chk, K_, 0, s);
+    gemm(P.l2l, dn, K_, chk, K_, 1, s);
+    if (P.n_x) {
+      SurfArgs a{};
+      a.nodes = P.x_nodes.as<int32_t>();
+      a.lptr = xptr_.as<int64_t>();
+      a.lidx = xidx_.as<int32_t>();
+      a.down = 0;
+      a.accumulate = 1;
+      a.corners = corners_.as<double>();
+      a.surf_idx = surf_idx_.as<int32_t>();
+      a.nq = nq_; a.n1 = res_ - 1; a.K = K_;
+      a.src_ptr = src_ptr_.as<int32_t>();
+      a.src_pos = src_pos_.as<double>();
+      a.src_q = src_q_.as<double>();
+      a.e2 = e2; a.c = c;
+      a.chk = chk;
+      hipLaunchKernelGGL(k_surf_eval, dim3(P.n_x), dim3(256), 0, s, a);
+      NNK_HIP(hipGetLastError());
+    }
+    gemm(P.dn_solve1, chk, K_, tmp, K_, 0, s);
+    gemm(P.dn_solve2, tmp, K_, dn, K_, 0, s);
+  }
+  if (n_tleaves_) {
+    LeafArgs a{};
+    a.leaves = tleaves_.as<int32_t>();
+    a.corners = corners_.as<double>();
+    a.surf_idx = surf_idx_.as<int32_t>();
+    a.nq = nq_; a.n1 = res_ - 1; a.K = K_;
+    a.tgt_ptr = tgt_ptr_.as<int32_t>();
+    a.tgt_pos = tgt_pos_.as<double>();
+    a.tgt_out = tgt_out_.as<int32_t>();
+    a.src_ptr = src_ptr_.as<int32_t>();
+    a.src_pos = src_pos_.as<double>();
+    a.src_q = src_q_.as<double>();
+    a.uptr = uptr_.as<int64_t>(); a.uidx = uidx_.as<int32_t>();
+    a.wptr = wptr_.as<int64_t>(); a.widx = widx_.as<int32_t>();
+    a.uppot = up; a.downpot = dn;
+    a.e2 = e2; a.c = c;
+    a.far = 1;
+    a.out = d_u;
+    hipLaunchKernelGGL(k_leaf, dim3(n_tleaves_), dim3(256), 0, s, a);
+    NNK_HIP(hipGetLastError());
+  }
+}
+
+void Fmm::debug_copy(int which, double *host) const {
+  const DBuf &b = which == 0 ? chk_ : which == 1 ? uppot_ : downpot_;
+  NNK_HIP(hipDeviceSynchronize());
+  NNK_HIP(hipMemcpy(host, b.p, sizeof(double) * (size_t)tree_.count * K_, hipMemcpyDeviceToHost));
+}
+
+void Fmm::reduced_apply(const double *d_f, double *d_u, hipStream_t s) {
+  const double e2 = eps_ * eps_, c = 1.0 / (8.0 * kPi * mu_);
+  strengths(d_f, s);
+  if (!n_tleaves_) return;
+  LeafArgs a{};
+  a.leaves = tleaves_.as<int32_t>();
+  a.corners = corners_.as<double>();
+  a.surf_idx = surf_idx_.as<int32_t>();
+  a.nq = nq_; a.n1 = res_ - 1; a.K = K_;
+  a.tgt_ptr = tgt_ptr_.as<int32_t>();
+  a.tgt_pos = tgt_pos_.as<double>();
+  a.tgt_out = tgt_out_.as<int32_t>();
+  a.src_ptr = src_ptr_.as<int32_t>();
+  a.src_pos = src_pos_.as<double>();
+  a.src_q = src_q_.as<double>();
+  a.uptr = uptr_.as<int64_t>(); a.uidx = uidx_.as<int32_t>();
+  a.wptr = wptr_.as<int64_t>(); a.widx = widx_.as<int32_t>();
+  a.e2 = e2; a.c = c;
+  a.far = 0;
+  a.out = d_u;
+  hipLaunchKernelGGL(k_leaf, dim3(n_tleaves_), dim3(256), 0, s, a);
+  NNK_HIP(hipGetLastError());
+}
+
+}  // namespace nnk

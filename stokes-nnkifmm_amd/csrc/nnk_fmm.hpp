@@ -1,0 +1,94 @@
+// Device-resident KIFMM operator for one tree (the reference's KIFMM object,
+// Classes/KIFMM.m, specialised to one corona resolution).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <array>
+#include <memory>
+#include <vector>
+
+#include "nnk_common.hpp"
+#include "nnk_tree.hpp"
+
+namespace nnk {
+
+struct GemmJob {
+  const double *const *A;  // nb operator pointers, each At[Kpad][Mpad] (k-major)
+  const int32_t *gidx;     // nb x N gather slots (-1 = none), or null = column index
+  const int32_t *cidx;     // N output slots, or null = column index
+  int nb, N;
+};
+
+struct GemmBatch {
+  DBuf jobs;                 // GemmJob[n]
+  std::vector<DBuf> owned;   // index/pointer arrays referenced by the jobs
+  int njobs = 0, maxN = 0;
+};
+
+struct LevelPlan {
+  int level = 0;
+  // upward: nodes whose equivalent density is computed at this level
+  int n_up = 0;
+  DBuf up_nodes;              // slots (id-1)
+  GemmBatch m2m;              // children -> parent check surface
+  GemmBatch up_solve1, up_solve2;
+  // downward
+  int n_down = 0;
+  DBuf down_nodes;
+  GemmBatch m2l;              // 8 classes x 189 translations (reads up densities)
+  GemmBatch l2l;              // 8 classes, parent down density (accumulates)
+  GemmBatch dn_solve1, dn_solve2;
+  int n_x = 0;
+  DBuf x_nodes;               // nodes with a non-empty X list
+};
+
+struct LevelOps {
+  DBuf Wup, Vup, Wdn, Udn;    // check-to-equivalent factors, At layout
+  DBuf M2M, L2L;              // 8 operators each (this level's child class)
+  DBuf T;                     // 316 translation operators
+  DBuf Tptr;                  // device pointer table for T (343 slots)
+  std::vector<double *> tslot;  // host copy: slot -> device pointer or null
+};
+
+class Fmm {
+ public:
+  Fmm(std::shared_ptr<const Tree> t, double eps, double mu, int res, int shells);
+  // f: npot x 3 (potential order), u: ntgt x 3 (target order); device ptrs.
+  void apply(const double *d_f, double *d_u, hipStream_t s);
+  // leaf-local product (reducedPass.m) on device pointers
+  void reduced_apply(const double *d_f, double *d_u, hipStream_t s);
+  int64_t npot() const { return npot_; }
+  int64_t ntgt() const { return ntgt_; }
+  int nq() const { return nq_; }
+  const Tree &tree() const { return *tp_; }
+  // copy an internal per-node array (0 check, 1 up, 2 down density) to host
+  void debug_copy(int which, double *host) const;
+  int K() const { return K_; }
+
+ private:
+  void plan();
+  void build_ops(int level);
+  void gemm(const GemmBatch &b, const double *X, int ldx, double *C, int ldc, int beta, hipStream_t s);
+  void strengths(const double *d_f, hipStream_t s);
+
+  std::shared_ptr<const Tree> tp_;
+  const Tree &tree_;
+  double eps_, mu_;
+  int res_, shells_, nq_, K_, Kpad_, Mpad_;
+  int64_t npot_, ntgt_, nsrc_ = 0;
+  int maxlev_ = 0;
+  double w0_ = 0;
+  // geometry and point data
+  DBuf corners_, surf_idx_;
+  DBuf src_ptr_, src_pos_, src_q_, src_from_, ctr_ptr_, ctr_col_, ctr_w_;
+  DBuf tgt_ptr_, tgt_pos_, tgt_out_;
+  DBuf uptr_, uidx_, wptr_, widx_, xptr_, xidx_;
+  DBuf pleaves_, tleaves_;
+  int n_pleaves_ = 0, n_tleaves_ = 0;
+  // work buffers
+  DBuf chk_, uppot_, downpot_, tmp_;
+  std::vector<LevelOps> ops_;
+  std::vector<LevelPlan> plans_;
+};
+
+}  // namespace nnk

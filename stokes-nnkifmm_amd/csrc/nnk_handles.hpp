@@ -1,0 +1,14 @@
+// Definitions behind the opaque C handles of include/nnkifmm.h.
+#pragma once
+#include <memory>
+
+#include "nnk_fmm.hpp"
+#include "nnk_tree.hpp"
+
+struct nnk_tree {
+  std::shared_ptr<nnk::Tree> t;
+};
+struct nnk_fmm {
+  std::unique_ptr<nnk::Fmm> f;
+  int64_t npot, ntgt;
+};

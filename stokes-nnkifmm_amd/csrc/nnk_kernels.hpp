@@ -1,0 +1,57 @@
+// Device kernels for the KIFMM matvec (FP64 form).
+//
+// Notation: a node's "up surface" is the res^3 grid on its box minus the
+// interior, its "down surface" the same on the box widened by its width on
+// every side (Tools/KIFMM/genupsurf.m, gendownsurf.m).  Surface point p has
+// grid indices packed in surf_idx[p] = ix | iy<<8 | iz<<16, ordered y
+// fastest, then x, then z.  Densities are stored per node as 3*Nq doubles,
+// interleaved [p0x p0y p0z p1x ...] as in the reference's uppot(node,:).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace nnk {
+
+constexpr double kPi = 3.14159265358979323846;
+
+// linspace(d1,d2,n)(k): d1 + (k*(d2-d1))/(n-1) with exact endpoints, as
+// MATLAB's linspace computes it.
+__device__ __forceinline__ double lin(double d1, double d2, int k, int n1) {
+  if (k == 0) return d1;
+  if (k == n1) return d2;
+  return d1 + ((double)k * (d2 - d1)) / (double)n1;
+}
+
+__device__ __forceinline__ void surf_point(const double *__restrict__ box, int down, int code,
+                                           int n1, double &x, double &y, double &z) {
+  double lo0 = box[0], lo1 = box[1], lo2 = box[2], hi0 = box[3], hi1 = box[4], hi2 = box[5];
+  if (down) {
+    double w0 = hi0 - lo0, w1 = hi1 - lo1, w2 = hi2 - lo2;
+    lo0 = lo0 - w0; hi0 = hi0 + w0;
+    lo1 = lo1 - w1; hi1 = hi1 + w1;
+    lo2 = lo2 - w2; hi2 = hi2 + w2;
+  }
+  x = lin(lo0, hi0, code & 255, n1);
+  y = lin(lo1, hi1, (code >> 8) & 255, n1);
+  z = lin(lo2, hi2, (code >> 16) & 255, n1);
+}
+
+// u += S(t <- s) q without the 1/(8 pi mu) factor:
+// S q = [(r^2 + 2 eps^2) q + r (r.q)] / (r^2 + eps^2)^{3/2},  r = t - s.
+__device__ __forceinline__ void stokeslet_acc(double tx, double ty, double tz, double sx, double sy,
+                                              double sz, double qx, double qy, double qz, double e2,
+                                              double &ux, double &uy, double &uz) {
+  double rx = tx - sx, ry = ty - sy, rz = tz - sz;
+  double r2 = rx * rx + ry * ry + rz * rz;
+  double re = r2 + e2;
+  double inv = rsqrt(re);
+  double inv3 = inv * inv * inv;
+  double iso = (re + e2) * inv3;
+  double rq = (rx * qx + ry * qy + rz * qz) * inv3;
+  ux += iso * qx + rq * rx;
+  uy += iso * qy + rq * ry;
+  uz += iso * qz + rq * rz;
+}
+
+}  // namespace nnk

@@ -1,0 +1,392 @@
+// Octree construction and interaction lists.
+//
+// Semantics follow the reference exactly (Classes/OcTree.m):
+//  * points = unique([pot; tgt],'rows','stable'); masks by ismember   (:80-82)
+//  * root cube [m m m M M M] from the min/max over all coordinates      (:97-98)
+//  * capacity = min(nodeCapacity, nPts-1)                              (:105)
+//  * Divide visits nodes in order, errors when level+1 >= maxDepth,
+//    splits when a node holds more than capacity points, recursing into
+//    the 8 new children before the next sibling (DFS creation order)  (:165-178)
+//  * centre = (lo+hi)/2, child k = 1 + 4[x>c] + 2[y>c] + [z>c]          (:191-221)
+//  * closed-box adjacency                       (Tools/Tree/nodeintersection.m)
+//  * U, V, W lists and the W coarsening, X = find(Wpadded == i)         (:265-366)
+// The lists are found by descending from the root through the nodes whose
+// closed box meets the query box (a child box lies inside its parent box,
+// because child corners are copies of the parent's lo/centre/hi), instead
+// of the reference's scan over all nodes.  The W coarsening is computed from
+// a per-node "complete subtree" flag; the result, including its order
+// (sorted, with the last merged node appended), is the reference's.
+#include "nnk_tree.hpp"
+
+#include <algorithm>
+#include <cstring>
+
+namespace nnk {
+
+namespace {
+
+struct RowHash {
+  std::vector<int64_t> slot;
+  uint64_t mask = 0;
+  const double *base = nullptr;
+  static uint64_t h(const double *p) {
+    uint64_t x = 0x243f6a8885a308d3ull;
+    for (int d = 0; d < 3; d++) {
+      double v = p[d] == 0.0 ? 0.0 : p[d];  // -0 and 0 compare equal
+      uint64_t b;
+      std::memcpy(&b, &v, 8);
+      x ^= b;
+      x *= 0x9e3779b97f4a7c15ull;
+      x ^= x >> 29;
+    }
+    return x;
+  }
+  void init(int64_t n, const double *b) {
+    uint64_t cap = 16;
+    while (cap < (uint64_t)(2 * n + 16)) cap <<= 1;
+    slot.assign(cap, -1);
+    mask = cap - 1;
+    base = b;
+  }
+  // index of an equal row already stored, or -1 (then inserts idx if asked)
+  int64_t find(const double *p, int64_t idx, bool insert) {
+    uint64_t s = h(p) & mask;
+    while (slot[s] >= 0) {
+      const double *q = base + 3 * slot[s];
+      if (q[0] == p[0] && q[1] == p[1] && q[2] == p[2]) return slot[s];
+      s = (s + 1) & mask;
+    }
+    if (insert) slot[s] = idx;
+    return -1;
+  }
+};
+
+inline bool isect(const double *c1, const double *c2) {
+  return !(c2[0] > c1[3] || c2[3] < c1[0] || c2[1] > c1[4] || c2[4] < c1[1] ||
+           c2[2] > c1[5] || c2[5] < c1[2]);
+}
+
+struct Builder {
+  Tree &t;
+  std::vector<int64_t> perm;               // point rows, grouped by node
+  std::vector<int64_t> beg, end;           // per node [id-1]
+  std::string &err;
+  Builder(Tree &tt, std::string &e) : t(tt), err(e) {}
+
+  void add_nodes(int k) {
+    size_t need = (size_t)t.count + k;
+    if (t.level.size() < need) {
+      size_t na = std::max<size_t>(need, 2 * t.level.size());
+      t.corners.resize(6 * na);
+      t.level.resize(na);
+      t.parent.resize(na);
+      t.children.resize(8 * na);
+      beg.resize(na);
+      end.resize(na);
+    }
+  }
+
+  void divide_node(int node) {
+    int base = t.count;
+    add_nodes(8);
+    const double *c = &t.corners[6 * (node - 1)];
+    double lo[3] = {c[0], c[1], c[2]}, hi[3] = {c[3], c[4], c[5]}, ce[3];
+    for (int d = 0; d < 3; d++) ce[d] = (lo[d] + hi[d]) / 2.0;
+    const double co[9] = {lo[0], lo[1], lo[2], ce[0], ce[1], ce[2], hi[0], hi[1], hi[2]};
+    static const int rows[8][6] = {
+        {0, 1, 2, 3, 4, 5}, {0, 1, 5, 3, 4, 8}, {0, 4, 2, 3, 7, 5}, {0, 4, 5, 3, 7, 8},
+        {3, 1, 2, 6, 4, 5}, {3, 1, 5, 6, 4, 8}, {3, 4, 2, 6, 7, 5}, {3, 4, 5, 6, 7, 8}};
+    for (int k = 0; k < 8; k++) {
+      int id = base + k + 1;
+      for (int e = 0; e < 6; e++) t.corners[6 * (id - 1) + e] = co[rows[k][e]];
+      t.level[id - 1] = t.level[node - 1] + 1;
+      t.parent[id - 1] = node;
+      std::fill(&t.children[8 * (id - 1)], &t.children[8 * id], 0);
+      t.children[8 * (node - 1) + k] = id;
+    }
+    // counting partition of this node's points into the 8 children
+    int64_t b = beg[node - 1], e = end[node - 1];
+    int64_t cnt[8] = {0};
+    std::vector<uint8_t> loc(e - b);
+    for (int64_t i = b; i < e; i++) {
+      const double *p = &t.points[3 * perm[i]];
+      int k = 4 * (p[0] > ce[0]) + 2 * (p[1] > ce[1]) + (p[2] > ce[2]);
+      loc[i - b] = (uint8_t)k;
+      cnt[k]++;
+    }
+    int64_t off[8];
+    int64_t run = b;
+    for (int k = 0; k < 8; k++) {
+      off[k] = run;
+      beg[base + k] = run;
+      run += cnt[k];
+      end[base + k] = run;
+    }
+    std::vector<int64_t> tmp(perm.begin() + b, perm.begin() + e);
+    for (int64_t i = 0; i < e - b; i++) {
+      int k = loc[i];
+      perm[off[k]++] = tmp[i];
+      t.point_index[tmp[i]] = base + k + 1;
+    }
+    t.count = base + 8;
+  }
+
+  int divide(const int *nodes, int nn) {
+    for (int i = 0; i < nn; i++) {
+      int node = nodes[i];
+      if (t.level[node - 1] + 1 >= t.max_depth) {
+        err = "Max level reached: Increase Max Depth";
+        return NNK_EMAXDEPTH;
+      }
+      int old = t.count;
+      if (end[node - 1] - beg[node - 1] > t.capacity) {
+        divide_node(node);
+        int kids[8];
+        for (int k = 0; k < 8; k++) kids[k] = old + k + 1;
+        int rc = divide(kids, 8);
+        if (rc) return rc;
+      }
+    }
+    return NNK_OK;
+  }
+};
+
+// Visit every node whose closed box meets q and whose level <= maxlev.
+template <class F>
+void descend(const Tree &t, const double *q, int maxlev, F &&visit) {
+  int stack[8 * 64];
+  int sp = 0;
+  stack[sp++] = 1;
+  while (sp) {
+    int n = stack[--sp];
+    if (!isect(q, t.box(n))) continue;
+    visit(n);
+    if (!t.is_leaf(n) && t.level[n - 1] < maxlev)
+      for (int k = 7; k >= 0; k--) stack[sp++] = t.children[8 * (n - 1) + k];
+  }
+}
+
+void build_lists(Tree &t) {
+  const int nc = t.count;
+  // complete(x): x is a potential leaf, or a non-leaf whose 8 children are
+  // complete -- exactly the nodes the W coarsening ends up merging.
+  std::vector<uint8_t> complete(nc + 1, 0);
+  {
+    std::vector<int> order(nc);
+    for (int i = 0; i < nc; i++) order[i] = i + 1;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int a, int b) { return t.level[a - 1] > t.level[b - 1]; });
+    for (int n : order) {
+      if (t.is_leaf(n)) complete[n] = t.pot_leaf[n];
+      else {
+        bool all = true;
+        for (int k = 0; k < 8; k++) all = all && complete[t.children[8 * (n - 1) + k]];
+        complete[n] = all;
+      }
+    }
+  }
+  std::vector<std::vector<int32_t>> L[3];
+  for (int q = 0; q < 3; q++) L[q].assign(nc + 1, {});
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int i = 2; i <= nc; i++) {
+    const double *bi = t.box(i);
+    const int li = t.level[i - 1];
+    const bool tl = t.tgt_leaf[i];
+    // U: potential leaves adjacent to i, sorted (OcTree.m:323-327)
+    if (tl) {
+      auto &U = L[0][i];
+      descend(t, bi, 1 << 30, [&](int n) {
+        if (t.is_leaf(n) && t.pot_leaf[n]) U.push_back(n);
+      });
+      std::sort(U.begin(), U.end());
+    }
+    // V: children of the non-leaf same-level neighbours of the parent that
+    // are not adjacent to i (OcTree.m:330-339)
+    {
+      auto &V = L[1][i];
+      int P = t.parent[i - 1];
+      int lp = t.level[P - 1];
+      descend(t, t.box(P), lp, [&](int n) {
+        if (n == P || t.level[n - 1] != lp || t.is_leaf(n)) return;
+        for (int k = 0; k < 8; k++) {
+          int c = t.children[8 * (n - 1) + k];
+          if (!isect(bi, t.box(c))) V.push_back(c);
+        }
+      });
+      std::sort(V.begin(), V.end());
+    }
+    // W: potential leaves under the non-leaf same-level neighbours of i
+    // that are not adjacent to i, coarsened (OcTree.m:343-360)
+    if (tl) {
+      auto &W = L[2][i];
+      std::vector<int> st;
+      descend(t, bi, li, [&](int n) {
+        if (t.level[n - 1] == li && !t.is_leaf(n)) st.push_back(n);
+      });
+      if (!st.empty()) {
+        std::vector<int> stack;
+        for (int c : st)
+          for (int k = 0; k < 8; k++) stack.push_back(t.children[8 * (c - 1) + k]);
+        int last = 0;  // last merged node: smallest level, then largest id
+        while (!stack.empty()) {
+          int y = stack.back();
+          stack.pop_back();
+          bool leaf = t.is_leaf(y);
+          if (isect(bi, t.box(y))) {
+            if (!leaf)
+              for (int k = 0; k < 8; k++) stack.push_back(t.children[8 * (y - 1) + k]);
+          } else if (complete[y]) {
+            W.push_back(y);
+            if (!leaf) {
+              if (!last || t.level[y - 1] < t.level[last - 1] ||
+                  (t.level[y - 1] == t.level[last - 1] && y > last))
+                last = y;
+            }
+          } else if (!leaf) {
+            for (int k = 0; k < 8; k++) stack.push_back(t.children[8 * (y - 1) + k]);
+          }
+        }
+        std::sort(W.begin(), W.end());
+        if (last) {
+          W.erase(std::find(W.begin(), W.end(), last));
+          W.push_back(last);
+        }
+      }
+    }
+  }
+  // X(i) = { j : i in W(j) }, in the order of find() on the padded W
+  // matrix: position in W(j) first, then j (OcTree.m:279-288).
+  std::vector<std::vector<int32_t>> X(nc + 1);
+  size_t maxw = 0;
+  for (int n = 1; n <= nc; n++) maxw = std::max(maxw, L[2][n].size());
+  {
+    std::vector<int32_t> withw;
+    for (int n = 1; n <= nc; n++)
+      if (!L[2][n].empty()) withw.push_back(n);
+    for (size_t k = 0; k < maxw; k++)
+      for (int j : withw)
+        if (L[2][j].size() > k) X[L[2][j][k]].push_back(j);
+  }
+  std::vector<std::vector<int32_t>> *all[4] = {&L[0], &L[1], &L[2], &X};
+  for (int q = 0; q < 4; q++) {
+    auto &src = *all[q];
+    t.lptr[q].assign(nc + 2, 0);
+    for (int n = 1; n <= nc; n++) t.lptr[q][n + 1] = t.lptr[q][n] + (int64_t)src[n].size();
+    t.lidx[q].resize(t.lptr[q][nc + 1]);
+    for (int n = 1; n <= nc; n++)
+      std::copy(src[n].begin(), src[n].end(), t.lidx[q].begin() + t.lptr[q][n]);
+  }
+}
+
+}  // namespace
+
+int build_tree(const double *pot, int64_t npot, const double *tgt, int64_t ntgt, int capacity,
+               int max_depth, Tree &t, std::string &err) {
+  t = Tree();
+  if (npot < 0 || ntgt < 0 || npot + ntgt < 2) {
+    err = "OcTree needs at least two points";
+    return NNK_EARG;
+  }
+  int64_t ntot = npot + ntgt;
+  for (int64_t i = 0; i < 3 * npot; i++)
+    if (pot[i] != pot[i]) { err = "OcTree: NaN in potential points"; return NNK_EARG; }
+  for (int64_t i = 0; i < 3 * ntgt; i++)
+    if (tgt[i] != tgt[i]) { err = "OcTree: NaN in target points"; return NNK_EARG; }
+  t.points.resize(3 * ntot);
+  RowHash h;
+  h.init(ntot, t.points.data());
+  int64_t np = 0;
+  for (int64_t i = 0; i < ntot; i++) {
+    const double *p = i < npot ? pot + 3 * i : tgt + 3 * (i - npot);
+    std::memcpy(&t.points[3 * np], p, 24);
+    if (h.find(&t.points[3 * np], np, true) < 0) np++;
+  }
+  t.points.resize(3 * np);
+  t.npts = np;
+  t.is_pot.assign(np, 0);
+  t.is_tgt.assign(np, 0);
+  for (int64_t i = 0; i < npot; i++) t.is_pot[h.find(pot + 3 * i, -1, false)] = 1;
+  for (int64_t i = 0; i < ntgt; i++) t.is_tgt[h.find(tgt + 3 * i, -1, false)] = 1;
+  t.pot_ord.assign(np, -1);
+  t.tgt_ord.assign(np, -1);
+  for (int64_t i = 0; i < np; i++) {
+    if (t.is_pot[i]) t.pot_ord[i] = t.npot++;
+    if (t.is_tgt[i]) t.tgt_ord[i] = t.ntgt++;
+  }
+  if (np < 2) {
+    err = "OcTree needs at least two distinct points";
+    return NNK_EARG;
+  }
+  double m = t.points[0], M = t.points[0];
+  for (int64_t i = 0; i < 3 * np; i++) {
+    m = std::min(m, t.points[i]);
+    M = std::max(M, t.points[i]);
+  }
+  t.max_depth = max_depth;
+  t.capacity = (int32_t)std::min<int64_t>(capacity, np - 1);
+  Builder b(t, err);
+  t.count = 0;
+  b.add_nodes(1);
+  t.count = 1;
+  const double rc[6] = {m, m, m, M, M, M};
+  std::memcpy(&t.corners[0], rc, sizeof rc);
+  t.level[0] = 0;
+  t.parent[0] = 0;
+  std::fill(&t.children[0], &t.children[8], 0);
+  t.point_index.assign(np, 1);
+  b.perm.resize(np);
+  for (int64_t i = 0; i < np; i++) b.perm[i] = i;
+  b.beg[0] = 0;
+  b.end[0] = np;
+  int root = 1;
+  int rcode = b.divide(&root, 1);
+  if (rcode) return rcode;
+  t.corners.resize(6 * t.count);
+  t.level.resize(t.count);
+  t.parent.resize(t.count);
+  t.children.resize(8 * t.count);
+  t.max_level = 0;
+  for (int n = 0; n < t.count; n++) t.max_level = std::max(t.max_level, t.level[n]);
+  t.pot_leaf.assign(t.count + 1, 0);
+  t.tgt_leaf.assign(t.count + 1, 0);
+  for (int64_t i = 0; i < np; i++) {
+    if (t.is_pot[i]) t.pot_leaf[t.point_index[i]] = 1;
+    if (t.is_tgt[i]) t.tgt_leaf[t.point_index[i]] = 1;
+  }
+  t.pt_ptr.assign(t.count + 2, 0);
+  for (int64_t i = 0; i < np; i++) t.pt_ptr[t.point_index[i] + 1]++;
+  for (int n = 1; n <= t.count; n++) t.pt_ptr[n + 1] += t.pt_ptr[n];
+  t.pt_rows.resize(np);
+  {
+    std::vector<int64_t> fill(t.count + 2, 0);
+    for (int64_t i = 0; i < np; i++) {
+      int n = t.point_index[i];
+      t.pt_rows[t.pt_ptr[n] + fill[n]++] = i;
+    }
+  }
+  build_lists(t);
+  return NNK_OK;
+}
+
+int set_nearest(Tree &t, const double *fine, int64_t nfine, const int64_t *colptr,
+                const int64_t *row, const double *val, std::string &err) {
+  if (nfine < 0) { err = "negative fine point count"; return NNK_EARG; }
+  if (t.npot != t.ntgt) {
+    err = "NEAREST mode needs potential points == target points";
+    return NNK_EARG;
+  }
+  int64_t nnz = colptr[t.npot];
+  for (int64_t c = 0; c < t.npot; c++)
+    if (colptr[c + 1] < colptr[c]) { err = "NN colptr not monotone"; return NNK_EARG; }
+  for (int64_t e = 0; e < nnz; e++)
+    if (row[e] < 0 || row[e] >= nfine) { err = "NN row index out of range"; return NNK_EARG; }
+  t.nearest = true;
+  t.nfine = nfine;
+  t.fine.assign(fine, fine + 3 * nfine);
+  t.nn_colptr.assign(colptr, colptr + t.npot + 1);
+  t.nn_row.assign(row, row + nnz);
+  t.nn_val.assign(val, val + nnz);
+  return NNK_OK;
+}
+
+}  // namespace nnk

@@ -1,0 +1,85 @@
+// Test-only entry points (not part of the C ABI): exercise the GEMM kernel
+// against a host product, and read back the FMM's per-node arrays.
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "../nnk_gemm.hpp"
+#include "../nnk_handles.hpp"
+
+using namespace nnk;
+
+extern "C" int nnk_selftest_gemm(int M, int K, int Mpad, int Kpad, const double *At, int nb,
+                                 const double *X, int nx, int ldx, const int *gidx,
+                                 const int *cidx, int N, double *C, int nc, int ldc, int beta) {
+  try {
+    DBuf dA, dX, dC, dG, dCi, dP, dJ;
+    size_t mat = (size_t)Kpad * Mpad;
+    dA.alloc(sizeof(double) * mat * nb);
+    NNK_HIP(hipMemcpy(dA.p, At, sizeof(double) * mat * nb, hipMemcpyHostToDevice));
+    dX.alloc(sizeof(double) * (size_t)nx * ldx);
+    NNK_HIP(hipMemcpy(dX.p, X, sizeof(double) * (size_t)nx * ldx, hipMemcpyHostToDevice));
+    dC.alloc(sizeof(double) * (size_t)nc * ldc);
+    NNK_HIP(hipMemcpy(dC.p, C, sizeof(double) * (size_t)nc * ldc, hipMemcpyHostToDevice));
+    std::vector<const double *> ptrs(nb);
+    for (int b = 0; b < nb; b++) ptrs[b] = dA.as<double>() + b * mat;
+    dP.upload(ptrs);
+    GemmJob j{};
+    j.A = dP.as<const double *>();
+    if (gidx) { dG.upload(std::vector<int>(gidx, gidx + (size_t)nb * N)); j.gidx = dG.as<int32_t>(); }
+    if (cidx) { dCi.upload(std::vector<int>(cidx, cidx + N)); j.cidx = dCi.as<int32_t>(); }
+    j.nb = nb;
+    j.N = N;
+    dJ.upload(std::vector<GemmJob>{j});
+    dim3 grid((N + 63) / 64, Mpad / 160, 1);
+    hipLaunchKernelGGL((k_gemm_gather<160, 64, 24>), grid, dim3(256), 0, 0, M, K, Mpad,
+                       dX.as<double>(), ldx, dC.as<double>(), ldc, beta, dJ.as<GemmJob>());
+    NNK_HIP(hipGetLastError());
+    NNK_HIP(hipDeviceSynchronize());
+    NNK_HIP(hipMemcpy(C, dC.p, sizeof(double) * (size_t)nc * ldc, hipMemcpyDeviceToHost));
+    return 0;
+  } catch (...) {
+    return 1;
+  }
+}
+
+extern "C" int nnk_selftest_fmm_dump(nnk_fmm *h, int which, double *out) {
+  try {
+    h->f->debug_copy(which, out);
+    return 0;
+  } catch (...) {
+    return 1;
+  }
+}
+
+extern "C" int nnk_selftest_fmm_K(nnk_fmm *h) { return h->f->K(); }
+
+// One v_mfma_f64_16x16x4_f64 per trial on a single wave: lane l supplies
+// a[t*64+l], b[t*64+l]; d[(t*64+l)*4+r] receives accumulator r.
+__global__ void k_mfma_probe(const double *a, const double *b, double *d, int trials) {
+  const int l = threadIdx.x;
+  for (int t = 0; t < trials; t++) {
+    d4 c = {0, 0, 0, 0};
+    c = __builtin_amdgcn_mfma_f64_16x16x4f64(a[t * 64 + l], b[t * 64 + l], c, 0, 0, 0);
+    for (int r = 0; r < 4; r++) d[(t * 64 + l) * 4 + r] = c[r];
+  }
+}
+
+extern "C" int nnk_selftest_mfma_probe(const double *a, const double *b, double *d, int trials) {
+  try {
+    DBuf da, db, dd;
+    da.alloc(sizeof(double) * 64 * trials);
+    db.alloc(sizeof(double) * 64 * trials);
+    dd.alloc(sizeof(double) * 256 * trials);
+    NNK_HIP(hipMemcpy(da.p, a, sizeof(double) * 64 * trials, hipMemcpyHostToDevice));
+    NNK_HIP(hipMemcpy(db.p, b, sizeof(double) * 64 * trials, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_mfma_probe, dim3(1), dim3(64), 0, 0, da.as<double>(), db.as<double>(),
+                       dd.as<double>(), trials);
+    NNK_HIP(hipGetLastError());
+    NNK_HIP(hipDeviceSynchronize());
+    NNK_HIP(hipMemcpy(d, dd.p, sizeof(double) * 256 * trials, hipMemcpyDeviceToHost));
+    return 0;
+  } catch (...) {
+    return 1;
+  }
+}
