@@ -101,6 +101,31 @@ int32_t nnk_fmm_reduced_apply_device(nnk_fmm *h, const double *d_f, double *d_u,
 
 void nnk_fmm_destroy(nnk_fmm *h);
 
+/* Replaces: blockcomputation(x, x0, q, blockSize, [eps mu])
+ *           Tools/Other/blockcomputation.m:17-28 (dense K(x->x0) q, here
+ *           without forming K).  src: ns x 3, q: ns x 3, tgt: nt x 3,
+ *           u: nt x 3. */
+int32_t nnk_direct_apply(const double *src, int64_t ns, const double *q, const double *tgt,
+                         int64_t nt, double eps, double mu, double *u);
+int32_t nnk_direct_apply_device(const double *d_src, int64_t ns, const double *d_q,
+                                const double *d_tgt, int64_t nt, double eps, double mu,
+                                double *d_u, void *stream);
+
+/* No counterpart in the reference (which times with tic/toc): per-phase
+ * HIP-event timing of nnk_fmm_apply* on the apply stream.  Phases:
+ * 0 strengths, 1 S2M, 2 upward GEMMs (M2M + solves), 3 M2L, 4 L2L, 5 P2L,
+ * 6 downward solves, 7 leaf L2T+P2P+M2T.  nnk_fmm_phase_times returns the
+ * totals (ms) and launch counts recorded since the previous call. */
+int32_t nnk_fmm_set_profiling(nnk_fmm *h, int32_t on);
+int32_t nnk_fmm_phase_times(nnk_fmm *h, double *ms, int64_t *count, int32_t nphases);
+
+/* No counterpart in the reference: work counts of one matvec, for roofline
+ * accounting.  0 M2L pairs, 1 P2P pairs, 2 S2M pairs, 3 L2T pairs,
+ * 4 M2T pairs, 5 P2L pairs, 6 up-solve nodes, 7 down-solve nodes, 8 M2M
+ * parents, 9 L2L nodes, 10 sources, 11 targets; out[nstats]; also returns
+ * the density length 3*Nq in out[nstats] if nstats > 12. */
+int32_t nnk_fmm_stats(const nnk_fmm *h, int64_t *out, int32_t nstats);
+
 #ifdef __cplusplus
 }
 #endif

@@ -12,6 +12,11 @@
 #include "nnk_tree.hpp"
 
 
+namespace nnk {
+void direct_device(const double *src, int64_t ns, const double *q, const double *tgt, int64_t nt,
+                   double eps, double mu, double *out, hipStream_t s);
+}
+
 namespace {
 thread_local std::string g_err;
 
@@ -168,5 +173,60 @@ int32_t nnk_fmm_reduced_apply_device(nnk_fmm *h, const double *d_f, double *d_u,
 }
 
 void nnk_fmm_destroy(nnk_fmm *h) { delete h; }
+
+int32_t nnk_direct_apply(const double *src, int64_t ns, const double *q, const double *tgt,
+                         int64_t nt, double eps, double mu, double *u) {
+  return guard([&]() -> int32_t {
+    if (ns < 0 || nt < 0 || (ns && (!src || !q)) || (nt && (!tgt || !u)))
+      return fail(NNK_EARG, "nnk_direct_apply: bad argument");
+    if (!(eps > 0) || !(mu > 0)) return fail(NNK_EARG, "nnk_direct_apply: eps and mu must be > 0");
+    nnk::DBuf ds, dq, dt, du;
+    ds.alloc(sizeof(double) * 3 * std::max<int64_t>(ns, 1));
+    dq.alloc(sizeof(double) * 3 * std::max<int64_t>(ns, 1));
+    dt.alloc(sizeof(double) * 3 * std::max<int64_t>(nt, 1));
+    du.alloc(sizeof(double) * 3 * std::max<int64_t>(nt, 1));
+    NNK_HIP(hipMemcpy(ds.p, src, sizeof(double) * 3 * ns, hipMemcpyHostToDevice));
+    NNK_HIP(hipMemcpy(dq.p, q, sizeof(double) * 3 * ns, hipMemcpyHostToDevice));
+    NNK_HIP(hipMemcpy(dt.p, tgt, sizeof(double) * 3 * nt, hipMemcpyHostToDevice));
+    nnk::direct_device(ds.as<double>(), ns, dq.as<double>(), dt.as<double>(), nt, eps, mu,
+                       du.as<double>(), 0);
+    NNK_HIP(hipMemcpy(u, du.p, sizeof(double) * 3 * nt, hipMemcpyDeviceToHost));
+    return NNK_OK;
+  });
+}
+
+int32_t nnk_direct_apply_device(const double *d_src, int64_t ns, const double *d_q,
+                                const double *d_tgt, int64_t nt, double eps, double mu,
+                                double *d_u, void *stream) {
+  return guard([&]() -> int32_t {
+    if (ns < 0 || nt < 0 || (ns && (!d_src || !d_q)) || (nt && (!d_tgt || !d_u)))
+      return fail(NNK_EARG, "nnk_direct_apply_device: bad argument");
+    if (!(eps > 0) || !(mu > 0)) return fail(NNK_EARG, "nnk_direct_apply_device: eps and mu must be > 0");
+    nnk::direct_device(d_src, ns, d_q, d_tgt, nt, eps, mu, d_u, (hipStream_t)stream);
+    return NNK_OK;
+  });
+}
+
+int32_t nnk_fmm_set_profiling(nnk_fmm *h, int32_t on) {
+  if (!h) return fail(NNK_EARG, "nnk_fmm_set_profiling: null handle");
+  h->f->set_profiling(on != 0);
+  return NNK_OK;
+}
+
+int32_t nnk_fmm_phase_times(nnk_fmm *h, double *ms, int64_t *count, int32_t nphases) {
+  return guard([&]() -> int32_t {
+    if (!h || !ms || !count || nphases < nnk::Fmm::kPhases)
+      return fail(NNK_EARG, "nnk_fmm_phase_times: bad argument");
+    h->f->phase_times(ms, count);
+    return NNK_OK;
+  });
+}
+
+int32_t nnk_fmm_stats(const nnk_fmm *h, int64_t *out, int32_t nstats) {
+  if (!h || !out || nstats < nnk::Fmm::kStats) return fail(NNK_EARG, "nnk_fmm_stats: bad argument");
+  h->f->stats(out);
+  if (nstats > nnk::Fmm::kStats) out[nnk::Fmm::kStats] = h->f->K();
+  return NNK_OK;
+}
 
 }  // extern "C"

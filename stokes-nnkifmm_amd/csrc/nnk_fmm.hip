@@ -311,12 +311,23 @@ template <class T> DBuf make_dbuf(const std::vector<T> &v) {
 }  // namespace
 
 void Fmm::gemm(const GemmBatch &b, const double *X, int ldx, double *C, int ldc, int beta,
-               hipStream_t s) {
+               hipStream_t s, int tag) {
   if (b.njobs == 0 || b.maxN == 0) return;
   dim3 grid((b.maxN + GBN - 1) / GBN, Mpad_ / GBM, b.njobs);
-  hipLaunchKernelGGL((k_gemm_gather<GBM, GBN, GBK>), grid, dim3(256), 0, s, K_, K_, Mpad_, X, ldx,
-                     C, ldc, beta, b.jobs.as<GemmJob>());
+  if (tag == 1)
+    hipLaunchKernelGGL((k_gemm_gather<GBM, GBN, GBK, 1>), grid, dim3(256), 0, s, K_, K_, Mpad_, X,
+                       ldx, C, ldc, beta, b.jobs.as<GemmJob>());
+  else
+    hipLaunchKernelGGL((k_gemm_gather<GBM, GBN, GBK, 0>), grid, dim3(256), 0, s, K_, K_, Mpad_, X,
+                       ldx, C, ldc, beta, b.jobs.as<GemmJob>());
   NNK_HIP(hipGetLastError());
+}
+
+Fmm::~Fmm() {
+  for (auto &m : marks_) {
+    (void)hipEventDestroy(m.a);
+    (void)hipEventDestroy(m.b);
+  }
 }
 
 Fmm::Fmm(std::shared_ptr<const Tree> t, double eps, double mu, int res, int shells)
@@ -647,6 +658,7 @@ void Fmm::plan() {
             if (it == pos.end() || std::max({std::abs(o[0]), std::abs(o[1]), std::abs(o[2])}) > 3)
               throw Error(NNK_ESTATE, "V-list node outside the class offset set");
             g[(size_t)it->second * N + j] = v - 1;
+            stats_[0]++;
           }
         }
         add_job(P.m2l, jobs, A, &g, &cls[a], N);
@@ -672,6 +684,63 @@ void Fmm::plan() {
     P.x_nodes.upload(xs);
   }
   tmp_.alloc(sizeof(double) * maxn * K_);
+  // remaining work counts
+  std::vector<int64_t> srcn(nc + 2, 0), tgtn(nc + 2, 0);
+  {
+    std::vector<int32_t> sp(nc + 2), tp(nc + 2);
+    NNK_HIP(hipMemcpy(sp.data(), src_ptr_.p, sizeof(int32_t) * (nc + 2), hipMemcpyDeviceToHost));
+    NNK_HIP(hipMemcpy(tp.data(), tgt_ptr_.p, sizeof(int32_t) * (nc + 2), hipMemcpyDeviceToHost));
+    for (int n = 1; n <= nc; n++) { srcn[n] = sp[n + 1] - sp[n]; tgtn[n] = tp[n + 1] - tp[n]; }
+  }
+  for (int n = 1; n <= nc; n++) {
+    stats_[10] += srcn[n];
+    stats_[11] += tgtn[n];
+    if (T.pot_leaf[n] && T.level[n - 1] >= 2) stats_[2] += srcn[n] * nq_;
+    if (T.tgt_leaf[n]) {
+      stats_[3] += tgtn[n] * nq_;
+      for (int64_t e = T.lptr[0][n]; e < T.lptr[0][n + 1]; e++) stats_[1] += tgtn[n] * srcn[T.lidx[0][e]];
+      stats_[4] += tgtn[n] * nq_ * (T.lptr[2][n + 1] - T.lptr[2][n]);
+    }
+    if (tgt_below[n] && T.level[n - 1] >= 2)
+      for (int64_t e = T.lptr[3][n]; e < T.lptr[3][n + 1]; e++) stats_[5] += nq_ * srcn[T.lidx[3][e]];
+  }
+  for (int l = 2; l <= maxlev_; l++) {
+    stats_[6] += plans_[l].n_up;
+    stats_[7] += plans_[l].n_down;
+    for (int n : bylev[l]) {
+      if (!T.is_leaf(n) && pot_below[n]) stats_[8]++;
+      if (l >= 3 && tgt_below[n]) stats_[9]++;
+    }
+  }
+}
+
+void Fmm::stats(int64_t *out) const {
+  for (int i = 0; i < kStats; i++) out[i] = stats_[i];
+}
+
+void Fmm::set_profiling(bool on) { prof_ = on; }
+
+void Fmm::mark(int phase, bool begin, hipStream_t s) {
+  if (!prof_) return;
+  hipEvent_t e;
+  NNK_HIP(hipEventCreate(&e));
+  NNK_HIP(hipEventRecord(e, s));
+  if (begin) open_ = e;
+  else marks_.push_back({phase, open_, e});
+}
+
+void Fmm::phase_times(double *ms, int64_t *count) {
+  for (int i = 0; i < kPhases; i++) { ms[i] = 0; count[i] = 0; }
+  for (auto &m : marks_) {
+    NNK_HIP(hipEventSynchronize(m.b));
+    float t = 0;
+    NNK_HIP(hipEventElapsedTime(&t, m.a, m.b));
+    ms[m.phase] += t;
+    count[m.phase]++;
+    (void)hipEventDestroy(m.a);
+    (void)hipEventDestroy(m.b);
+  }
+  marks_.clear();
 }
 
 void Fmm::strengths(const double *d_f, hipStream_t s) {
@@ -684,7 +753,9 @@ void Fmm::strengths(const double *d_f, hipStream_t s) {
 
 void Fmm::apply(const double *d_f, double *d_u, hipStream_t s) {
   const double e2 = eps_ * eps_, c = 1.0 / (8.0 * kPi * mu_);
+  mark(0, true, s);
   strengths(d_f, s);
+  mark(0, false, s);
   double *chk = chk_.as<double>(), *up = uppot_.as<double>(), *dn = downpot_.as<double>(),
          *tmp = tmp_.as<double>();
   // S2M for every potential leaf at level >= 2 (coarser up densities are unused)
@@ -700,9 +771,12 @@ void Fmm::apply(const double *d_f, double *d_u, hipStream_t s) {
     a.src_q = src_q_.as<double>();
     a.e2 = e2; a.c = c;
     a.chk = chk;
+    mark(1, true, s);
     hipLaunchKernelGGL(k_surf_eval, dim3(n_pleaves_), dim3(256), 0, s, a);
     NNK_HIP(hipGetLastError());
+    mark(1, false, s);
   }
+  mark(2, true, s);
   for (int l = maxlev_; l >= 2; l--) {
     LevelPlan &P = plans_[l];
     if (!P.n_up) continue;
@@ -710,12 +784,18 @@ void Fmm::apply(const double *d_f, double *d_u, hipStream_t s) {
     gemm(P.up_solve1, chk, K_, tmp, K_, 0, s);
     gemm(P.up_solve2, tmp, K_, up, K_, 0, s);
   }
+  mark(2, false, s);
   for (int l = 2; l <= maxlev_; l++) {
     LevelPlan &P = plans_[l];
     if (!P.n_down) continue;
-    gemm(P.m2l, up, K_, chk, K_, 0, s);
+    mark(3, true, s);
+    gemm(P.m2l, up, K_, chk, K_, 0, s, 1);
+    mark(3, false, s);
+    mark(4, true, s);
     gemm(P.l2l, dn, K_, chk, K_, 1, s);
+    mark(4, false, s);
     if (P.n_x) {
+      mark(5, true, s);
       SurfArgs a{};
       a.nodes = P.x_nodes.as<int32_t>();
       a.lptr = xptr_.as<int64_t>();
@@ -732,9 +812,12 @@ void Fmm::apply(const double *d_f, double *d_u, hipStream_t s) {
       a.chk = chk;
       hipLaunchKernelGGL(k_surf_eval, dim3(P.n_x), dim3(256), 0, s, a);
       NNK_HIP(hipGetLastError());
+      mark(5, false, s);
     }
+    mark(6, true, s);
     gemm(P.dn_solve1, chk, K_, tmp, K_, 0, s);
     gemm(P.dn_solve2, tmp, K_, dn, K_, 0, s);
+    mark(6, false, s);
   }
   if (n_tleaves_) {
     LeafArgs a{};
@@ -754,8 +837,10 @@ void Fmm::apply(const double *d_f, double *d_u, hipStream_t s) {
     a.e2 = e2; a.c = c;
     a.far = 1;
     a.out = d_u;
+    mark(7, true, s);
     hipLaunchKernelGGL(k_leaf, dim3(n_tleaves_), dim3(256), 0, s, a);
     NNK_HIP(hipGetLastError());
+    mark(7, false, s);
   }
 }
 

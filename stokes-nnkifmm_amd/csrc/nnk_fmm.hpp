@@ -53,6 +53,9 @@ struct LevelOps {
 class Fmm {
  public:
   Fmm(std::shared_ptr<const Tree> t, double eps, double mu, int res, int shells);
+  ~Fmm();
+  Fmm(const Fmm &) = delete;
+  Fmm &operator=(const Fmm &) = delete;
   // f: npot x 3 (potential order), u: ntgt x 3 (target order); device ptrs.
   void apply(const double *d_f, double *d_u, hipStream_t s);
   // leaf-local product (reducedPass.m) on device pointers
@@ -64,12 +67,26 @@ class Fmm {
   // copy an internal per-node array (0 check, 1 up, 2 down density) to host
   void debug_copy(int which, double *host) const;
   int K() const { return K_; }
+  // Phase timing with HIP events on the apply stream (no host sync inside
+  // apply).  Phases: 0 strengths, 1 S2M, 2 upward GEMMs, 3 M2L, 4 L2L,
+  // 5 P2L, 6 downward solves, 7 leaf L2T+P2P+M2T.
+  static constexpr int kPhases = 8;
+  void set_profiling(bool on);
+  // totals in ms and launch counts since the last call; resets
+  void phase_times(double *ms, int64_t *count);
+  // work counts of one matvec: 0 M2L pairs, 1 P2P pairs, 2 S2M pairs,
+  // 3 L2T pairs, 4 M2T pairs, 5 P2L pairs, 6 up-solve nodes,
+  // 7 down-solve nodes, 8 M2M parents, 9 L2L nodes, 10 sources, 11 targets
+  static constexpr int kStats = 12;
+  void stats(int64_t *out) const;
 
  private:
   void plan();
   void build_ops(int level);
-  void gemm(const GemmBatch &b, const double *X, int ldx, double *C, int ldc, int beta, hipStream_t s);
+  void gemm(const GemmBatch &b, const double *X, int ldx, double *C, int ldc, int beta, hipStream_t s,
+            int tag = 0);
   void strengths(const double *d_f, hipStream_t s);
+  void mark(int phase, bool begin, hipStream_t s);
 
   std::shared_ptr<const Tree> tp_;
   const Tree &tree_;
@@ -89,6 +106,11 @@ class Fmm {
   DBuf chk_, uppot_, downpot_, tmp_;
   std::vector<LevelOps> ops_;
   std::vector<LevelPlan> plans_;
+  int64_t stats_[kStats] = {0};
+  bool prof_ = false;
+  struct Mark { int phase; hipEvent_t a, b; };
+  std::vector<Mark> marks_;
+  hipEvent_t open_ = nullptr;
 };
 
 }  // namespace nnk

@@ -15,7 +15,8 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 // computing a (BM/2) x (BN/2) tile with v_mfma_f64_16x16x4_f64.  Operand
 // layout (measured): lane l holds A[l%16][l/16] and B[l/16][l%16];
 // accumulator register r of lane l holds D[l/16 + 4r][l%16].
-template <int BM, int BN, int BK>
+// TAG only separates launches by role in profiles (1 = M2L translations).
+template <int BM, int BN, int BK, int TAG>
 __global__ __launch_bounds__(256) void k_gemm_gather(int M, int K, int Mpad,
                                                      const double *__restrict__ X, int ldx,
                                                      double *__restrict__ C, int ldc, int beta,

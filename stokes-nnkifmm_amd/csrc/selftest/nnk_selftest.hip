@@ -32,7 +32,7 @@ extern "C" int nnk_selftest_gemm(int M, int K, int Mpad, int Kpad, const double 
     j.N = N;
     dJ.upload(std::vector<GemmJob>{j});
     dim3 grid((N + 63) / 64, Mpad / 160, 1);
-    hipLaunchKernelGGL((k_gemm_gather<160, 64, 24>), grid, dim3(256), 0, 0, M, K, Mpad,
+    hipLaunchKernelGGL((k_gemm_gather<160, 64, 24, 0>), grid, dim3(256), 0, 0, M, K, Mpad,
                        dX.as<double>(), ldx, dC.as<double>(), ldc, beta, dJ.as<GemmJob>());
     NNK_HIP(hipGetLastError());
     NNK_HIP(hipDeviceSynchronize());
