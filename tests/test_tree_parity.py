@@ -1,0 +1,87 @@
+"""libnnkifmm host side against the oracle (CPU; no GPU call is made).
+
+The tree and its interaction lists must be bit-exact: same ids, corners,
+point-to-leaf map and list contents in the same order.  Also: error paths
+and the exported C ABI.
+"""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from geometry import blobs, cube_sphere, grid16
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _cases():
+    rng = np.random.default_rng(11)
+    p = rng.random((4000, 3))
+    yield "uniform", p, p, 32
+    b = blobs(rng)
+    yield "blobs", b, b, 12
+    s = cube_sphere(24) + 0.01 * rng.standard_normal((6 * 24 * 24, 3))
+    yield "shell", s, s, 20
+    q = rng.random((2500, 3))
+    yield "distinct", q, np.concatenate([rng.random((1500, 3)), q[:400]]), 30
+    d = np.concatenate([p[:500], p[:200]])   # duplicated rows
+    yield "duplicates", d, d, 16
+    g = grid16()
+    yield "complete", g, g, 1
+    c = cube_sphere(8)
+    yield "sphere", c, c, 64
+
+
+@pytest.mark.parametrize("name,pot,tgt,cap", list(_cases()), ids=[c[0] for c in _cases()])
+def test_tree_bit_exact(oracle, nnk, name, pot, tgt, cap):
+    a = nnk.OcTree(pot, tgt, "nodeCapacity", cap)
+    b = oracle.Tree(pot, tgt, nodeCapacity=cap)
+    assert a.nodeCount == b.nodeCount and a.nodeCapacity == b.nodeCapacity
+    for attr in ("points", "potentials", "targets", "pointIndex", "nodeCorners", "nodeLevel",
+                 "nodeParents", "nodeChildren"):
+        np.testing.assert_array_equal(getattr(a, attr), getattr(b, attr), err_msg=attr)
+    for kind in range(4):
+        for n in range(1, a.nodeCount + 1):
+            np.testing.assert_array_equal(a.interactions(n, kind + 1), b.interactions(n, kind),
+                                          err_msg=f"list {'UVWX'[kind]} of node {n}")
+
+
+def test_max_depth_and_bad_input(nnk):
+    p = np.concatenate([np.zeros((1, 3)), np.ones((1, 3)), 1e-9 * np.random.default_rng(0).random((30, 3))])
+    with pytest.raises(nnk.NNKError, match="Max level reached"):
+        nnk.OcTree(p, p, "nodeCapacity", 4, "maxDepth", 6)
+    bad = np.random.default_rng(1).random((10, 3))
+    bad[3, 1] = np.nan
+    with pytest.raises(nnk.NNKError, match="NaN"):
+        nnk.OcTree(bad, bad)
+    with pytest.raises(nnk.NNKError):
+        nnk.OcTree(bad[:1] * 0, bad[:1] * 0)   # a single distinct point
+    with pytest.raises(ValueError):
+        nnk.OcTree(p, p, "nodeCapacity")        # unpaired option
+    with pytest.raises(ValueError):
+        nnk.OcTree(p, p, "noSuchOption", 3)
+
+
+def test_options_name_value_and_keywords(nnk):
+    p = np.random.default_rng(2).random((500, 3))
+    a = nnk.OcTree(p, p, "nodeCapacity", 20, "maxDepth", 10)
+    b = nnk.OcTree(p, p, nodeCapacity=20, maxDepth=10)
+    assert a.nodeCount == b.nodeCount and a.arguments["nodeCapacity"] == 20
+    np.testing.assert_array_equal(a.getPotentialPoints(), p)
+    np.testing.assert_array_equal(a.getTargetPoints(), p)
+
+
+def test_library_exports_header_symbols(nnk):
+    with open(os.path.join(ROOT, "include", "nnkifmm.h")) as fh:
+        hdr = fh.read()
+    names = set(re.findall(r"\b(nnk_[a-z_]+)\s*\(", hdr))
+    assert len(names) >= 20
+    lib = ctypes.CDLL(nnk.library_path())
+    missing = [n for n in sorted(names) if not hasattr(lib, n)]
+    assert not missing, missing
+    from importlib import import_module
+    sig = import_module("stokes-nnkifmm_amd._lib").SIGNATURES
+    assert names == set(sig), names ^ set(sig)
+    assert lib.nnk_version() >= 100
