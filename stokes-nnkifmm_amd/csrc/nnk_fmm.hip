@@ -21,6 +21,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <cstdlib>
 #include <map>
 
 #include "nnk_fmm.hpp"
@@ -242,7 +243,7 @@ __global__ __launch_bounds__(256) void k_leaf(LeafArgs a) {
   }
 }
 
-constexpr int GBM = 160, GBN = 64, GBK = 24;
+constexpr int GBM = 160, GBK = 16;
 
 // ---------------------------------------------------------------------------
 // host side
@@ -310,17 +311,27 @@ template <class T> DBuf make_dbuf(const std::vector<T> &v) {
 
 }  // namespace
 
+// tile configurations of the gather GEMM (BM x BN); BK = 16
+template <int BN, int TAG>
+static void launch_gemm(int nrb, int M, int K, int Mpad, int Kpad, const double *X, int ldx,
+                        double *C, int ldc, int beta, const GemmBatch &b, hipStream_t s) {
+  const int nct = (b.maxN + BN - 1) / BN;
+  dim3 grid((unsigned)(b.njobs * nrb * nct));
+  hipLaunchKernelGGL((k_gemm_gather<GBM, BN, GBK, TAG>), grid, dim3(256), 0, s, M, K, Mpad, Kpad,
+                     X, ldx, C, ldc, beta, b.jobs.as<GemmJob>(), b.njobs, nrb);
+  NNK_HIP(hipGetLastError());
+}
+
 void Fmm::gemm(const GemmBatch &b, const double *X, int ldx, double *C, int ldc, int beta,
                hipStream_t s, int tag) {
   if (b.njobs == 0 || b.maxN == 0) return;
-  dim3 grid((b.maxN + GBN - 1) / GBN, Mpad_ / GBM, b.njobs);
-  if (tag == 1)
-    hipLaunchKernelGGL((k_gemm_gather<GBM, GBN, GBK, 1>), grid, dim3(256), 0, s, K_, K_, Mpad_, X,
-                       ldx, C, ldc, beta, b.jobs.as<GemmJob>());
-  else
-    hipLaunchKernelGGL((k_gemm_gather<GBM, GBN, GBK, 0>), grid, dim3(256), 0, s, K_, K_, Mpad_, X,
-                       ldx, C, ldc, beta, b.jobs.as<GemmJob>());
-  NNK_HIP(hipGetLastError());
+  const int nrb = Mpad_ / GBM;
+  if (tag == 1) {
+    if (m2l_cfg_ == 1) launch_gemm<96, 1>(nrb, K_, K_, Mpad_, Kpad_, X, ldx, C, ldc, beta, b, s);
+    else launch_gemm<64, 1>(nrb, K_, K_, Mpad_, Kpad_, X, ldx, C, ldc, beta, b, s);
+  } else {
+    launch_gemm<64, 0>(nrb, K_, K_, Mpad_, Kpad_, X, ldx, C, ldc, beta, b, s);
+  }
 }
 
 Fmm::~Fmm() {
@@ -340,6 +351,7 @@ Fmm::Fmm(std::shared_ptr<const Tree> t, double eps, double mu, int res, int shel
   nq_ = (int)codes.size();
   K_ = 3 * nq_;
   Kpad_ = (K_ + GBK - 1) / GBK * GBK;
+  if (const char *e = getenv("NNK_M2L_TILE")) m2l_cfg_ = atoi(e);
   Mpad_ = (K_ + GBM - 1) / GBM * GBM;
   npot_ = T.npot;
   ntgt_ = T.ntgt;
@@ -569,6 +581,7 @@ void Fmm::plan() {
     if (gidx) { B.owned.emplace_back(make_dbuf(*gidx)); j.gidx = B.owned.back().as<int32_t>(); }
     if (cidx) { B.owned.emplace_back(make_dbuf(*cidx)); j.cidx = B.owned.back().as<int32_t>(); }
     j.nb = (int)A.size();
+    if (j.nb > 512) throw Error(NNK_ESTATE, "too many operator blocks in one GEMM job");
     j.N = N;
     jobs.push_back(j);
     B.maxN = std::max(B.maxN, N);

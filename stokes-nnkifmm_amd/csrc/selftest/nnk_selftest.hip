@@ -31,9 +31,11 @@ extern "C" int nnk_selftest_gemm(int M, int K, int Mpad, int Kpad, const double 
     j.nb = nb;
     j.N = N;
     dJ.upload(std::vector<GemmJob>{j});
-    dim3 grid((N + 63) / 64, Mpad / 160, 1);
-    hipLaunchKernelGGL((k_gemm_gather<160, 64, 24, 0>), grid, dim3(256), 0, 0, M, K, Mpad,
-                       dX.as<double>(), ldx, dC.as<double>(), ldc, beta, dJ.as<GemmJob>());
+    const int nrb = Mpad / 160;
+    if (Kpad % 16) return 2;
+    dim3 grid(nrb * ((N + 63) / 64));
+    hipLaunchKernelGGL((k_gemm_gather<160, 64, 16, 0>), grid, dim3(256), 0, 0, M, K, Mpad, Kpad,
+                       dX.as<double>(), ldx, dC.as<double>(), ldc, beta, dJ.as<GemmJob>(), 1, nrb);
     NNK_HIP(hipGetLastError());
     NNK_HIP(hipDeviceSynchronize());
     NNK_HIP(hipMemcpy(C, dC.p, sizeof(double) * (size_t)nc * ldc, hipMemcpyDeviceToHost));

@@ -157,7 +157,7 @@ def test_gemm_selftest(selftest):
     d = ctypes.POINTER(ctypes.c_double)
     ip = ctypes.POINTER(ctypes.c_int)
     rng = np.random.default_rng(7)
-    M = K = 456; Mpad = 480; Kpad = 456; nb, nx, N, nc = 3, 50, 101, 120
+    M = K = 456; Mpad = 480; Kpad = 464; nb, nx, N, nc = 3, 50, 101, 120
     At = np.zeros((nb, Kpad, Mpad)); At[:, :K, :M] = rng.standard_normal((nb, K, M))
     X = rng.standard_normal((nx, K))
     g = rng.integers(-1, nx, (nb, N)).astype(np.int32)
