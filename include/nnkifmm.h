@@ -29,6 +29,7 @@ extern "C" {
 
 typedef struct nnk_tree nnk_tree;
 typedef struct nnk_fmm nnk_fmm;
+typedef struct nnk_mobility nnk_mobility;
 
 enum {
   NNK_OK = 0,
@@ -99,7 +100,56 @@ int32_t nnk_fmm_apply_device(nnk_fmm *h, const double *d_f, double *d_u, void *s
 int32_t nnk_fmm_reduced_apply(nnk_fmm *h, const double *f, double *u);
 int32_t nnk_fmm_reduced_apply_device(nnk_fmm *h, const double *d_f, double *d_u, void *stream);
 
+/* Replaces: KIFMM.reducedSolve  Classes/KIFMM.m:200-242 (Mobility/ReducedSolve.m):
+ *           per target leaf, K(leaf sources -> leaf targets) \ v, with the NN
+ *           map folded in NEAREST mode.  Needs potentials == targets.  v, f:
+ *           N x 3.  The leaf blocks are LU-factored once, on first use. */
+int32_t nnk_fmm_reduced_solve(nnk_fmm *h, const double *v, double *f);
+int32_t nnk_fmm_reduced_solve_device(nnk_fmm *h, const double *d_v, double *d_f, void *stream);
+
 void nnk_fmm_destroy(nnk_fmm *h);
+
+/* Replaces: NN = nearestMatrix(coarse, fine, blockSize, threads)
+ *           Tools/Nearest/nearestMatrix.m:17-41.  NN is q x n in CSC form:
+ *           colptr (n+1), rowidx (0-based), val; rows ascending in a column.
+ *           If cap < nnz the call fails with NNK_EARG and sets *nnz. */
+int32_t nnk_nearest_matrix(const double *coarse, int64_t n, const double *fine, int64_t q,
+                           int64_t *colptr, int64_t *rowidx, double *val, int64_t cap,
+                           int64_t *nnz);
+
+/* Replaces: the operators of mobilityProblem (Mobility/mobilityProblem.m:35-57):
+ *           B, BT from genMatrixComponets (Mobility/genMatrixComponets.m:334-337)
+ *           as CSR (B: nb6 x 3N, BT: 3N x nb6, nb6 = 6*nsw), A1 = B*B' and
+ *           A2 = BT'*BT dense row-major (nb6 x nb6); fmm is the computeVel
+ *           operator and fmm_sparse its coronaRes-4 twin (computeVelSparse,
+ *           KIFMM.m:107-155; may be null if no preconditioner is used).  Both
+ *           handles must outlive the mobility handle.  Vectors are
+ *           [f (3N, all x, all y, all z); U (3 nsw); Omega (3 nsw)]. */
+int32_t nnk_mobility_create(nnk_fmm *fmm, nnk_fmm *fmm_sparse, int64_t nb6,
+                            const int64_t *B_rowptr, const int32_t *B_col, const double *B_val,
+                            const int64_t *BT_rowptr, const int32_t *BT_col, const double *BT_val,
+                            const double *A1, const double *A2, nnk_mobility **out);
+
+/* Replaces: mobilityMatrixMulti(f, swimmers, B, BT)  Mobility/MobilityMatrixMulti.m:17-36. */
+int32_t nnk_mobility_apply(nnk_mobility *h, const double *x, double *y);
+int32_t nnk_mobility_apply_device(nnk_mobility *h, const double *d_x, double *d_y, void *stream);
+
+/* Replaces: mobilityPrecon(f, swimmers, B, BT, A1, A2, tol, maxIter)
+ *           Mobility/MobilityPrecon.m:17-39 (tol and maxIter are unused there). */
+int32_t nnk_mobility_precon(nnk_mobility *h, const double *x, double *y);
+int32_t nnk_mobility_precon_device(nnk_mobility *h, const double *d_x, double *d_y, void *stream);
+
+/* Replaces: x = gmres(afunc, input, [], tol, maxIter) and, with precon != 0,
+ *           afunc = A(P(x)) followed by x = P(x)  (Mobility/mobilityProblem.m:47-58).
+ *           Unrestarted GMRES from x0 = 0.  rhs, x: host vectors of length
+ *           3N + nb6.  resvec (optional) receives up to maxit+1 residual norms;
+ *           *iters is the iteration count, *relres the true relative residual
+ *           of the (preconditioned) system, *flag 0 if relres <= tol else 1. */
+int32_t nnk_mobility_solve(nnk_mobility *h, const double *rhs, double tol, int32_t maxit,
+                           int32_t precon, double *x, int32_t *flag, double *relres,
+                           int32_t *iters, double *resvec);
+
+void nnk_mobility_destroy(nnk_mobility *h);
 
 /* Replaces: blockcomputation(x, x0, q, blockSize, [eps mu])
  *           Tools/Other/blockcomputation.m:17-28 (dense K(x->x0) q, here

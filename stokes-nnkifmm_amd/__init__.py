@@ -6,7 +6,11 @@ Import as ``importlib.import_module("stokes-nnkifmm_amd")``.
 """
 from ._lib import NNKError, lib  # noqa: F401
 from .kifmm import KIFMM  # noqa: F401
+from .mobility import (genMatrixComponets, mobilityMatrixMulti, mobilityPrecon,  # noqa: F401
+                       mobilityProblem, mobilityProblemGravity)
+from .nearest import nearestMatrix  # noqa: F401
 from .octree import OcTree  # noqa: F401
+from .swimmers import Swimmers  # noqa: F401
 
 
 def library_path():

@@ -44,6 +44,19 @@ SIGNATURES = {
     "nnk_fmm_set_profiling": (ctypes.c_int32, [_vp, ctypes.c_int32]),
     "nnk_fmm_phase_times": (ctypes.c_int32, [_vp, _d, _i64, ctypes.c_int32]),
     "nnk_fmm_stats": (ctypes.c_int32, [_vp, _i64, ctypes.c_int32]),
+    "nnk_fmm_reduced_solve": (ctypes.c_int32, [_vp, _d, _d]),
+    "nnk_fmm_reduced_solve_device": (ctypes.c_int32, [_vp, _vp, _vp, _vp]),
+    "nnk_nearest_matrix": (ctypes.c_int32, [_d, ctypes.c_int64, _d, ctypes.c_int64, _i64, _i64, _d,
+                                            ctypes.c_int64, _i64]),
+    "nnk_mobility_create": (ctypes.c_int32, [_vp, _vp, ctypes.c_int64, _i64, _i32, _d, _i64, _i32, _d,
+                                             _d, _d, _h]),
+    "nnk_mobility_apply": (ctypes.c_int32, [_vp, _d, _d]),
+    "nnk_mobility_apply_device": (ctypes.c_int32, [_vp, _vp, _vp, _vp]),
+    "nnk_mobility_precon": (ctypes.c_int32, [_vp, _d, _d]),
+    "nnk_mobility_precon_device": (ctypes.c_int32, [_vp, _vp, _vp, _vp]),
+    "nnk_mobility_solve": (ctypes.c_int32, [_vp, _d, ctypes.c_double, ctypes.c_int32, ctypes.c_int32, _d,
+                                            _i32, _d, _i32, _d]),
+    "nnk_mobility_destroy": (None, [_vp]),
 }
 
 _lib = None

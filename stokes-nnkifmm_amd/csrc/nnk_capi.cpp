@@ -9,10 +9,17 @@
 #include "../../include/nnkifmm.h"
 #include "nnk_fmm.hpp"
 #include "nnk_handles.hpp"
+#include "nnk_mobility.hpp"
 #include "nnk_tree.hpp"
 
 
+struct nnk_mobility {
+  std::unique_ptr<nnk::Mobility> m;
+};
+
 namespace nnk {
+int64_t nearest_matrix(const double *coarse, int64_t n, const double *fine, int64_t q,
+                       int64_t *colptr, int64_t *row, double *val, int64_t cap);
 void direct_device(const double *src, int64_t ns, const double *q, const double *tgt, int64_t nt,
                    double eps, double mu, double *out, hipStream_t s);
 }
@@ -173,6 +180,113 @@ int32_t nnk_fmm_reduced_apply_device(nnk_fmm *h, const double *d_f, double *d_u,
 }
 
 void nnk_fmm_destroy(nnk_fmm *h) { delete h; }
+
+int32_t nnk_fmm_reduced_solve(nnk_fmm *h, const double *v, double *f) {
+  return guard([&]() -> int32_t {
+    if (!h || !v || !f) return fail(NNK_EARG, "nnk_fmm_reduced_solve: null argument");
+    if (h->npot != h->ntgt) return fail(NNK_EARG, "reducedSolve needs potentials == targets");
+    nnk::DBuf dv, df;
+    dv.alloc(sizeof(double) * 3 * std::max<int64_t>(h->ntgt, 1));
+    df.alloc(sizeof(double) * 3 * std::max<int64_t>(h->npot, 1));
+    NNK_HIP(hipMemcpy(dv.p, v, sizeof(double) * 3 * h->ntgt, hipMemcpyHostToDevice));
+    h->f->reduced_solve(dv.as<double>(), df.as<double>(), 0);
+    NNK_HIP(hipMemcpy(f, df.p, sizeof(double) * 3 * h->npot, hipMemcpyDeviceToHost));
+    return NNK_OK;
+  });
+}
+
+int32_t nnk_fmm_reduced_solve_device(nnk_fmm *h, const double *d_v, double *d_f, void *stream) {
+  return guard([&]() -> int32_t {
+    if (!h || !d_v || !d_f) return fail(NNK_EARG, "nnk_fmm_reduced_solve_device: null argument");
+    h->f->reduced_solve(d_v, d_f, (hipStream_t)stream);
+    return NNK_OK;
+  });
+}
+
+int32_t nnk_nearest_matrix(const double *coarse, int64_t n, const double *fine, int64_t q,
+                           int64_t *colptr, int64_t *rowidx, double *val, int64_t cap,
+                           int64_t *nnz) {
+  return guard([&]() -> int32_t {
+    if (!coarse || n <= 0 || q < 0 || (q && !fine) || !colptr || !nnz)
+      return fail(NNK_EARG, "nnk_nearest_matrix: bad argument");
+    int64_t k = nnk::nearest_matrix(coarse, n, fine, q, colptr, rowidx, val, cap);
+    *nnz = k;
+    if (k > cap) return fail(NNK_EARG, "nnk_nearest_matrix: output capacity too small");
+    return NNK_OK;
+  });
+}
+
+int32_t nnk_mobility_create(nnk_fmm *fmm, nnk_fmm *fmm_sparse, int64_t nb6, const int64_t *Brp,
+                            const int32_t *Bci, const double *Bv, const int64_t *BTrp,
+                            const int32_t *BTci, const double *BTv, const double *A1,
+                            const double *A2, nnk_mobility **out) {
+  return guard([&]() -> int32_t {
+    if (!fmm || !out || !Brp || !BTrp || !A1 || !A2) return fail(NNK_EARG, "nnk_mobility_create: null argument");
+    *out = nullptr;
+    auto m = std::make_unique<nnk::Mobility>(fmm->f.get(), fmm_sparse ? fmm_sparse->f.get() : nullptr, nb6,
+                                             Brp, Bci, Bv, BTrp, BTci, BTv, A1, A2);
+    *out = new nnk_mobility{std::move(m)};
+    return NNK_OK;
+  });
+}
+
+static int32_t mob_host(nnk_mobility *h, const double *x, double *y, bool pre) {
+  return guard([&]() -> int32_t {
+    if (!h || !x || !y) return fail(NNK_EARG, "nnk_mobility: null argument");
+    const int64_t n = h->m->size();
+    nnk::DBuf dx, dy;
+    dx.alloc(sizeof(double) * n);
+    dy.alloc(sizeof(double) * n);
+    NNK_HIP(hipMemcpy(dx.p, x, sizeof(double) * n, hipMemcpyHostToDevice));
+    if (pre) h->m->precon(dx.as<double>(), dy.as<double>(), 0);
+    else h->m->apply(dx.as<double>(), dy.as<double>(), 0);
+    NNK_HIP(hipMemcpy(y, dy.p, sizeof(double) * n, hipMemcpyDeviceToHost));
+    return NNK_OK;
+  });
+}
+
+int32_t nnk_mobility_apply(nnk_mobility *h, const double *x, double *y) { return mob_host(h, x, y, false); }
+int32_t nnk_mobility_precon(nnk_mobility *h, const double *x, double *y) { return mob_host(h, x, y, true); }
+
+int32_t nnk_mobility_apply_device(nnk_mobility *h, const double *d_x, double *d_y, void *stream) {
+  return guard([&]() -> int32_t {
+    if (!h || !d_x || !d_y) return fail(NNK_EARG, "nnk_mobility_apply_device: null argument");
+    h->m->apply(d_x, d_y, (hipStream_t)stream);
+    return NNK_OK;
+  });
+}
+
+int32_t nnk_mobility_precon_device(nnk_mobility *h, const double *d_x, double *d_y, void *stream) {
+  return guard([&]() -> int32_t {
+    if (!h || !d_x || !d_y) return fail(NNK_EARG, "nnk_mobility_precon_device: null argument");
+    h->m->precon(d_x, d_y, (hipStream_t)stream);
+    return NNK_OK;
+  });
+}
+
+int32_t nnk_mobility_solve(nnk_mobility *h, const double *rhs, double tol, int32_t maxit,
+                           int32_t precon, double *x, int32_t *flag, double *relres, int32_t *iters,
+                           double *resvec) {
+  return guard([&]() -> int32_t {
+    if (!h || !rhs || !x || maxit < 1 || !(tol > 0)) return fail(NNK_EARG, "nnk_mobility_solve: bad argument");
+    const int64_t n = h->m->size();
+    nnk::DBuf db, dx;
+    db.alloc(sizeof(double) * n);
+    dx.alloc(sizeof(double) * n);
+    NNK_HIP(hipMemcpy(db.p, rhs, sizeof(double) * n, hipMemcpyHostToDevice));
+    std::vector<double> rv;
+    nnk::GmresResult R = h->m->gmres(db.as<double>(), dx.as<double>(), tol, maxit, precon != 0, 0, &rv);
+    NNK_HIP(hipMemcpy(x, dx.p, sizeof(double) * n, hipMemcpyDeviceToHost));
+    if (flag) *flag = R.flag;
+    if (relres) *relres = R.relres;
+    if (iters) *iters = R.iter;
+    if (resvec)
+      for (size_t i = 0; i < rv.size() && i <= (size_t)maxit; i++) resvec[i] = rv[i];
+    return NNK_OK;
+  });
+}
+
+void nnk_mobility_destroy(nnk_mobility *h) { delete h; }
 
 int32_t nnk_direct_apply(const double *src, int64_t ns, const double *q, const double *tgt,
                          int64_t nt, double eps, double mu, double *u) {

@@ -12,6 +12,8 @@
 
 namespace nnk {
 
+struct RedArgs;
+
 struct GemmJob {
   const double *const *A;  // nb operator pointers, each At[Kpad][Mpad] (k-major)
   const int32_t *gidx;     // nb x N gather slots (-1 = none), or null = column index
@@ -60,6 +62,9 @@ class Fmm {
   void apply(const double *d_f, double *d_u, hipStream_t s);
   // leaf-local product (reducedPass.m) on device pointers
   void reduced_apply(const double *d_f, double *d_u, hipStream_t s);
+  // leaf-block solve (ReducedSolve.m): f = blockdiag(K_leaf)^-1 v, with the
+  // per-leaf LU factors formed on first use; needs potentials == targets
+  void reduced_solve(const double *d_v, double *d_f, hipStream_t s);
   int64_t npot() const { return npot_; }
   int64_t ntgt() const { return ntgt_; }
   int nq() const { return nq_; }
@@ -106,6 +111,11 @@ class Fmm {
   DBuf chk_, uppot_, downpot_, tmp_;
   std::vector<LevelOps> ops_;
   std::vector<LevelPlan> plans_;
+  void reduced_factor();
+  bool red_ready_ = false;
+  DBuf red_blk_, red_off_, red_piv_, red_poff_, red_ord2loc_;
+  struct RedArgs red_args() const;
+  int red_maxn_ = 0;
   int64_t stats_[kStats] = {0};
   int m2l_cfg_ = 1;  // M2L tile: 0 = 160x64, 1 = 160x96
   bool prof_ = false;

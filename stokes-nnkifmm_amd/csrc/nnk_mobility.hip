@@ -1,0 +1,392 @@
+// Saddle-point mobility operator, its block preconditioner and GMRES on the
+// device (Mobility/MobilityMatrixMulti.m, MobilityPrecon.m, mobilityProblem.m).
+//
+// Vector layout (mobilityProblem.m:37-46): x = [f (3N, component-major:
+// all x, then all y, then all z); U (3 nsw, same); Omega (3 nsw)].
+//   apply:  y = [ S f + BT (U;Omega) ;  B f ]            (MobilityMatrixMulti.m:31-35)
+//   precon: y_f  = R^-1 f,  y_UO = -A2 \ (BT^T S4(B^T (A1 \ UO))),
+//           y_f -= R^-1 (BT y_UO)                          (MobilityPrecon.m:29-38)
+// with S the KIFMM matvec, S4 the same at 4 points per corona edge and R^-1
+// the leaf-block solve.  B, BT come from genMatrixComponets as CSR; A1 and
+// A2 (6 nsw square) are LU-factored on the host once.
+// GMRES: unrestarted, zero initial guess, classical Gram-Schmidt with one
+// reorthogonalisation on the device, Givens rotations on the host
+// (Saad & Schultz 1986), stopping on the estimated relative residual and
+// reporting the true one, like MATLAB's gmres(A, b, [], tol, maxit).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "nnk_fmm.hpp"
+#include "nnk_mobility.hpp"
+
+namespace nnk {
+
+// ---------------------------------------------------------------------------
+// kernels
+
+// y[r] (+)= sum_e val[e] x[col[e]] over row r: one wavefront per row, lanes
+// stride the row and combine in a fixed order.
+__global__ __launch_bounds__(256) void k_spmv(int64_t rows, const int64_t *__restrict__ rp,
+                                              const int32_t *__restrict__ ci,
+                                              const double *__restrict__ v,
+                                              const double *__restrict__ x, double *__restrict__ y,
+                                              int accumulate, double alpha) {
+  const int64_t r = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (r >= rows) return;
+  double s = 0.0;
+  for (int64_t e = rp[r] + lane; e < rp[r + 1]; e += 64) s += v[e] * x[ci[e]];
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+  if (lane == 0) y[r] = accumulate ? y[r] + alpha * s : alpha * s;
+}
+
+__global__ void k_soa2aos(int64_t n, const double *__restrict__ a, double *__restrict__ b) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  b[3 * i] = a[i];
+  b[3 * i + 1] = a[n + i];
+  b[3 * i + 2] = a[2 * n + i];
+}
+
+__global__ void k_aos2soa(int64_t n, const double *__restrict__ b, double *__restrict__ a,
+                          int accumulate) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  for (int d = 0; d < 3; d++) a[d * n + i] = accumulate ? a[d * n + i] + b[3 * i + d] : b[3 * i + d];
+}
+
+__global__ void k_axpby(int64_t n, double a, const double *__restrict__ x, double b,
+                        double *__restrict__ y) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = a * x[i] + b * y[i];
+}
+
+// partial[j * nblk + blk] = sum over this block's slice of V_j . w
+__global__ __launch_bounds__(256) void k_multidot(int64_t n, int k, const double *__restrict__ V,
+                                                  const double *__restrict__ w,
+                                                  double *__restrict__ partial) {
+  __shared__ double red[256];
+  for (int j = 0; j < k; j++) {
+    const double *vj = V + (size_t)j * n;
+    double s = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
+      s += vj[i] * w[i];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+      if (threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[(size_t)j * gridDim.x + blockIdx.x] = red[0];
+    __syncthreads();
+  }
+}
+
+// w -= sum_j h[j] V_j
+__global__ void k_subtract_combo(int64_t n, int k, const double *__restrict__ V,
+                                 const double *__restrict__ h, double *__restrict__ w) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double s = 0.0;
+  for (int j = 0; j < k; j++) s += h[j] * V[(size_t)j * n + i];
+  w[i] -= s;
+}
+
+// small dense LU solve x = A^-1 b (factors from the host: row-major LU and
+// pivots), one block
+__global__ void k_small_lu_solve(int n, const double *__restrict__ LU, const int32_t *__restrict__ piv,
+                                 const double *__restrict__ b, double *__restrict__ x, double scale) {
+  extern __shared__ double y[];
+  for (int i = threadIdx.x; i < n; i += blockDim.x) y[i] = b[i];
+  __syncthreads();
+  if (threadIdx.x == 0)
+    for (int k = 0; k < n; k++) {
+      int p = piv[k];
+      if (p != k) { double t = y[k]; y[k] = y[p]; y[p] = t; }
+    }
+  __syncthreads();
+  for (int k = 0; k < n; k++) {
+    double yk = y[k];
+    for (int i = k + 1 + threadIdx.x; i < n; i += blockDim.x) y[i] -= LU[(size_t)i * n + k] * yk;
+    __syncthreads();
+  }
+  for (int k = n - 1; k >= 0; k--) {
+    if (threadIdx.x == 0) y[k] /= LU[(size_t)k * n + k];
+    __syncthreads();
+    double yk = y[k];
+    for (int i = threadIdx.x; i < k; i += blockDim.x) y[i] -= LU[(size_t)i * n + k] * yk;
+    __syncthreads();
+  }
+  for (int i = threadIdx.x; i < n; i += blockDim.x) x[i] = scale * y[i];
+}
+
+// ---------------------------------------------------------------------------
+
+namespace {
+
+inline unsigned nblk(int64_t n, int bs = 256) { return (unsigned)((n + bs - 1) / bs); }
+
+void host_lu(std::vector<double> &A, std::vector<int32_t> &piv, int n) {
+  piv.resize(n);
+  for (int k = 0; k < n; k++) {
+    int p = k;
+    double amax = std::fabs(A[(size_t)k * n + k]);
+    for (int i = k + 1; i < n; i++)
+      if (std::fabs(A[(size_t)i * n + k]) > amax) { amax = std::fabs(A[(size_t)i * n + k]); p = i; }
+    piv[k] = p;
+    if (p != k)
+      for (int j = 0; j < n; j++) std::swap(A[(size_t)k * n + j], A[(size_t)p * n + j]);
+    double d = A[(size_t)k * n + k];
+    if (d == 0.0) continue;
+    for (int i = k + 1; i < n; i++) {
+      double l = A[(size_t)i * n + k] / d;
+      A[(size_t)i * n + k] = l;
+      if (l != 0.0)
+        for (int j = k + 1; j < n; j++) A[(size_t)i * n + j] -= l * A[(size_t)k * n + j];
+    }
+  }
+}
+
+// CSR transpose on the host
+void csr_transpose(int64_t rows, int64_t cols, const std::vector<int64_t> &rp,
+                   const std::vector<int32_t> &ci, const std::vector<double> &v,
+                   std::vector<int64_t> &trp, std::vector<int32_t> &tci, std::vector<double> &tv) {
+  trp.assign(cols + 1, 0);
+  for (size_t e = 0; e < ci.size(); e++) trp[ci[e] + 1]++;
+  for (int64_t c = 0; c < cols; c++) trp[c + 1] += trp[c];
+  tci.resize(ci.size());
+  tv.resize(v.size());
+  std::vector<int64_t> fill(trp.begin(), trp.end() - 1);
+  for (int64_t r = 0; r < rows; r++)
+    for (int64_t e = rp[r]; e < rp[r + 1]; e++) {
+      int64_t k = fill[ci[e]]++;
+      tci[k] = (int32_t)r;
+      tv[k] = v[e];
+    }
+}
+
+}  // namespace
+
+void DevCsr::upload(int64_t r, int64_t c, const std::vector<int64_t> &rp, const std::vector<int32_t> &ci,
+                    const std::vector<double> &v) {
+  rows = r;
+  cols = c;
+  rowptr.upload(rp);
+  col.upload(ci);
+  val.upload(v);
+}
+
+void DevCsr::mul(const double *x, double *y, bool accumulate, double alpha, hipStream_t s) const {
+  if (rows == 0) return;
+  hipLaunchKernelGGL(k_spmv, dim3(nblk(rows, 4)), dim3(256), 0, s, rows, rowptr.as<int64_t>(),
+                     col.as<int32_t>(), val.as<double>(), x, y, accumulate ? 1 : 0, alpha);
+  NNK_HIP(hipGetLastError());
+}
+
+Mobility::Mobility(Fmm *fmm, Fmm *fmm_sparse, int64_t nb6, const int64_t *Brp, const int32_t *Bci,
+                   const double *Bv, const int64_t *BTrp, const int32_t *BTci, const double *BTv,
+                   const double *A1, const double *A2)
+    : F_(fmm), F4_(fmm_sparse), nb6_(nb6) {
+  if (!F_) throw Error(NNK_EARG, "mobility: null KIFMM operator");
+  N_ = F_->npot();
+  if (F_->ntgt() != N_ || (F4_ && (F4_->npot() != N_ || F4_->ntgt() != N_)))
+    throw Error(NNK_EARG, "mobility: operators must map the N force points to themselves");
+  if (nb6 <= 0 || nb6 % 6) throw Error(NNK_EARG, "mobility: 6*nsw must be a positive multiple of 6");
+  n3_ = 3 * N_;
+  n_ = n3_ + nb6_;
+  std::vector<int64_t> rp(Brp, Brp + nb6 + 1);
+  std::vector<int32_t> ci(Bci, Bci + rp.back());
+  std::vector<double> v(Bv, Bv + rp.back());
+  for (int32_t c : ci)
+    if (c < 0 || c >= n3_) throw Error(NNK_EARG, "mobility: B column out of range");
+  std::vector<int64_t> trp;
+  std::vector<int32_t> tci;
+  std::vector<double> tv;
+  B_.upload(nb6, n3_, rp, ci, v);
+  csr_transpose(nb6, n3_, rp, ci, v, trp, tci, tv);
+  Bt_.upload(n3_, nb6, trp, tci, tv);
+  std::vector<int64_t> rp2(BTrp, BTrp + n3_ + 1);
+  std::vector<int32_t> ci2(BTci, BTci + rp2.back());
+  std::vector<double> v2(BTv, BTv + rp2.back());
+  for (int32_t c : ci2)
+    if (c < 0 || c >= nb6) throw Error(NNK_EARG, "mobility: BT column out of range");
+  BT_.upload(n3_, nb6, rp2, ci2, v2);
+  csr_transpose(n3_, nb6, rp2, ci2, v2, trp, tci, tv);
+  BTt_.upload(nb6, n3_, trp, tci, tv);
+  for (int q = 0; q < 2; q++) {
+    std::vector<double> A(q == 0 ? A1 : A2, (q == 0 ? A1 : A2) + (size_t)nb6 * nb6);
+    std::vector<int32_t> piv;
+    host_lu(A, piv, (int)nb6);
+    (q == 0 ? A1lu_ : A2lu_).upload(A);
+    (q == 0 ? A1piv_ : A2piv_).upload(piv);
+  }
+  aos_a_.alloc(sizeof(double) * n3_);
+  aos_b_.alloc(sizeof(double) * n3_);
+  t3a_.alloc(sizeof(double) * n3_);
+  t3b_.alloc(sizeof(double) * n3_);
+  s6a_.alloc(sizeof(double) * nb6_);
+  s6b_.alloc(sizeof(double) * nb6_);
+}
+
+// u (3N, component-major) = S f (3N, component-major) with operator Fm
+void Mobility::kifmm(Fmm *Fm, const double *f, double *u, bool accumulate, hipStream_t s) {
+  hipLaunchKernelGGL(k_soa2aos, dim3(nblk(N_)), dim3(256), 0, s, N_, f, aos_a_.as<double>());
+  Fm->apply(aos_a_.as<double>(), aos_b_.as<double>(), s);
+  hipLaunchKernelGGL(k_aos2soa, dim3(nblk(N_)), dim3(256), 0, s, N_, aos_b_.as<double>(), u,
+                     accumulate ? 1 : 0);
+  NNK_HIP(hipGetLastError());
+}
+
+void Mobility::reduced(const double *v, double *f, hipStream_t s) {
+  hipLaunchKernelGGL(k_soa2aos, dim3(nblk(N_)), dim3(256), 0, s, N_, v, aos_a_.as<double>());
+  F_->reduced_solve(aos_a_.as<double>(), aos_b_.as<double>(), s);
+  hipLaunchKernelGGL(k_aos2soa, dim3(nblk(N_)), dim3(256), 0, s, N_, aos_b_.as<double>(), f, 0);
+  NNK_HIP(hipGetLastError());
+}
+
+void Mobility::small_solve(const DBuf &lu, const DBuf &piv, const double *b, double *x, double scale,
+                           hipStream_t s) {
+  hipLaunchKernelGGL(k_small_lu_solve, dim3(1), dim3(256), sizeof(double) * nb6_, s, (int)nb6_,
+                     lu.as<double>(), piv.as<int32_t>(), b, x, scale);
+  NNK_HIP(hipGetLastError());
+}
+
+void Mobility::apply(const double *x, double *y, hipStream_t s) {
+  kifmm(F_, x, y, false, s);                 // S f
+  BT_.mul(x + n3_, y, true, 1.0, s);          // + BT (U; Omega)
+  B_.mul(x, y + n3_, false, 1.0, s);          // B f
+}
+
+void Mobility::precon(const double *x, double *y, hipStream_t s) {
+  if (!F4_) throw Error(NNK_ESTATE, "mobility: no coronaRes-4 operator for the preconditioner");
+  reduced(x, y, s);                                        // y_f = R^-1 f
+  small_solve(A1lu_, A1piv_, x + n3_, s6a_.as<double>(), 1.0, s);   // A1 \ UO
+  Bt_.mul(s6a_.as<double>(), t3a_.as<double>(), false, 1.0, s);     // B^T (.)
+  kifmm(F4_, t3a_.as<double>(), t3b_.as<double>(), false, s);       // S4 (.)
+  BTt_.mul(t3b_.as<double>(), s6b_.as<double>(), false, 1.0, s);    // BT^T (.)
+  small_solve(A2lu_, A2piv_, s6b_.as<double>(), y + n3_, -1.0, s);  // -A2 \ (.)
+  BT_.mul(y + n3_, t3a_.as<double>(), false, 1.0, s);               // BT out
+  reduced(t3a_.as<double>(), t3b_.as<double>(), s);                 // R^-1 (.)
+  hipLaunchKernelGGL(k_axpby, dim3(nblk(n3_)), dim3(256), 0, s, n3_, -1.0, t3b_.as<double>(), 1.0, y);
+  NNK_HIP(hipGetLastError());
+}
+
+void Mobility::op(const double *x, double *y, bool use_precon, hipStream_t s) {
+  if (!use_precon) { apply(x, y, s); return; }
+  if (!px_.p) px_.alloc(sizeof(double) * n_);
+  precon(x, px_.as<double>(), s);
+  apply(px_.as<double>(), y, s);
+}
+
+double Mobility::dot(const double *a, const double *b, hipStream_t s) {
+  const int nb = 256;
+  if (!part_.p || part_.bytes < sizeof(double) * nb) part_.alloc(sizeof(double) * nb);
+  hipLaunchKernelGGL(k_multidot, dim3(nb), dim3(256), 0, s, n_, 1, a, b, part_.as<double>());
+  NNK_HIP(hipGetLastError());
+  std::vector<double> h(nb);
+  NNK_HIP(hipMemcpyAsync(h.data(), part_.p, sizeof(double) * nb, hipMemcpyDeviceToHost, s));
+  NNK_HIP(hipStreamSynchronize(s));
+  double t = 0;
+  for (double v : h) t += v;
+  return t;
+}
+
+GmresResult Mobility::gmres(const double *b, double *x, double tol, int maxit, bool use_precon,
+                            hipStream_t s, std::vector<double> *resvec) {
+  GmresResult R{};
+  const int64_t n = n_;
+  const int m = std::max(1, (int)std::min<int64_t>(maxit, n));
+  const int nb = 256;
+  DBuf V, w, hdev, part;
+  V.alloc(sizeof(double) * (size_t)(m + 1) * n);
+  w.alloc(sizeof(double) * n);
+  hdev.alloc(sizeof(double) * (m + 1));
+  part.alloc(sizeof(double) * (size_t)(m + 1) * nb);
+  NNK_HIP(hipMemsetAsync(x, 0, sizeof(double) * n, s));
+  const double bnorm = std::sqrt(dot(b, b, s));
+  if (resvec) resvec->assign(1, bnorm);
+  if (bnorm == 0.0) {  // MATLAB: x = 0, flag 0
+    R.flag = 0; R.relres = 0; R.iter = 0;
+    if (use_precon) { /* P(0) = 0 */ }
+    return R;
+  }
+  double *Vp = V.as<double>();
+  hipLaunchKernelGGL(k_axpby, dim3(nblk(n)), dim3(256), 0, s, n, 1.0 / bnorm, b, 0.0, Vp);
+  std::vector<double> H((size_t)(m + 1) * m, 0.0), cs(m), sn(m), g(m + 1, 0.0), h(m + 1), h2(m + 1),
+      hp((size_t)(m + 1) * nb);
+  g[0] = bnorm;
+  int k = 0;
+  bool done = false;
+  auto project = [&](int kk, std::vector<double> &coef) {
+    hipLaunchKernelGGL(k_multidot, dim3(nb), dim3(256), 0, s, n, kk, Vp, w.as<double>(), part.as<double>());
+    NNK_HIP(hipGetLastError());
+    NNK_HIP(hipMemcpyAsync(hp.data(), part.p, sizeof(double) * (size_t)kk * nb, hipMemcpyDeviceToHost, s));
+    NNK_HIP(hipStreamSynchronize(s));
+    for (int j = 0; j < kk; j++) {
+      double t = 0;
+      for (int q = 0; q < nb; q++) t += hp[(size_t)j * nb + q];
+      coef[j] = t;
+    }
+    NNK_HIP(hipMemcpyAsync(hdev.p, coef.data(), sizeof(double) * kk, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_subtract_combo, dim3(nblk(n)), dim3(256), 0, s, n, kk, Vp, hdev.as<double>(),
+                       w.as<double>());
+    NNK_HIP(hipGetLastError());
+  };
+  for (k = 0; k < m && !done; k++) {
+    op(Vp + (size_t)k * n, w.as<double>(), use_precon, s);
+    project(k + 1, h);
+    project(k + 1, h2);
+    for (int j = 0; j <= k; j++) h[j] += h2[j];
+    double hn = std::sqrt(dot(w.as<double>(), w.as<double>(), s));
+    h[k + 1] = hn;
+    if (hn > 0)
+      hipLaunchKernelGGL(k_axpby, dim3(nblk(n)), dim3(256), 0, s, n, 1.0 / hn, w.as<double>(), 0.0,
+                         Vp + (size_t)(k + 1) * n);
+    for (int j = 0; j < k; j++) {  // previous rotations
+      double t = cs[j] * h[j] + sn[j] * h[j + 1];
+      h[j + 1] = -sn[j] * h[j] + cs[j] * h[j + 1];
+      h[j] = t;
+    }
+    double r = std::hypot(h[k], h[k + 1]);
+    cs[k] = r > 0 ? h[k] / r : 1.0;
+    sn[k] = r > 0 ? h[k + 1] / r : 0.0;
+    h[k] = r;
+    h[k + 1] = 0.0;
+    g[k + 1] = -sn[k] * g[k];
+    g[k] = cs[k] * g[k];
+    for (int j = 0; j <= k; j++) H[(size_t)j * m + k] = h[j];
+    if (resvec) resvec->push_back(std::fabs(g[k + 1]));
+    if (std::fabs(g[k + 1]) <= tol * bnorm || hn == 0.0) done = true;
+  }
+  const int kk = k;
+  std::vector<double> y(kk, 0.0);
+  for (int i = kk - 1; i >= 0; i--) {
+    double t = g[i];
+    for (int j = i + 1; j < kk; j++) t -= H[(size_t)i * m + j] * y[j];
+    y[i] = H[(size_t)i * m + i] != 0 ? t / H[(size_t)i * m + i] : 0.0;
+  }
+  // x = V y (then the true residual)
+  NNK_HIP(hipMemcpyAsync(hdev.p, y.data(), sizeof(double) * kk, hipMemcpyHostToDevice, s));
+  NNK_HIP(hipMemsetAsync(w.p, 0, sizeof(double) * n, s));
+  hipLaunchKernelGGL(k_subtract_combo, dim3(nblk(n)), dim3(256), 0, s, n, kk, Vp, hdev.as<double>(),
+                     w.as<double>());
+  hipLaunchKernelGGL(k_axpby, dim3(nblk(n)), dim3(256), 0, s, n, -1.0, w.as<double>(), 0.0, x);
+  NNK_HIP(hipGetLastError());
+  op(x, w.as<double>(), use_precon, s);
+  hipLaunchKernelGGL(k_axpby, dim3(nblk(n)), dim3(256), 0, s, n, 1.0, b, -1.0, w.as<double>());
+  R.relres = std::sqrt(dot(w.as<double>(), w.as<double>(), s)) / bnorm;
+  R.iter = kk;
+  R.flag = R.relres <= tol ? 0 : 1;
+  if (use_precon) {  // x = P(x)  (mobilityProblem.m:54)
+    precon(x, w.as<double>(), s);
+    NNK_HIP(hipMemcpyAsync(x, w.p, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+  }
+  NNK_HIP(hipStreamSynchronize(s));
+  return R;
+}
+
+}  // namespace nnk

@@ -1,0 +1,56 @@
+// Device mobility operator, block preconditioner and GMRES driver.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "nnk_common.hpp"
+#include "nnk_fmm.hpp"
+
+namespace nnk {
+
+struct DevCsr {
+  int64_t rows = 0, cols = 0;
+  DBuf rowptr, col, val;
+  void upload(int64_t r, int64_t c, const std::vector<int64_t> &rp, const std::vector<int32_t> &ci,
+              const std::vector<double> &v);
+  // y (+)= alpha A x
+  void mul(const double *x, double *y, bool accumulate, double alpha, hipStream_t s) const;
+};
+
+struct GmresResult {
+  int flag = 1;
+  double relres = 0;
+  int iter = 0;
+};
+
+class Mobility {
+ public:
+  Mobility(Fmm *fmm, Fmm *fmm_sparse, int64_t nb6, const int64_t *Brp, const int32_t *Bci,
+           const double *Bv, const int64_t *BTrp, const int32_t *BTci, const double *BTv,
+           const double *A1, const double *A2);
+  int64_t size() const { return n_; }
+  // mobilityMatrixMulti: y = [S f + BT UO; B f]
+  void apply(const double *x, double *y, hipStream_t s);
+  // mobilityPrecon
+  void precon(const double *x, double *y, hipStream_t s);
+  // gmres(A (or A*P), b, [], tol, maxit); x = P(x) when preconditioned
+  GmresResult gmres(const double *b, double *x, double tol, int maxit, bool use_precon, hipStream_t s,
+                    std::vector<double> *resvec);
+
+ private:
+  void op(const double *x, double *y, bool use_precon, hipStream_t s);
+  void kifmm(Fmm *Fm, const double *f, double *u, bool accumulate, hipStream_t s);
+  void reduced(const double *v, double *f, hipStream_t s);
+  void small_solve(const DBuf &lu, const DBuf &piv, const double *b, double *x, double scale,
+                   hipStream_t s);
+  double dot(const double *a, const double *b, hipStream_t s);
+
+  Fmm *F_, *F4_;
+  int64_t nb6_, N_ = 0, n3_ = 0, n_ = 0;
+  DevCsr B_, Bt_, BT_, BTt_;
+  DBuf A1lu_, A1piv_, A2lu_, A2piv_;
+  DBuf aos_a_, aos_b_, t3a_, t3b_, s6a_, s6b_, px_, part_;
+};
+
+}  // namespace nnk

@@ -1,0 +1,236 @@
+// Leaf-block solve of the preconditioner (Mobility/ReducedSolve.m:164-209):
+// for every target leaf, solve K(leaf sources -> leaf targets) x = v, or in
+// NEAREST mode (K(fine -> targets) kron(NN_leaf, I3)) x = v (ReducedSolve.m
+// :170-187).  The 3n x 3n blocks are formed and LU-factored with partial
+// pivoting once per tree on the GPU (one workgroup per leaf); each solve is
+// a pair of triangular substitutions.  MATLAB's mldivide would use Cholesky
+// on the symmetric non-NEAREST blocks; LU gives the same solution up to
+// rounding.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "nnk_fmm.hpp"
+#include "nnk_kernels.hpp"
+
+namespace nnk {
+
+struct RedArgs {
+  const int32_t *leaves;
+  const int32_t *tgt_ptr;
+  const double *tgt_pos;
+  const int32_t *tgt_out;
+  const int32_t *src_ptr;
+  const double *src_pos;
+  const int64_t *ctr_ptr;   // NEAREST: contributions per source instance
+  const int32_t *ctr_col;
+  const double *ctr_w;
+  const int32_t *ord2loc;   // target ordinal -> index within its leaf
+  const int64_t *off;       // block offset (doubles) per leaf slot
+  double *blk;
+  int32_t *piv;             // 3n pivots per leaf at off3[leaf]
+  const int64_t *poff;
+  double e2, c;
+  int nearest;
+};
+
+// form the leaf block: rows 3t+a (targets), columns 3j+b (the leaf's coarse
+// points in target order); block stored row-major
+__global__ __launch_bounds__(256) void k_red_form(RedArgs a) {
+  const int li = blockIdx.x;
+  const int n = a.leaves[li];
+  const int tb = a.tgt_ptr[n], nt = a.tgt_ptr[n + 1] - tb;
+  const int n3 = 3 * nt;
+  double *A = a.blk + a.off[li];
+  for (int i = threadIdx.x; i < n3 * n3; i += 256) A[i] = 0.0;
+  __syncthreads();
+  const int sb = a.src_ptr[n], se = a.src_ptr[n + 1];
+  for (int t = threadIdx.x; t < nt; t += 256) {
+    const double *tp = a.tgt_pos + 3 * (size_t)(tb + t);
+    double *row = A + (size_t)(3 * t) * n3;
+    for (int s = sb; s < se; s++) {
+      const double *sp = a.src_pos + 3 * (size_t)s;
+      double rx = tp[0] - sp[0], ry = tp[1] - sp[1], rz = tp[2] - sp[2];
+      double r2 = rx * rx + ry * ry + rz * rz;
+      double inv = rsqrt(r2 + a.e2);
+      double w3 = a.c * inv * inv * inv;
+      double iso = r2 + 2.0 * a.e2;
+      double r[3] = {rx, ry, rz};
+      if (!a.nearest) {
+        const int j = s - sb;  // sources and targets share the leaf order
+        for (int p = 0; p < 3; p++)
+          for (int q = 0; q < 3; q++)
+            row[(size_t)p * n3 + 3 * j + q] = (r[p] * r[q] + (p == q ? iso : 0.0)) * w3;
+      } else {
+        for (int64_t e = a.ctr_ptr[s]; e < a.ctr_ptr[s + 1]; e++) {
+          const int j = a.ord2loc[a.ctr_col[e]];
+          const double w = a.ctr_w[e] * w3;
+          for (int p = 0; p < 3; p++)
+            for (int q = 0; q < 3; q++)
+              row[(size_t)p * n3 + 3 * j + q] += (r[p] * r[q] + (p == q ? iso : 0.0)) * w;
+        }
+      }
+    }
+  }
+}
+
+// in-place LU with partial pivoting (first maximal |a_ik|), one block per leaf
+__global__ __launch_bounds__(256) void k_red_lu(RedArgs a) {
+  const int li = blockIdx.x;
+  const int n = a.leaves[li];
+  const int n3 = 3 * (a.tgt_ptr[n + 1] - a.tgt_ptr[n]);
+  double *A = a.blk + a.off[li];
+  int32_t *piv = a.piv + a.poff[li];
+  __shared__ double sv[256];
+  __shared__ int si[256];
+  for (int k = 0; k < n3; k++) {
+    double best = -1.0;
+    int bi = n3;
+    for (int i = k + threadIdx.x; i < n3; i += 256) {
+      double v = fabs(A[(size_t)i * n3 + k]);
+      if (v > best) { best = v; bi = i; }
+    }
+    sv[threadIdx.x] = best;
+    si[threadIdx.x] = bi;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+      if (threadIdx.x < st) {
+        double v2 = sv[threadIdx.x + st];
+        int i2 = si[threadIdx.x + st];
+        if (v2 > sv[threadIdx.x] || (v2 == sv[threadIdx.x] && i2 < si[threadIdx.x])) {
+          sv[threadIdx.x] = v2;
+          si[threadIdx.x] = i2;
+        }
+      }
+      __syncthreads();
+    }
+    const int p = si[0];
+    if (threadIdx.x == 0) piv[k] = p;
+    if (p != k)
+      for (int j = threadIdx.x; j < n3; j += 256) {
+        double t = A[(size_t)k * n3 + j];
+        A[(size_t)k * n3 + j] = A[(size_t)p * n3 + j];
+        A[(size_t)p * n3 + j] = t;
+      }
+    __syncthreads();
+    const double d = A[(size_t)k * n3 + k];
+    if (d != 0.0)
+      for (int i = k + 1 + threadIdx.x; i < n3; i += 256) A[(size_t)i * n3 + k] /= d;
+    __syncthreads();
+    const int m = n3 - k - 1;
+    for (int idx = threadIdx.x; idx < m * m; idx += 256) {
+      const int i = k + 1 + idx / m, j = k + 1 + idx % m;
+      A[(size_t)i * n3 + j] -= A[(size_t)i * n3 + k] * A[(size_t)k * n3 + j];
+    }
+    __syncthreads();
+  }
+}
+
+// x = A^-1 v for each leaf (v, x indexed by target ordinal, 3 per point)
+__global__ __launch_bounds__(256) void k_red_solve(RedArgs a, const double *__restrict__ v,
+                                                   double *__restrict__ x) {
+  extern __shared__ double b[];
+  const int li = blockIdx.x;
+  const int n = a.leaves[li];
+  const int tb = a.tgt_ptr[n], nt = a.tgt_ptr[n + 1] - tb;
+  const int n3 = 3 * nt;
+  const double *A = a.blk + a.off[li];
+  const int32_t *piv = a.piv + a.poff[li];
+  for (int i = threadIdx.x; i < n3; i += 256) b[i] = v[3 * (size_t)a.tgt_out[tb + i / 3] + i % 3];
+  __syncthreads();
+  if (threadIdx.x == 0)
+    for (int k = 0; k < n3; k++) {
+      const int p = piv[k];
+      if (p != k) { double t = b[k]; b[k] = b[p]; b[p] = t; }
+    }
+  __syncthreads();
+  for (int k = 0; k < n3; k++) {          // unit lower
+    const double bk = b[k];
+    for (int i = k + 1 + threadIdx.x; i < n3; i += 256) b[i] -= A[(size_t)i * n3 + k] * bk;
+    __syncthreads();
+  }
+  for (int k = n3 - 1; k >= 0; k--) {     // upper
+    if (threadIdx.x == 0) b[k] /= A[(size_t)k * n3 + k];
+    __syncthreads();
+    const double bk = b[k];
+    for (int i = threadIdx.x; i < k; i += 256) b[i] -= A[(size_t)i * n3 + k] * bk;
+    __syncthreads();
+  }
+  for (int i = threadIdx.x; i < n3; i += 256) x[3 * (size_t)a.tgt_out[tb + i / 3] + i % 3] = b[i];
+}
+
+void Fmm::reduced_factor() {
+  const Tree &T = tree_;
+  if (T.npot != T.ntgt || T.npts != T.npot)
+    throw Error(NNK_EARG, "reducedSolve needs potential points == target points");
+  std::vector<int32_t> tl(n_tleaves_), tp(T.count + 2);
+  NNK_HIP(hipMemcpy(tl.data(), tleaves_.p, sizeof(int32_t) * n_tleaves_, hipMemcpyDeviceToHost));
+  NNK_HIP(hipMemcpy(tp.data(), tgt_ptr_.p, sizeof(int32_t) * (T.count + 2), hipMemcpyDeviceToHost));
+  std::vector<int64_t> off(n_tleaves_ + 1, 0), poff(n_tleaves_ + 1, 0);
+  red_maxn_ = 0;
+  for (int i = 0; i < n_tleaves_; i++) {
+    int64_t n3 = 3 * (tp[tl[i] + 1] - tp[tl[i]]);
+    off[i + 1] = off[i] + n3 * n3;
+    poff[i + 1] = poff[i] + n3;
+    red_maxn_ = std::max<int>(red_maxn_, (int)n3);
+  }
+  if ((size_t)red_maxn_ * sizeof(double) > 64 * 1024)
+    throw Error(NNK_ESTATE, "reducedSolve: leaf block larger than 8192 unknowns");
+  std::vector<int32_t> o2l(T.ntgt, 0);
+  {
+    std::vector<int32_t> to(T.ntgt);
+    NNK_HIP(hipMemcpy(to.data(), tgt_out_.p, sizeof(int32_t) * T.ntgt, hipMemcpyDeviceToHost));
+    for (int i = 0; i < n_tleaves_; i++)
+      for (int k = tp[tl[i]]; k < tp[tl[i] + 1]; k++) o2l[to[k]] = k - tp[tl[i]];
+  }
+  red_off_.upload(off);
+  DBuf dpoff;
+  dpoff.upload(poff);
+  red_ord2loc_.upload(o2l);
+  red_blk_.alloc(sizeof(double) * std::max<int64_t>(off.back(), 1));
+  red_piv_.alloc(sizeof(int32_t) * std::max<int64_t>(poff.back(), 1));
+  red_poff_ = std::move(dpoff);
+  RedArgs a = red_args();
+  if (n_tleaves_) {
+    hipLaunchKernelGGL(k_red_form, dim3(n_tleaves_), dim3(256), 0, 0, a);
+    NNK_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_red_lu, dim3(n_tleaves_), dim3(256), 0, 0, a);
+    NNK_HIP(hipGetLastError());
+  }
+  NNK_HIP(hipDeviceSynchronize());
+  red_ready_ = true;
+}
+
+RedArgs Fmm::red_args() const {
+  RedArgs a{};
+  a.leaves = tleaves_.as<int32_t>();
+  a.tgt_ptr = tgt_ptr_.as<int32_t>();
+  a.tgt_pos = tgt_pos_.as<double>();
+  a.tgt_out = tgt_out_.as<int32_t>();
+  a.src_ptr = src_ptr_.as<int32_t>();
+  a.src_pos = src_pos_.as<double>();
+  a.ctr_ptr = ctr_ptr_.as<int64_t>();
+  a.ctr_col = ctr_col_.as<int32_t>();
+  a.ctr_w = ctr_w_.as<double>();
+  a.ord2loc = red_ord2loc_.as<int32_t>();
+  a.off = red_off_.as<int64_t>();
+  a.blk = red_blk_.as<double>();
+  a.piv = red_piv_.as<int32_t>();
+  a.poff = red_poff_.as<int64_t>();
+  a.e2 = eps_ * eps_;
+  a.c = 1.0 / (8.0 * kPi * mu_);
+  a.nearest = tree_.nearest ? 1 : 0;
+  return a;
+}
+
+void Fmm::reduced_solve(const double *d_v, double *d_f, hipStream_t s) {
+  if (!red_ready_) reduced_factor();
+  if (!n_tleaves_) return;
+  RedArgs a = red_args();
+  hipLaunchKernelGGL(k_red_solve, dim3(n_tleaves_), dim3(256), sizeof(double) * red_maxn_, s, a, d_v,
+                     d_f);
+  NNK_HIP(hipGetLastError());
+}
+
+}  // namespace nnk

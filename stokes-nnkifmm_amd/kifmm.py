@@ -89,18 +89,22 @@ class KIFMM:
                 return u.T.reshape(-1)
         return u
 
-    def _run(self, potentials, h, reduced=False):
+    def _run(self, potentials, h, reduced=False, solve=False):
         if _is_torch(potentials):
             import torch
             f = self._to_rows(potentials).to(torch.float64).contiguous()
             u = torch.empty((self.tree.targets.sum(), 3), dtype=torch.float64, device=f.device)
             stream = ctypes.c_void_p(torch.cuda.current_stream(f.device).cuda_stream)
             fn = lib().nnk_fmm_reduced_apply_device if reduced else lib().nnk_fmm_apply_device
+            if solve:
+                fn = lib().nnk_fmm_reduced_solve_device
             check(fn(h.h, ctypes.c_void_p(f.data_ptr()), ctypes.c_void_p(u.data_ptr()), stream))
             return self._from_rows(u, potentials).contiguous()
         f = c64(self._to_rows(np.asarray(potentials)))
         u = np.empty((int(self.tree.targets.sum()), 3))
         fn = lib().nnk_fmm_reduced_apply if reduced else lib().nnk_fmm_apply
+        if solve:
+            fn = lib().nnk_fmm_reduced_solve
         check(fn(h.h, ptr(f), ptr(u)))
         return np.ascontiguousarray(self._from_rows(u, potentials))
 
@@ -115,3 +119,7 @@ class KIFMM:
     def reducedComputeVel(self, potentials):
         """KIFMM.m:157-198: leaf-local product (reducedPass.m)."""
         return self._run(potentials, self._handle(), reduced=True)
+
+    def reducedSolve(self, vel):
+        """KIFMM.m:200-242: per-leaf block solve (Mobility/ReducedSolve.m)."""
+        return self._run(vel, self._handle(), solve=True)
