@@ -1,9 +1,9 @@
 /*
  * nnkifmm.h -- C ABI of the MI355X KIFMM library (libnnkifmm.so).
  *
- * The reference (Walkersb101/Stokes-NNKIFMM) is MATLAB; a MEX gateway (see
- * INTEGRATION.md) binds these entry points so that the reference's classes
- * and functions keep their signatures.  Each declaration names the reference
+ * The reference (Walkersb101/Stokes-NNKIFMM) is MATLAB; a MEX gateway binds
+ * these entry points so that the reference's classes and functions keep
+ * their signatures.  Each declaration names the reference
  * interface it replaces (paths relative to the reference root).
  *
  * Conventions
