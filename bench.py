@@ -2,14 +2,20 @@
 
 Workload (the FMM-matvec configuration): N = 2^20 points i.i.d. uniform in
 the unit cube, forces i.i.d. N(0,1) in 3 components (numpy PCG64, fixed
-seed), eps = 1e-3, mu = 1, nodeCapacity 64, coronaRes 6 / coronaShells 1.
+seed), eps = 1e-3, mu = 1, nodeCapacity 64, coronaShells 1.
 One step = one full KIFMM matvec (upward + downward pass) with inputs and
 outputs resident in HBM.  Tree build and operator precompute are setup.
+Two forms are timed: the plain FP64 form (the reference's numbers,
+coronaRes 6) and the mixed form (truncated-SVD pseudo-inverses, split-BF16
+M2L on the matrix cores, FP32-math direct sums); the mixed form is the
+headline `value`, both are reported under "forms".
 
 Prints one JSON line.  Besides the throughput it reports:
-  roofline      the M2L translation GEMM (the dominant kernel): algorithmic
-                FP64 flops (2 * (3Nq)^2 per V-list pair) over its HIP-event
-                time in the timed region, against MI355X's FP64 matrix peak;
+  roofline      the headline form's M2L translation GEMM (the dominant
+                kernel): its MFMA flops (FP64: 2 (3Nq)^2 per V-list pair;
+                mixed: 3 BF16 products of 2 R^2 per pair, R the
+                pseudo-inverse rank) over its HIP-event time in the timed
+                region, against the MI355X dense MFMA peak of that type;
   cpu_baseline  the CPU oracle (the reference's algorithm restated in C with
                 OpenMP) timed on this host on a bounded sample;
   rel_err       ||u - u_direct|| / ||u_direct|| on 4096 sampled targets,
@@ -29,6 +35,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 FP64_MFMA_PEAK_TFLOPS = 78.6
+BF16_MFMA_PEAK_TFLOPS = 2500.0
 HBM_PEAK_GBS = 8000.0
 
 
@@ -38,7 +45,11 @@ def parse():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=1 << 20)
-    ap.add_argument("--res", type=int, default=6)
+    ap.add_argument("--res", type=int, default=6, help="corona resolution of the FP64 form")
+    ap.add_argument("--mixed-res", type=int, default=6, help="corona resolution of the mixed form")
+    ap.add_argument("--svd-tol", type=float, default=1e-10, help="pseudo-inverse cut of the mixed form")
+    ap.add_argument("--precision", choices=["fp64", "mixed", "both"], default="both")
+    ap.add_argument("--no-solve", action="store_true", help="skip the one-sphere mobility solve")
     ap.add_argument("--seed", type=int, default=20240611)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--cpu-n", type=int, default=1 << 14, help="CPU baseline sample size")
@@ -65,6 +76,91 @@ def cpu_baseline(args, eps, mu):
                       f"uniform points, same eps/nodeCapacity/coronaRes; {dt:.2f} s for one matvec"}
 
 
+def run_form(nnk, lib, tree, fd, args, eps, mu, precision, res, svd_tol):
+    """Time one form of the matvec; returns (u, record)."""
+    import torch
+    P = ctypes.POINTER
+    fmm = nnk.KIFMM(tree, [eps, mu], "coronaRes", res, "precision", precision, "svdTol", svd_tol)
+    t0 = time.perf_counter()
+    h = fmm._handle()
+    torch.cuda.synchronize()
+    t_setup = time.perf_counter() - t0
+    for _ in range(args.warmup):
+        u = fmm.computeVel(fd)
+    torch.cuda.synchronize()
+    assert lib.nnk_fmm_set_profiling(h.h, 1) == 0
+    ms = np.zeros(8)
+    cnt = np.zeros(8, np.int64)
+    lib.nnk_fmm_phase_times(h.h, ms.ctypes.data_as(P(ctypes.c_double)), cnt.ctypes.data_as(P(ctypes.c_int64)), 8)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        u = fmm.computeVel(fd)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    lib.nnk_fmm_phase_times(h.h, ms.ctypes.data_as(P(ctypes.c_double)), cnt.ctypes.data_as(P(ctypes.c_int64)), 8)
+    lib.nnk_fmm_set_profiling(h.h, 0)
+    st = np.zeros(14, np.int64)
+    lib.nnk_fmm_stats(h.h, st.ctypes.data_as(P(ctypes.c_int64)), 14)
+    K, R = int(st[12]), int(st[13])
+    ms_step = dt / args.steps * 1e3
+    phases = ["strengths", "S2M", "up", "M2L", "L2L", "P2L", "down", "leaf"]
+    if precision == "fp64":
+        flops = 2.0 * K * K * st[0]
+        peak, kern = FP64_MFMA_PEAK_TFLOPS, "k_gemm_gather<160,64,16,1> (M2L, FP64 MFMA)"
+    else:
+        flops = 3 * 2.0 * R * R * st[0]    # three BF16 products per split product
+        peak, kern = BF16_MFMA_PEAK_TFLOPS, "k_gemm_bf16x3<128,64,1> (M2L, split-BF16 MFMA)"
+    m2l_ms = ms[3] / args.steps
+    achieved = flops / (m2l_ms * 1e-3) / 1e12 if m2l_ms > 0 else 0.0
+    rec = {"precision": precision, "coronaRes": res, "svd_tol": svd_tol if precision != "fp64" else None,
+           "rank": R if precision != "fp64" else K, "Mpts_s": round(tree.points.shape[0] / (ms_step * 1e-3) / 1e6, 4),
+           "ms_per_step": round(ms_step, 3), "setup_s": round(t_setup, 3),
+           "phase_ms_per_step": {p: round(float(ms[i] / args.steps), 3) for i, p in enumerate(phases)},
+           "m2l": {"kernel": kern, "flops_per_step": flops, "ms_per_step": round(m2l_ms, 3),
+                   "launches_per_step": float(cnt[3] / args.steps), "achieved_tflops": round(achieved, 3),
+                   "peak_tflops": peak}}
+    return u, rec
+
+
+def direct_sample(lib, pts, fd, idx, eps, mu):
+    import torch
+    dev = fd.device
+    tg = torch.tensor(pts[idx], device=dev)
+    sd = torch.tensor(pts, device=dev)
+    ud = torch.empty((len(idx), 3), dtype=torch.float64, device=dev)
+    stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    assert lib.nnk_direct_apply_device(ctypes.c_void_p(sd.data_ptr()), len(pts), ctypes.c_void_p(fd.data_ptr()),
+                                       ctypes.c_void_p(tg.data_ptr()), len(idx), eps, mu,
+                                       ctypes.c_void_p(ud.data_ptr()), stream) == 0
+    torch.cuda.synchronize()
+    return ud.cpu().numpy()
+
+
+def sphere_solve(nnk):
+    """One rigid sphere (radius 1, mu 1, eps 0.01), 384 force / 6144 NEAREST
+    quadrature points, driven by F = 6 pi e_x: GMRES to 1e-6 (FP64)."""
+    import importlib as il
+    geo = il.import_module("stokes-nnkifmm_amd.geometries")
+    mob = il.import_module("stokes-nnkifmm_amd.mobility")
+    c, fine = geo.cube_sphere(8), geo.cube_sphere(32)
+    out = {}
+    for pre in (False, True):
+        sw = nnk.Swimmers()
+        t0 = time.perf_counter()
+        NN = nnk.nearestMatrix(c, fine)
+        sw.addSwimmer(c, fine, NN, [0, 0, 0], [1, 0, 0, 0, 1, 0], np.zeros_like(c), [6 * np.pi, 0, 0], [0, 0, 0])
+        U = nnk.mobilityProblem(sw, lambda t, x: np.zeros_like(x), 0.0, np.r_[0, 0, 0, 1, 0, 0, 0, 1, 0],
+                                [0.01, 1.0], ["nodeCapacity", 64, "nearest", 1], 0, 1, 0.1, 1e-6, 200,
+                                [1e-6, 100] if pre else [])
+        dt = time.perf_counter() - t0
+        info = mob.last_solve
+        out["precon" if pre else "plain"] = {"wall_s": round(dt, 4), "iterations": info["iter"],
+                                             "relres": info["relres"], "U_x": float(U[0]),
+                                             "stokes_law_rel_err": abs(float(U[0]) - 1.0)}
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -81,87 +177,58 @@ def main():
     t0 = time.perf_counter()
     tree = nnk.OcTree(pts, pts, "nodeCapacity", 64)
     t_tree = time.perf_counter() - t0
-    fmm = nnk.KIFMM(tree, [eps, mu], "coronaRes", args.res)
     dev = torch.device("cuda:0")
     fd = torch.tensor(f, device=dev)
-    t0 = time.perf_counter()
-    h = fmm._handle()
-    torch.cuda.synchronize()
-    t_setup = time.perf_counter() - t0
-    for _ in range(args.warmup):
-        u = fmm.computeVel(fd)
-    torch.cuda.synchronize()
-    check = lib.nnk_fmm_set_profiling(h.h, 1)
-    assert check == 0
-    ms = np.zeros(8)
-    cnt = np.zeros(8, np.int64)
-    lib.nnk_fmm_phase_times(h.h, ms.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
-                            cnt.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), 8)  # reset
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        u = fmm.computeVel(fd)
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
-    lib.nnk_fmm_phase_times(h.h, ms.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
-                            cnt.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), 8)
-    lib.nnk_fmm_set_profiling(h.h, 0)
-    st = np.zeros(13, np.int64)
-    lib.nnk_fmm_stats(h.h, st.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), 13)
-    K = int(st[12])
-    ms_step = dt / args.steps * 1e3
-    # ---- roofline of the M2L translation GEMM ----
-    m2l_flops = 2.0 * K * K * st[0]              # per matvec
-    m2l_ms = ms[3] / args.steps                  # per matvec (4 launches)
-    launches = cnt[3] / args.steps
-    achieved = m2l_flops / (m2l_ms * 1e-3) / 1e12
+    idx = np.sort(rng.choice(N, 4096, replace=False))
+    ud = direct_sample(lib, pts, fd, idx, eps, mu)
+    forms = {}
+    for prec, res, tol in (("fp64", args.res, 0.0), ("mixed", args.mixed_res, args.svd_tol)):
+        if args.precision not in ("both", prec):
+            continue
+        u, rec = run_form(nnk, lib, tree, fd, args, eps, mu, prec, res, tol)
+        uo = u.cpu().numpy()[idx]
+        rec["rel_err_vs_direct"] = float(np.linalg.norm(uo - ud) / np.linalg.norm(ud))
+        forms[prec] = rec
+        del u
+        torch.cuda.empty_cache()
+    head = forms.get("mixed") or forms["fp64"]
+    m = head["m2l"]
     traffic = None
-    tfile = os.path.join(ROOT, "profiles", "m2l_fp64_hbm_traffic.json")
+    tfile = os.path.join(ROOT, "profiles", f"m2l_{head['precision']}_hbm_traffic.json")
     if os.path.exists(tfile):
         with open(tfile) as fh:
             traffic = json.load(fh).get("bytes_per_launch")
-    roof = {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP64_MFMA_PEAK_TFLOPS,
-            "unit": "TFLOP/s", "frac": round(achieved / FP64_MFMA_PEAK_TFLOPS, 4), "traffic": traffic,
-            "kernel": "k_gemm_gather<160,64,24,1> (M2L)", "launches_per_step": float(launches),
-            "flops_per_step": m2l_flops}
-    # ---- accuracy on 4096 sampled targets vs the GPU direct sum ----
-    idx = np.sort(rng.choice(N, 4096, replace=False))
-    tg = torch.tensor(pts[idx], device=dev)
-    sd = torch.tensor(pts, device=dev)
-    ud = torch.empty((4096, 3), dtype=torch.float64, device=dev)
-    stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
-    rc = lib.nnk_direct_apply_device(ctypes.c_void_p(sd.data_ptr()), N, ctypes.c_void_p(fd.data_ptr()),
-                                     ctypes.c_void_p(tg.data_ptr()), 4096, eps, mu,
-                                     ctypes.c_void_p(ud.data_ptr()), stream)
-    assert rc == 0
-    torch.cuda.synchronize()
-    uo = u.cpu().numpy()[idx]
-    udn = ud.cpu().numpy()
-    rel_err = float(np.linalg.norm(uo - udn) / np.linalg.norm(udn))
-    phases = ["strengths", "S2M", "up_gemm", "M2L", "L2L", "P2L", "down_solve", "leaf"]
+    roof = {"bound": "mfma", "achieved": m["achieved_tflops"], "peak": m["peak_tflops"], "unit": "TFLOP/s",
+            "frac": round(m["achieved_tflops"] / m["peak_tflops"], 4), "traffic": traffic,
+            "kernel": m["kernel"], "launches_per_step": m["launches_per_step"],
+            "flops_per_step": m["flops_per_step"]}
     out = {
         "metric": "FMM matvec Mpts/s + mobility solve time at 1e-6 rel err, FP32/MFMA vs FP64",
-        "value": round(N / (ms_step * 1e-3) / 1e6, 4),
+        "value": head["Mpts_s"],
         "unit": "Mpts/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_step, 3),
+        "ms_per_step": head["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "bf16x3-mfma/f32/f64" if head["precision"] == "mixed" else "f64",
         "data": "synthetic",
         "config": {"workload": f"KIFMM matvec, N={N} uniform points in the unit cube, forces N(0,1), "
-                               f"eps={eps}, mu={mu}, nodeCapacity 64, coronaRes {args.res}/shells 1, "
-                               f"plain FP64 form",
-                   "N": N, "eps": eps, "coronaRes": args.res, "nodeCapacity": 64, "precision": "fp64",
-                   "nodes": int(tree.nodeCount), "parallelism": "single GPU"},
-        "rel_err_vs_direct": rel_err,
-        "phase_ms_per_step": {p: round(float(ms[i] / args.steps), 3) for i, p in enumerate(phases)},
-        "setup_s": {"tree": round(t_tree, 3), "operators": round(t_setup, 3)},
+                               f"eps={eps}, mu={mu}, nodeCapacity 64",
+                   "N": N, "eps": eps, "nodeCapacity": 64, "nodes": int(tree.nodeCount),
+                   "headline_form": head["precision"], "coronaRes": head["coronaRes"],
+                   "parallelism": "single GPU"},
+        "rel_err_vs_direct": head["rel_err_vs_direct"],
+        "forms": forms,
+        "setup_s": {"tree": round(t_tree, 3)},
         "roofline": roof,
     }
+    if "mixed" in forms and "fp64" in forms:
+        out["speedup_mixed_vs_fp64"] = round(forms["fp64"]["ms_per_step"] / forms["mixed"]["ms_per_step"], 3)
+    if not args.no_solve:
+        out["mobility_solve_one_sphere"] = sphere_solve(nnk)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args, eps, mu)
     print(json.dumps(out), flush=True)

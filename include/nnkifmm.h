@@ -41,7 +41,10 @@ enum {
 };
 
 enum { NNK_LIST_U = 0, NNK_LIST_V = 1, NNK_LIST_W = 2, NNK_LIST_X = 3 };
-enum { NNK_FP64 = 0 };
+/* NNK_FP64: plain FP64 form, reproduces the reference's velocities.
+ * NNK_MIXED: truncated-SVD pseudo-inverses (FP64), split-BF16 MFMA M2L with
+ * FP32 accumulation, FP32-math direct sums with FP64 accumulation. */
+enum { NNK_FP64 = 0, NNK_MIXED = 1 };
 
 /* Library version (major*10000 + minor*100 + patch). */
 int32_t nnk_version(void);
@@ -84,10 +87,12 @@ void nnk_tree_destroy(nnk_tree *t);
 /* Replaces: KIFMM(tree, [eps mu], 'coronaRes', r, 'coronaShells', s)
  *           Classes/KIFMM.m:46-61.  Builds the device-resident operator:
  *           per-level check-to-equivalent factors and translation
- *           operators, point data in leaf order.  The tree must stay alive
- *           while the handle exists. */
+ *           operators, point data in leaf order.  precision is NNK_FP64 or
+ *           NNK_MIXED; svd_tol (mixed form only, <= 0 for the default 1e-10)
+ *           is the relative singular-value cut of the pseudo-inverses.  The
+ *           tree must stay alive while the handle exists. */
 int32_t nnk_fmm_create(const nnk_tree *t, double eps, double mu, int32_t corona_res,
-                       int32_t corona_shells, int32_t precision, nnk_fmm **out);
+                       int32_t corona_shells, int32_t precision, double svd_tol, nnk_fmm **out);
 
 /* Replaces: KIFMM.computeVel  Classes/KIFMM.m:63-105 (upwardPass +
  *           downwardPass).  f: npot x 3 host array in points(potentials)
@@ -149,6 +154,18 @@ int32_t nnk_mobility_solve(nnk_mobility *h, const double *rhs, double tol, int32
                            int32_t precon, double *x, int32_t *flag, double *relres,
                            int32_t *iters, double *resvec);
 
+/* No counterpart in the reference (which solves in FP64 only): the same
+ *           solve by iterative refinement.  Outer residuals use the FP64
+ *           operator of nnk_mobility_create; each correction is a GMRES solve
+ *           on `inner` (a KIFMM handle on the same tree, usually
+ *           NNK_MIXED) to relative tolerance inner_tol.  Stops when the true
+ *           FP64 relative residual of the unpreconditioned system is <= tol
+ *           or maxit inner iterations are used.  *iters = inner iterations,
+ *           *outer = refinement steps. */
+int32_t nnk_mobility_solve_refined(nnk_mobility *h, nnk_fmm *inner, const double *rhs, double tol,
+                                   int32_t maxit, double inner_tol, int32_t precon, double *x,
+                                   int32_t *flag, double *relres, int32_t *iters, int32_t *outer);
+
 void nnk_mobility_destroy(nnk_mobility *h);
 
 /* Replaces: blockcomputation(x, x0, q, blockSize, [eps mu])
@@ -173,7 +190,8 @@ int32_t nnk_fmm_phase_times(nnk_fmm *h, double *ms, int64_t *count, int32_t npha
  * accounting.  0 M2L pairs, 1 P2P pairs, 2 S2M pairs, 3 L2T pairs,
  * 4 M2T pairs, 5 P2L pairs, 6 up-solve nodes, 7 down-solve nodes, 8 M2M
  * parents, 9 L2L nodes, 10 sources, 11 targets; out[nstats]; also returns
- * the density length 3*Nq in out[nstats] if nstats > 12. */
+ * the density length 3*Nq in out[12] if nstats > 12 and the pseudo-inverse
+ * rank R of the mixed form in out[13] if nstats > 13. */
 int32_t nnk_fmm_stats(const nnk_fmm *h, int64_t *out, int32_t nstats);
 
 #ifdef __cplusplus

@@ -6,7 +6,8 @@ Import as ``importlib.import_module("stokes-nnkifmm_amd")``.
 """
 from ._lib import NNKError, lib  # noqa: F401
 from .kifmm import KIFMM  # noqa: F401
-from .mobility import (genMatrixComponets, mobilityMatrixMulti, mobilityPrecon,  # noqa: F401
+from .mobility import (genMatrixComponets, mobility_gmres, mobility_refined, mobilityMatrixMulti,  # noqa: F401
+                       mobilityPrecon,
                        mobilityProblem, mobilityProblemGravity)
 from .nearest import nearestMatrix  # noqa: F401
 from .octree import OcTree  # noqa: F401

@@ -31,7 +31,7 @@ SIGNATURES = {
     "nnk_tree_list_export": (ctypes.c_int32, [_vp, ctypes.c_int32, _i64, _i32]),
     "nnk_tree_destroy": (None, [_vp]),
     "nnk_fmm_create": (ctypes.c_int32, [_vp, ctypes.c_double, ctypes.c_double, ctypes.c_int32,
-                                        ctypes.c_int32, ctypes.c_int32, _h]),
+                                        ctypes.c_int32, ctypes.c_int32, ctypes.c_double, _h]),
     "nnk_fmm_apply": (ctypes.c_int32, [_vp, _d, _d]),
     "nnk_fmm_apply_device": (ctypes.c_int32, [_vp, _vp, _vp, _vp]),
     "nnk_fmm_reduced_apply": (ctypes.c_int32, [_vp, _d, _d]),
@@ -56,6 +56,9 @@ SIGNATURES = {
     "nnk_mobility_precon_device": (ctypes.c_int32, [_vp, _vp, _vp, _vp]),
     "nnk_mobility_solve": (ctypes.c_int32, [_vp, _d, ctypes.c_double, ctypes.c_int32, ctypes.c_int32, _d,
                                             _i32, _d, _i32, _d]),
+    "nnk_mobility_solve_refined": (ctypes.c_int32, [_vp, _vp, _d, ctypes.c_double, ctypes.c_int32,
+                                                    ctypes.c_double, ctypes.c_int32, _d, _i32, _d, _i32,
+                                                    _i32]),
     "nnk_mobility_destroy": (None, [_vp]),
 }
 

@@ -138,12 +138,14 @@ int32_t nnk_tree_list_export(const nnk_tree *h, int32_t kind, int64_t *ptr, int3
 void nnk_tree_destroy(nnk_tree *t) { delete t; }
 
 int32_t nnk_fmm_create(const nnk_tree *t, double eps, double mu, int32_t res, int32_t shells,
-                       int32_t precision, nnk_fmm **out) {
+                       int32_t precision, double svd_tol, nnk_fmm **out) {
   return guard([&]() -> int32_t {
     if (!t || !out) return fail(NNK_EARG, "nnk_fmm_create: null argument");
     *out = nullptr;
-    if (precision != NNK_FP64) return fail(NNK_EARG, "nnk_fmm_create: unknown precision");
-    auto f = std::make_unique<nnk::Fmm>(t->t, eps, mu, res, shells);
+    if (precision != NNK_FP64 && precision != NNK_MIXED)
+      return fail(NNK_EARG, "nnk_fmm_create: unknown precision");
+    auto f = std::make_unique<nnk::Fmm>(t->t, eps, mu, res, shells, precision,
+                                        svd_tol > 0 ? svd_tol : 1e-10);
     *out = new nnk_fmm{std::move(f), t->t->npot, t->t->ntgt};
     return NNK_OK;
   });
@@ -286,6 +288,33 @@ int32_t nnk_mobility_solve(nnk_mobility *h, const double *rhs, double tol, int32
   });
 }
 
+int32_t nnk_mobility_solve_refined(nnk_mobility *h, nnk_fmm *inner, const double *rhs, double tol,
+                                   int32_t maxit, double inner_tol, int32_t precon, double *x,
+                                   int32_t *flag, double *relres, int32_t *iters, int32_t *outer) {
+  return guard([&]() -> int32_t {
+    if (!h || !inner || !rhs || !x || maxit < 1 || !(tol > 0) || !(inner_tol > 0))
+      return fail(NNK_EARG, "nnk_mobility_solve_refined: bad argument");
+    if (inner->npot != h->m->points() || inner->ntgt != h->m->points())
+      return fail(NNK_EARG, "nnk_mobility_solve_refined: inner operator is not on the mobility tree");
+    const int64_t n = h->m->size();
+    nnk::DBuf db, dx;
+    db.alloc(sizeof(double) * n);
+    dx.alloc(sizeof(double) * n);
+    NNK_HIP(hipMemcpy(db.p, rhs, sizeof(double) * n, hipMemcpyHostToDevice));
+    h->m->set_inner(inner->f.get());
+    int nout = 0;
+    nnk::GmresResult R =
+        h->m->refine(db.as<double>(), dx.as<double>(), tol, maxit, inner_tol, precon != 0, 0, nullptr, &nout);
+    h->m->set_inner(nullptr);
+    NNK_HIP(hipMemcpy(x, dx.p, sizeof(double) * n, hipMemcpyDeviceToHost));
+    if (flag) *flag = R.flag;
+    if (relres) *relres = R.relres;
+    if (iters) *iters = R.iter;
+    if (outer) *outer = nout;
+    return NNK_OK;
+  });
+}
+
 void nnk_mobility_destroy(nnk_mobility *h) { delete h; }
 
 int32_t nnk_direct_apply(const double *src, int64_t ns, const double *q, const double *tgt,
@@ -340,6 +369,7 @@ int32_t nnk_fmm_stats(const nnk_fmm *h, int64_t *out, int32_t nstats) {
   if (!h || !out || nstats < nnk::Fmm::kStats) return fail(NNK_EARG, "nnk_fmm_stats: bad argument");
   h->f->stats(out);
   if (nstats > nnk::Fmm::kStats) out[nnk::Fmm::kStats] = h->f->K();
+  if (nstats > nnk::Fmm::kStats + 1) out[nnk::Fmm::kStats + 1] = h->f->rank();
   return NNK_OK;
 }
 

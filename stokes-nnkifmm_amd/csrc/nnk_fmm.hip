@@ -98,6 +98,7 @@ struct SurfArgs {
 // Check potential on a node surface from the sources of a set of leaves
 // (S2M: the leaf itself onto its down surface; P2L: the X list onto the up
 // surface).  One block per node; sources staged in LDS.
+template <bool F32>
 __global__ __launch_bounds__(256) void k_surf_eval(SurfArgs a) {
   const int n = a.nodes[blockIdx.x];
   __shared__ double sp[256 * 6];
@@ -125,8 +126,8 @@ __global__ __launch_bounds__(256) void k_surf_eval(SurfArgs a) {
         __syncthreads();
         if (p < a.nq)
           for (int j = 0; j < cnt; j++)
-            stokeslet_acc(tx, ty, tz, sp[6 * j], sp[6 * j + 1], sp[6 * j + 2], sp[6 * j + 3],
-                          sp[6 * j + 4], sp[6 * j + 5], a.e2, ux, uy, uz);
+            stk<F32>(tx, ty, tz, sp[6 * j], sp[6 * j + 1], sp[6 * j + 2], sp[6 * j + 3],
+                     sp[6 * j + 4], sp[6 * j + 5], a.e2, ux, uy, uz);
         __syncthreads();
       }
     }
@@ -163,6 +164,7 @@ struct LeafArgs {
 // its U-list sources, M2T from its W-list up densities (Downleaf.m:37-40).
 // Each target is shared by nsub lanes that split the sources; partial sums
 // are combined in a fixed order.
+template <bool F32>
 __global__ __launch_bounds__(256) void k_leaf(LeafArgs a) {
   const int n = a.leaves[blockIdx.x];
   __shared__ double sp[256 * 6];
@@ -183,8 +185,8 @@ __global__ __launch_bounds__(256) void k_leaf(LeafArgs a) {
     auto consume = [&](int cnt) {
       if (act)
         for (int j = sub; j < cnt; j += nsub)
-          stokeslet_acc(tx, ty, tz, sp[6 * j], sp[6 * j + 1], sp[6 * j + 2], sp[6 * j + 3],
-                        sp[6 * j + 4], sp[6 * j + 5], a.e2, ux, uy, uz);
+          stk<F32>(tx, ty, tz, sp[6 * j], sp[6 * j + 1], sp[6 * j + 2], sp[6 * j + 3],
+                   sp[6 * j + 4], sp[6 * j + 5], a.e2, ux, uy, uz);
     };
     auto surface = [&](int node, int down, const double *dens) {
       const double *box = a.corners + 6 * (size_t)(node - 1);
@@ -322,6 +324,13 @@ static void launch_gemm(int nrb, int M, int K, int Mpad, int Kpad, const double 
   NNK_HIP(hipGetLastError());
 }
 
+void Fmm::gemm_dims(const GemmBatch &b, const double *X, int ldx, double *C, int ldc, int beta,
+                    hipStream_t s, int M, int K, int Mpad, int Kpad) {
+  if (b.njobs == 0 || b.maxN == 0) return;
+  if (Mpad % GBM || Kpad % GBK) throw Error(NNK_ESTATE, "gemm_dims: bad padding");
+  launch_gemm<64, 0>(Mpad / GBM, M, K, Mpad, Kpad, X, ldx, C, ldc, beta, b, s);
+}
+
 void Fmm::gemm(const GemmBatch &b, const double *X, int ldx, double *C, int ldc, int beta,
                hipStream_t s, int tag) {
   if (b.njobs == 0 || b.maxN == 0) return;
@@ -341,9 +350,13 @@ Fmm::~Fmm() {
   }
 }
 
-Fmm::Fmm(std::shared_ptr<const Tree> t, double eps, double mu, int res, int shells)
-    : tp_(std::move(t)), tree_(*tp_), eps_(eps), mu_(mu), res_(res), shells_(shells) {
+Fmm::Fmm(std::shared_ptr<const Tree> t, double eps, double mu, int res, int shells, int precision,
+         double tau)
+    : tp_(std::move(t)), tree_(*tp_), eps_(eps), mu_(mu), res_(res), shells_(shells),
+      prec_(precision), tau_(tau) {
   const Tree &T = tree_;
+  if (precision != 0 && precision != 1) throw Error(NNK_EARG, "unknown precision");
+  if (precision == 1 && !(tau > 0 && tau < 1)) throw Error(NNK_EARG, "truncation tolerance must be in (0,1)");
   if (res < 2 || res > 16 || shells < 1 || 2 * shells > res)
     throw Error(NNK_EARG, "coronaRes must be in [2,16] and 1 <= coronaShells <= res/2");
   if (!(eps > 0) || !(mu > 0)) throw Error(NNK_EARG, "kernel parameters eps and mu must be > 0");
@@ -456,8 +469,17 @@ Fmm::Fmm(std::shared_ptr<const Tree> t, double eps, double mu, int res, int shel
   uppot_.zero();
   downpot_.zero();
   ops_.resize(maxlev_ + 1);
+  if (prec_ == 1) {
+    svdU_.resize(maxlev_ + 1);
+    svdS_.resize(maxlev_ + 1);
+    svdV_.resize(maxlev_ + 1);
+  }
   for (int l = 2; l <= maxlev_; l++) build_ops(l);
   plan();
+  if (prec_ == 1) {
+    build_mixed();
+    plan_mixed();
+  }
   NNK_HIP(hipDeviceSynchronize());
 }
 
@@ -484,6 +506,11 @@ void Fmm::build_ops(int l) {
             A[(size_t)(3 * p + a) * n + 3 * q + b] = (r[a] * r[b] + (a == b ? r2 + 2 * e2 : 0.0)) * invs;
       }
     svd_jacobi(A.data(), n, U.data(), S.data(), V.data());
+    if (prec_ == 1) {
+      svdU_[l] = U;
+      svdS_[l] = S;
+      svdV_[l] = V;
+    }
     // up:   x = V (S^-1 U^T b);  down (K(down->up) = A^T): x = U (S^-1 V^T b)
     std::vector<double> Wup(mat, 0.0), Vup(mat, 0.0), Wdn(mat, 0.0), Udn(mat, 0.0);
     for (int k = 0; k < n; k++)
@@ -570,6 +597,17 @@ void Fmm::plan() {
   }
   plans_.clear();
   plans_.resize(maxlev_ + 1);
+  hl_.assign(maxlev_ + 1, HostLevel());
+  {
+    std::vector<uint8_t> inw(nc + 1, 0);
+    for (int32_t w : T.lidx[2]) inw[w] = 1;
+    for (int n = 1; n <= nc; n++) {
+      const int l = T.level[n - 1];
+      if (l < 2) continue;
+      if (inw[n]) hl_[l].wnodes.push_back(n - 1);
+      if (T.tgt_leaf[n]) hl_[l].tleaves.push_back(n - 1);
+    }
+  }
   size_t maxn = 1;
   std::vector<std::vector<int>> bylev(maxlev_ + 1);
   for (int n = 1; n <= nc; n++) bylev[T.level[n - 1]].push_back(n);
@@ -617,7 +655,10 @@ void Fmm::plan() {
       std::vector<GemmJob> jobs;
       add_job(P.m2m, jobs, A, &g, &nl, (int)nl.size());
       finish(P.m2m, jobs);
+      hl_[l].m2m_g = g;
     }
+    hl_[l].up_leaves = up;
+    hl_[l].up_nonleaves = nl;
     std::vector<int32_t> upall = up;
     upall.insert(upall.end(), nl.begin(), nl.end());
     P.n_up = (int)upall.size();
@@ -675,11 +716,13 @@ void Fmm::plan() {
           }
         }
         add_job(P.m2l, jobs, A, &g, &cls[a], N);
-        if (l >= 3) {  // parent's down density onto the child's up surface
-          std::vector<int32_t> gp(N);
-          for (int j = 0; j < N; j++) gp[j] = T.parent[clsnode[a][j] - 1] - 1;
+        std::vector<int32_t> gp(N);
+        for (int j = 0; j < N; j++) gp[j] = T.parent[clsnode[a][j] - 1] - 1;
+        if (l >= 3)  // parent's down density onto the child's up surface
           add_job(P.l2l, ljobs, {O.L2L.as<double>() + a * mat}, &gp, &cls[a], N);
-        }
+        hl_[l].cls[a] = cls[a];
+        hl_[l].gv[a] = std::move(g);
+        hl_[l].gp[a] = std::move(gp);
       }
       finish(P.m2l, jobs);
       finish(P.l2l, ljobs);
@@ -695,6 +738,8 @@ void Fmm::plan() {
       if (tgt_below[n] && T.lptr[3][n + 1] > T.lptr[3][n]) xs.push_back(n);
     P.n_x = (int)xs.size();
     P.x_nodes.upload(xs);
+    hl_[l].dn = dn;
+    hl_[l].xnodes = xs;
   }
   tmp_.alloc(sizeof(double) * maxn * K_);
   // remaining work counts
@@ -764,29 +809,67 @@ void Fmm::strengths(const double *d_f, hipStream_t s) {
   NNK_HIP(hipGetLastError());
 }
 
+void Fmm::launch_surf(const int32_t *nodes, int n, const int64_t *lptr, const int32_t *lidx, int down,
+                      int accumulate, double *chk, hipStream_t s) {
+  if (n <= 0) return;
+  SurfArgs a{};
+  a.nodes = nodes;
+  a.lptr = lptr;
+  a.lidx = lidx;
+  a.down = down;
+  a.accumulate = accumulate;
+  a.corners = corners_.as<double>();
+  a.surf_idx = surf_idx_.as<int32_t>();
+  a.nq = nq_; a.n1 = res_ - 1; a.K = K_;
+  a.src_ptr = src_ptr_.as<int32_t>();
+  a.src_pos = src_pos_.as<double>();
+  a.src_q = src_q_.as<double>();
+  a.e2 = eps_ * eps_;
+  a.c = 1.0 / (8.0 * kPi * mu_);
+  a.chk = chk;
+  if (prec_ == 1) hipLaunchKernelGGL(k_surf_eval<true>, dim3(n), dim3(256), 0, s, a);
+  else hipLaunchKernelGGL(k_surf_eval<false>, dim3(n), dim3(256), 0, s, a);
+  NNK_HIP(hipGetLastError());
+}
+
+void Fmm::launch_leaf(int far, double *d_u, hipStream_t s) {
+  if (!n_tleaves_) return;
+  LeafArgs a{};
+  a.leaves = tleaves_.as<int32_t>();
+  a.corners = corners_.as<double>();
+  a.surf_idx = surf_idx_.as<int32_t>();
+  a.nq = nq_; a.n1 = res_ - 1; a.K = K_;
+  a.tgt_ptr = tgt_ptr_.as<int32_t>();
+  a.tgt_pos = tgt_pos_.as<double>();
+  a.tgt_out = tgt_out_.as<int32_t>();
+  a.src_ptr = src_ptr_.as<int32_t>();
+  a.src_pos = src_pos_.as<double>();
+  a.src_q = src_q_.as<double>();
+  a.uptr = uptr_.as<int64_t>(); a.uidx = uidx_.as<int32_t>();
+  a.wptr = wptr_.as<int64_t>(); a.widx = widx_.as<int32_t>();
+  a.uppot = uppot_.as<double>();
+  a.downpot = downpot_.as<double>();
+  a.e2 = eps_ * eps_;
+  a.c = 1.0 / (8.0 * kPi * mu_);
+  a.far = far;
+  a.out = d_u;
+  if (prec_ == 1) hipLaunchKernelGGL(k_leaf<true>, dim3(n_tleaves_), dim3(256), 0, s, a);
+  else hipLaunchKernelGGL(k_leaf<false>, dim3(n_tleaves_), dim3(256), 0, s, a);
+  NNK_HIP(hipGetLastError());
+}
+
 void Fmm::apply(const double *d_f, double *d_u, hipStream_t s) {
-  const double e2 = eps_ * eps_, c = 1.0 / (8.0 * kPi * mu_);
+  if (prec_ == 1) { apply_mixed(d_f, d_u, s); return; }
   mark(0, true, s);
   strengths(d_f, s);
   mark(0, false, s);
   double *chk = chk_.as<double>(), *up = uppot_.as<double>(), *dn = downpot_.as<double>(),
          *tmp = tmp_.as<double>();
-  // S2M for every potential leaf at level >= 2 (coarser up densities are unused)
+  // S2M for every potential leaf (up densities of levels 0-1 are never used,
+  // but computing them is harmless and keeps one launch)
   if (maxlev_ >= 2 && n_pleaves_) {
-    SurfArgs a{};
-    a.nodes = pleaves_.as<int32_t>();
-    a.down = 1;
-    a.corners = corners_.as<double>();
-    a.surf_idx = surf_idx_.as<int32_t>();
-    a.nq = nq_; a.n1 = res_ - 1; a.K = K_;
-    a.src_ptr = src_ptr_.as<int32_t>();
-    a.src_pos = src_pos_.as<double>();
-    a.src_q = src_q_.as<double>();
-    a.e2 = e2; a.c = c;
-    a.chk = chk;
     mark(1, true, s);
-    hipLaunchKernelGGL(k_surf_eval, dim3(n_pleaves_), dim3(256), 0, s, a);
-    NNK_HIP(hipGetLastError());
+    launch_surf(pleaves_.as<int32_t>(), n_pleaves_, nullptr, nullptr, 1, 0, chk, s);
     mark(1, false, s);
   }
   mark(2, true, s);
@@ -809,22 +892,7 @@ void Fmm::apply(const double *d_f, double *d_u, hipStream_t s) {
     mark(4, false, s);
     if (P.n_x) {
       mark(5, true, s);
-      SurfArgs a{};
-      a.nodes = P.x_nodes.as<int32_t>();
-      a.lptr = xptr_.as<int64_t>();
-      a.lidx = xidx_.as<int32_t>();
-      a.down = 0;
-      a.accumulate = 1;
-      a.corners = corners_.as<double>();
-      a.surf_idx = surf_idx_.as<int32_t>();
-      a.nq = nq_; a.n1 = res_ - 1; a.K = K_;
-      a.src_ptr = src_ptr_.as<int32_t>();
-      a.src_pos = src_pos_.as<double>();
-      a.src_q = src_q_.as<double>();
-      a.e2 = e2; a.c = c;
-      a.chk = chk;
-      hipLaunchKernelGGL(k_surf_eval, dim3(P.n_x), dim3(256), 0, s, a);
-      NNK_HIP(hipGetLastError());
+      launch_surf(P.x_nodes.as<int32_t>(), P.n_x, xptr_.as<int64_t>(), xidx_.as<int32_t>(), 0, 1, chk, s);
       mark(5, false, s);
     }
     mark(6, true, s);
@@ -832,29 +900,9 @@ void Fmm::apply(const double *d_f, double *d_u, hipStream_t s) {
     gemm(P.dn_solve2, tmp, K_, dn, K_, 0, s);
     mark(6, false, s);
   }
-  if (n_tleaves_) {
-    LeafArgs a{};
-    a.leaves = tleaves_.as<int32_t>();
-    a.corners = corners_.as<double>();
-    a.surf_idx = surf_idx_.as<int32_t>();
-    a.nq = nq_; a.n1 = res_ - 1; a.K = K_;
-    a.tgt_ptr = tgt_ptr_.as<int32_t>();
-    a.tgt_pos = tgt_pos_.as<double>();
-    a.tgt_out = tgt_out_.as<int32_t>();
-    a.src_ptr = src_ptr_.as<int32_t>();
-    a.src_pos = src_pos_.as<double>();
-    a.src_q = src_q_.as<double>();
-    a.uptr = uptr_.as<int64_t>(); a.uidx = uidx_.as<int32_t>();
-    a.wptr = wptr_.as<int64_t>(); a.widx = widx_.as<int32_t>();
-    a.uppot = up; a.downpot = dn;
-    a.e2 = e2; a.c = c;
-    a.far = 1;
-    a.out = d_u;
-    mark(7, true, s);
-    hipLaunchKernelGGL(k_leaf, dim3(n_tleaves_), dim3(256), 0, s, a);
-    NNK_HIP(hipGetLastError());
-    mark(7, false, s);
-  }
+  mark(7, true, s);
+  launch_leaf(1, d_u, s);
+  mark(7, false, s);
 }
 
 void Fmm::debug_copy(int which, double *host) const {
@@ -864,27 +912,8 @@ void Fmm::debug_copy(int which, double *host) const {
 }
 
 void Fmm::reduced_apply(const double *d_f, double *d_u, hipStream_t s) {
-  const double e2 = eps_ * eps_, c = 1.0 / (8.0 * kPi * mu_);
   strengths(d_f, s);
-  if (!n_tleaves_) return;
-  LeafArgs a{};
-  a.leaves = tleaves_.as<int32_t>();
-  a.corners = corners_.as<double>();
-  a.surf_idx = surf_idx_.as<int32_t>();
-  a.nq = nq_; a.n1 = res_ - 1; a.K = K_;
-  a.tgt_ptr = tgt_ptr_.as<int32_t>();
-  a.tgt_pos = tgt_pos_.as<double>();
-  a.tgt_out = tgt_out_.as<int32_t>();
-  a.src_ptr = src_ptr_.as<int32_t>();
-  a.src_pos = src_pos_.as<double>();
-  a.src_q = src_q_.as<double>();
-  a.uptr = uptr_.as<int64_t>(); a.uidx = uidx_.as<int32_t>();
-  a.wptr = wptr_.as<int64_t>(); a.widx = widx_.as<int32_t>();
-  a.e2 = e2; a.c = c;
-  a.far = 0;
-  a.out = d_u;
-  hipLaunchKernelGGL(k_leaf, dim3(n_tleaves_), dim3(256), 0, s, a);
-  NNK_HIP(hipGetLastError());
+  launch_leaf(0, d_u, s);
 }
 
 }  // namespace nnk

@@ -42,6 +42,27 @@ struct LevelPlan {
   GemmBatch dn_solve1, dn_solve2;
   int n_x = 0;
   DBuf x_nodes;               // nodes with a non-empty X list
+  // mixed form
+  GemmBatch mx_m2m, mx_upP, mx_l2l, mx_xP, mx_downU, mx_wV;
+  DBuf mx_m2l;                // GemmJobBf16[8]
+  std::vector<DBuf> mx_owned;
+  int mx_m2l_jobs = 0, mx_m2l_maxN = 0;
+};
+
+// host copies of a level's node lists (slots = id-1), shared by both forms
+struct HostLevel {
+  std::vector<int32_t> up_leaves, up_nonleaves, m2m_g;   // m2m_g: 8 x nonleaves
+  std::vector<int32_t> dn, tleaves, wnodes, xnodes;       // xnodes: ids
+  std::vector<int32_t> cls[8], gv[8], gp[8];              // gv: 189 x |cls|
+};
+
+// compressed (truncated-SVD) operators of one level for the mixed form
+struct MixedOps {
+  DBuf Pup, Vr, Pdn, Ur;      // FP64 At layout: R x K, K x R, R x K, K x R
+  DBuf VrT, UrT;              // R rows of length K (precompute inputs)
+  DBuf Mc, Lc;                // 8 each, FP64 At (R x R)
+  DBuf Chi, Clo;              // 316 M2L operators, bf16 row-major [RPm][RP]
+  std::vector<const uint16_t *> chi, clo;  // per offset slot (343)
 };
 
 struct LevelOps {
@@ -54,7 +75,10 @@ struct LevelOps {
 
 class Fmm {
  public:
-  Fmm(std::shared_ptr<const Tree> t, double eps, double mu, int res, int shells);
+  // precision 0: FP64 form (reference parity); 1: mixed form (truncated SVD
+  // at relative tolerance tau, BF16x3 M2L, FP32-math direct sums)
+  Fmm(std::shared_ptr<const Tree> t, double eps, double mu, int res, int shells, int precision = 0,
+      double tau = 1e-10);
   ~Fmm();
   Fmm(const Fmm &) = delete;
   Fmm &operator=(const Fmm &) = delete;
@@ -72,6 +96,7 @@ class Fmm {
   // copy an internal per-node array (0 check, 1 up, 2 down density) to host
   void debug_copy(int which, double *host) const;
   int K() const { return K_; }
+  int rank() const { return R_; }
   // Phase timing with HIP events on the apply stream (no host sync inside
   // apply).  Phases: 0 strengths, 1 S2M, 2 upward GEMMs, 3 M2L, 4 L2L,
   // 5 P2L, 6 downward solves, 7 leaf L2T+P2P+M2T.
@@ -90,6 +115,14 @@ class Fmm {
   void build_ops(int level);
   void gemm(const GemmBatch &b, const double *X, int ldx, double *C, int ldc, int beta, hipStream_t s,
             int tag = 0);
+  void gemm_dims(const GemmBatch &b, const double *X, int ldx, double *C, int ldc, int beta,
+                 hipStream_t s, int M, int K, int Mpad, int Kpad);
+  void build_mixed();
+  void plan_mixed();
+  void apply_mixed(const double *d_f, double *d_u, hipStream_t s);
+  void launch_surf(const int32_t *nodes, int n, const int64_t *lptr, const int32_t *lidx, int down,
+                   int accumulate, double *chk, hipStream_t s);
+  void launch_leaf(int far, double *d_u, hipStream_t s);
   void strengths(const double *d_f, hipStream_t s);
   void mark(int phase, bool begin, hipStream_t s);
 
@@ -111,6 +144,14 @@ class Fmm {
   DBuf chk_, uppot_, downpot_, tmp_;
   std::vector<LevelOps> ops_;
   std::vector<LevelPlan> plans_;
+  std::vector<HostLevel> hl_;
+  int prec_ = 0;
+  double tau_ = 1e-10;
+  int R_ = 0, RP_ = 0, RPm_ = 0, MpadR_ = 0, KpadR_ = 0;
+  std::vector<std::vector<double>> svdU_, svdS_, svdV_;  // host, per level
+  std::vector<MixedOps> mops_;
+  DBuf chat_, dhat_, chat_hi_, chat_lo_, upall_;
+  int n_upall_ = 0;
   void reduced_factor();
   bool red_ready_ = false;
   DBuf red_blk_, red_off_, red_piv_, red_poff_, red_ord2loc_;

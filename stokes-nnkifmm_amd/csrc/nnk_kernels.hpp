@@ -54,4 +54,33 @@ __device__ __forceinline__ void stokeslet_acc(double tx, double ty, double tz, d
   uz += iso * qz + rq * rz;
 }
 
+// Mixed form: the separation r = t - s is formed in FP64 (so near pairs keep
+// their digits), the kernel is evaluated in FP32 with the hardware reciprocal
+// square root, and the result is accumulated in FP64.
+__device__ __forceinline__ void stokeslet_acc_f32(double tx, double ty, double tz, double sx,
+                                                  double sy, double sz, double qx, double qy,
+                                                  double qz, double e2, double &ux, double &uy,
+                                                  double &uz) {
+  const float rx = (float)(tx - sx), ry = (float)(ty - sy), rz = (float)(tz - sz);
+  const float e2f = (float)e2;
+  const float r2 = rx * rx + ry * ry + rz * rz;
+  const float re = r2 + e2f;
+  const float inv = __builtin_amdgcn_rsqf(re);
+  const float inv3 = inv * inv * inv;
+  const float iso = (re + e2f) * inv3;
+  const float fx = (float)qx, fy = (float)qy, fz = (float)qz;
+  const float rq = (rx * fx + ry * fy + rz * fz) * inv3;
+  ux += (double)(iso * fx + rq * rx);
+  uy += (double)(iso * fy + rq * ry);
+  uz += (double)(iso * fz + rq * rz);
+}
+
+template <bool F32>
+__device__ __forceinline__ void stk(double tx, double ty, double tz, double sx, double sy, double sz,
+                                    double qx, double qy, double qz, double e2, double &ux, double &uy,
+                                    double &uz) {
+  if constexpr (F32) stokeslet_acc_f32(tx, ty, tz, sx, sy, sz, qx, qy, qz, e2, ux, uy, uz);
+  else stokeslet_acc(tx, ty, tz, sx, sy, sz, qx, qy, qz, e2, ux, uy, uz);
+}
+
 }  // namespace nnk

@@ -1,0 +1,393 @@
+// Mixed-precision form of the KIFMM matvec.
+//
+// Check-to-equivalent solves use a truncated SVD pseudo-inverse (singular
+// values below tau * s_max dropped; a common rank R over the levels), in
+// FP64.  Densities are carried as coefficients in the singular bases:
+//   up:   c = S_R^-1 U_R^T chk_up   (equivalent density V_R c)
+//   down: d = S_R^-1 V_R^T chk_dn   (equivalent density U_R d)
+// so every translation becomes an R x R operator:
+//   M2M_k <- Pup(l-1) K(child up -> parent down) V_R(l)
+//   L2L_k <- Pdn(l)   K(parent down -> child up) U_R(l-1)
+//   M2L_o <- Pdn(l)   K(up at offset o -> up)    V_R(l)
+// formed once on the device with the FP64 GEMM.  M2L (the dominant work)
+// runs as split-BF16 MFMA GEMMs with FP32 accumulation promoted to FP64 per
+// translation; M2M, L2L and the expansions to densities stay FP64; the
+// direct sums (S2M, P2L, L2T, P2P, M2T) evaluate the kernel in FP32 from
+// FP64 separations and accumulate in FP64.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "nnk_fmm.hpp"
+#include "nnk_gemm_bf16.hpp"
+#include "nnk_kernels.hpp"
+
+namespace nnk {
+
+namespace {
+
+__device__ __forceinline__ uint16_t f2bf(float f) {  // round to nearest even
+  uint32_t u = __float_as_uint(f);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+__device__ __forceinline__ void split2(double v, uint16_t &h, uint16_t &l) {
+  h = f2bf((float)v);
+  l = f2bf((float)(v - (double)bf2f(h)));
+}
+
+}  // namespace
+
+// rows[i] of X (length R, stride ldx) -> hi/lo bf16 rows of length ldo
+// (zero beyond R)
+__global__ void k_split_rows(const int32_t *__restrict__ rows, int nrows, const double *__restrict__ X,
+                             int ldx, int R, uint16_t *__restrict__ hi, uint16_t *__restrict__ lo,
+                             int ldo) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (int64_t)nrows * ldo) return;
+  const int i = (int)(idx / ldo), k = (int)(idx % ldo);
+  const int64_t r = rows[i];
+  uint16_t h = 0, l = 0;
+  if (k < R) split2(X[r * ldx + k], h, l);
+  hi[r * ldo + k] = h;
+  lo[r * ldo + k] = l;
+}
+
+// operator j: CT[(j*R + n) * ldct + m] = C_j[m][n]  ->  bf16 parts [m][n],
+// rows padded to RPm, columns to RP
+__global__ void k_split_op(const double *__restrict__ CT, int ldct, int R, int nops,
+                           uint16_t *__restrict__ hi, uint16_t *__restrict__ lo, int RPm, int RP) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t per = (int64_t)RPm * RP;
+  if (idx >= per * nops) return;
+  const int j = (int)(idx / per);
+  const int m = (int)((idx % per) / RP), n = (int)(idx % RP);
+  uint16_t h = 0, l = 0;
+  if (m < R && n < R) split2(CT[((size_t)j * R + n) * ldct + m], h, l);
+  hi[idx] = h;
+  lo[idx] = l;
+}
+
+constexpr int MBM = 128, MBN = 64;   // split-BF16 M2L tile
+
+void Fmm::build_mixed() {
+  const int K = K_;
+  R_ = 0;
+  for (int l = 2; l <= maxlev_; l++) {
+    const auto &S = svdS_[l];
+    int r = 0;
+    while (r < K && S[r] >= tau_ * S[0]) r++;
+    R_ = std::max(R_, r);
+  }
+  if (R_ == 0) R_ = 32;
+  RP_ = (R_ + 31) / 32 * 32;
+  RPm_ = (R_ + MBM - 1) / MBM * MBM;
+  MpadR_ = (R_ + 159) / 160 * 160;
+  KpadR_ = (R_ + 15) / 16 * 16;
+  const int R = R_;
+  mops_.resize(maxlev_ + 1);
+  const size_t matKR = (size_t)Kpad_ * MpadR_;   // R x K operators (Pup, Pdn)
+  const size_t matRK = (size_t)KpadR_ * Mpad_;   // K x R operators (Vr, Ur)
+  const size_t matRR = (size_t)KpadR_ * MpadR_;  // R x R operators
+  for (int l = 2; l <= maxlev_; l++) {
+    MixedOps &X = mops_[l];
+    const auto &U = svdU_[l], &S = svdS_[l], &V = svdV_[l];
+    std::vector<double> Pup(matKR, 0.0), Pdn(matKR, 0.0), Vr(matRK, 0.0), Ur(matRK, 0.0),
+        VrT((size_t)R * K), UrT((size_t)R * K);
+    for (int k = 0; k < K; k++)
+      for (int m = 0; m < R; m++) {
+        Pup[(size_t)k * MpadR_ + m] = U[(size_t)k * K + m] / S[m];
+        Pdn[(size_t)k * MpadR_ + m] = V[(size_t)k * K + m] / S[m];
+        Vr[(size_t)m * Mpad_ + k] = V[(size_t)k * K + m];
+        Ur[(size_t)m * Mpad_ + k] = U[(size_t)k * K + m];
+        VrT[(size_t)m * K + k] = V[(size_t)k * K + m];
+        UrT[(size_t)m * K + k] = U[(size_t)k * K + m];
+      }
+    X.Pup.upload(Pup);
+    X.Pdn.upload(Pdn);
+    X.Vr.upload(Vr);
+    X.Ur.upload(Ur);
+    X.VrT.upload(VrT);
+    X.UrT.upload(UrT);
+  }
+  // compressed translations; T_o, M2M_k, L2L_k are the FP64 kernel matrices
+  auto batch = [&](int njobs, const std::vector<const double *> &A, int N,
+                   const std::vector<std::vector<int32_t>> *g, const std::vector<std::vector<int32_t>> *c) {
+    GemmBatch B;
+    std::vector<GemmJob> jobs(njobs);
+    for (int j = 0; j < njobs; j++) {
+      GemmJob &J = jobs[j];
+      B.owned.emplace_back();
+      B.owned.back().upload(std::vector<const double *>{A[j]});
+      J.A = B.owned.back().as<const double *>();
+      if (g) { B.owned.emplace_back(); B.owned.back().upload((*g)[j]); J.gidx = B.owned.back().as<int32_t>(); }
+      if (c) { B.owned.emplace_back(); B.owned.back().upload((*c)[j]); J.cidx = B.owned.back().as<int32_t>(); }
+      J.nb = 1;
+      J.N = N;
+    }
+    B.njobs = njobs;
+    B.maxN = N;
+    B.jobs.upload(jobs);
+    return B;
+  };
+  auto offs_ids = [&](int nj, int stride) {
+    std::vector<std::vector<int32_t>> v(nj, std::vector<int32_t>(R));
+    for (int j = 0; j < nj; j++)
+      for (int n = 0; n < R; n++) v[j][n] = j * stride + n;
+    return v;
+  };
+  for (int l = 2; l <= maxlev_; l++) {
+    LevelOps &O = ops_[l];
+    MixedOps &X = mops_[l];
+    // ---- M2L: C_o = Pdn T_o V_R ----
+    std::vector<const double *> Ts;
+    std::vector<int> slots;
+    for (int sl = 0; sl < 343; sl++)
+      if (O.tslot[sl]) { Ts.push_back(O.tslot[sl]); slots.push_back(sl); }
+    const int no = (int)Ts.size();
+    {
+      DBuf Y, CT;
+      Y.alloc(sizeof(double) * (size_t)no * R * K);
+      CT.alloc(sizeof(double) * (size_t)no * R * MpadR_);
+      auto ids = offs_ids(no, R);
+      GemmBatch b1 = batch(no, Ts, R, nullptr, &ids);
+      gemm_dims(b1, X.VrT.as<double>(), K, Y.as<double>(), K, 0, 0, K, K, Mpad_, Kpad_);
+      std::vector<const double *> P(no, X.Pdn.as<double>());
+      GemmBatch b2 = batch(no, P, R, &ids, &ids);
+      gemm_dims(b2, Y.as<double>(), K, CT.as<double>(), MpadR_, 0, 0, R, K, MpadR_, Kpad_);
+      const size_t per = (size_t)RPm_ * RP_;
+      X.Chi.alloc(sizeof(uint16_t) * per * no);
+      X.Clo.alloc(sizeof(uint16_t) * per * no);
+      const int64_t tot = (int64_t)per * no;
+      hipLaunchKernelGGL(k_split_op, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, 0, CT.as<double>(),
+                         MpadR_, R, no, X.Chi.as<uint16_t>(), X.Clo.as<uint16_t>(), RPm_, RP_);
+      NNK_HIP(hipGetLastError());
+      NNK_HIP(hipDeviceSynchronize());
+      X.chi.assign(343, nullptr);
+      X.clo.assign(343, nullptr);
+      for (int j = 0; j < no; j++) {
+        X.chi[slots[j]] = X.Chi.as<uint16_t>() + per * j;
+        X.clo[slots[j]] = X.Clo.as<uint16_t>() + per * j;
+      }
+    }
+    // ---- M2M (this level is the child level; parent level l-1 >= 2) ----
+    if (l >= 3) {
+      const size_t mat = (size_t)Kpad_ * Mpad_;
+      std::vector<const double *> Ms(8);
+      for (int k = 0; k < 8; k++) Ms[k] = O.M2M.as<double>() + k * mat;
+      DBuf Y;
+      Y.alloc(sizeof(double) * (size_t)8 * R * K);
+      auto ids = offs_ids(8, R);
+      GemmBatch b1 = batch(8, Ms, R, nullptr, &ids);
+      gemm_dims(b1, X.VrT.as<double>(), K, Y.as<double>(), K, 0, 0, K, K, Mpad_, Kpad_);
+      X.Mc.alloc(sizeof(double) * matRR * 8);
+      X.Mc.zero();
+      auto outs = offs_ids(8, KpadR_);
+      std::vector<const double *> P(8, mops_[l - 1].Pup.as<double>());
+      GemmBatch b2 = batch(8, P, R, &ids, &outs);
+      gemm_dims(b2, Y.as<double>(), K, X.Mc.as<double>(), MpadR_, 0, 0, R, K, MpadR_, Kpad_);
+      // ---- L2L: Lc_a = Pdn(l) L2L_a U_R(l-1) ----
+      std::vector<const double *> Ls(8);
+      for (int k = 0; k < 8; k++) Ls[k] = O.L2L.as<double>() + k * mat;
+      GemmBatch b3 = batch(8, Ls, R, nullptr, &ids);
+      gemm_dims(b3, mops_[l - 1].UrT.as<double>(), K, Y.as<double>(), K, 0, 0, K, K, Mpad_, Kpad_);
+      X.Lc.alloc(sizeof(double) * matRR * 8);
+      X.Lc.zero();
+      std::vector<const double *> P2(8, X.Pdn.as<double>());
+      GemmBatch b4 = batch(8, P2, R, &ids, &outs);
+      gemm_dims(b4, Y.as<double>(), K, X.Lc.as<double>(), MpadR_, 0, 0, R, K, MpadR_, Kpad_);
+      NNK_HIP(hipDeviceSynchronize());
+    }
+  }
+  // the FP64 translation operators are no longer needed in this form
+  for (int l = 2; l <= maxlev_; l++) {
+    ops_[l].T.release();
+    ops_[l].tslot.assign(343, nullptr);
+  }
+  const int nc = tree_.count;
+  chat_.alloc(sizeof(double) * (size_t)nc * RP_);
+  dhat_.alloc(sizeof(double) * (size_t)nc * RP_);
+  chat_hi_.alloc(sizeof(uint16_t) * (size_t)nc * RP_);
+  chat_lo_.alloc(sizeof(uint16_t) * (size_t)nc * RP_);
+  chat_.zero();
+  dhat_.zero();
+  chat_hi_.zero();
+  chat_lo_.zero();
+  NNK_HIP(hipDeviceSynchronize());
+}
+
+void Fmm::plan_mixed() {
+  const size_t matRR = (size_t)KpadR_ * MpadR_;
+  std::vector<int32_t> upall;
+  auto job1 = [&](GemmBatch &B, const double *A, const std::vector<int32_t> &g,
+                  const std::vector<int32_t> &c) {
+    if (g.empty()) return;
+    std::vector<GemmJob> jobs(1);
+    B.owned.emplace_back();
+    B.owned.back().upload(std::vector<const double *>{A});
+    jobs[0].A = B.owned.back().as<const double *>();
+    B.owned.emplace_back();
+    B.owned.back().upload(g);
+    jobs[0].gidx = B.owned.back().as<int32_t>();
+    B.owned.emplace_back();
+    B.owned.back().upload(c);
+    jobs[0].cidx = B.owned.back().as<int32_t>();
+    jobs[0].nb = 1;
+    jobs[0].N = (int)c.size();
+    B.njobs = 1;
+    B.maxN = jobs[0].N;
+    B.jobs.upload(jobs);
+  };
+  for (int l = 2; l <= maxlev_; l++) {
+    LevelPlan &P = plans_[l];
+    HostLevel &H = hl_[l];
+    MixedOps &X = mops_[l];
+    upall.insert(upall.end(), H.up_leaves.begin(), H.up_leaves.end());
+    upall.insert(upall.end(), H.up_nonleaves.begin(), H.up_nonleaves.end());
+    if (!H.up_nonleaves.empty()) {
+      const int N = (int)H.up_nonleaves.size();
+      std::vector<GemmJob> jobs(1);
+      std::vector<const double *> A(8);
+      for (int k = 0; k < 8; k++) A[k] = mops_[l + 1].Mc.as<double>() + k * matRR;
+      P.mx_m2m.owned.emplace_back();
+      P.mx_m2m.owned.back().upload(A);
+      jobs[0].A = P.mx_m2m.owned.back().as<const double *>();
+      P.mx_m2m.owned.emplace_back();
+      P.mx_m2m.owned.back().upload(H.m2m_g);
+      jobs[0].gidx = P.mx_m2m.owned.back().as<int32_t>();
+      P.mx_m2m.owned.emplace_back();
+      P.mx_m2m.owned.back().upload(H.up_nonleaves);
+      jobs[0].cidx = P.mx_m2m.owned.back().as<int32_t>();
+      jobs[0].nb = 8;
+      jobs[0].N = N;
+      P.mx_m2m.njobs = 1;
+      P.mx_m2m.maxN = N;
+      P.mx_m2m.jobs.upload(jobs);
+    }
+    job1(P.mx_upP, X.Pup.as<double>(), H.up_leaves, H.up_leaves);
+    job1(P.mx_wV, X.Vr.as<double>(), H.wnodes, H.wnodes);
+    job1(P.mx_downU, X.Ur.as<double>(), H.tleaves, H.tleaves);
+    {
+      std::vector<int32_t> xs;
+      for (int32_t n : H.xnodes) xs.push_back(n - 1);
+      job1(P.mx_xP, X.Pdn.as<double>(), xs, xs);
+    }
+    // split-BF16 M2L jobs (one per child class) and FP64 L2L jobs
+    std::vector<GemmJobBf16> mj;
+    std::vector<GemmJob> lj;
+    for (int a = 0; a < 8; a++) {
+      if (H.cls[a].empty()) continue;
+      const int g[3] = {(a >> 2) & 1, (a >> 1) & 1, a & 1};
+      std::vector<const uint16_t *> hi, lo;
+      for (int ox = -2 - g[0]; ox <= 3 - g[0]; ox++)
+        for (int oy = -2 - g[1]; oy <= 3 - g[1]; oy++)
+          for (int oz = -2 - g[2]; oz <= 3 - g[2]; oz++) {
+            if (std::max({std::abs(ox), std::abs(oy), std::abs(oz)}) < 2) continue;
+            const int sl = (ox + 3) * 49 + (oy + 3) * 7 + (oz + 3);
+            hi.push_back(X.chi[sl]);
+            lo.push_back(X.clo[sl]);
+          }
+      GemmJobBf16 J{};
+      P.mx_owned.emplace_back();
+      P.mx_owned.back().upload(hi);
+      J.Ahi = P.mx_owned.back().as<const uint16_t *>();
+      P.mx_owned.emplace_back();
+      P.mx_owned.back().upload(lo);
+      J.Alo = P.mx_owned.back().as<const uint16_t *>();
+      P.mx_owned.emplace_back();
+      P.mx_owned.back().upload(H.gv[a]);
+      J.gidx = P.mx_owned.back().as<int32_t>();
+      P.mx_owned.emplace_back();
+      P.mx_owned.back().upload(H.cls[a]);
+      J.cidx = P.mx_owned.back().as<int32_t>();
+      J.nb = (int)hi.size();
+      J.N = (int)H.cls[a].size();
+      mj.push_back(J);
+      P.mx_m2l_maxN = std::max(P.mx_m2l_maxN, J.N);
+      if (l >= 3) {
+        GemmJob L{};
+        P.mx_l2l.owned.emplace_back();
+        P.mx_l2l.owned.back().upload(std::vector<const double *>{X.Lc.as<double>() + a * matRR});
+        L.A = P.mx_l2l.owned.back().as<const double *>();
+        P.mx_l2l.owned.emplace_back();
+        P.mx_l2l.owned.back().upload(H.gp[a]);
+        L.gidx = P.mx_l2l.owned.back().as<int32_t>();
+        P.mx_l2l.owned.emplace_back();
+        P.mx_l2l.owned.back().upload(H.cls[a]);
+        L.cidx = P.mx_l2l.owned.back().as<int32_t>();
+        L.nb = 1;
+        L.N = (int)H.cls[a].size();
+        lj.push_back(L);
+        P.mx_l2l.maxN = std::max(P.mx_l2l.maxN, L.N);
+      }
+    }
+    P.mx_m2l_jobs = (int)mj.size();
+    if (!mj.empty()) P.mx_m2l.upload(mj);
+    P.mx_l2l.njobs = (int)lj.size();
+    if (!lj.empty()) P.mx_l2l.jobs.upload(lj);
+  }
+  n_upall_ = (int)upall.size();
+  upall_.upload(upall);
+}
+
+void Fmm::apply_mixed(const double *d_f, double *d_u, hipStream_t s) {
+  const int K = K_, R = R_;
+  double *chk = chk_.as<double>(), *c = chat_.as<double>(), *d = dhat_.as<double>();
+  mark(0, true, s);
+  strengths(d_f, s);
+  mark(0, false, s);
+  if (maxlev_ >= 2 && n_pleaves_) {
+    mark(1, true, s);
+    launch_surf(pleaves_.as<int32_t>(), n_pleaves_, nullptr, nullptr, 1, 0, chk, s);
+    mark(1, false, s);
+  }
+  mark(2, true, s);
+  for (int l = maxlev_; l >= 2; l--) {
+    LevelPlan &P = plans_[l];
+    gemm_dims(P.mx_m2m, c, RP_, c, RP_, 0, s, R, R, MpadR_, KpadR_);
+    gemm_dims(P.mx_upP, chk, K, c, RP_, 0, s, R, K, MpadR_, Kpad_);
+  }
+  if (n_upall_) {
+    const int64_t tot = (int64_t)n_upall_ * RP_;
+    hipLaunchKernelGGL(k_split_rows, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s,
+                       upall_.as<int32_t>(), n_upall_, c, RP_, R, chat_hi_.as<uint16_t>(),
+                       chat_lo_.as<uint16_t>(), RP_);
+    NNK_HIP(hipGetLastError());
+  }
+  for (int l = maxlev_; l >= 2; l--)
+    gemm_dims(plans_[l].mx_wV, c, RP_, uppot_.as<double>(), K, 0, s, K, R, Mpad_, KpadR_);
+  mark(2, false, s);
+  for (int l = 2; l <= maxlev_; l++) {
+    LevelPlan &P = plans_[l];
+    if (P.mx_m2l_jobs) {
+      mark(3, true, s);
+      const int nrb = RPm_ / MBM, nct = (P.mx_m2l_maxN + MBN - 1) / MBN;
+      hipLaunchKernelGGL((k_gemm_bf16x3<MBM, MBN, 1>), dim3((unsigned)(P.mx_m2l_jobs * nrb * nct)), dim3(256),
+                         0, s, R, RP_, chat_hi_.as<uint16_t>(), chat_lo_.as<uint16_t>(), RP_, d, RP_,
+                         P.mx_m2l.as<GemmJobBf16>(), P.mx_m2l_jobs, nrb);
+      NNK_HIP(hipGetLastError());
+      mark(3, false, s);
+    }
+    mark(4, true, s);
+    gemm_dims(P.mx_l2l, d, RP_, d, RP_, 1, s, R, R, MpadR_, KpadR_);
+    mark(4, false, s);
+    if (P.n_x) {
+      mark(5, true, s);
+      launch_surf(P.x_nodes.as<int32_t>(), P.n_x, xptr_.as<int64_t>(), xidx_.as<int32_t>(), 0, 0, chk, s);
+      gemm_dims(P.mx_xP, chk, K, d, RP_, 1, s, R, K, MpadR_, Kpad_);
+      mark(5, false, s);
+    }
+    mark(6, true, s);
+    gemm_dims(P.mx_downU, d, RP_, downpot_.as<double>(), K, 0, s, K, R, Mpad_, KpadR_);
+    mark(6, false, s);
+  }
+  mark(7, true, s);
+  launch_leaf(1, d_u, s);
+  mark(7, false, s);
+}
+
+}  // namespace nnk

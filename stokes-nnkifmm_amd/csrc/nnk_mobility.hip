@@ -256,7 +256,7 @@ void Mobility::small_solve(const DBuf &lu, const DBuf &piv, const double *b, dou
 }
 
 void Mobility::apply(const double *x, double *y, hipStream_t s) {
-  kifmm(F_, x, y, false, s);                 // S f
+  kifmm(Fcur_ ? Fcur_ : F_, x, y, false, s);  // S f
   BT_.mul(x + n3_, y, true, 1.0, s);          // + BT (U; Omega)
   B_.mul(x, y + n3_, false, 1.0, s);          // B f
 }
@@ -385,6 +385,41 @@ GmresResult Mobility::gmres(const double *b, double *x, double tol, int maxit, b
     precon(x, w.as<double>(), s);
     NNK_HIP(hipMemcpyAsync(x, w.p, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
   }
+  NNK_HIP(hipStreamSynchronize(s));
+  return R;
+}
+
+GmresResult Mobility::refine(const double *b, double *x, double tol, int maxit, double inner_tol,
+                            bool use_precon, hipStream_t s, std::vector<double> *resvec, int *outer) {
+  if (!Fi_) throw Error(NNK_ESTATE, "mobility: no inner operator for refinement");
+  const int64_t n = n_;
+  DBuf r, d;
+  r.alloc(sizeof(double) * n);
+  d.alloc(sizeof(double) * n);
+  NNK_HIP(hipMemsetAsync(x, 0, sizeof(double) * n, s));
+  NNK_HIP(hipMemcpyAsync(r.p, b, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+  const double bnorm = std::sqrt(dot(b, b, s));
+  GmresResult R{};
+  if (resvec) resvec->assign(1, bnorm);
+  double rn = bnorm;
+  int used = 0, it = 0;
+  for (; it < 50 && rn > tol * bnorm && used < maxit; it++) {
+    Fcur_ = Fi_;
+    GmresResult g = gmres(r.as<double>(), d.as<double>(), std::max(inner_tol, tol * bnorm / rn), maxit - used,
+                          use_precon, s, nullptr);
+    Fcur_ = nullptr;
+    used += std::max(g.iter, 1);
+    hipLaunchKernelGGL(k_axpby, dim3(nblk(n)), dim3(256), 0, s, n, 1.0, d.as<double>(), 1.0, x);
+    apply(x, r.as<double>(), s);  // FP64 operator
+    hipLaunchKernelGGL(k_axpby, dim3(nblk(n)), dim3(256), 0, s, n, 1.0, b, -1.0, r.as<double>());
+    NNK_HIP(hipGetLastError());
+    rn = std::sqrt(dot(r.as<double>(), r.as<double>(), s));
+    if (resvec) resvec->push_back(rn);
+  }
+  R.relres = rn / bnorm;
+  R.iter = used;
+  R.flag = R.relres <= tol ? 0 : 1;
+  if (outer) *outer = it;
   NNK_HIP(hipStreamSynchronize(s));
   return R;
 }

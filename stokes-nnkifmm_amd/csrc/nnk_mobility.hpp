@@ -30,6 +30,7 @@ class Mobility {
            const double *Bv, const int64_t *BTrp, const int32_t *BTci, const double *BTv,
            const double *A1, const double *A2);
   int64_t size() const { return n_; }
+  int64_t points() const { return N_; }
   // mobilityMatrixMulti: y = [S f + BT UO; B f]
   void apply(const double *x, double *y, hipStream_t s);
   // mobilityPrecon
@@ -37,6 +38,13 @@ class Mobility {
   // gmres(A (or A*P), b, [], tol, maxit); x = P(x) when preconditioned
   GmresResult gmres(const double *b, double *x, double tol, int maxit, bool use_precon, hipStream_t s,
                     std::vector<double> *resvec);
+  // Iterative refinement: outer residuals r = b - A x with the FP64
+  // operator, corrections from GMRES on the inner (mixed-form) operator to
+  // inner_tol; stops when ||r|| <= tol ||b||.  Returns the total inner
+  // iteration count in iter and the outer count in outer.
+  GmresResult refine(const double *b, double *x, double tol, int maxit, double inner_tol, bool use_precon,
+                     hipStream_t s, std::vector<double> *resvec, int *outer);
+  void set_inner(Fmm *inner) { Fi_ = inner; }
 
  private:
   void op(const double *x, double *y, bool use_precon, hipStream_t s);
@@ -46,7 +54,8 @@ class Mobility {
                    hipStream_t s);
   double dot(const double *a, const double *b, hipStream_t s);
 
-  Fmm *F_, *F4_;
+  Fmm *F_, *F4_, *Fi_ = nullptr;
+  Fmm *Fcur_ = nullptr;  // operator used by apply(): F_ or Fi_
   int64_t nb6_, N_ = 0, n3_ = 0, n_ = 0;
   DevCsr B_, Bt_, BT_, BTt_;
   DBuf A1lu_, A1piv_, A2lu_, A2piv_;

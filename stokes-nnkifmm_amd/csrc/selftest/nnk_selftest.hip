@@ -85,3 +85,42 @@ extern "C" int nnk_selftest_mfma_probe(const double *a, const double *b, double 
     return 1;
   }
 }
+
+// One v_mfma_f32_16x16x32_bf16 per trial on a single wave: lane l supplies
+// 8 bf16 values a[(t*64+l)*8 + j], b[...]; d[(t*64+l)*4+r] receives
+// accumulator r.  Inputs are bf16 bit patterns (uint16).
+typedef __bf16 bf8v __attribute__((ext_vector_type(8)));
+typedef float f4v __attribute__((ext_vector_type(4)));
+__global__ void k_mfma_bf16_probe(const uint16_t *a, const uint16_t *b, float *d, int trials) {
+  const int l = threadIdx.x;
+  for (int t = 0; t < trials; t++) {
+    bf8v av, bv;
+    for (int j = 0; j < 8; j++) {
+      uint16_t x = a[(t * 64 + l) * 8 + j], y = b[(t * 64 + l) * 8 + j];
+      av[j] = __builtin_bit_cast(__bf16, x);
+      bv[j] = __builtin_bit_cast(__bf16, y);
+    }
+    f4v c = {0, 0, 0, 0};
+    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, c, 0, 0, 0);
+    for (int r = 0; r < 4; r++) d[(t * 64 + l) * 4 + r] = c[r];
+  }
+}
+
+extern "C" int nnk_selftest_mfma_bf16_probe(const uint16_t *a, const uint16_t *b, float *d, int trials) {
+  try {
+    DBuf da, db, dd;
+    da.alloc(sizeof(uint16_t) * 512 * trials);
+    db.alloc(sizeof(uint16_t) * 512 * trials);
+    dd.alloc(sizeof(float) * 256 * trials);
+    NNK_HIP(hipMemcpy(da.p, a, sizeof(uint16_t) * 512 * trials, hipMemcpyHostToDevice));
+    NNK_HIP(hipMemcpy(db.p, b, sizeof(uint16_t) * 512 * trials, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_mfma_bf16_probe, dim3(1), dim3(64), 0, 0, da.as<uint16_t>(), db.as<uint16_t>(),
+                       dd.as<float>(), trials);
+    NNK_HIP(hipGetLastError());
+    NNK_HIP(hipDeviceSynchronize());
+    NNK_HIP(hipMemcpy(d, dd.p, sizeof(float) * 256 * trials, hipMemcpyDeviceToHost));
+    return 0;
+  } catch (...) {
+    return 1;
+  }
+}

@@ -15,8 +15,8 @@ from ._lib import c64, check, lib, ptr
 from .octree import parse_options
 
 _OPTS = {"GPU": 0, "parThreads": 0, "blockSize": 0.1, "GMRES": 0, "coronaRes": 6, "coronaShells": 1,
-         "format": 1, "precision": "fp64"}
-_PREC = {"fp64": 0}
+         "format": 1, "precision": "fp64", "svdTol": 1e-10}
+_PREC = {"fp64": 0, "mixed": 1}
 
 
 def _is_torch(x):
@@ -24,10 +24,10 @@ def _is_torch(x):
 
 
 class _Handle:
-    def __init__(self, tree, eps, mu, res, shells, precision):
+    def __init__(self, tree, eps, mu, res, shells, precision, svd_tol=1e-10):
         h = ctypes.c_void_p()
         check(lib().nnk_fmm_create(tree.handle, float(eps), float(mu), int(res), int(shells),
-                                   _PREC[precision], ctypes.byref(h)))
+                                   _PREC[precision], float(svd_tol), ctypes.byref(h)))
         self.h = h
         self.tree = tree  # the tree must outlive the handle
 
@@ -45,7 +45,9 @@ class KIFMM:
     'GMRES', 0|1, 'coronaRes', r, 'coronaShells', s, 'format', 1|2)
     Classes/KIFMM.m:46-61.  'GPU', 'parThreads' and 'blockSize' are
     accepted for compatibility; the matvec always runs on the GPU and never
-    materialises a kernel block."""
+    materialises a kernel block.  Build-only options: 'precision' ('fp64',
+    the reference's numbers, or 'mixed') and 'svdTol' (mixed form's
+    pseudo-inverse cut)."""
 
     def __init__(self, tree, kernelPar, *args, **kwargs):
         self.tree = tree
@@ -63,10 +65,12 @@ class KIFMM:
         a = self.arguments
         res = a["coronaRes"] if res is None else res
         shells = a["coronaShells"] if shells is None else shells
-        key = (int(res), int(shells), a["precision"], float(a["kernelPar"][0]), float(a["kernelPar"][1]))
+        key = (int(res), int(shells), a["precision"], float(a["kernelPar"][0]), float(a["kernelPar"][1]),
+               float(a["svdTol"]))
         h = self._handles.get(key)
         if h is None:
-            h = _Handle(self.tree, a["kernelPar"][0], a["kernelPar"][1], res, shells, a["precision"])
+            h = _Handle(self.tree, a["kernelPar"][0], a["kernelPar"][1], res, shells, a["precision"],
+                        a["svdTol"])
             self._handles[key] = h
         return h
 

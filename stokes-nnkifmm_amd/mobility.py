@@ -145,6 +145,36 @@ def mobility_gmres(swimmers, B, BT, A1, A2, rhs, tol, maxIter, precon):
     return x
 
 
+def mobility_refined(swimmers, B, BT, A1, A2, rhs, tol, maxIter, precon, innerTol=1e-4, svdTol=1e-10):
+    """The solve of mobility_gmres by iterative refinement (no counterpart in
+    the reference): residuals with the FP64 operator, corrections by GMRES on
+    the mixed-precision KIFMM of the same tree to innerTol.  The answer meets
+    tol on the unpreconditioned FP64 system.  Records flag, relres, iter
+    (inner iterations) and outer in mobility.last_solve."""
+    from .kifmm import _Handle
+    m = _mobility(swimmers, B, BT, A1, A2, precon=bool(precon))
+    fmm = swimmers.FMM
+    a = fmm.arguments
+    key = ("mixed", float(svdTol))
+    inner = getattr(m, "inner", {}).get(key)
+    if inner is None:
+        inner = _Handle(fmm.tree, a["kernelPar"][0], a["kernelPar"][1], a["coronaRes"], a["coronaShells"],
+                        "mixed", svdTol)
+        m.inner = {key: inner}
+    b = c64(rhs).ravel()
+    x = np.empty_like(b)
+    flag = np.zeros(1, np.int32)
+    it = np.zeros(1, np.int32)
+    outer = np.zeros(1, np.int32)
+    rel = np.zeros(1)
+    check(lib().nnk_mobility_solve_refined(m.h, inner.h, ptr(b), float(tol), int(maxIter), float(innerTol),
+                                           int(bool(precon)), ptr(x), ptr(flag, _lib._i32), ptr(rel),
+                                           ptr(it, _lib._i32), ptr(outer, _lib._i32)))
+    last_solve.clear()
+    last_solve.update(flag=int(flag[0]), relres=float(rel[0]), iter=int(it[0]), outer=int(outer[0]))
+    return x
+
+
 def _rhs(swimmers, fluidFunc, t):
     pointVel = swimmers.getVelocities()
     pts = swimmers.getSwimmers()[0]

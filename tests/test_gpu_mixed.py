@@ -1,0 +1,87 @@
+"""Mixed-precision form (GPU): truncated-SVD pseudo-inverses, split-BF16
+M2L, FP32-math direct sums.
+
+Its numbers differ from the reference's by design, so it is checked against
+the FP64 direct sum (the accuracy the form promises) and against the FP64
+form of this library (the two share everything but precision and the
+pseudo-inverse cut).  Tolerances: 1e-5 relative against the direct sum at 6
+points per corona edge (the FP64 form itself is at 2e-5 to 4e-6 there), and
+the mixed form may add at most 5e-6 to the FP64 form's error.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from geometry import blobs, cube_sphere
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+@pytest.mark.parametrize("case", ["uniform", "blobs", "distinct"])
+def test_mixed_against_direct_and_fp64(oracle, nnk, case):
+    rng = np.random.default_rng(31)
+    if case == "uniform":
+        p = t = rng.random((20000, 3)); cap, eps = 64, 1e-3
+    elif case == "blobs":
+        p = t = blobs(rng); cap, eps = 16, 1e-3
+    else:
+        p = rng.random((8000, 3)); t = np.concatenate([rng.random((3000, 3)), p[:1000]]); cap, eps = 40, 5e-3
+    tree = nnk.OcTree(p, t, "nodeCapacity", cap)
+    q = rng.standard_normal((int(tree.potentials.sum()), 3))
+    u64 = nnk.KIFMM(tree, [eps, 1.0]).computeVel(q)
+    um = nnk.KIFMM(tree, [eps, 1.0], "precision", "mixed").computeVel(q)
+    ud = oracle.direct(tree.getPotentialPoints(), q, tree.getTargetPoints(), eps)
+    e64, em = rel(u64, ud), rel(um, ud)
+    assert em < 1e-5 and em < e64 + 5e-6, (e64, em)
+    assert rel(um, u64) < 1e-5
+
+
+def test_mixed_nearest_sphere_pair(oracle, nnk):
+    p = np.concatenate([cube_sphere(12), cube_sphere(12, centre=(2.5, 0.3, 0.1))])
+    fine = np.concatenate([cube_sphere(48), cube_sphere(48, centre=(2.5, 0.3, 0.1))])
+    NN = nnk.nearestMatrix(p, fine)
+    q = np.random.default_rng(3).standard_normal(p.shape)
+    t = nnk.OcTree(p, p, "nodeCapacity", 40, "nearest", 1, "finePoints", fine, "NN", NN)
+    um = nnk.KIFMM(t, [0.01, 1.0], "precision", "mixed").computeVel(q)
+    ud = oracle.direct(fine, NN @ q, p, 0.01)
+    assert rel(um, ud) < 1e-5
+
+
+def test_mixed_device_path_and_stats(nnk):
+    import torch
+    rng = np.random.default_rng(5)
+    p = rng.random((30000, 3))
+    t = nnk.OcTree(p, p, "nodeCapacity", 64)
+    f = nnk.KIFMM(t, [1e-3, 1.0], "precision", "mixed", "svdTol", 1e-9)
+    q = rng.standard_normal(p.shape)
+    u = f.computeVel(q)
+    ud = f.computeVel(torch.tensor(q, device="cuda")).cpu().numpy()
+    np.testing.assert_array_equal(u, ud)
+    st = np.zeros(14, np.int64)
+    assert nnk.lib().nnk_fmm_stats(f._handle().h, st.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), 14) == 0
+    assert st[12] == 456 and 0 < st[13] <= 456
+
+
+@pytest.mark.parametrize("precon", [False, True])
+def test_refined_solve_reaches_fp64_accuracy(oracle, nnk, precon):
+    """Iterative refinement with mixed-form inner solves must meet an FP64
+    tolerance on the FP64 system and agree with the plain FP64 GMRES."""
+    from importlib import import_module
+
+    from test_gpu_mobility import _system
+    sw, B, BT, A1, A2, _ = _system(nnk, oracle, True)
+    mob = import_module("stokes-nnkifmm_amd.mobility")
+    b = mob._rhs(sw, lambda t, x: np.zeros_like(x), 0.0)
+    x64 = mob.mobility_gmres(sw, B, BT, A1, A2, b, 1e-10, 300, precon)
+    xr = mob.mobility_refined(sw, B, BT, A1, A2, b, 1e-10, 600, precon, innerTol=1e-4)
+    info = dict(mob.last_solve)
+    assert info["flag"] == 0 and info["relres"] <= 1e-10 and info["outer"] >= 2, info
+    r = nnk.mobilityMatrixMulti(xr, sw, B, BT) - b
+    assert np.linalg.norm(r) <= 1.5e-10 * np.linalg.norm(b)
+    assert rel(xr, x64) < 1e-7
