@@ -47,12 +47,12 @@ def parse():
     ap.add_argument("--n", type=int, default=1 << 20)
     ap.add_argument("--res", type=int, default=6, help="corona resolution of the FP64 form")
     ap.add_argument("--mixed-res", type=int, default=6, help="corona resolution of the mixed form")
-    ap.add_argument("--svd-tol", type=float, default=1e-10, help="pseudo-inverse cut of the mixed form")
+    ap.add_argument("--svd-tol", type=float, default=1e-8, help="pseudo-inverse cut of the mixed form")
     ap.add_argument("--precision", choices=["fp64", "mixed", "both"], default="both")
     ap.add_argument("--no-solve", action="store_true", help="skip the one-sphere mobility solve")
     ap.add_argument("--seed", type=int, default=20240611)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
-    ap.add_argument("--cpu-n", type=int, default=1 << 14, help="CPU baseline sample size")
+    ap.add_argument("--cpu-n", type=int, default=1 << 16, help="CPU baseline sample size")
     return ap.parse_args()
 
 

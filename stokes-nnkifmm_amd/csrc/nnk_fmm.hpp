@@ -158,7 +158,7 @@ class Fmm {
   struct RedArgs red_args() const;
   int red_maxn_ = 0;
   int64_t stats_[kStats] = {0};
-  int m2l_cfg_ = 1;  // M2L tile: 0 = 160x64, 1 = 160x96
+  int m2l_cfg_ = 0;  // M2L tile: 0 = 160x64, 1 = 160x96
   bool prof_ = false;
   struct Mark { int phase; hipEvent_t a, b; };
   std::vector<Mark> marks_;

@@ -15,7 +15,7 @@ from ._lib import c64, check, lib, ptr
 from .octree import parse_options
 
 _OPTS = {"GPU": 0, "parThreads": 0, "blockSize": 0.1, "GMRES": 0, "coronaRes": 6, "coronaShells": 1,
-         "format": 1, "precision": "fp64", "svdTol": 1e-10}
+         "format": 1, "precision": "fp64", "svdTol": 1e-8}
 _PREC = {"fp64": 0, "mixed": 1}
 
 
@@ -24,7 +24,7 @@ def _is_torch(x):
 
 
 class _Handle:
-    def __init__(self, tree, eps, mu, res, shells, precision, svd_tol=1e-10):
+    def __init__(self, tree, eps, mu, res, shells, precision, svd_tol=1e-8):
         h = ctypes.c_void_p()
         check(lib().nnk_fmm_create(tree.handle, float(eps), float(mu), int(res), int(shells),
                                    _PREC[precision], float(svd_tol), ctypes.byref(h)))

@@ -145,7 +145,7 @@ def mobility_gmres(swimmers, B, BT, A1, A2, rhs, tol, maxIter, precon):
     return x
 
 
-def mobility_refined(swimmers, B, BT, A1, A2, rhs, tol, maxIter, precon, innerTol=1e-4, svdTol=1e-10):
+def mobility_refined(swimmers, B, BT, A1, A2, rhs, tol, maxIter, precon, innerTol=1e-4, svdTol=1e-8):
     """The solve of mobility_gmres by iterative refinement (no counterpart in
     the reference): residuals with the FP64 operator, corrections by GMRES on
     the mixed-precision KIFMM of the same tree to innerTol.  The answer meets
