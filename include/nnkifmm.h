@@ -42,9 +42,12 @@ enum {
 
 enum { NNK_LIST_U = 0, NNK_LIST_V = 1, NNK_LIST_W = 2, NNK_LIST_X = 3 };
 /* NNK_FP64: plain FP64 form, reproduces the reference's velocities.
- * NNK_MIXED: truncated-SVD pseudo-inverses (FP64), split-BF16 MFMA M2L with
- * FP32 accumulation, FP32-math direct sums with FP64 accumulation. */
-enum { NNK_FP64 = 0, NNK_MIXED = 1 };
+ * NNK_MIXED: truncated-SVD pseudo-inverses (FP64), M2L on the BF16 matrix
+ * cores with operands split in two BF16 parts (3 products, ~16 bits) and
+ * FP32 accumulation promoted to FP64 per translation, FP32-math direct sums
+ * with FP64 accumulation.
+ * NNK_MIXED3: as NNK_MIXED with three BF16 parts (6 products, ~24 bits). */
+enum { NNK_FP64 = 0, NNK_MIXED = 1, NNK_MIXED3 = 2 };
 
 /* Library version (major*10000 + minor*100 + patch). */
 int32_t nnk_version(void);
@@ -87,8 +90,9 @@ void nnk_tree_destroy(nnk_tree *t);
 /* Replaces: KIFMM(tree, [eps mu], 'coronaRes', r, 'coronaShells', s)
  *           Classes/KIFMM.m:46-61.  Builds the device-resident operator:
  *           per-level check-to-equivalent factors and translation
- *           operators, point data in leaf order.  precision is NNK_FP64 or
- *           NNK_MIXED; svd_tol (mixed form only, <= 0 for the default 1e-10)
+ *           operators, point data in leaf order.  precision is NNK_FP64,
+ *           NNK_MIXED or NNK_MIXED3; svd_tol (mixed forms only, <= 0 for the
+ *           default 1e-8)
  *           is the relative singular-value cut of the pseudo-inverses.  The
  *           tree must stay alive while the handle exists. */
 int32_t nnk_fmm_create(const nnk_tree *t, double eps, double mu, int32_t corona_res,

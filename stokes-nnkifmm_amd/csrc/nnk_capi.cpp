@@ -142,10 +142,10 @@ int32_t nnk_fmm_create(const nnk_tree *t, double eps, double mu, int32_t res, in
   return guard([&]() -> int32_t {
     if (!t || !out) return fail(NNK_EARG, "nnk_fmm_create: null argument");
     *out = nullptr;
-    if (precision != NNK_FP64 && precision != NNK_MIXED)
+    if (precision != NNK_FP64 && precision != NNK_MIXED && precision != NNK_MIXED3)
       return fail(NNK_EARG, "nnk_fmm_create: unknown precision");
     auto f = std::make_unique<nnk::Fmm>(t->t, eps, mu, res, shells, precision,
-                                        svd_tol > 0 ? svd_tol : 1e-10);
+                                        svd_tol > 0 ? svd_tol : 1e-8);
     *out = new nnk_fmm{std::move(f), t->t->npot, t->t->ntgt};
     return NNK_OK;
   });

@@ -316,11 +316,45 @@ template <class T> DBuf make_dbuf(const std::vector<T> &v) {
 // tile configurations of the gather GEMM (BM x BN); BK = 16
 template <int BN, int TAG>
 static void launch_gemm(int nrb, int M, int K, int Mpad, int Kpad, const double *X, int ldx,
-                        double *C, int ldc, int beta, const GemmBatch &b, hipStream_t s) {
+                        double *C, int ldc, int beta, const GemmBatch &b, hipStream_t s, int nsplit = 1,
+                        double *P = nullptr) {
   const int nct = (b.maxN + BN - 1) / BN;
-  dim3 grid((unsigned)(b.njobs * nrb * nct));
+  dim3 grid((unsigned)(b.njobs * nsplit * nrb * nct));
   hipLaunchKernelGGL((k_gemm_gather<GBM, BN, GBK, TAG>), grid, dim3(256), 0, s, M, K, Mpad, Kpad,
-                     X, ldx, C, ldc, beta, b.jobs.as<GemmJob>(), b.njobs, nrb);
+                     X, ldx, C, ldc, beta, b.jobs.as<GemmJob>(), b.njobs, nrb, nsplit, P, b.maxN);
+  NNK_HIP(hipGetLastError());
+}
+
+__global__ void k_split_reduce(const int32_t *__restrict__ slot, int M, int ldc, int nsplit,
+                               const double *__restrict__ P, size_t pstride, double *__restrict__ C) {
+  const int c = blockIdx.x;
+  const int cs = slot[c];
+  if (cs < 0) return;
+  for (int row = threadIdx.x; row < M; row += blockDim.x) {
+    double v = 0.0;
+    for (int sp = 0; sp < nsplit; sp++) v += P[sp * pstride + (size_t)c * ldc + row];
+    C[(size_t)cs * ldc + row] = v;
+  }
+}
+
+int Fmm::split_for(int64_t tiles, int nb) {
+  // aim at ~4 workgroups per CU, with at least 4 operator blocks per split
+  const int64_t want = 1024;
+  if (tiles >= want / 2) return 1;
+  int s = (int)((want + tiles - 1) / tiles);
+  s = std::min(s, std::max(1, nb / 4));
+  return std::max(1, std::min(s, 64));
+}
+
+void Fmm::reserve_part(size_t elems) {
+  if (elems <= part_elems_) return;
+  part_.alloc(sizeof(double) * elems);
+  part_elems_ = elems;
+}
+
+void Fmm::split_reduce(const DBuf &slot, int ncols, int M, int ldc, int nsplit, double *C, hipStream_t s) {
+  hipLaunchKernelGGL(k_split_reduce, dim3((unsigned)ncols), dim3(256), 0, s, slot.as<int32_t>(), M, ldc,
+                     nsplit, part_.as<double>(), (size_t)ncols * ldc, C);
   NNK_HIP(hipGetLastError());
 }
 
@@ -332,9 +366,18 @@ void Fmm::gemm_dims(const GemmBatch &b, const double *X, int ldx, double *C, int
 }
 
 void Fmm::gemm(const GemmBatch &b, const double *X, int ldx, double *C, int ldc, int beta,
-               hipStream_t s, int tag) {
+               hipStream_t s, int tag, int nsplit, const DBuf *slot) {
   if (b.njobs == 0 || b.maxN == 0) return;
   const int nrb = Mpad_ / GBM;
+  if (nsplit > 1) {
+    // partial tiles, then the fixed-order reduction (stores: beta must be 0)
+    if (beta != 0 || !slot) throw Error(NNK_ESTATE, "split GEMM needs beta = 0 and a slot list");
+    if ((size_t)nsplit * b.njobs * b.maxN * ldc > part_elems_) throw Error(NNK_ESTATE, "split buffer too small");
+    if (tag == 1) launch_gemm<64, 1>(nrb, K_, K_, Mpad_, Kpad_, X, ldx, C, ldc, 0, b, s, nsplit, part_.as<double>());
+    else launch_gemm<64, 0>(nrb, K_, K_, Mpad_, Kpad_, X, ldx, C, ldc, 0, b, s, nsplit, part_.as<double>());
+    split_reduce(*slot, b.njobs * b.maxN, K_, ldc, nsplit, C, s);
+    return;
+  }
   if (tag == 1) {
     if (m2l_cfg_ == 1) launch_gemm<96, 1>(nrb, K_, K_, Mpad_, Kpad_, X, ldx, C, ldc, beta, b, s);
     else launch_gemm<64, 1>(nrb, K_, K_, Mpad_, Kpad_, X, ldx, C, ldc, beta, b, s);
@@ -355,8 +398,8 @@ Fmm::Fmm(std::shared_ptr<const Tree> t, double eps, double mu, int res, int shel
     : tp_(std::move(t)), tree_(*tp_), eps_(eps), mu_(mu), res_(res), shells_(shells),
       prec_(precision), tau_(tau) {
   const Tree &T = tree_;
-  if (precision != 0 && precision != 1) throw Error(NNK_EARG, "unknown precision");
-  if (precision == 1 && !(tau > 0 && tau < 1)) throw Error(NNK_EARG, "truncation tolerance must be in (0,1)");
+  if (precision < 0 || precision > 2) throw Error(NNK_EARG, "unknown precision");
+  if (precision >= 1 && !(tau > 0 && tau < 1)) throw Error(NNK_EARG, "truncation tolerance must be in (0,1)");
   if (res < 2 || res > 16 || shells < 1 || 2 * shells > res)
     throw Error(NNK_EARG, "coronaRes must be in [2,16] and 1 <= coronaShells <= res/2");
   if (!(eps > 0) || !(mu > 0)) throw Error(NNK_EARG, "kernel parameters eps and mu must be > 0");
@@ -469,14 +512,14 @@ Fmm::Fmm(std::shared_ptr<const Tree> t, double eps, double mu, int res, int shel
   uppot_.zero();
   downpot_.zero();
   ops_.resize(maxlev_ + 1);
-  if (prec_ == 1) {
+  if (prec_ >= 1) {
     svdU_.resize(maxlev_ + 1);
     svdS_.resize(maxlev_ + 1);
     svdV_.resize(maxlev_ + 1);
   }
   for (int l = 2; l <= maxlev_; l++) build_ops(l);
   plan();
-  if (prec_ == 1) {
+  if (prec_ >= 1) {
     build_mixed();
     plan_mixed();
   }
@@ -506,7 +549,7 @@ void Fmm::build_ops(int l) {
             A[(size_t)(3 * p + a) * n + 3 * q + b] = (r[a] * r[b] + (a == b ? r2 + 2 * e2 : 0.0)) * invs;
       }
     svd_jacobi(A.data(), n, U.data(), S.data(), V.data());
-    if (prec_ == 1) {
+    if (prec_ >= 1) {
       svdU_[l] = U;
       svdS_[l] = S;
       svdV_[l] = V;
@@ -726,6 +769,22 @@ void Fmm::plan() {
       }
       finish(P.m2l, jobs);
       finish(P.l2l, ljobs);
+      {  // split-K for levels with few output tiles
+        const int nj = P.m2l.njobs, mN = P.m2l.maxN;
+        std::vector<int32_t> slot((size_t)nj * mN, -1);
+        int j = 0, minnb = 512;
+        for (int a = 0; a < 8; a++) {
+          if (cls[a].empty()) continue;
+          for (size_t c = 0; c < cls[a].size(); c++) slot[(size_t)j * mN + c] = cls[a][c];
+          minnb = std::min(minnb, (int)class_offsets(a).size());
+          j++;
+        }
+        P.m2l_split = split_for((int64_t)nj * (Mpad_ / GBM) * ((mN + 63) / 64), minnb);
+        if (P.m2l_split > 1) {
+          P.m2l_slot.upload(slot);
+          reserve_part((size_t)P.m2l_split * nj * mN * K_);
+        }
+      }
       std::vector<GemmJob> j1, j2;
       add_job(P.dn_solve1, j1, {O.Wdn.as<double>()}, &dn, nullptr, P.n_down);
       finish(P.dn_solve1, j1);
@@ -827,7 +886,7 @@ void Fmm::launch_surf(const int32_t *nodes, int n, const int64_t *lptr, const in
   a.e2 = eps_ * eps_;
   a.c = 1.0 / (8.0 * kPi * mu_);
   a.chk = chk;
-  if (prec_ == 1) hipLaunchKernelGGL(k_surf_eval<true>, dim3(n), dim3(256), 0, s, a);
+  if (prec_ >= 1) hipLaunchKernelGGL(k_surf_eval<true>, dim3(n), dim3(256), 0, s, a);
   else hipLaunchKernelGGL(k_surf_eval<false>, dim3(n), dim3(256), 0, s, a);
   NNK_HIP(hipGetLastError());
 }
@@ -853,13 +912,13 @@ void Fmm::launch_leaf(int far, double *d_u, hipStream_t s) {
   a.c = 1.0 / (8.0 * kPi * mu_);
   a.far = far;
   a.out = d_u;
-  if (prec_ == 1) hipLaunchKernelGGL(k_leaf<true>, dim3(n_tleaves_), dim3(256), 0, s, a);
+  if (prec_ >= 1) hipLaunchKernelGGL(k_leaf<true>, dim3(n_tleaves_), dim3(256), 0, s, a);
   else hipLaunchKernelGGL(k_leaf<false>, dim3(n_tleaves_), dim3(256), 0, s, a);
   NNK_HIP(hipGetLastError());
 }
 
 void Fmm::apply(const double *d_f, double *d_u, hipStream_t s) {
-  if (prec_ == 1) { apply_mixed(d_f, d_u, s); return; }
+  if (prec_ >= 1) { apply_mixed(d_f, d_u, s); return; }
   mark(0, true, s);
   strengths(d_f, s);
   mark(0, false, s);
@@ -885,7 +944,7 @@ void Fmm::apply(const double *d_f, double *d_u, hipStream_t s) {
     LevelPlan &P = plans_[l];
     if (!P.n_down) continue;
     mark(3, true, s);
-    gemm(P.m2l, up, K_, chk, K_, 0, s, 1);
+    gemm(P.m2l, up, K_, chk, K_, 0, s, 1, P.m2l_split, &P.m2l_slot);
     mark(3, false, s);
     mark(4, true, s);
     gemm(P.l2l, dn, K_, chk, K_, 1, s);

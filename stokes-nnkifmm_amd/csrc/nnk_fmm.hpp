@@ -38,6 +38,8 @@ struct LevelPlan {
   int n_down = 0;
   DBuf down_nodes;
   GemmBatch m2l;              // 8 classes x 189 translations (reads up densities)
+  int m2l_split = 1;          // split-K factor of the M2L launch (both forms)
+  DBuf m2l_slot;              // job * maxN + column -> output slot (-1 none), for the split reduction
   GemmBatch l2l;              // 8 classes, parent down density (accumulates)
   GemmBatch dn_solve1, dn_solve2;
   int n_x = 0;
@@ -45,6 +47,8 @@ struct LevelPlan {
   // mixed form
   GemmBatch mx_m2m, mx_upP, mx_l2l, mx_xP, mx_downU, mx_wV;
   DBuf mx_m2l;                // GemmJobBf16[8]
+  int mx_split = 1;
+  DBuf mx_slot;
   std::vector<DBuf> mx_owned;
   int mx_m2l_jobs = 0, mx_m2l_maxN = 0;
 };
@@ -61,8 +65,8 @@ struct MixedOps {
   DBuf Pup, Vr, Pdn, Ur;      // FP64 At layout: R x K, K x R, R x K, K x R
   DBuf VrT, UrT;              // R rows of length K (precompute inputs)
   DBuf Mc, Lc;                // 8 each, FP64 At (R x R)
-  DBuf Chi, Clo;              // 316 M2L operators, bf16 row-major [RPm][RP]
-  std::vector<const uint16_t *> chi, clo;  // per offset slot (343)
+  DBuf Cf;                    // 316 M2L operators, NP bf16 parts in MFMA fragment order
+  std::vector<const uint16_t *> cop;  // per offset slot (343)
 };
 
 struct LevelOps {
@@ -76,7 +80,8 @@ struct LevelOps {
 class Fmm {
  public:
   // precision 0: FP64 form (reference parity); 1: mixed form (truncated SVD
-  // at relative tolerance tau, BF16x3 M2L, FP32-math direct sums)
+  // at relative tolerance tau, two-part split-BF16 M2L, FP32-math direct
+  // sums); 2: as 1 with three-part split-BF16 M2L
   Fmm(std::shared_ptr<const Tree> t, double eps, double mu, int res, int shells, int precision = 0,
       double tau = 1e-10);
   ~Fmm();
@@ -114,7 +119,12 @@ class Fmm {
   void plan();
   void build_ops(int level);
   void gemm(const GemmBatch &b, const double *X, int ldx, double *C, int ldc, int beta, hipStream_t s,
-            int tag = 0);
+            int tag = 0, int nsplit = 1, const DBuf *slot = nullptr);
+  // split-K factor for a launch of `tiles` workgroups over blocks of nb
+  // operators, and the reduction of its partial tiles into C
+  static int split_for(int64_t tiles, int nb);
+  void split_reduce(const DBuf &slot, int ncols, int M, int ldc, int nsplit, double *C, hipStream_t s);
+  void reserve_part(size_t elems);
   void gemm_dims(const GemmBatch &b, const double *X, int ldx, double *C, int ldc, int beta,
                  hipStream_t s, int M, int K, int Mpad, int Kpad);
   void build_mixed();
@@ -150,7 +160,10 @@ class Fmm {
   int R_ = 0, RP_ = 0, RPm_ = 0, MpadR_ = 0, KpadR_ = 0;
   std::vector<std::vector<double>> svdU_, svdS_, svdV_;  // host, per level
   std::vector<MixedOps> mops_;
-  DBuf chat_, dhat_, chat_hi_, chat_lo_, upall_;
+  DBuf chat_, dhat_, chat_x_, upall_;
+  int np_ = 2, KCm_ = 0, RT_ = 0;   // mixed M2L: bf16 parts, 32-deep k chunks, 16-row tiles
+  DBuf part_;                       // split-K partial tiles
+  size_t part_elems_ = 0;
   int n_upall_ = 0;
   void reduced_factor();
   bool red_ready_ = false;

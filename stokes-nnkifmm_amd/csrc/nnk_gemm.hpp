@@ -14,7 +14,10 @@
 // form a 2x2 grid of (BM/2) x (BN/2) sub-tiles.  K is walked in BK-deep
 // chunks over the operator blocks that have at least one source in the tile,
 // through a two-stage LDS ping-pong fed by register prefetch (one barrier per
-// chunk).  MFMA operand layout (measured on gfx950): lane l holds A[l%16][l/16]
+// chunk).  With nsplit > 1 the active blocks are divided over nsplit
+// workgroups (split-K, for levels with few tiles); each writes its partial
+// tile to P[(split, job, column)] and a fixed-order reduction forms C.
+// MFMA operand layout (measured on gfx950): lane l holds A[l%16][l/16]
 // and B[l/16][l%16]; accumulator register r of lane l holds D[l/16 + 4r][l%16].
 #pragma once
 #include <hip/hip_runtime.h>
@@ -34,7 +37,8 @@ __global__ __launch_bounds__(256, 2) void k_gemm_gather(int M, int K, int Mpad, 
                                                         const double *__restrict__ X, int ldx,
                                                         double *__restrict__ C, int ldc, int beta,
                                                         const GemmJob *__restrict__ jobs,
-                                                        int njobs, int nrb) {
+                                                        int njobs, int nrb, int nsplit,
+                                                        double *__restrict__ P, int maxN) {
   static_assert(BK == 16 && BM % 32 == 0 && BN % 32 == 0, "tile shape");
   constexpr int WM = BM / 2, WN = BN / 2, TM = WM / 16, TN = WN / 16;
   constexpr int A2 = BK * BM / 2;       // double2 per A chunk
@@ -49,7 +53,9 @@ __global__ __launch_bounds__(256, 2) void k_gemm_gather(int M, int K, int Mpad, 
 
   const int bid = blockIdx.x;
   const int jb = bid % njobs;
-  const int rr = bid / njobs;
+  int rr = bid / njobs;
+  const int sp = rr % nsplit;
+  rr /= nsplit;
   const int rb = rr % nrb, ct = rr / nrb;
   const GemmJob J = jobs[jb];
   const int col0 = ct * BN;
@@ -95,14 +101,15 @@ __global__ __launch_bounds__(256, 2) void k_gemm_gather(int M, int K, int Mpad, 
     for (int j = 0; j < TN; j++) acc[i][j] = d4{0, 0, 0, 0};
 
   const int KC = Kpad / BK;
-  const int total = nact * KC;
+  const int a0 = (int)((int64_t)nact * sp / nsplit), a1 = (int)((int64_t)nact * (sp + 1) / nsplit);
+  const int total = (a1 - a0) * KC;
   d2v ra[AR], rbv[BR];
   int gcol[BR];
   int cur_block = -1;
 
   auto fetch = [&](int c) {
-    const int bi = c / KC, kc = c - bi * KC;
-    const int b = act[bi];
+    const int q = c / KC, kc = c - q * KC;
+    const int b = act[a0 + q];
     const int k0 = kc * BK;
     if (b != cur_block) {
       cur_block = b;
@@ -187,10 +194,19 @@ __global__ __launch_bounds__(256, 2) void k_gemm_gather(int M, int K, int Mpad, 
       for (int r = 0; r < 4; r++) {
         const int row = m0 + wm * WM + i * 16 + lk + 4 * r;
         if (row >= M) continue;
-        double *p = C + (size_t)cs * ldc + row;
-        *p = beta ? *p + acc[i][j][r] : acc[i][j][r];
+        if (nsplit > 1) {
+          P[((size_t)(sp * njobs + jb) * maxN + col) * ldc + row] = acc[i][j][r];
+        } else {
+          double *p = C + (size_t)cs * ldc + row;
+          *p = beta ? *p + acc[i][j][r] : acc[i][j][r];
+        }
       }
     }
 }
+
+// C[slot[c]][row] = sum over splits of P[split][c][row], splits in order
+// (c runs over job * maxN + column; slot -1 = no column)
+__global__ void k_split_reduce(const int32_t *__restrict__ slot, int M, int ldc, int nsplit,
+                               const double *__restrict__ P, size_t pstride, double *__restrict__ C);
 
 }  // namespace nnk

@@ -3,17 +3,30 @@
 //
 //   D[:, cidx[n]] = sum_b A_b X[:, gidx[b][n]]      (D in FP64)
 //
-// Every operand is stored as two BF16 parts, hi = bf16(v), lo = bf16(v - hi)
-// (16 significant bits together).  Each tile product takes the three terms
-// hi*hi + hi*lo + lo*hi with FP32 accumulation; after every operator block
-// (one translation, K = R) the FP32 accumulators are added into FP64 ones,
-// so rounding does not grow with the 189-block sum.
+// Every operand is held as NP BF16 parts, v ~ p0 + p1 (+ p2), each part the
+// rounding of what the previous ones leave (16 or 24 significant bits).
+// Products keep the terms down to the order of the last part:
+//   NP = 2:  p1*q0 + p0*q1 + p0*q0                      (3 MFMAs)
+//   NP = 3:  p2*q0 + p1*q1 + p0*q2 + p1*q0 + p0*q1 + p0*q0   (6 MFMAs)
+// accumulated in FP32 over one operator block (K = R) and then added into
+// FP64 accumulators, so rounding does not grow with the 189-block sum.
 //
-// A_b parts are row-major [Mrows][KP] (k contiguous), X parts node-major
-// [node][KP]: both MFMA operands are then 16-byte contiguous per lane
-// (lane l holds A[l%16][8(l/16) .. 8(l/16)+7] and B[8(l/16) .. +7][l%16]).
-// Grid walk, active-block lists and the register-prefetch LDS ping-pong
-// follow the FP64 kernel (nnk_gemm.hpp).
+// Storage is in MFMA fragment order, so that every global load, LDS store
+// and LDS load is a contiguous 1 KB per wavefront:
+//   A_b:  [KC][NP][RT][64 lanes][8]   lane q of row tile t holds
+//                                     A[16t + q%16][32kc + 8(q/16) + e]
+//   X:    [node][KC][NP][32]          (the 64-byte k-slice of one node)
+// A gather slot of -1 reads node `zrow`, an all-zero row.
+//
+// Work decomposition: one 512-thread workgroup owns a 128 x 128 tile of D
+// (8 waves as 2 x 4, 64 x 32 each).  The flattened grid is walked
+// job-fastest, so with the round-robin dispatch of workgroups to the 8 XCDs
+// the 8 child-class jobs each stay on one XCD.  The operator blocks with at
+// least one source in the tile are walked in order in 32-deep chunks through
+// a two-stage LDS ping-pong fed by a ring of D register stages (loads run
+// D-1 chunks ahead of the MFMAs).  Optionally the active blocks are split
+// over `nsplit` workgroups (split-K, for levels with few tiles); each split
+// then writes its partial tile to P for a fixed-order reduction.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -21,59 +34,54 @@
 
 #include "nnk_fmm.hpp"
 
-#ifndef NNK_BF16_DROW
-// accumulator register r of lane l holds D[NNK_BF16_DROW(l/16, r)][l%16]
-#define NNK_BF16_DROW(g, r) (4 * (g) + (r))
-#endif
-
 namespace nnk {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+// global-address-space views (pointers read from memory are otherwise
+// generic, and generic loads also hold the LDS counter the MFMA loop waits on)
+typedef const u32x4 __attribute__((address_space(1))) *gu32x4p;
+typedef const int32_t __attribute__((address_space(1))) *gi32p;
 
 struct GemmJobBf16 {
-  const uint16_t *const *Ahi;  // nb operator parts, [Mrows][KP] bf16
-  const uint16_t *const *Alo;
-  const int32_t *gidx;         // nb x N (-1 = none)
-  const int32_t *cidx;         // N output slots
+  const uint16_t *const *A;  // nb operators in fragment order
+  const int32_t *gidx;       // nb x N gather slots (-1 = none)
+  const int32_t *cidx;       // N output slots
   int nb, N;
 };
 
-template <int BM, int BN, int TAG>
-__global__ __launch_bounds__(256, 2) void k_gemm_bf16x3(int M, int KP, const uint16_t *__restrict__ Xhi,
-                                                        const uint16_t *__restrict__ Xlo, int ldx,
-                                                        double *__restrict__ C, int ldc,
-                                                        const GemmJobBf16 *__restrict__ jobs, int njobs,
-                                                        int nrb) {
-  constexpr int BK = 32;                       // one MFMA k-step
-  constexpr int WM = BM / 2, WN = BN / 2, TM = WM / 16, TN = WN / 16;
-  constexpr int AROW = BK + 8;                 // padded LDS row (bf16)
-  constexpr int AV = BM * BK / 8;              // 16-byte vectors per A part per chunk
-  constexpr int BV = BN * BK / 8;
-  constexpr int AR = AV / 256 > 0 ? AV / 256 : 1, BR = BV / 256 > 0 ? BV / 256 : 1;
-  static_assert(AV % 256 == 0 || AV < 256, "A chunk split");
-  static_assert(BV % 256 == 0 || BV < 256, "B chunk split");
-  __shared__ __attribute__((aligned(16))) uint16_t As[2][2][BM][AROW];
-  __shared__ __attribute__((aligned(16))) uint16_t Bs[2][2][BN][AROW];
+template <int NP, int D, int TAG>
+__global__ __launch_bounds__(512, 1) void k_m2l_bf16(const int M, const int KC, const int RT,
+                                                     const uint16_t *__restrict__ X, const int zrow,
+                                                     double *__restrict__ C, const int ldc,
+                                                     const GemmJobBf16 *__restrict__ jobs, const int njobs,
+                                                     const int nrb, const int nsplit, double *__restrict__ P,
+                                                     const int maxN) {
+  constexpr int BN = 128, TM = 4, TN = 2, TILE = 512;
+  __shared__ __attribute__((aligned(16))) uint16_t As[2][NP][8][TILE];
+  __shared__ __attribute__((aligned(16))) uint16_t Xs[2][NP][8][TILE];
   __shared__ int act[512];
+  __shared__ const uint16_t *aptr[512];
   __shared__ int nact_s;
 
   const int bid = blockIdx.x;
   const int jb = bid % njobs;
-  const int rr = bid / njobs;
+  int rr = bid / njobs;
+  const int sp = rr % nsplit;
+  rr /= nsplit;
   const int rb = rr % nrb, ct = rr / nrb;
   const GemmJobBf16 J = jobs[jb];
   const int col0 = ct * BN;
   if (col0 >= J.N) return;
-  const int m0 = rb * BM;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wm = wave >> 1, wn = wave & 1;
+  const int wm = wave >> 2, wn = wave & 3;
   const int lr = lane & 15, lg = lane >> 4;
 
+  // ---- operator blocks with at least one source column in this tile ----
   if (tid == 0) nact_s = 0;
   __syncthreads();
-  for (int b = tid; b < J.nb; b += 256) {
+  for (int b = tid; b < J.nb; b += 512) {
     bool any = false;
     const int32_t *g = J.gidx + (size_t)b * J.N;
     for (int n = 0; n < BN && !any; n++)
@@ -82,13 +90,64 @@ __global__ __launch_bounds__(256, 2) void k_gemm_bf16x3(int M, int KP, const uin
   }
   __syncthreads();
   const int nact = nact_s;
-  if (tid == 0)
+  if (tid == 0)  // block order: a fixed summation order
     for (int i = 1; i < nact; i++) {
       int v = act[i], j = i - 1;
       while (j >= 0 && act[j] > v) { act[j + 1] = act[j]; j--; }
       act[j + 1] = v;
     }
   __syncthreads();
+  for (int i = tid; i < nact; i += 512) aptr[i] = J.A[act[i]];
+  __syncthreads();
+  const int a0 = (int)((int64_t)nact * sp / nsplit), a1 = (int)((int64_t)nact * (sp + 1) / nsplit);
+  const int total = (a1 - a0) * KC;
+
+  // this thread's load slot: fragment tile ti (0..7), lane slot tq
+  const int ti = tid >> 6, tq = tid & 63;
+  const int my_col = col0 + ti * 16 + (tq & 15);
+  const int kof = 8 * (tq >> 4);
+  const size_t aoff = (size_t)(rb * 8 + ti) * TILE + (size_t)tq * 8;
+  const size_t slab = (size_t)RT * TILE;  // one (kc, part) slab of an operator
+
+  // gather slot of this thread's column in active block bi (clamped
+  // indices, so the load is issued unconditionally); the raw slot is only
+  // examined when the block is reached, a block after the load
+  const gi32p gidx = (gi32p)J.gidx;
+  auto gload = [&](int bi) -> int {
+    const int bc = bi < a1 ? bi : a1 - 1;
+    const int cc = my_col < J.N ? my_col : J.N - 1;
+    return gidx[(size_t)act[bc] * J.N + cc];
+  };
+  auto gslot = [&](int raw, int bi) -> int {
+    return (bi < a1 && my_col < J.N && raw >= 0) ? raw : zrow;
+  };
+  int fblk = -1, gcur = zrow, gnext = total > 0 ? gload(a0) : -1;
+
+  // loads of chunk c (clamped to the last chunk, so that every stage issues
+  // the same loads and the wait counts stay static)
+  auto fetch = [&](int c, u32x4(&R)[2 * NP]) {
+    const int cq = c < total ? c : total - 1;
+    const int q = cq / KC, kc = cq - q * KC, bi = a0 + q;
+    if (bi != fblk) {  // first chunk of a new block: its slot was loaded a block ahead
+      fblk = bi;
+      gcur = gslot(gnext, bi);
+      gnext = gload(bi + 1);
+    }
+    const gu32x4p Ab = (gu32x4p)(aptr[bi] + aoff + (size_t)kc * NP * slab);
+    const gu32x4p Xb = (gu32x4p)(X + ((size_t)gcur * KC + kc) * NP * 32 + kof);
+#pragma unroll
+    for (int p = 0; p < NP; p++) {
+      R[p] = Ab[p * slab / 8];
+      R[NP + p] = Xb[p * 4];
+    }
+  };
+  auto stash = [&](const u32x4(&R)[2 * NP], int s) {
+#pragma unroll
+    for (int p = 0; p < NP; p++) {
+      *reinterpret_cast<u32x4 *>(&As[s][p][ti][tq * 8]) = R[p];
+      *reinterpret_cast<u32x4 *>(&Xs[s][p][ti][tq * 8]) = R[NP + p];
+    }
+  };
 
   f32x4 acc[TM][TN];
   double acc64[TM][TN][4];
@@ -101,123 +160,78 @@ __global__ __launch_bounds__(256, 2) void k_gemm_bf16x3(int M, int KP, const uin
       for (int r = 0; r < 4; r++) acc64[i][j][r] = 0.0;
     }
 
-  const int KC = KP / BK;
-  const int total = nact * KC;
-  u32x4 ra[2][AR], rbv[2][BR];
-  int gcol[BR];
-  int cur_block = -1;
-
-  auto fetch = [&](int c) {
-    const int bi = c / KC, kc = c - bi * KC;
-    const int b = act[bi];
-    const int k0 = kc * BK;
-    if (b != cur_block) {
-      cur_block = b;
+  auto compute = [&](int s) {
+    bf16x8 a[NP][TM], x[NP][TN];
 #pragma unroll
-      for (int r = 0; r < BR; r++) {
-        const int i = tid + 256 * r;
-        const int n = i / (BK / 8);
-        const int col = col0 + n;
-        gcol[r] = (i < BV && col < J.N) ? J.gidx[(size_t)b * J.N + col] : -1;
-      }
+    for (int p = 0; p < NP; p++) {
+#pragma unroll
+      for (int i = 0; i < TM; i++) a[p][i] = *reinterpret_cast<const bf16x8 *>(&As[s][p][wm * 4 + i][lane * 8]);
+#pragma unroll
+      for (int j = 0; j < TN; j++) x[p][j] = *reinterpret_cast<const bf16x8 *>(&Xs[s][p][wn * 2 + j][lane * 8]);
     }
 #pragma unroll
-    for (int p = 0; p < 2; p++) {
-      const uint16_t *__restrict__ Ab = p ? J.Alo[b] : J.Ahi[b];
+    for (int i = 0; i < TM; i++)
 #pragma unroll
-      for (int r = 0; r < AR; r++) {
-        const int i = tid + 256 * r;
-        if (i < AV) {
-          const int m = i / (BK / 8), k8 = i % (BK / 8);
-          ra[p][r] = *reinterpret_cast<const u32x4 *>(Ab + (size_t)(m0 + m) * KP + k0 + 8 * k8);
+      for (int j = 0; j < TN; j++) {
+        f32x4 t = acc[i][j];
+        if constexpr (NP == 3) {
+          t = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2][i], x[0][j], t, 0, 0, 0);
+          t = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1][i], x[1][j], t, 0, 0, 0);
+          t = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][i], x[2][j], t, 0, 0, 0);
         }
+        t = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1][i], x[0][j], t, 0, 0, 0);
+        t = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][i], x[1][j], t, 0, 0, 0);
+        t = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][i], x[0][j], t, 0, 0, 0);
+        acc[i][j] = t;
       }
-      const uint16_t *__restrict__ Xp = p ? Xlo : Xhi;
-#pragma unroll
-      for (int r = 0; r < BR; r++) {
-        const int i = tid + 256 * r;
-        const int k8 = i % (BK / 8);
-        u32x4 v = {0, 0, 0, 0};
-        if (gcol[r] >= 0) v = *reinterpret_cast<const u32x4 *>(Xp + (size_t)gcol[r] * ldx + k0 + 8 * k8);
-        rbv[p][r] = v;
-      }
-    }
   };
-  auto stash = [&](int buf) {
+  auto promote = [&]() {
 #pragma unroll
-    for (int p = 0; p < 2; p++) {
+    for (int i = 0; i < TM; i++)
 #pragma unroll
-      for (int r = 0; r < AR; r++) {
-        const int i = tid + 256 * r;
-        if (i < AV) {
-          const int m = i / (BK / 8), k8 = i % (BK / 8);
-          *reinterpret_cast<u32x4 *>(&As[buf][p][m][8 * k8]) = ra[p][r];
-        }
+      for (int j = 0; j < TN; j++) {
+#pragma unroll
+        for (int r = 0; r < 4; r++) acc64[i][j][r] += (double)acc[i][j][r];
+        acc[i][j] = f32x4{0, 0, 0, 0};
       }
-#pragma unroll
-      for (int r = 0; r < BR; r++) {
-        const int i = tid + 256 * r;
-        if (i < BV) {
-          const int n = i / (BK / 8), k8 = i % (BK / 8);
-          *reinterpret_cast<u32x4 *>(&Bs[buf][p][n][8 * k8]) = rbv[p][r];
-        }
-      }
-    }
   };
 
   if (total > 0) {
-    fetch(0);
-    stash(0);
+    u32x4 R[D][2 * NP];
+#pragma unroll
+    for (int d = 0; d < D; d++) fetch(d, R[d]);
+    stash(R[0], 0);
     __syncthreads();
-    for (int c = 0; c < total; c++) {
-      const int buf = c & 1;
-      if (c + 1 < total) fetch(c + 1);
-      bf16x8 ah[TM], al[TM], bh[TN], bl[TN];
+    // ring: at chunk cc, R[d] (chunk cc) is already in LDS and is refilled
+    // with chunk cc + D; chunk cc + 1 moves from R[d+1] to the other stage
+    for (int c = 0; c < total; c += D) {
 #pragma unroll
-      for (int i = 0; i < TM; i++) {
-        ah[i] = *reinterpret_cast<const bf16x8 *>(&As[buf][0][wm * WM + i * 16 + lr][8 * lg]);
-        al[i] = *reinterpret_cast<const bf16x8 *>(&As[buf][1][wm * WM + i * 16 + lr][8 * lg]);
-      }
-#pragma unroll
-      for (int j = 0; j < TN; j++) {
-        bh[j] = *reinterpret_cast<const bf16x8 *>(&Bs[buf][0][wn * WN + j * 16 + lr][8 * lg]);
-        bl[j] = *reinterpret_cast<const bf16x8 *>(&Bs[buf][1][wn * WN + j * 16 + lr][8 * lg]);
-      }
-#pragma unroll
-      for (int i = 0; i < TM; i++)
-#pragma unroll
-        for (int j = 0; j < TN; j++) {
-          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
-          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
-          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+      for (int d = 0; d < D; d++) {
+        const int cc = c + d;
+        fetch(cc + D, R[d]);
+        if (cc < total) {
+          compute(cc & 1);
+          if ((cc + 1) % KC == 0) promote();  // end of an operator block
         }
-      // end of an operator block: promote to FP64
-      if ((c + 1) % KC == 0) {
-#pragma unroll
-        for (int i = 0; i < TM; i++)
-#pragma unroll
-          for (int j = 0; j < TN; j++) {
-#pragma unroll
-            for (int r = 0; r < 4; r++) acc64[i][j][r] += (double)acc[i][j][r];
-            acc[i][j] = f32x4{0, 0, 0, 0};
-          }
+        stash(R[(d + 1) % D], (cc + 1) & 1);
+        __syncthreads();
       }
-      if (c + 1 < total) stash(buf ^ 1);
-      __syncthreads();
     }
   }
+
+  const int m0 = rb * 128 + wm * 64;
 #pragma unroll
   for (int i = 0; i < TM; i++)
 #pragma unroll
     for (int j = 0; j < TN; j++) {
-      const int col = col0 + wn * WN + j * 16 + lr;
+      const int col = col0 + wn * 32 + j * 16 + lr;
       if (col >= J.N) continue;
-      const int cs = J.cidx ? J.cidx[col] : col;
+      double *out = nsplit > 1 ? P + ((size_t)(sp * njobs + jb) * maxN + col) * ldc
+                               : C + (size_t)J.cidx[col] * ldc;
 #pragma unroll
       for (int r = 0; r < 4; r++) {
-        const int row = m0 + wm * WM + i * 16 + NNK_BF16_DROW(lg, r);
-        if (row >= M) continue;
-        C[(size_t)cs * ldc + row] = acc64[i][j][r];
+        const int row = m0 + i * 16 + 4 * lg + r;
+        if (row < M) out[row] = acc64[i][j][r];
       }
     }
 }

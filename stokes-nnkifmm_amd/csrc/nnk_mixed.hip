@@ -10,8 +10,9 @@
 //   L2L_k <- Pdn(l)   K(parent down -> child up) U_R(l-1)
 //   M2L_o <- Pdn(l)   K(up at offset o -> up)    V_R(l)
 // formed once on the device with the FP64 GEMM.  M2L (the dominant work)
-// runs as split-BF16 MFMA GEMMs with FP32 accumulation promoted to FP64 per
-// translation; M2M, L2L and the expansions to densities stay FP64; the
+// runs as split-BF16 MFMA GEMMs (two or three BF16 parts per operand) with
+// FP32 accumulation promoted to FP64 per translation; M2M, L2L and the
+// expansions to densities stay FP64; the
 // direct sums (S2M, P2L, L2T, P2P, M2T) evaluate the kernel in FP32 from
 // FP64 separations and accumulate in FP64.
 #include <hip/hip_runtime.h>
@@ -35,44 +36,54 @@ __device__ __forceinline__ uint16_t f2bf(float f) {  // round to nearest even
 }
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
-__device__ __forceinline__ void split2(double v, uint16_t &h, uint16_t &l) {
-  h = f2bf((float)v);
-  l = f2bf((float)(v - (double)bf2f(h)));
+// v ~ parts[0] + ... + parts[np-1], each part the BF16 rounding of the
+// remainder (the remainders are exact in FP64)
+__device__ __forceinline__ uint16_t split_part(double v, int np, int p) {
+  uint16_t h = 0;
+  for (int i = 0; i <= p && i < np; i++) {
+    h = f2bf((float)v);
+    v -= (double)bf2f(h);
+  }
+  return h;
 }
 
 }  // namespace
 
-// rows[i] of X (length R, stride ldx) -> hi/lo bf16 rows of length ldo
+// rows[i] of X (length R, stride ldx) -> parts in [node][KC][np][32] order
 // (zero beyond R)
 __global__ void k_split_rows(const int32_t *__restrict__ rows, int nrows, const double *__restrict__ X,
-                             int ldx, int R, uint16_t *__restrict__ hi, uint16_t *__restrict__ lo,
-                             int ldo) {
+                             int ldx, int R, int KC, int np, uint16_t *__restrict__ out) {
   const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= (int64_t)nrows * ldo) return;
-  const int i = (int)(idx / ldo), k = (int)(idx % ldo);
+  const int kk = KC * 32;
+  if (idx >= (int64_t)nrows * kk) return;
+  const int i = (int)(idx / kk), k = (int)(idx % kk);
   const int64_t r = rows[i];
-  uint16_t h = 0, l = 0;
-  if (k < R) split2(X[r * ldx + k], h, l);
-  hi[r * ldo + k] = h;
-  lo[r * ldo + k] = l;
+  const double v = k < R ? X[r * ldx + k] : 0.0;
+  const int kc = k >> 5, e = k & 31;
+  for (int p = 0; p < np; p++) out[((r * KC + kc) * np + p) * 32 + e] = split_part(v, np, p);
 }
 
-// operator j: CT[(j*R + n) * ldct + m] = C_j[m][n]  ->  bf16 parts [m][n],
-// rows padded to RPm, columns to RP
-__global__ void k_split_op(const double *__restrict__ CT, int ldct, int R, int nops,
-                           uint16_t *__restrict__ hi, uint16_t *__restrict__ lo, int RPm, int RP) {
+// operator j: CT[(j*R + n) * ldct + m] = C_j[m][n] -> parts in MFMA fragment
+// order [KC][np][RT][64][8]: lane q of row tile t holds
+// C_j[16t + q%16][32kc + 8(q/16) + e]
+__global__ void k_split_op(const double *__restrict__ CT, int ldct, int R, int nops, int KC, int np, int RT,
+                           uint16_t *__restrict__ out) {
   const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t per = (int64_t)RPm * RP;
+  const int64_t per = (int64_t)KC * np * RT * 512;
   if (idx >= per * nops) return;
   const int j = (int)(idx / per);
-  const int m = (int)((idx % per) / RP), n = (int)(idx % RP);
-  uint16_t h = 0, l = 0;
-  if (m < R && n < R) split2(CT[((size_t)j * R + n) * ldct + m], h, l);
-  hi[idx] = h;
-  lo[idx] = l;
+  int64_t rem = idx % per;
+  const int e = (int)(rem & 7), q = (int)((rem >> 3) & 63);
+  rem >>= 9;
+  const int t = (int)(rem % RT);
+  rem /= RT;
+  const int p = (int)(rem % np), kc = (int)(rem / np);
+  const int m = 16 * t + (q & 15), k = 32 * kc + 8 * (q >> 4) + e;
+  const double v = (m < R && k < R) ? CT[((size_t)j * R + k) * ldct + m] : 0.0;
+  out[idx] = split_part(v, np, p);
 }
 
-constexpr int MBM = 128, MBN = 64;   // split-BF16 M2L tile
+constexpr int MBM = 128, MBN = 128;   // split-BF16 M2L tile
 
 void Fmm::build_mixed() {
   const int K = K_;
@@ -84,8 +95,11 @@ void Fmm::build_mixed() {
     R_ = std::max(R_, r);
   }
   if (R_ == 0) R_ = 32;
+  np_ = prec_ == 2 ? 3 : 2;
   RP_ = (R_ + 31) / 32 * 32;
   RPm_ = (R_ + MBM - 1) / MBM * MBM;
+  KCm_ = RP_ / 32;
+  RT_ = RPm_ / 16;
   MpadR_ = (R_ + 159) / 160 * 160;
   KpadR_ = (R_ + 15) / 16 * 16;
   const int R = R_;
@@ -159,20 +173,15 @@ void Fmm::build_mixed() {
       std::vector<const double *> P(no, X.Pdn.as<double>());
       GemmBatch b2 = batch(no, P, R, &ids, &ids);
       gemm_dims(b2, Y.as<double>(), K, CT.as<double>(), MpadR_, 0, 0, R, K, MpadR_, Kpad_);
-      const size_t per = (size_t)RPm_ * RP_;
-      X.Chi.alloc(sizeof(uint16_t) * per * no);
-      X.Clo.alloc(sizeof(uint16_t) * per * no);
+      const size_t per = (size_t)KCm_ * np_ * RT_ * 512;
+      X.Cf.alloc(sizeof(uint16_t) * per * no);
       const int64_t tot = (int64_t)per * no;
       hipLaunchKernelGGL(k_split_op, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, 0, CT.as<double>(),
-                         MpadR_, R, no, X.Chi.as<uint16_t>(), X.Clo.as<uint16_t>(), RPm_, RP_);
+                         MpadR_, R, no, KCm_, np_, RT_, X.Cf.as<uint16_t>());
       NNK_HIP(hipGetLastError());
       NNK_HIP(hipDeviceSynchronize());
-      X.chi.assign(343, nullptr);
-      X.clo.assign(343, nullptr);
-      for (int j = 0; j < no; j++) {
-        X.chi[slots[j]] = X.Chi.as<uint16_t>() + per * j;
-        X.clo[slots[j]] = X.Clo.as<uint16_t>() + per * j;
-      }
+      X.cop.assign(343, nullptr);
+      for (int j = 0; j < no; j++) X.cop[slots[j]] = X.Cf.as<uint16_t>() + per * j;
     }
     // ---- M2M (this level is the child level; parent level l-1 >= 2) ----
     if (l >= 3) {
@@ -211,12 +220,11 @@ void Fmm::build_mixed() {
   const int nc = tree_.count;
   chat_.alloc(sizeof(double) * (size_t)nc * RP_);
   dhat_.alloc(sizeof(double) * (size_t)nc * RP_);
-  chat_hi_.alloc(sizeof(uint16_t) * (size_t)nc * RP_);
-  chat_lo_.alloc(sizeof(uint16_t) * (size_t)nc * RP_);
+  // split coefficients, one all-zero row (slot nc) for absent sources
+  chat_x_.alloc(sizeof(uint16_t) * (size_t)(nc + 1) * np_ * RP_);
   chat_.zero();
   dhat_.zero();
-  chat_hi_.zero();
-  chat_lo_.zero();
+  chat_x_.zero();
   NNK_HIP(hipDeviceSynchronize());
 }
 
@@ -282,29 +290,26 @@ void Fmm::plan_mixed() {
     for (int a = 0; a < 8; a++) {
       if (H.cls[a].empty()) continue;
       const int g[3] = {(a >> 2) & 1, (a >> 1) & 1, a & 1};
-      std::vector<const uint16_t *> hi, lo;
+      std::vector<const uint16_t *> ops;
       for (int ox = -2 - g[0]; ox <= 3 - g[0]; ox++)
         for (int oy = -2 - g[1]; oy <= 3 - g[1]; oy++)
           for (int oz = -2 - g[2]; oz <= 3 - g[2]; oz++) {
             if (std::max({std::abs(ox), std::abs(oy), std::abs(oz)}) < 2) continue;
             const int sl = (ox + 3) * 49 + (oy + 3) * 7 + (oz + 3);
-            hi.push_back(X.chi[sl]);
-            lo.push_back(X.clo[sl]);
+            if (!X.cop[sl]) throw Error(NNK_ESTATE, "mixed M2L: missing translation operator");
+            ops.push_back(X.cop[sl]);
           }
       GemmJobBf16 J{};
       P.mx_owned.emplace_back();
-      P.mx_owned.back().upload(hi);
-      J.Ahi = P.mx_owned.back().as<const uint16_t *>();
-      P.mx_owned.emplace_back();
-      P.mx_owned.back().upload(lo);
-      J.Alo = P.mx_owned.back().as<const uint16_t *>();
+      P.mx_owned.back().upload(ops);
+      J.A = P.mx_owned.back().as<const uint16_t *>();
       P.mx_owned.emplace_back();
       P.mx_owned.back().upload(H.gv[a]);
       J.gidx = P.mx_owned.back().as<int32_t>();
       P.mx_owned.emplace_back();
       P.mx_owned.back().upload(H.cls[a]);
       J.cidx = P.mx_owned.back().as<int32_t>();
-      J.nb = (int)hi.size();
+      J.nb = (int)ops.size();
       J.N = (int)H.cls[a].size();
       mj.push_back(J);
       P.mx_m2l_maxN = std::max(P.mx_m2l_maxN, J.N);
@@ -326,7 +331,25 @@ void Fmm::plan_mixed() {
       }
     }
     P.mx_m2l_jobs = (int)mj.size();
-    if (!mj.empty()) P.mx_m2l.upload(mj);
+    if (!mj.empty()) {
+      P.mx_m2l.upload(mj);
+      const int nj = P.mx_m2l_jobs, mN = P.mx_m2l_maxN;
+      std::vector<int32_t> slot((size_t)nj * mN, -1);
+      int minnb = 512;
+      for (int j = 0; j < nj; j++) {
+        minnb = std::min(minnb, mj[j].nb);
+      }
+      for (int j = 0, a = 0; a < 8; a++) {
+        if (H.cls[a].empty()) continue;
+        for (size_t c = 0; c < H.cls[a].size(); c++) slot[(size_t)j * mN + c] = H.cls[a][c];
+        j++;
+      }
+      P.mx_split = split_for((int64_t)nj * (RPm_ / MBM) * ((mN + MBN - 1) / MBN), minnb);
+      if (P.mx_split > 1) {
+        P.mx_slot.upload(slot);
+        reserve_part((size_t)P.mx_split * nj * mN * RP_);
+      }
+    }
     P.mx_l2l.njobs = (int)lj.size();
     if (!lj.empty()) P.mx_l2l.jobs.upload(lj);
   }
@@ -352,10 +375,9 @@ void Fmm::apply_mixed(const double *d_f, double *d_u, hipStream_t s) {
     gemm_dims(P.mx_upP, chk, K, c, RP_, 0, s, R, K, MpadR_, Kpad_);
   }
   if (n_upall_) {
-    const int64_t tot = (int64_t)n_upall_ * RP_;
+    const int64_t tot = (int64_t)n_upall_ * KCm_ * 32;
     hipLaunchKernelGGL(k_split_rows, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s,
-                       upall_.as<int32_t>(), n_upall_, c, RP_, R, chat_hi_.as<uint16_t>(),
-                       chat_lo_.as<uint16_t>(), RP_);
+                       upall_.as<int32_t>(), n_upall_, c, RP_, R, KCm_, np_, chat_x_.as<uint16_t>());
     NNK_HIP(hipGetLastError());
   }
   for (int l = maxlev_; l >= 2; l--)
@@ -365,11 +387,18 @@ void Fmm::apply_mixed(const double *d_f, double *d_u, hipStream_t s) {
     LevelPlan &P = plans_[l];
     if (P.mx_m2l_jobs) {
       mark(3, true, s);
-      const int nrb = RPm_ / MBM, nct = (P.mx_m2l_maxN + MBN - 1) / MBN;
-      hipLaunchKernelGGL((k_gemm_bf16x3<MBM, MBN, 1>), dim3((unsigned)(P.mx_m2l_jobs * nrb * nct)), dim3(256),
-                         0, s, R, RP_, chat_hi_.as<uint16_t>(), chat_lo_.as<uint16_t>(), RP_, d, RP_,
-                         P.mx_m2l.as<GemmJobBf16>(), P.mx_m2l_jobs, nrb);
+      const int nrb = RPm_ / MBM, nct = (P.mx_m2l_maxN + MBN - 1) / MBN, sp = P.mx_split;
+      const dim3 grid((unsigned)(P.mx_m2l_jobs * sp * nrb * nct));
+      const int zrow = tree_.count;
+      double *pp = sp > 1 ? part_.as<double>() : nullptr;
+      if (np_ == 3)
+        hipLaunchKernelGGL((k_m2l_bf16<3, 2, 1>), grid, dim3(512), 0, s, R, KCm_, RT_, chat_x_.as<uint16_t>(),
+                           zrow, d, RP_, P.mx_m2l.as<GemmJobBf16>(), P.mx_m2l_jobs, nrb, sp, pp, P.mx_m2l_maxN);
+      else
+        hipLaunchKernelGGL((k_m2l_bf16<2, 3, 1>), grid, dim3(512), 0, s, R, KCm_, RT_, chat_x_.as<uint16_t>(),
+                           zrow, d, RP_, P.mx_m2l.as<GemmJobBf16>(), P.mx_m2l_jobs, nrb, sp, pp, P.mx_m2l_maxN);
       NNK_HIP(hipGetLastError());
+      if (sp > 1) split_reduce(P.mx_slot, P.mx_m2l_jobs * P.mx_m2l_maxN, R, RP_, sp, d, s);
       mark(3, false, s);
     }
     mark(4, true, s);

@@ -35,7 +35,8 @@ extern "C" int nnk_selftest_gemm(int M, int K, int Mpad, int Kpad, const double 
     if (Kpad % 16) return 2;
     dim3 grid(nrb * ((N + 63) / 64));
     hipLaunchKernelGGL((k_gemm_gather<160, 64, 16, 0>), grid, dim3(256), 0, 0, M, K, Mpad, Kpad,
-                       dX.as<double>(), ldx, dC.as<double>(), ldc, beta, dJ.as<GemmJob>(), 1, nrb);
+                       dX.as<double>(), ldx, dC.as<double>(), ldc, beta, dJ.as<GemmJob>(), 1, nrb, 1,
+                       nullptr, N);
     NNK_HIP(hipGetLastError());
     NNK_HIP(hipDeviceSynchronize());
     NNK_HIP(hipMemcpy(C, dC.p, sizeof(double) * (size_t)nc * ldc, hipMemcpyDeviceToHost));

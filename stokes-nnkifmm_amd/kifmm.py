@@ -16,7 +16,7 @@ from .octree import parse_options
 
 _OPTS = {"GPU": 0, "parThreads": 0, "blockSize": 0.1, "GMRES": 0, "coronaRes": 6, "coronaShells": 1,
          "format": 1, "precision": "fp64", "svdTol": 1e-8}
-_PREC = {"fp64": 0, "mixed": 1}
+_PREC = {"fp64": 0, "mixed": 1, "mixed3": 2}
 
 
 def _is_torch(x):
@@ -46,7 +46,8 @@ class KIFMM:
     Classes/KIFMM.m:46-61.  'GPU', 'parThreads' and 'blockSize' are
     accepted for compatibility; the matvec always runs on the GPU and never
     materialises a kernel block.  Build-only options: 'precision' ('fp64',
-    the reference's numbers, or 'mixed') and 'svdTol' (mixed form's
+    the reference's numbers, 'mixed' or 'mixed3': two- or three-part
+    split-BF16 M2L) and 'svdTol' (the mixed forms'
     pseudo-inverse cut)."""
 
     def __init__(self, tree, kernelPar, *args, **kwargs):
