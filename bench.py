@@ -5,10 +5,14 @@ the unit cube, forces i.i.d. N(0,1) in 3 components (numpy PCG64, fixed
 seed), eps = 1e-3, mu = 1, nodeCapacity 64, coronaShells 1.
 One step = one full KIFMM matvec (upward + downward pass) with inputs and
 outputs resident in HBM.  Tree build and operator precompute are setup.
-Two forms are timed: the plain FP64 form (the reference's numbers,
-coronaRes 6) and the mixed form (truncated-SVD pseudo-inverses, split-BF16
-M2L on the matrix cores, FP32-math direct sums); the mixed form is the
-headline `value`, both are reported under "forms".
+Forms timed (all reported under "forms"):
+  fp64_res6    the plain FP64 form at the reference's coronaRes 6
+  fp64_res8    the same at coronaRes 8 (its error does not improve: the
+               exact check-to-equivalent inverses are ill-conditioned)
+  mixed_res6   truncated-SVD pseudo-inverses, two-part split-BF16 M2L on the
+               matrix cores, FP32-math direct sums
+  mixed3_res8  the same with three-part split-BF16 M2L at coronaRes 8: the
+               first form below 1e-6 relative error, the headline `value`
 
 Prints one JSON line.  Besides the throughput it reports:
   roofline      the headline form's M2L translation GEMM (the dominant
@@ -45,14 +49,17 @@ def parse():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=1 << 20)
-    ap.add_argument("--res", type=int, default=6, help="corona resolution of the FP64 form")
-    ap.add_argument("--mixed-res", type=int, default=6, help="corona resolution of the mixed form")
-    ap.add_argument("--svd-tol", type=float, default=1e-8, help="pseudo-inverse cut of the mixed form")
-    ap.add_argument("--precision", choices=["fp64", "mixed", "both"], default="both")
+    ap.add_argument("--res", type=int, default=6, help="corona resolution of the CPU baseline")
+    ap.add_argument("--svd-tol", type=float, default=1e-8, help="pseudo-inverse cut of the mixed forms")
+    ap.add_argument("--forms", default="fp64_res6,fp64_res8,mixed_res6,mixed3_res8",
+                    help="comma-separated forms to time; the last one is the headline")
     ap.add_argument("--no-solve", action="store_true", help="skip the one-sphere mobility solve")
     ap.add_argument("--seed", type=int, default=20240611)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--cpu-n", type=int, default=1 << 16, help="CPU baseline sample size")
+    ap.add_argument("--copy-mb", type=int, default=0,
+                    help="only run a device-to-device copy of this many MiB (3 times): a kernel of "
+                         "known traffic for scaling the FETCH_SIZE / WRITE_SIZE counters")
     return ap.parse_args()
 
 
@@ -108,9 +115,12 @@ def run_form(nnk, lib, tree, fd, args, eps, mu, precision, res, svd_tol):
     if precision == "fp64":
         flops = 2.0 * K * K * st[0]
         peak, kern = FP64_MFMA_PEAK_TFLOPS, "k_gemm_gather<160,64,16,1> (M2L, FP64 MFMA)"
-    else:
+    elif precision == "mixed":
         flops = 3 * 2.0 * R * R * st[0]    # three BF16 products per split product
-        peak, kern = BF16_MFMA_PEAK_TFLOPS, "k_gemm_bf16x3<128,64,1> (M2L, split-BF16 MFMA)"
+        peak, kern = BF16_MFMA_PEAK_TFLOPS, "k_m2l_bf16<2,3,1> (M2L, two-part split-BF16 MFMA)"
+    else:
+        flops = 6 * 2.0 * R * R * st[0]    # six BF16 products per split product
+        peak, kern = BF16_MFMA_PEAK_TFLOPS, "k_m2l_bf16<3,2,1> (M2L, three-part split-BF16 MFMA)"
     m2l_ms = ms[3] / args.steps
     achieved = flops / (m2l_ms * 1e-3) / 1e12 if m2l_ms > 0 else 0.0
     rec = {"precision": precision, "coronaRes": res, "svd_tol": svd_tol if precision != "fp64" else None,
@@ -168,6 +178,15 @@ def main():
               "multi-GPU sharding is a design note, not a measured path.", file=sys.stderr)
         sys.exit(2)
     import torch
+    if args.copy_mb > 0:
+        n = args.copy_mb << 18   # doubles
+        a = torch.ones(n, dtype=torch.float64, device="cuda:0")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": 8 * n, "copy_bytes_written": 8 * n}), flush=True)
+        return
     nnk = importlib.import_module("stokes-nnkifmm_amd")
     lib = nnk.lib()
     eps, mu, N = 1e-3, 1.0, args.n
@@ -182,19 +201,24 @@ def main():
     idx = np.sort(rng.choice(N, 4096, replace=False))
     ud = direct_sample(lib, pts, fd, idx, eps, mu)
     forms = {}
-    for prec, res, tol in (("fp64", args.res, 0.0), ("mixed", args.mixed_res, args.svd_tol)):
-        if args.precision not in ("both", prec):
-            continue
-        u, rec = run_form(nnk, lib, tree, fd, args, eps, mu, prec, res, tol)
+    names = [f for f in args.forms.split(",") if f]
+    for name in names:
+        prec, res = name.split("_res")
+        if prec not in ("fp64", "mixed", "mixed3"):
+            raise SystemExit(f"unknown form {name}")
+        u, rec = run_form(nnk, lib, tree, fd, args, eps, mu, prec, int(res),
+                          0.0 if prec == "fp64" else args.svd_tol)
         uo = u.cpu().numpy()[idx]
         rec["rel_err_vs_direct"] = float(np.linalg.norm(uo - ud) / np.linalg.norm(ud))
-        forms[prec] = rec
+        forms[name] = rec
+        print(f"# {name}: {rec['Mpts_s']} Mpts/s, {rec['ms_per_step']} ms/step, "
+              f"rel err {rec['rel_err_vs_direct']:.3e}", file=sys.stderr, flush=True)
         del u
         torch.cuda.empty_cache()
-    head = forms.get("mixed") or forms["fp64"]
+    head = forms[names[-1]]
     m = head["m2l"]
     traffic = None
-    tfile = os.path.join(ROOT, "profiles", f"m2l_{head['precision']}_hbm_traffic.json")
+    tfile = os.path.join(ROOT, "profiles", f"m2l_{names[-1]}_hbm_traffic.json")
     if os.path.exists(tfile):
         with open(tfile) as fh:
             traffic = json.load(fh).get("bytes_per_launch")
@@ -213,20 +237,22 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "bf16x3-mfma/f32/f64" if head["precision"] == "mixed" else "f64",
+        "dtype": {"mixed": "bf16x2-split-mfma/f32/f64", "mixed3": "bf16x3-split-mfma/f32/f64"}.get(
+            head["precision"], "f64"),
         "data": "synthetic",
         "config": {"workload": f"KIFMM matvec, N={N} uniform points in the unit cube, forces N(0,1), "
                                f"eps={eps}, mu={mu}, nodeCapacity 64",
                    "N": N, "eps": eps, "nodeCapacity": 64, "nodes": int(tree.nodeCount),
-                   "headline_form": head["precision"], "coronaRes": head["coronaRes"],
+                   "headline_form": names[-1], "coronaRes": head["coronaRes"],
                    "parallelism": "single GPU"},
         "rel_err_vs_direct": head["rel_err_vs_direct"],
         "forms": forms,
         "setup_s": {"tree": round(t_tree, 3)},
         "roofline": roof,
     }
-    if "mixed" in forms and "fp64" in forms:
-        out["speedup_mixed_vs_fp64"] = round(forms["fp64"]["ms_per_step"] / forms["mixed"]["ms_per_step"], 3)
+    for name in names[:-1]:
+        if name.startswith("fp64"):
+            out[f"speedup_vs_{name}"] = round(forms[name]["ms_per_step"] / head["ms_per_step"], 3)
     if not args.no_solve:
         out["mobility_solve_one_sphere"] = sphere_solve(nnk)
     if not args.no_cpu:

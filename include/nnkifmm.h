@@ -158,6 +158,17 @@ int32_t nnk_mobility_solve(nnk_mobility *h, const double *rhs, double tol, int32
                            int32_t precon, double *x, int32_t *flag, double *relres,
                            int32_t *iters, double *resvec);
 
+/* No counterpart in the reference (MATLAB launches nothing): route every
+ *           later nnk_fmm_apply / nnk_fmm_apply_device call (on = 1) through
+ *           a HIP graph of the whole matvec, captured on the first call and
+ *           replayed with one launch; results are bit-identical to on = 0. */
+int32_t nnk_fmm_set_graph(nnk_fmm *h, int32_t on);
+
+/* No counterpart in the reference: the same for the GMRES operator (A or
+ *           A P) of nnk_mobility_solve / nnk_mobility_solve_refined, one
+ *           graph per operator form, captured on first use. */
+int32_t nnk_mobility_set_graph(nnk_mobility *h, int32_t on);
+
 /* No counterpart in the reference (which solves in FP64 only): the same
  *           solve by iterative refinement.  Outer residuals use the FP64
  *           operator of nnk_mobility_create; each correction is a GMRES solve

@@ -55,7 +55,7 @@ int32_t apply_host(nnk_fmm *h, const double *f, double *u, bool reduced) {
   NNK_HIP(hipMemcpy(df.p, f, sizeof(double) * 3 * h->npot, hipMemcpyHostToDevice));
   du.zero();
   if (reduced) h->f->reduced_apply(df.as<double>(), du.as<double>(), 0);
-  else h->f->apply(df.as<double>(), du.as<double>(), 0);
+  else h->apply(df.as<double>(), du.as<double>(), 0);
   NNK_HIP(hipMemcpy(u, du.p, sizeof(double) * 3 * h->ntgt, hipMemcpyDeviceToHost));
   return NNK_OK;
 }
@@ -161,7 +161,7 @@ int32_t nnk_fmm_apply(nnk_fmm *h, const double *f, double *u) {
 int32_t nnk_fmm_apply_device(nnk_fmm *h, const double *d_f, double *d_u, void *stream) {
   return guard([&]() -> int32_t {
     if (!h || (!d_f && h->npot) || (!d_u && h->ntgt)) return fail(NNK_EARG, "nnk_fmm_apply_device: null argument");
-    h->f->apply(d_f, d_u, (hipStream_t)stream);
+    h->apply(d_f, d_u, (hipStream_t)stream);
     return NNK_OK;
   });
 }
@@ -284,6 +284,22 @@ int32_t nnk_mobility_solve(nnk_mobility *h, const double *rhs, double tol, int32
     if (iters) *iters = R.iter;
     if (resvec)
       for (size_t i = 0; i < rv.size() && i <= (size_t)maxit; i++) resvec[i] = rv[i];
+    return NNK_OK;
+  });
+}
+
+int32_t nnk_fmm_set_graph(nnk_fmm *h, int32_t on) {
+  return guard([&]() -> int32_t {
+    if (!h) return fail(NNK_EARG, "nnk_fmm_set_graph: null handle");
+    h->graph = on != 0;
+    return NNK_OK;
+  });
+}
+
+int32_t nnk_mobility_set_graph(nnk_mobility *h, int32_t on) {
+  return guard([&]() -> int32_t {
+    if (!h) return fail(NNK_EARG, "nnk_mobility_set_graph: null handle");
+    h->m->set_graph(on != 0);
     return NNK_OK;
   });
 }

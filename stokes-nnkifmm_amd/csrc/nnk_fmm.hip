@@ -337,6 +337,22 @@ __global__ void k_split_reduce(const int32_t *__restrict__ slot, int M, int ldc,
   }
 }
 
+void tile_lists(const std::vector<int32_t> &gidx, int nb, int N, int BN, std::vector<int32_t> &ptr,
+                std::vector<int32_t> &list) {
+  const int nct = (N + BN - 1) / BN;
+  ptr.assign(nct + 1, 0);
+  list.clear();
+  for (int ct = 0; ct < nct; ct++) {
+    const int c0 = ct * BN, c1 = std::min(N, c0 + BN);
+    for (int b = 0; b < nb; b++) {
+      const int32_t *g = gidx.data() + (size_t)b * N;
+      for (int c = c0; c < c1; c++)
+        if (g[c] >= 0) { list.push_back(b); break; }
+    }
+    ptr[ct + 1] = (int32_t)list.size();
+  }
+}
+
 int Fmm::split_for(int64_t tiles, int nb) {
   // aim at ~4 workgroups per CU, with at least 4 operator blocks per split
   const int64_t want = 1024;
@@ -379,8 +395,7 @@ void Fmm::gemm(const GemmBatch &b, const double *X, int ldx, double *C, int ldc,
     return;
   }
   if (tag == 1) {
-    if (m2l_cfg_ == 1) launch_gemm<96, 1>(nrb, K_, K_, Mpad_, Kpad_, X, ldx, C, ldc, beta, b, s);
-    else launch_gemm<64, 1>(nrb, K_, K_, Mpad_, Kpad_, X, ldx, C, ldc, beta, b, s);
+    launch_gemm<64, 1>(nrb, K_, K_, Mpad_, Kpad_, X, ldx, C, ldc, beta, b, s);
   } else {
     launch_gemm<64, 0>(nrb, K_, K_, Mpad_, Kpad_, X, ldx, C, ldc, beta, b, s);
   }
@@ -407,7 +422,6 @@ Fmm::Fmm(std::shared_ptr<const Tree> t, double eps, double mu, int res, int shel
   nq_ = (int)codes.size();
   K_ = 3 * nq_;
   Kpad_ = (K_ + GBK - 1) / GBK * GBK;
-  if (const char *e = getenv("NNK_M2L_TILE")) m2l_cfg_ = atoi(e);
   Mpad_ = (K_ + GBM - 1) / GBM * GBM;
   npot_ = T.npot;
   ntgt_ = T.ntgt;
@@ -759,6 +773,14 @@ void Fmm::plan() {
           }
         }
         add_job(P.m2l, jobs, A, &g, &cls[a], N);
+        {  // active blocks per 64-column tile of the M2L launch
+          std::vector<int32_t> tp, tl;
+          tile_lists(g, nb, N, 64, tp, tl);
+          P.m2l.owned.emplace_back(make_dbuf(tp));
+          jobs.back().tptr = P.m2l.owned.back().as<int32_t>();
+          P.m2l.owned.emplace_back(make_dbuf(tl));
+          jobs.back().tlist = P.m2l.owned.back().as<int32_t>();
+        }
         std::vector<int32_t> gp(N);
         for (int j = 0; j < N; j++) gp[j] = T.parent[clsnode[a][j] - 1] - 1;
         if (l >= 3)  // parent's down density onto the child's up surface
@@ -962,6 +984,21 @@ void Fmm::apply(const double *d_f, double *d_u, hipStream_t s) {
   mark(7, true, s);
   launch_leaf(1, d_u, s);
   mark(7, false, s);
+}
+
+void Fmm::apply_graph(const double *d_f, double *d_u, hipStream_t s) {
+  if (!gop_.ready()) {
+    const bool prof = prof_;
+    prof_ = false;  // no event records inside the graph
+    try {
+      gop_.capture(3 * npot_, 3 * ntgt_, [&](const double *in, double *out, hipStream_t cs) { apply(in, out, cs); });
+    } catch (...) {
+      prof_ = prof;
+      throw;
+    }
+    prof_ = prof;
+  }
+  gop_.run(d_f, d_u, s);
 }
 
 void Fmm::debug_copy(int which, double *host) const {

@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "nnk_common.hpp"
+#include "nnk_graph.hpp"
 #include "nnk_tree.hpp"
 
 namespace nnk {
@@ -19,7 +20,14 @@ struct GemmJob {
   const int32_t *gidx;     // nb x N gather slots (-1 = none), or null = column index
   const int32_t *cidx;     // N output slots, or null = column index
   int nb, N;
+  // optional, per column tile ct: the operator blocks with a source in the
+  // tile, in increasing order, tlist[tptr[ct] .. tptr[ct+1])
+  const int32_t *tptr = nullptr, *tlist = nullptr;
 };
+
+// the column-tile active lists of a gather table (nb x N, -1 = none)
+void tile_lists(const std::vector<int32_t> &gidx, int nb, int N, int BN, std::vector<int32_t> &ptr,
+                std::vector<int32_t> &list);
 
 struct GemmBatch {
   DBuf jobs;                 // GemmJob[n]
@@ -89,6 +97,9 @@ class Fmm {
   Fmm &operator=(const Fmm &) = delete;
   // f: npot x 3 (potential order), u: ntgt x 3 (target order); device ptrs.
   void apply(const double *d_f, double *d_u, hipStream_t s);
+  // the same matvec replayed from a HIP graph captured on first use (one
+  // launch per call instead of one per kernel); bit-identical to apply
+  void apply_graph(const double *d_f, double *d_u, hipStream_t s);
   // leaf-local product (reducedPass.m) on device pointers
   void reduced_apply(const double *d_f, double *d_u, hipStream_t s);
   // leaf-block solve (ReducedSolve.m): f = blockdiag(K_leaf)^-1 v, with the
@@ -171,8 +182,8 @@ class Fmm {
   struct RedArgs red_args() const;
   int red_maxn_ = 0;
   int64_t stats_[kStats] = {0};
-  int m2l_cfg_ = 0;  // M2L tile: 0 = 160x64, 1 = 160x96
   bool prof_ = false;
+  GraphOp gop_;
   struct Mark { int phase; hipEvent_t a, b; };
   std::vector<Mark> marks_;
   hipEvent_t open_ = nullptr;

@@ -66,33 +66,42 @@ __global__ __launch_bounds__(256, 2) void k_gemm_gather(int M, int K, int Mpad, 
   const int lr = lane & 15, lk = lane >> 4;
 
   // ---- operator blocks with at least one source column in this tile ----
-  if (tid == 0) nact_s = 0;
-  __syncthreads();
-  for (int b = tid; b < J.nb; b += 256) {
-    bool any = false;
-    if (J.gidx) {
-      const int32_t *g = J.gidx + (size_t)b * J.N;
-      for (int n = 0; n < BN && !any; n++)
-        if (col0 + n < J.N && g[col0 + n] >= 0) any = true;
-    } else {
-      any = true;
+  int nact;
+  if (J.tptr) {  // precomputed per tile
+    const int t0 = J.tptr[ct];
+    nact = J.tptr[ct + 1] - t0;
+    for (int i = tid; i < nact; i += 256) act[i] = J.tlist[t0 + i];
+    __syncthreads();
+  } else {
+    if (tid == 0) nact_s = 0;
+    __syncthreads();
+    for (int b = tid; b < J.nb; b += 256) {
+      bool any = false;
+      if (J.gidx) {
+        const int32_t *g = J.gidx + (size_t)b * J.N;
+        for (int n = 0; n < BN && !any; n++)
+          if (col0 + n < J.N && g[col0 + n] >= 0) any = true;
+      } else {
+        any = true;
+      }
+      if (any) {
+        int slot = atomicAdd(&nact_s, 1);
+        act[slot] = b;
+      }
     }
-    if (any) {
-      int slot = atomicAdd(&nact_s, 1);
-      act[slot] = b;
+    __syncthreads();
+    nact = nact_s;
+    // keep the walk in block order (deterministic summation order); only
+    // small jobs come here (at most 8 blocks)
+    if (tid == 0) {
+      for (int i = 1; i < nact; i++) {
+        int v = act[i], j = i - 1;
+        while (j >= 0 && act[j] > v) { act[j + 1] = act[j]; j--; }
+        act[j + 1] = v;
+      }
     }
+    __syncthreads();
   }
-  __syncthreads();
-  const int nact = nact_s;
-  // keep the walk in block order (deterministic summation order)
-  if (tid == 0) {
-    for (int i = 1; i < nact; i++) {
-      int v = act[i], j = i - 1;
-      while (j >= 0 && act[j] > v) { act[j + 1] = act[j]; j--; }
-      act[j + 1] = v;
-    }
-  }
-  __syncthreads();
 
   d4 acc[TM][TN];
 #pragma unroll

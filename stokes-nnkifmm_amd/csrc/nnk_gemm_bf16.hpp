@@ -49,6 +49,7 @@ struct GemmJobBf16 {
   const int32_t *gidx;       // nb x N gather slots (-1 = none)
   const int32_t *cidx;       // N output slots
   int nb, N;
+  const int32_t *tptr, *tlist;  // per 128-column tile: its active blocks, in order
 };
 
 template <int NP, int D, int TAG>
@@ -63,7 +64,6 @@ __global__ __launch_bounds__(512, 1) void k_m2l_bf16(const int M, const int KC, 
   __shared__ __attribute__((aligned(16))) uint16_t Xs[2][NP][8][TILE];
   __shared__ int act[512];
   __shared__ const uint16_t *aptr[512];
-  __shared__ int nact_s;
 
   const int bid = blockIdx.x;
   const int jb = bid % njobs;
@@ -79,25 +79,13 @@ __global__ __launch_bounds__(512, 1) void k_m2l_bf16(const int M, const int KC, 
   const int lr = lane & 15, lg = lane >> 4;
 
   // ---- operator blocks with at least one source column in this tile ----
-  if (tid == 0) nact_s = 0;
-  __syncthreads();
-  for (int b = tid; b < J.nb; b += 512) {
-    bool any = false;
-    const int32_t *g = J.gidx + (size_t)b * J.N;
-    for (int n = 0; n < BN && !any; n++)
-      if (col0 + n < J.N && g[col0 + n] >= 0) any = true;
-    if (any) act[atomicAdd(&nact_s, 1)] = b;
+  const int t0 = J.tptr[ct];
+  const int nact = J.tptr[ct + 1] - t0;
+  for (int i = tid; i < nact; i += 512) {
+    const int b = J.tlist[t0 + i];
+    act[i] = b;
+    aptr[i] = J.A[b];
   }
-  __syncthreads();
-  const int nact = nact_s;
-  if (tid == 0)  // block order: a fixed summation order
-    for (int i = 1; i < nact; i++) {
-      int v = act[i], j = i - 1;
-      while (j >= 0 && act[j] > v) { act[j + 1] = act[j]; j--; }
-      act[j + 1] = v;
-    }
-  __syncthreads();
-  for (int i = tid; i < nact; i += 512) aptr[i] = J.A[act[i]];
   __syncthreads();
   const int a0 = (int)((int64_t)nact * sp / nsplit), a1 = (int)((int64_t)nact * (sp + 1) / nsplit);
   const int total = (a1 - a0) * KC;

@@ -311,6 +311,16 @@ void Fmm::plan_mixed() {
       J.cidx = P.mx_owned.back().as<int32_t>();
       J.nb = (int)ops.size();
       J.N = (int)H.cls[a].size();
+      {
+        std::vector<int32_t> tp, tl;
+        tile_lists(H.gv[a], J.nb, J.N, MBN, tp, tl);
+        P.mx_owned.emplace_back();
+        P.mx_owned.back().upload(tp);
+        J.tptr = P.mx_owned.back().as<int32_t>();
+        P.mx_owned.emplace_back();
+        P.mx_owned.back().upload(tl);
+        J.tlist = P.mx_owned.back().as<int32_t>();
+      }
       mj.push_back(J);
       P.mx_m2l_maxN = std::max(P.mx_m2l_maxN, J.N);
       if (l >= 3) {

@@ -282,6 +282,18 @@ void Mobility::op(const double *x, double *y, bool use_precon, hipStream_t s) {
   apply(px_.as<double>(), y, s);
 }
 
+void Mobility::op_any(const double *x, double *y, bool use_precon, hipStream_t s) {
+  if (!graph_) { op(x, y, use_precon, s); return; }
+  GraphOp &G = gops_[(use_precon ? 1 : 0) + (Fcur_ ? 2 : 0)];
+  if (!G.ready()) {
+    op(x, y, use_precon, s);  // eager first call: settles lazy set-up (leaf LU, buffers)
+    NNK_HIP(hipStreamSynchronize(s));
+    G.capture(n_, n_, [&](const double *in, double *out, hipStream_t cs) { op(in, out, use_precon, cs); });
+    return;
+  }
+  G.run(x, y, s);
+}
+
 double Mobility::dot(const double *a, const double *b, hipStream_t s) {
   const int nb = 256;
   if (!part_.p || part_.bytes < sizeof(double) * nb) part_.alloc(sizeof(double) * nb);
@@ -337,7 +349,7 @@ GmresResult Mobility::gmres(const double *b, double *x, double tol, int maxit, b
     NNK_HIP(hipGetLastError());
   };
   for (k = 0; k < m && !done; k++) {
-    op(Vp + (size_t)k * n, w.as<double>(), use_precon, s);
+    op_any(Vp + (size_t)k * n, w.as<double>(), use_precon, s);
     project(k + 1, h);
     project(k + 1, h2);
     for (int j = 0; j <= k; j++) h[j] += h2[j];
@@ -376,7 +388,7 @@ GmresResult Mobility::gmres(const double *b, double *x, double tol, int maxit, b
                      w.as<double>());
   hipLaunchKernelGGL(k_axpby, dim3(nblk(n)), dim3(256), 0, s, n, -1.0, w.as<double>(), 0.0, x);
   NNK_HIP(hipGetLastError());
-  op(x, w.as<double>(), use_precon, s);
+  op_any(x, w.as<double>(), use_precon, s);
   hipLaunchKernelGGL(k_axpby, dim3(nblk(n)), dim3(256), 0, s, n, 1.0, b, -1.0, w.as<double>());
   R.relres = std::sqrt(dot(w.as<double>(), w.as<double>(), s)) / bnorm;
   R.iter = kk;

@@ -44,10 +44,17 @@ class Mobility {
   // iteration count in iter and the outer count in outer.
   GmresResult refine(const double *b, double *x, double tol, int maxit, double inner_tol, bool use_precon,
                      hipStream_t s, std::vector<double> *resvec, int *outer);
-  void set_inner(Fmm *inner) { Fi_ = inner; }
+  void set_inner(Fmm *inner) {
+    if (inner != Fi_) { gops_[2].reset(); gops_[3].reset(); }
+    Fi_ = inner;
+  }
+  // replay the GMRES operator (A or A P) from HIP graphs, one per
+  // (preconditioned, inner operator) combination, captured on first use
+  void set_graph(bool on) { graph_ = on; }
 
  private:
   void op(const double *x, double *y, bool use_precon, hipStream_t s);
+  void op_any(const double *x, double *y, bool use_precon, hipStream_t s);
   void kifmm(Fmm *Fm, const double *f, double *u, bool accumulate, hipStream_t s);
   void reduced(const double *v, double *f, hipStream_t s);
   void small_solve(const DBuf &lu, const DBuf &piv, const double *b, double *x, double scale,
@@ -60,6 +67,8 @@ class Mobility {
   DevCsr B_, Bt_, BT_, BTt_;
   DBuf A1lu_, A1piv_, A2lu_, A2piv_;
   DBuf aos_a_, aos_b_, t3a_, t3b_, s6a_, s6b_, px_, part_;
+  bool graph_ = false;
+  GraphOp gops_[4];
 };
 
 }  // namespace nnk

@@ -78,6 +78,8 @@ class _MobHandle:
                                         ptr(arrs[6]), ptr(arrs[7]), ctypes.byref(h)))
         self.h = h
         self.ids = (id(B), id(BT))
+        if fmm.arguments.get("hipGraph"):
+            check(lib().nnk_mobility_set_graph(h, 1))
 
     def __del__(self):
         if getattr(self, "h", None):
