@@ -30,6 +30,7 @@ extern "C" {
 typedef struct nnk_tree nnk_tree;
 typedef struct nnk_fmm nnk_fmm;
 typedef struct nnk_mobility nnk_mobility;
+typedef struct nnk_batch nnk_batch;
 
 enum {
   NNK_OK = 0,
@@ -208,6 +209,53 @@ int32_t nnk_fmm_phase_times(nnk_fmm *h, double *ms, int64_t *count, int32_t npha
  * the density length 3*Nq in out[12] if nstats > 12 and the pseudo-inverse
  * rank R of the mixed form in out[13] if nstats > 13. */
 int32_t nnk_fmm_stats(const nnk_fmm *h, int64_t *out, int32_t nstats);
+
+/* Batched mobility problems and their adjoints.  No counterpart in the
+ * reference, which solves one problem per mobilityProblem call
+ * (Mobility/mobilityProblem.m:24-72): each problem here is that call's
+ * system for its own bodies (and optional fixed boundary), with S the
+ * direct regularized-Stokeslet sum from the problem's quadrature points
+ * (nearest map) onto its force points -- what the KIFMM reduces to when the
+ * tree is the root and 8 leaves.  Problem p owns force points
+ * [pt_off[p], pt_off[p+1]), quadrature points [fine_off[p], fine_off[p+1])
+ * (world coordinates, n x 3) and bodies [body_off[p], body_off[p+1]).  The
+ * nearest map is CSR over quadrature points (nn_ptr: Q+1 entries, nn_col:
+ * global force-point index in the same problem).  Per force point: weight
+ * (NN column sum), arm (NN-weighted rotated quadrature position) and rot
+ * (rotated, untranslated position), as genMatrixComponets forms AF, AM and
+ * AOm (Mobility/genMatrixComponets.m:34-76), and pt_body (body index local
+ * to the problem, -1 for boundary points; a body's points are contiguous).
+ * eps: per problem.  precision NNK_FP64 or NNK_MIXED (FP32 kernel
+ * evaluation, FP64 accumulation).  Vectors concatenate the problems, each
+ * [f (3 N_p: all x, all y, all z); U (3 nsw_p: all x, all y, all z);
+ * Omega (3 nsw_p)] as in mobilityProblem; nnk_batch_size gives the total. */
+int32_t nnk_batch_create(int32_t nprob, const int64_t *pt_off, const double *pts,
+                         const int64_t *fine_off, const double *fine, const int64_t *nn_ptr,
+                         const int32_t *nn_col, const double *nn_val, const double *weight,
+                         const double *arm, const double *rot, const int32_t *pt_body,
+                         const int64_t *body_off, const double *eps, double mu, int32_t precision,
+                         nnk_batch **out);
+int32_t nnk_batch_size(const nnk_batch *h, int64_t *n);
+/* y = A x for every problem (mobilityMatrixMulti, MobilityMatrixMulti.m:17-36,
+ * per problem); the _transpose form applies A^T. */
+int32_t nnk_batch_apply(nnk_batch *h, const double *x, double *y);
+int32_t nnk_batch_apply_transpose(nnk_batch *h, const double *x, double *y);
+int32_t nnk_batch_apply_device(nnk_batch *h, const double *d_x, double *d_y, void *stream);
+/* on = 1: replay the batched operators from HIP graphs captured on first use. */
+int32_t nnk_batch_set_graph(nnk_batch *h, int32_t on);
+/* Unrestarted GMRES from x = 0 on every problem at once, on A (transpose =
+ * 0) or A^T (transpose = 1): lockstep iterations, a problem stops when its
+ * residual estimate reaches tol.  flag, relres (true relative residual),
+ * iters: nprob entries each. */
+int32_t nnk_batch_solve(nnk_batch *h, const double *rhs, double tol, int32_t maxit, int32_t transpose,
+                        double *x, int32_t *flag, double *relres, int32_t *iters);
+/* Adjoint gradient: with A z = b and A^T lambda = dJ/dz, writes
+ * dJ/dtheta = -d(lambda^T A z)/dtheta for the force-point positions (gx,
+ * 3 per point), the quadrature-point positions (gy, 3 per point) and eps
+ * (geps, one per problem), holding b and the nearest map fixed. */
+int32_t nnk_batch_gradient(nnk_batch *h, const double *z, const double *lambda, double *gx, double *gy,
+                           double *geps);
+void nnk_batch_destroy(nnk_batch *h);
 
 #ifdef __cplusplus
 }

@@ -5,6 +5,7 @@ Classes/KIFMM.m) over the C ABI of libnnkifmm (include/nnkifmm.h).
 Import as ``importlib.import_module("stokes-nnkifmm_amd")``.
 """
 from ._lib import NNKError, lib  # noqa: F401
+from .batch import BatchedMobility  # noqa: F401
 from .kifmm import KIFMM  # noqa: F401
 from .mobility import (genMatrixComponets, mobility_gmres, mobility_refined, mobilityMatrixMulti,  # noqa: F401
                        mobilityPrecon,

@@ -62,6 +62,17 @@ SIGNATURES = {
     "nnk_fmm_set_graph": (ctypes.c_int32, [_vp, ctypes.c_int32]),
     "nnk_mobility_set_graph": (ctypes.c_int32, [_vp, ctypes.c_int32]),
     "nnk_mobility_destroy": (None, [_vp]),
+    "nnk_batch_create": (ctypes.c_int32, [ctypes.c_int32, _i64, _d, _i64, _d, _i64, _i32, _d, _d, _d, _d, _i32,
+                                          _i64, _d, ctypes.c_double, ctypes.c_int32, _vp]),
+    "nnk_batch_size": (ctypes.c_int32, [_vp, _i64]),
+    "nnk_batch_apply": (ctypes.c_int32, [_vp, _d, _d]),
+    "nnk_batch_apply_transpose": (ctypes.c_int32, [_vp, _d, _d]),
+    "nnk_batch_apply_device": (ctypes.c_int32, [_vp, _vp, _vp, _vp]),
+    "nnk_batch_set_graph": (ctypes.c_int32, [_vp, ctypes.c_int32]),
+    "nnk_batch_solve": (ctypes.c_int32, [_vp, _d, ctypes.c_double, ctypes.c_int32, ctypes.c_int32, _d, _i32, _d,
+                                         _i32]),
+    "nnk_batch_gradient": (ctypes.c_int32, [_vp, _d, _d, _d, _d, _d]),
+    "nnk_batch_destroy": (None, [_vp]),
 }
 
 _lib = None

@@ -1,0 +1,906 @@
+// Batched mobility problems and their adjoints (see nnk_batch.hpp).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <numeric>
+#include <vector>
+
+#include "nnk_batch.hpp"
+#include "nnk_kernels.hpp"
+
+namespace nnk {
+
+namespace {
+
+inline unsigned nblk(int64_t n, int bs = 256) { return (unsigned)((n + bs - 1) / bs); }
+
+constexpr int kTile = 64;  // targets per P2P work item
+
+// slots of body lb's rigid-body unknowns (or rows) in problem p's segment
+struct Slots {
+  int64_t base, np;
+  int nsw;
+  __device__ int64_t u(int d, int lb) const { return base + 3 * np + d * nsw + lb; }
+  __device__ int64_t w(int d, int lb) const { return base + 3 * np + 3 * nsw + d * nsw + lb; }
+  __device__ int64_t f(int d, int64_t loc) const { return base + d * np + loc; }
+};
+
+__device__ __forceinline__ Slots slots(int p, const int64_t *poff, const int64_t *voff, const int32_t *nsw) {
+  return Slots{voff[p], poff[p + 1] - poff[p], nsw[p]};
+}
+
+// d/dr and d/deps of c lambda^T S(r) g, S(r) = [(r^2+2e^2) I + r r^T] / s^3,
+// s^2 = r^2 + e^2:
+//   grad = c [(2 r (l.g) + l (r.g) + g (l.r)) / s^3 - 3 N r / s^5]
+//   deps = c [4 e (l.g) / s^3 - 3 N e / s^5],  N = (r^2+2e^2)(l.g) + (l.r)(r.g)
+template <bool F32>
+__device__ __forceinline__ void grad_acc(double rxd, double ryd, double rzd, double lx, double ly, double lz,
+                                         double gx, double gy, double gz, double ev, double &Gx, double &Gy,
+                                         double &Gz, double &E) {
+  if constexpr (F32) {
+    const float rx = (float)rxd, ry = (float)ryd, rz = (float)rzd, e = (float)ev;
+    const float Lx = (float)lx, Ly = (float)ly, Lz = (float)lz, Fx = (float)gx, Fy = (float)gy, Fz = (float)gz;
+    const float r2 = rx * rx + ry * ry + rz * rz, s2 = r2 + e * e;
+    const float is = __builtin_amdgcn_rsqf(s2), is3 = is * is * is, is5 = is3 * is * is;
+    const float lg = Lx * Fx + Ly * Fy + Lz * Fz, lr = Lx * rx + Ly * ry + Lz * rz, rg = rx * Fx + ry * Fy + rz * Fz;
+    const float N = (r2 + 2.0f * e * e) * lg + lr * rg;
+    const float a = 2.0f * lg * is3 - 3.0f * N * is5;
+    Gx += (double)(a * rx + (Lx * rg + Fx * lr) * is3);
+    Gy += (double)(a * ry + (Ly * rg + Fy * lr) * is3);
+    Gz += (double)(a * rz + (Lz * rg + Fz * lr) * is3);
+    E += (double)(e * (4.0f * lg * is3 - 3.0f * N * is5));
+  } else {
+    const double rx = rxd, ry = ryd, rz = rzd, e = ev;
+    const double r2 = rx * rx + ry * ry + rz * rz, s2 = r2 + e * e;
+    const double is = rsqrt(s2), is3 = is * is * is, is5 = is3 * is * is;
+    const double lg = lx * gx + ly * gy + lz * gz, lr = lx * rx + ly * ry + lz * rz, rg = rx * gx + ry * gy + rz * gz;
+    const double N = (r2 + 2.0 * e * e) * lg + lr * rg;
+    const double a = 2.0 * lg * is3 - 3.0 * N * is5;
+    Gx += a * rx + (lx * rg + gx * lr) * is3;
+    Gy += a * ry + (ly * rg + gy * lr) * is3;
+    Gz += a * rz + (lz * rg + gz * lr) * is3;
+    E += e * (4.0 * lg * is3 - 3.0 * N * is5);
+  }
+}
+
+}  // namespace
+
+// g[q] = sum_e NN(q, col e) f(col e)    (the quadrature strengths, 3 each)
+__global__ void k_bgather(int64_t qtot, const int64_t *__restrict__ ptr, const int32_t *__restrict__ col,
+                          const double *__restrict__ val, const double *__restrict__ x,
+                          const int64_t *__restrict__ xbase, const int64_t *__restrict__ xstride,
+                          double *__restrict__ g) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= qtot) return;
+  double gx = 0, gy = 0, gz = 0;
+  for (int64_t e = ptr[q]; e < ptr[q + 1]; e++) {
+    const int j = col[e];
+    const double v = val[e];
+    const int64_t b = xbase[j], st = xstride[j];
+    gx += v * x[b];
+    gy += v * x[b + st];
+    gz += v * x[b + 2 * st];
+  }
+  g[3 * q] = gx;
+  g[3 * q + 1] = gy;
+  g[3 * q + 2] = gz;
+}
+
+__global__ void k_bcounter_reset(int *c) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *c = 0;
+}
+
+// Common frame of the persistent kernels: each workgroup takes work items
+// (problem, 64-target tile) from a counter until none is left; the tile's
+// targets sit on lanes, the four waves split every LDS-staged block of 256
+// sources and their partial sums are added in a fixed order.
+#define NNK_PERSISTENT_LOOP_BEGIN                                   \
+  for (;;) {                                                        \
+    if (tid == 0) item_s = atomicAdd(counter, 1);                   \
+    __syncthreads();                                                \
+    const int it = item_s;                                          \
+    __syncthreads(); /* item_s is rewritten only after all read */  \
+    if (it >= nitems) break;                                        \
+    const int p = items[it].x, tile = items[it].y;                  \
+    if (!active[p]) continue;
+
+#define NNK_PERSISTENT_LOOP_END \
+  __syncthreads();              \
+  }
+
+// forward P2P: u_i = c sum_q S(x_i - y_q) g_q, then + BT (U; W) on body points
+template <bool F32>
+__global__ __launch_bounds__(256) void k_bp2p(int nitems, const int2 *__restrict__ items, int *counter,
+                                              const int64_t *__restrict__ poff, const int64_t *__restrict__ foff,
+                                              const int64_t *__restrict__ voff, const int32_t *__restrict__ nsw,
+                                              const double *__restrict__ pts, const double *__restrict__ fine,
+                                              const double *__restrict__ g, const double *__restrict__ rot,
+                                              const int32_t *__restrict__ pbody, const double *__restrict__ eps,
+                                              const int32_t *__restrict__ active, double c,
+                                              const double *__restrict__ x, double *__restrict__ y) {
+  __shared__ double sp[256 * 6];
+  __shared__ double red[4][kTile * 3];
+  __shared__ int item_s;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  NNK_PERSISTENT_LOOP_BEGIN
+  const int64_t pb = poff[p], Np = poff[p + 1] - pb;
+  const int64_t i0 = pb + (int64_t)kTile * tile;
+  const int nt = (int)min((int64_t)kTile, Np - (int64_t)kTile * tile);
+  double tx = 0, ty = 0, tz = 0;
+  if (lane < nt) {
+    tx = pts[3 * (i0 + lane)];
+    ty = pts[3 * (i0 + lane) + 1];
+    tz = pts[3 * (i0 + lane) + 2];
+  }
+  const double e2 = eps[p] * eps[p];
+  double ux = 0, uy = 0, uz = 0;
+  const int64_t qb = foff[p], qe = foff[p + 1];
+  for (int64_t q0 = qb; q0 < qe; q0 += 256) {
+    const int cnt = (int)min((int64_t)256, qe - q0);
+    if (tid < cnt) {
+      const int64_t q = q0 + tid;
+      for (int d = 0; d < 3; d++) {
+        sp[6 * tid + d] = fine[3 * q + d];
+        sp[6 * tid + 3 + d] = g[3 * q + d];
+      }
+    }
+    __syncthreads();
+    if (lane < nt)
+      for (int j = wv; j < cnt; j += 4)
+        stk<F32>(tx, ty, tz, sp[6 * j], sp[6 * j + 1], sp[6 * j + 2], sp[6 * j + 3], sp[6 * j + 4], sp[6 * j + 5],
+                 e2, ux, uy, uz);
+    __syncthreads();
+  }
+  red[wv][3 * lane] = ux;
+  red[wv][3 * lane + 1] = uy;
+  red[wv][3 * lane + 2] = uz;
+  __syncthreads();
+  if (wv == 0 && lane < nt) {
+    double u[3];
+    for (int d = 0; d < 3; d++)
+      u[d] = c * (((red[0][3 * lane + d] + red[1][3 * lane + d]) + red[2][3 * lane + d]) + red[3][3 * lane + d]);
+    const Slots S = slots(p, poff, voff, nsw);
+    const int64_t i = i0 + lane, loc = i - pb;
+    const int b = pbody[i];
+    if (b >= 0) {
+      const double rx = rot[3 * i], ry = rot[3 * i + 1], rz = rot[3 * i + 2];
+      const double wx = x[S.w(0, b)], wy = x[S.w(1, b)], wz = x[S.w(2, b)];
+      u[0] -= x[S.u(0, b)] + (wy * rz - wz * ry);
+      u[1] -= x[S.u(1, b)] + (wz * rx - wx * rz);
+      u[2] -= x[S.u(2, b)] + (wx * ry - wy * rx);
+    }
+    for (int d = 0; d < 3; d++) y[S.f(d, loc)] = u[d];
+  }
+  NNK_PERSISTENT_LOOP_END
+}
+
+// B f per body: [sum_j w_j f_j ; sum_j pm_j x f_j] into the body's rows
+__global__ __launch_bounds__(256) void k_bB(const int32_t *__restrict__ bprob, const int32_t *__restrict__ blocal,
+                                            const int64_t *__restrict__ bpt, const int64_t *__restrict__ poff,
+                                            const int64_t *__restrict__ voff, const int32_t *__restrict__ nsw,
+                                            const double *__restrict__ w, const double *__restrict__ pm,
+                                            const int32_t *__restrict__ active, const double *__restrict__ x,
+                                            double *__restrict__ y) {
+  const int B = blockIdx.x, p = bprob[B], lb = blocal[B];
+  if (!active[p]) return;
+  __shared__ double red[6][256];
+  const Slots S = slots(p, poff, voff, nsw);
+  const int64_t pb = poff[p];
+  double s[6] = {0, 0, 0, 0, 0, 0};
+  for (int64_t j = bpt[2 * B] + threadIdx.x; j < bpt[2 * B + 1]; j += 256) {
+    const int64_t loc = j - pb;
+    const double fx = x[S.f(0, loc)], fy = x[S.f(1, loc)], fz = x[S.f(2, loc)];
+    const double wj = w[j], mx = pm[3 * j], my = pm[3 * j + 1], mz = pm[3 * j + 2];
+    s[0] += wj * fx;
+    s[1] += wj * fy;
+    s[2] += wj * fz;
+    s[3] += my * fz - mz * fy;
+    s[4] += mz * fx - mx * fz;
+    s[5] += mx * fy - my * fx;
+  }
+  for (int k = 0; k < 6; k++) red[k][threadIdx.x] = s[k];
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st)
+      for (int k = 0; k < 6; k++) red[k][threadIdx.x] += red[k][threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x < 3) y[S.u(threadIdx.x, lb)] = red[threadIdx.x][0];
+  else if (threadIdx.x < 6) y[S.w(threadIdx.x - 3, lb)] = red[threadIdx.x][0];
+}
+
+// transpose P2P: v_q = c sum_i S(y_q - x_i) l_i over the problem's force
+// points (l = the force-point block of the adjoint vector)
+template <bool F32>
+__global__ __launch_bounds__(256) void k_bp2pT(int nitems, const int2 *__restrict__ items, int *counter,
+                                               const int64_t *__restrict__ poff, const int64_t *__restrict__ foff,
+                                               const int64_t *__restrict__ voff, const double *__restrict__ pts,
+                                               const double *__restrict__ fine, const double *__restrict__ eps,
+                                               const int32_t *__restrict__ active, double c,
+                                               const double *__restrict__ x, double *__restrict__ v) {
+  __shared__ double sp[256 * 6];
+  __shared__ double red[4][kTile * 3];
+  __shared__ int item_s;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  NNK_PERSISTENT_LOOP_BEGIN
+  const int64_t qb = foff[p], Qp = foff[p + 1] - qb;
+  const int64_t q0t = qb + (int64_t)kTile * tile;
+  const int nt = (int)min((int64_t)kTile, Qp - (int64_t)kTile * tile);
+  double tx = 0, ty = 0, tz = 0;
+  if (lane < nt) {
+    tx = fine[3 * (q0t + lane)];
+    ty = fine[3 * (q0t + lane) + 1];
+    tz = fine[3 * (q0t + lane) + 2];
+  }
+  const double e2 = eps[p] * eps[p];
+  const int64_t pb = poff[p], Np = poff[p + 1] - pb, base = voff[p];
+  double ux = 0, uy = 0, uz = 0;
+  for (int64_t s0 = 0; s0 < Np; s0 += 256) {
+    const int cnt = (int)min((int64_t)256, Np - s0);
+    if (tid < cnt) {
+      const int64_t i = pb + s0 + tid;
+      for (int d = 0; d < 3; d++) {
+        sp[6 * tid + d] = pts[3 * i + d];
+        sp[6 * tid + 3 + d] = x[base + d * Np + s0 + tid];
+      }
+    }
+    __syncthreads();
+    if (lane < nt)
+      for (int j = wv; j < cnt; j += 4)
+        stk<F32>(tx, ty, tz, sp[6 * j], sp[6 * j + 1], sp[6 * j + 2], sp[6 * j + 3], sp[6 * j + 4], sp[6 * j + 5],
+                 e2, ux, uy, uz);
+    __syncthreads();
+  }
+  red[wv][3 * lane] = ux;
+  red[wv][3 * lane + 1] = uy;
+  red[wv][3 * lane + 2] = uz;
+  __syncthreads();
+  if (wv == 0 && lane < nt)
+    for (int d = 0; d < 3; d++)
+      v[3 * (q0t + lane) + d] =
+          c * (((red[0][3 * lane + d] + red[1][3 * lane + d]) + red[2][3 * lane + d]) + red[3][3 * lane + d]);
+  NNK_PERSISTENT_LOOP_END
+}
+
+// transpose force-point rows: mu_j = sum_q NN(q,j) v_q + [body b] (w_j lF_b + lM_b x pm_j)
+__global__ void k_bgatherT(int64_t ntot, const int32_t *__restrict__ ppid, const int64_t *__restrict__ tptr,
+                           const int32_t *__restrict__ trow, const double *__restrict__ tval,
+                           const double *__restrict__ v, const int64_t *__restrict__ poff,
+                           const int64_t *__restrict__ voff, const int32_t *__restrict__ nsw,
+                           const double *__restrict__ w, const double *__restrict__ pm,
+                           const int32_t *__restrict__ pbody, const int32_t *__restrict__ active,
+                           const double *__restrict__ x, double *__restrict__ y) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= ntot) return;
+  const int p = ppid[j];
+  if (!active[p]) return;
+  double m[3] = {0, 0, 0};
+  for (int64_t e = tptr[j]; e < tptr[j + 1]; e++) {
+    const int64_t q = trow[e];
+    const double a = tval[e];
+    m[0] += a * v[3 * q];
+    m[1] += a * v[3 * q + 1];
+    m[2] += a * v[3 * q + 2];
+  }
+  const Slots S = slots(p, poff, voff, nsw);
+  const int b = pbody[j];
+  if (b >= 0) {
+    const double wj = w[j], mx = pm[3 * j], my = pm[3 * j + 1], mz = pm[3 * j + 2];
+    const double lx = x[S.w(0, b)], ly = x[S.w(1, b)], lz = x[S.w(2, b)];
+    m[0] += wj * x[S.u(0, b)] + (ly * mz - lz * my);
+    m[1] += wj * x[S.u(1, b)] + (lz * mx - lx * mz);
+    m[2] += wj * x[S.u(2, b)] + (lx * my - ly * mx);
+  }
+  const int64_t loc = j - poff[p];
+  for (int d = 0; d < 3; d++) y[S.f(d, loc)] = m[d];
+}
+
+// transpose rigid-body rows: BT^T l = [-sum_i l_i ; -sum_i r_i x l_i] per body
+__global__ __launch_bounds__(256) void k_bBTt(const int32_t *__restrict__ bprob, const int32_t *__restrict__ blocal,
+                                              const int64_t *__restrict__ bpt, const int64_t *__restrict__ poff,
+                                              const int64_t *__restrict__ voff, const int32_t *__restrict__ nsw,
+                                              const double *__restrict__ rot, const int32_t *__restrict__ active,
+                                              const double *__restrict__ x, double *__restrict__ y) {
+  const int B = blockIdx.x, p = bprob[B], lb = blocal[B];
+  if (!active[p]) return;
+  __shared__ double red[6][256];
+  const Slots S = slots(p, poff, voff, nsw);
+  const int64_t pb = poff[p];
+  double s[6] = {0, 0, 0, 0, 0, 0};
+  for (int64_t j = bpt[2 * B] + threadIdx.x; j < bpt[2 * B + 1]; j += 256) {
+    const int64_t loc = j - pb;
+    const double lx = x[S.f(0, loc)], ly = x[S.f(1, loc)], lz = x[S.f(2, loc)];
+    const double rx = rot[3 * j], ry = rot[3 * j + 1], rz = rot[3 * j + 2];
+    s[0] -= lx;
+    s[1] -= ly;
+    s[2] -= lz;
+    s[3] -= ry * lz - rz * ly;
+    s[4] -= rz * lx - rx * lz;
+    s[5] -= rx * ly - ry * lx;
+  }
+  for (int k = 0; k < 6; k++) red[k][threadIdx.x] = s[k];
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st)
+      for (int k = 0; k < 6; k++) red[k][threadIdx.x] += red[k][threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x < 3) y[S.u(threadIdx.x, lb)] = red[threadIdx.x][0];
+  else if (threadIdx.x < 6) y[S.w(threadIdx.x - 3, lb)] = red[threadIdx.x][0];
+}
+
+// gradient on force points: gx_i = -(sum_q grad_r(l_i^T c S g_q) - l_i x W_b),
+// ge_i = -sum_q d/deps(l_i^T c S g_q)
+template <bool F32>
+__global__ __launch_bounds__(256) void k_bgradx(int nitems, const int2 *__restrict__ items, int *counter,
+                                                const int64_t *__restrict__ poff, const int64_t *__restrict__ foff,
+                                                const int64_t *__restrict__ voff, const int32_t *__restrict__ nsw,
+                                                const double *__restrict__ pts, const double *__restrict__ fine,
+                                                const double *__restrict__ g, const int32_t *__restrict__ pbody,
+                                                const double *__restrict__ eps, const int32_t *__restrict__ active,
+                                                double c, const double *__restrict__ z, const double *__restrict__ lam,
+                                                double *__restrict__ gx, double *__restrict__ ge) {
+  __shared__ double sp[256 * 6];
+  __shared__ double red[4][kTile * 4];
+  __shared__ int item_s;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  NNK_PERSISTENT_LOOP_BEGIN
+  const int64_t pb = poff[p], Np = poff[p + 1] - pb;
+  const int64_t i0 = pb + (int64_t)kTile * tile;
+  const int nt = (int)min((int64_t)kTile, Np - (int64_t)kTile * tile);
+  const Slots S = slots(p, poff, voff, nsw);
+  double tx = 0, ty = 0, tz = 0, lx = 0, ly = 0, lz = 0;
+  if (lane < nt) {
+    const int64_t i = i0 + lane, loc = i - pb;
+    tx = pts[3 * i];
+    ty = pts[3 * i + 1];
+    tz = pts[3 * i + 2];
+    lx = lam[S.f(0, loc)];
+    ly = lam[S.f(1, loc)];
+    lz = lam[S.f(2, loc)];
+  }
+  const double ev = eps[p];
+  double Gx = 0, Gy = 0, Gz = 0, E = 0;
+  const int64_t qb = foff[p], qe = foff[p + 1];
+  for (int64_t q0 = qb; q0 < qe; q0 += 256) {
+    const int cnt = (int)min((int64_t)256, qe - q0);
+    if (tid < cnt) {
+      const int64_t q = q0 + tid;
+      for (int d = 0; d < 3; d++) {
+        sp[6 * tid + d] = fine[3 * q + d];
+        sp[6 * tid + 3 + d] = g[3 * q + d];
+      }
+    }
+    __syncthreads();
+    if (lane < nt)
+      for (int j = wv; j < cnt; j += 4)
+        grad_acc<F32>(tx - sp[6 * j], ty - sp[6 * j + 1], tz - sp[6 * j + 2], lx, ly, lz, sp[6 * j + 3],
+                      sp[6 * j + 4], sp[6 * j + 5], ev, Gx, Gy, Gz, E);
+    __syncthreads();
+  }
+  red[wv][4 * lane] = Gx;
+  red[wv][4 * lane + 1] = Gy;
+  red[wv][4 * lane + 2] = Gz;
+  red[wv][4 * lane + 3] = E;
+  __syncthreads();
+  if (wv == 0 && lane < nt) {
+    double G[4];
+    for (int d = 0; d < 4; d++)
+      G[d] = c * (((red[0][4 * lane + d] + red[1][4 * lane + d]) + red[2][4 * lane + d]) + red[3][4 * lane + d]);
+    const int64_t i = i0 + lane;
+    const int b = pbody[i];
+    if (b >= 0) {  // d/dx_i of -l_i . (W x r_i) is -(l_i x W)
+      const double wx = z[S.w(0, b)], wy = z[S.w(1, b)], wz = z[S.w(2, b)];
+      G[0] -= ly * wz - lz * wy;
+      G[1] -= lz * wx - lx * wz;
+      G[2] -= lx * wy - ly * wx;
+    }
+    for (int d = 0; d < 3; d++) gx[3 * i + d] = -G[d];
+    ge[i] = -G[3];
+  }
+  NNK_PERSISTENT_LOOP_END
+}
+
+// gradient on quadrature points: gy_q = sum_i grad_r(l_i^T c S(x_i - y_q) g_q)
+// - g_q x lM_b (the moment rows of B hold (y_q - x0) x g_q)
+template <bool F32>
+__global__ __launch_bounds__(256) void k_bgrady(int nitems, const int2 *__restrict__ items, int *counter,
+                                                const int64_t *__restrict__ poff, const int64_t *__restrict__ foff,
+                                                const int64_t *__restrict__ voff, const int32_t *__restrict__ nsw,
+                                                const double *__restrict__ pts, const double *__restrict__ fine,
+                                                const double *__restrict__ g, const int32_t *__restrict__ fbody,
+                                                const double *__restrict__ eps, const int32_t *__restrict__ active,
+                                                double c, const double *__restrict__ lam, double *__restrict__ gy) {
+  __shared__ double sp[256 * 6];
+  __shared__ double red[4][kTile * 3];
+  __shared__ int item_s;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  NNK_PERSISTENT_LOOP_BEGIN
+  const int64_t qb = foff[p], Qp = foff[p + 1] - qb;
+  const int64_t q0t = qb + (int64_t)kTile * tile;
+  const int nt = (int)min((int64_t)kTile, Qp - (int64_t)kTile * tile);
+  double yx = 0, yy = 0, yz = 0, gqx = 0, gqy = 0, gqz = 0;
+  if (lane < nt) {
+    const int64_t q = q0t + lane;
+    yx = fine[3 * q];
+    yy = fine[3 * q + 1];
+    yz = fine[3 * q + 2];
+    gqx = g[3 * q];
+    gqy = g[3 * q + 1];
+    gqz = g[3 * q + 2];
+  }
+  const double ev = eps[p];
+  const int64_t pb = poff[p], Np = poff[p + 1] - pb, base = voff[p];
+  double Gx = 0, Gy = 0, Gz = 0, E = 0;
+  for (int64_t s0 = 0; s0 < Np; s0 += 256) {
+    const int cnt = (int)min((int64_t)256, Np - s0);
+    if (tid < cnt) {
+      const int64_t i = pb + s0 + tid;
+      for (int d = 0; d < 3; d++) {
+        sp[6 * tid + d] = pts[3 * i + d];
+        sp[6 * tid + 3 + d] = lam[base + d * Np + s0 + tid];
+      }
+    }
+    __syncthreads();
+    if (lane < nt)
+      for (int j = wv; j < cnt; j += 4)
+        grad_acc<F32>(sp[6 * j] - yx, sp[6 * j + 1] - yy, sp[6 * j + 2] - yz, sp[6 * j + 3], sp[6 * j + 4],
+                      sp[6 * j + 5], gqx, gqy, gqz, ev, Gx, Gy, Gz, E);
+    __syncthreads();
+  }
+  red[wv][3 * lane] = Gx;
+  red[wv][3 * lane + 1] = Gy;
+  red[wv][3 * lane + 2] = Gz;
+  __syncthreads();
+  if (wv == 0 && lane < nt) {
+    double G[3];
+    for (int d = 0; d < 3; d++)
+      G[d] = c * (((red[0][3 * lane + d] + red[1][3 * lane + d]) + red[2][3 * lane + d]) + red[3][3 * lane + d]);
+    const int64_t q = q0t + lane;
+    const int b = fbody[q];
+    if (b >= 0) {
+      const Slots S = slots(p, poff, voff, nsw);
+      const double mx = lam[S.w(0, b)], my = lam[S.w(1, b)], mz = lam[S.w(2, b)];
+      G[0] -= gqy * mz - gqz * my;
+      G[1] -= gqz * mx - gqx * mz;
+      G[2] -= gqx * my - gqy * mx;
+    }
+    for (int d = 0; d < 3; d++) gy[3 * q + d] = G[d];
+  }
+  NNK_PERSISTENT_LOOP_END
+}
+
+#undef NNK_PERSISTENT_LOOP_BEGIN
+#undef NNK_PERSISTENT_LOOP_END
+
+// out[j * nprob + p] = V_j . w over problem p's segment   (grid: nprob x k)
+__global__ __launch_bounds__(256) void k_bdots(int64_t n, int nprob, const int64_t *__restrict__ voff,
+                                               const double *__restrict__ V, const double *__restrict__ w,
+                                               double *__restrict__ out) {
+  const int p = blockIdx.x, j = blockIdx.y;
+  __shared__ double red[256];
+  const double *vj = V + (size_t)j * n;
+  double s = 0;
+  for (int64_t e = voff[p] + threadIdx.x; e < voff[p + 1]; e += 256) s += vj[e] * w[e];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[(size_t)j * nprob + p] = red[0];
+}
+
+// w[e] -= sum_j h[j * nprob + pid[e]] V_j[e]
+__global__ void k_bcombo(int64_t n, int k, int nprob, const double *__restrict__ V, const double *__restrict__ h,
+                         const int32_t *__restrict__ pid, double *__restrict__ w) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  const int p = pid[e];
+  double s = 0;
+  for (int j = 0; j < k; j++) s += h[(size_t)j * nprob + p] * V[(size_t)j * n + e];
+  w[e] -= s;
+}
+
+// dst[e] = a[pid[e]] * src[e] + b * dst[e]
+__global__ void k_bscale(int64_t n, const int32_t *__restrict__ pid, const double *__restrict__ a,
+                         const double *__restrict__ src, double b, double *__restrict__ dst) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  dst[e] = a[pid[e]] * src[e] + (b != 0.0 ? b * dst[e] : 0.0);
+}
+
+// ---------------------------------------------------------------------------
+
+Batch::Batch(int nprob, const int64_t *pt_off, const double *pts, const int64_t *fine_off, const double *fine,
+             const int64_t *nn_ptr, const int32_t *nn_col, const double *nn_val, const double *w,
+             const double *pm, const double *rot, const int32_t *pt_body, const int64_t *body_off,
+             const double *eps, double mu, int f32)
+    : np_(nprob), mu_(mu), f32_(f32) {
+  if (nprob <= 0) throw Error(NNK_EARG, "batch: no problems");
+  if (!(mu > 0)) throw Error(NNK_EARG, "batch: mu must be > 0");
+  hpoff_.assign(pt_off, pt_off + nprob + 1);
+  std::vector<int64_t> hfoff(fine_off, fine_off + nprob + 1), hboff(body_off, body_off + nprob + 1);
+  if (hpoff_[0] != 0 || hfoff[0] != 0 || hboff[0] != 0) throw Error(NNK_EARG, "batch: offsets must start at 0");
+  for (int p = 0; p < nprob; p++) {
+    if (hpoff_[p + 1] <= hpoff_[p] || hfoff[p + 1] < hfoff[p] || hboff[p + 1] < hboff[p])
+      throw Error(NNK_EARG, "batch: bad offsets");
+    if (!(eps[p] > 0)) throw Error(NNK_EARG, "batch: eps must be > 0");
+  }
+  ntot_ = hpoff_[nprob];
+  qtot_ = hfoff[nprob];
+  nbody_ = (int)hboff[nprob];
+  if (ntot_ > INT32_MAX || qtot_ > INT32_MAX) throw Error(NNK_EARG, "batch: too many points");
+  // problems of force points; bodies: contiguous point ranges
+  std::vector<int32_t> ppid(ntot_), bprob(std::max(nbody_, 1)), blocal(std::max(nbody_, 1)), nsw(nprob),
+      pbody(pt_body, pt_body + ntot_);
+  for (int p = 0; p < nprob; p++) {
+    const int ns = (int)(hboff[p + 1] - hboff[p]);
+    nsw[p] = ns;
+    std::vector<int64_t> lo(ns, -1), hi(ns, -1);
+    for (int64_t j = hpoff_[p]; j < hpoff_[p + 1]; j++) {
+      ppid[j] = p;
+      const int b = pbody[j];
+      if (b < -1 || b >= ns) throw Error(NNK_EARG, "batch: point body out of range");
+      if (b < 0) continue;
+      if (lo[b] < 0) lo[b] = j;
+      else if (hi[b] != j) throw Error(NNK_EARG, "batch: the points of a body must be contiguous");
+      hi[b] = j + 1;
+    }
+    for (int b = 0; b < ns; b++) {
+      const int B = (int)hboff[p] + b;
+      bprob[B] = p;
+      blocal[B] = b;
+      if (lo[b] < 0) throw Error(NNK_EARG, "batch: a body without points");
+    }
+  }
+  // point range of each body: [branges[2B], branges[2B+1])
+  std::vector<int64_t> branges(2 * (size_t)std::max(nbody_, 1), 0);
+  for (int p = 0; p < nprob; p++)
+    for (int64_t j = hpoff_[p]; j < hpoff_[p + 1]; j++) {
+      const int b = pbody[j];
+      if (b < 0) continue;
+      const int B = (int)hboff[p] + b;
+      if (branges[2 * B + 1] == 0) branges[2 * B] = j;
+      branges[2 * B + 1] = j + 1;
+    }
+  std::vector<int64_t> hptr(nn_ptr, nn_ptr + qtot_ + 1);
+  if (hptr[0] != 0) throw Error(NNK_EARG, "batch: NN row pointer must start at 0");
+  std::vector<int32_t> fbody(qtot_, -1);
+  for (int p = 0; p < nprob; p++)
+    for (int64_t q = hfoff[p]; q < hfoff[p + 1]; q++)
+      for (int64_t e = hptr[q]; e < hptr[q + 1]; e++) {
+        if (nn_col[e] < 0 || nn_col[e] >= ntot_ || ppid[nn_col[e]] != p)
+          throw Error(NNK_EARG, "batch: NN entry outside its problem");
+        if (e == hptr[q]) fbody[q] = pbody[nn_col[e]];
+        else if (fbody[q] != pbody[nn_col[e]]) throw Error(NNK_EARG, "batch: a quadrature point maps to two bodies");
+      }
+  // NN^T (per force point: quadrature rows and weights, in row order)
+  const int64_t nnz = hptr[qtot_];
+  std::vector<int64_t> tptr(ntot_ + 1, 0);
+  for (int64_t e = 0; e < nnz; e++) tptr[nn_col[e] + 1]++;
+  for (int64_t j = 0; j < ntot_; j++) tptr[j + 1] += tptr[j];
+  std::vector<int32_t> trow(nnz);
+  std::vector<double> tval(nnz);
+  {
+    std::vector<int64_t> fill(tptr.begin(), tptr.end() - 1);
+    for (int64_t q = 0; q < qtot_; q++)
+      for (int64_t e = hptr[q]; e < hptr[q + 1]; e++) {
+        const int64_t k = fill[nn_col[e]]++;
+        trow[k] = (int32_t)q;
+        tval[k] = nn_val[e];
+      }
+  }
+  hvoff_.assign(nprob + 1, 0);
+  for (int p = 0; p < nprob; p++) hvoff_[p + 1] = hvoff_[p] + 3 * (hpoff_[p + 1] - hpoff_[p]) + 6 * nsw[p];
+  n_ = hvoff_[nprob];
+  std::vector<int64_t> xb(ntot_), xs(ntot_);
+  std::vector<int32_t> pid(n_);
+  for (int p = 0; p < nprob; p++) {
+    const int64_t Np = hpoff_[p + 1] - hpoff_[p];
+    for (int64_t j = 0; j < Np; j++) {
+      xb[hpoff_[p] + j] = hvoff_[p] + j;
+      xs[hpoff_[p] + j] = Np;
+    }
+    for (int64_t e = hvoff_[p]; e < hvoff_[p + 1]; e++) pid[e] = p;
+  }
+  // work items, most expensive first: forward (force-point tiles) and
+  // transpose (quadrature-point tiles)
+  auto make_items = [&](bool fine_tiles) {
+    std::vector<int2> items;
+    std::vector<double> cost;
+    for (int p = 0; p < nprob; p++) {
+      const int64_t Np = hpoff_[p + 1] - hpoff_[p], Qp = hfoff[p + 1] - hfoff[p];
+      const int64_t nt = fine_tiles ? Qp : Np, ns = fine_tiles ? Np : Qp;
+      for (int64_t t = 0; t * kTile < nt; t++) {
+        items.push_back(make_int2(p, (int)t));
+        cost.push_back((double)std::min<int64_t>(kTile, nt - t * kTile) * (double)ns);
+      }
+    }
+    std::vector<int> ord(items.size());
+    std::iota(ord.begin(), ord.end(), 0);
+    std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return cost[a] > cost[b]; });
+    std::vector<int2> sorted(items.size());
+    for (size_t i = 0; i < ord.size(); i++) sorted[i] = items[ord[i]];
+    return sorted;
+  };
+  auto it_f = make_items(false), it_t = make_items(true);
+  nitems_ = (int)it_f.size();
+  nitemsT_ = (int)it_t.size();
+  poff_.upload(hpoff_);
+  foff_.upload(hfoff);
+  voff_.upload(hvoff_);
+  nsw_.upload(nsw);
+  pts_.upload(std::vector<double>(pts, pts + 3 * ntot_));
+  fine_.upload(std::vector<double>(fine, fine + 3 * qtot_));
+  nnptr_.upload(hptr);
+  nncol_.upload(std::vector<int32_t>(nn_col, nn_col + nnz));
+  nnval_.upload(std::vector<double>(nn_val, nn_val + nnz));
+  nntptr_.upload(tptr);
+  nntrow_.upload(trow);
+  nntval_.upload(tval);
+  w_.upload(std::vector<double>(w, w + ntot_));
+  pm_.upload(std::vector<double>(pm, pm + 3 * ntot_));
+  rot_.upload(std::vector<double>(rot, rot + 3 * ntot_));
+  pbody_.upload(pbody);
+  fbody_.upload(fbody);
+  eps_.upload(std::vector<double>(eps, eps + nprob));
+  bprob_.upload(bprob);
+  blocal_.upload(blocal);
+  bpt_.upload(branges);
+  xbase_.upload(xb);
+  xstride_.upload(xs);
+  pid_.upload(pid);
+  part_.alloc(16);
+  items_.upload(it_f);
+  itemsT_.upload(it_t);
+  counter_.alloc(sizeof(int));
+  g_.alloc(sizeof(double) * 3 * std::max<int64_t>(qtot_, 1));
+  v_.alloc(sizeof(double) * 3 * std::max<int64_t>(qtot_, 1));
+  active_.upload(std::vector<int32_t>(nprob, 1));
+  pidf_.upload(ppid);
+}
+
+void Batch::op(const double *x, double *y, hipStream_t s) {
+  if (qtot_)
+    hipLaunchKernelGGL(k_bgather, dim3(nblk(qtot_)), dim3(256), 0, s, qtot_, nnptr_.as<int64_t>(),
+                       nncol_.as<int32_t>(), nnval_.as<double>(), x, xbase_.as<int64_t>(),
+                       xstride_.as<int64_t>(), g_.as<double>());
+  hipLaunchKernelGGL(k_bcounter_reset, dim3(1), dim3(64), 0, s, counter_.as<int>());
+  const unsigned grid = (unsigned)std::max(1, std::min(nitems_, 1024));
+  const double c = 1.0 / (8.0 * kPi * mu_);
+  auto launch = [&](auto kern) {
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(256), 0, s, nitems_, items_.as<int2>(), counter_.as<int>(),
+                       poff_.as<int64_t>(), foff_.as<int64_t>(), voff_.as<int64_t>(), nsw_.as<int32_t>(),
+                       pts_.as<double>(), fine_.as<double>(), g_.as<double>(), rot_.as<double>(),
+                       pbody_.as<int32_t>(), eps_.as<double>(), active_.as<int32_t>(), c, x, y);
+  };
+  if (f32_) launch(k_bp2p<true>);
+  else launch(k_bp2p<false>);
+  if (nbody_)
+    hipLaunchKernelGGL(k_bB, dim3(nbody_), dim3(256), 0, s, bprob_.as<int32_t>(), blocal_.as<int32_t>(),
+                       bpt_.as<int64_t>(), poff_.as<int64_t>(), voff_.as<int64_t>(), nsw_.as<int32_t>(),
+                       w_.as<double>(), pm_.as<double>(), active_.as<int32_t>(), x, y);
+  NNK_HIP(hipGetLastError());
+}
+
+void Batch::opT(const double *x, double *y, hipStream_t s) {
+  const double c = 1.0 / (8.0 * kPi * mu_);
+  if (nitemsT_) {
+    hipLaunchKernelGGL(k_bcounter_reset, dim3(1), dim3(64), 0, s, counter_.as<int>());
+    const unsigned grid = (unsigned)std::max(1, std::min(nitemsT_, 1024));
+    auto launch = [&](auto kern) {
+      hipLaunchKernelGGL(kern, dim3(grid), dim3(256), 0, s, nitemsT_, itemsT_.as<int2>(), counter_.as<int>(),
+                         poff_.as<int64_t>(), foff_.as<int64_t>(), voff_.as<int64_t>(), pts_.as<double>(),
+                         fine_.as<double>(), eps_.as<double>(), active_.as<int32_t>(), c, x, v_.as<double>());
+    };
+    if (f32_) launch(k_bp2pT<true>);
+    else launch(k_bp2pT<false>);
+  }
+  hipLaunchKernelGGL(k_bgatherT, dim3(nblk(ntot_)), dim3(256), 0, s, ntot_, pidf_.as<int32_t>(),
+                     nntptr_.as<int64_t>(), nntrow_.as<int32_t>(), nntval_.as<double>(), v_.as<double>(),
+                     poff_.as<int64_t>(), voff_.as<int64_t>(), nsw_.as<int32_t>(), w_.as<double>(), pm_.as<double>(),
+                     pbody_.as<int32_t>(), active_.as<int32_t>(), x, y);
+  if (nbody_)
+    hipLaunchKernelGGL(k_bBTt, dim3(nbody_), dim3(256), 0, s, bprob_.as<int32_t>(), blocal_.as<int32_t>(),
+                       bpt_.as<int64_t>(), poff_.as<int64_t>(), voff_.as<int64_t>(), nsw_.as<int32_t>(),
+                       rot_.as<double>(), active_.as<int32_t>(), x, y);
+  NNK_HIP(hipGetLastError());
+}
+
+void Batch::apply(const double *x, double *y, bool transpose, hipStream_t s) {
+  if (!graph_) {
+    if (transpose) opT(x, y, s);
+    else op(x, y, s);
+    return;
+  }
+  GraphOp &G = gop_[transpose ? 1 : 0];
+  if (!G.ready())
+    G.capture(n_, n_, [&](const double *in, double *out, hipStream_t cs) {
+      if (transpose) opT(in, out, cs);
+      else op(in, out, cs);
+    });
+  G.run(x, y, s);
+}
+
+void Batch::segdots(int k, const double *V, const double *w, double *host_out, hipStream_t s) {
+  const size_t need = sizeof(double) * (size_t)k * np_;
+  if (part_.bytes < need) part_.alloc(need);
+  hipLaunchKernelGGL(k_bdots, dim3(np_, k), dim3(256), 0, s, n_, np_, voff_.as<int64_t>(), V, w, part_.as<double>());
+  NNK_HIP(hipGetLastError());
+  NNK_HIP(hipMemcpyAsync(host_out, part_.p, need, hipMemcpyDeviceToHost, s));
+  NNK_HIP(hipStreamSynchronize(s));
+}
+
+void Batch::gmres(const double *b, double *x, double tol, int maxit, bool transpose, hipStream_t s,
+                  int32_t *flag, double *relres, int32_t *iters) {
+  const int P = np_;
+  const int64_t n = n_;
+  int64_t minlen = INT64_MAX;
+  for (int p = 0; p < P; p++) minlen = std::min(minlen, hvoff_[p + 1] - hvoff_[p]);
+  const int m = std::max(1, (int)std::min<int64_t>(maxit, minlen));
+  DBuf V, w, hd;
+  V.alloc(sizeof(double) * (size_t)(m + 1) * n);
+  w.alloc(sizeof(double) * n);
+  hd.alloc(sizeof(double) * (size_t)(m + 1) * P);
+  double *Vp = V.as<double>();
+  auto upload = [&](const std::vector<double> &v) {
+    NNK_HIP(hipMemcpyAsync(hd.p, v.data(), sizeof(double) * v.size(), hipMemcpyHostToDevice, s));
+  };
+  std::vector<int32_t> act(P, 1);
+  // b norms, V_0 = b / |b|
+  std::vector<double> bn(P);
+  segdots(1, b, b, bn.data(), s);
+  std::vector<double> scale(P);
+  for (int p = 0; p < P; p++) {
+    bn[p] = std::sqrt(bn[p]);
+    scale[p] = bn[p] > 0 ? 1.0 / bn[p] : 0.0;
+  }
+  upload(scale);
+  hipLaunchKernelGGL(k_bscale, dim3(nblk(n)), dim3(256), 0, s, n, pid_.as<int32_t>(), hd.as<double>(), b, 0.0, Vp);
+  // per-problem Hessenberg (Givens-reduced), rotations, residual estimates
+  std::vector<double> H((size_t)P * (m + 1) * m, 0.0), cs((size_t)P * m), sn((size_t)P * m),
+      g((size_t)P * (m + 1), 0.0), h((size_t)(m + 1) * P), h2((size_t)(m + 1) * P), hn(P);
+  std::vector<int> kp(P, 0);
+  std::vector<char> done(P, 0);
+  for (int p = 0; p < P; p++) {
+    g[(size_t)p * (m + 1)] = bn[p];
+    if (bn[p] == 0) { done[p] = 1; act[p] = 0; }
+  }
+  NNK_HIP(hipMemcpyAsync(active_.p, act.data(), sizeof(int32_t) * P, hipMemcpyHostToDevice, s));
+  int k = 0;
+  for (; k < m; k++) {
+    bool any = false;
+    for (int p = 0; p < P; p++) any |= !done[p];
+    if (!any) break;
+    apply(Vp + (size_t)k * n, w.as<double>(), transpose, s);
+    for (int pass = 0; pass < 2; pass++) {  // CGS with one reorthogonalisation
+      std::vector<double> &hh = pass ? h2 : h;
+      segdots(k + 1, Vp, w.as<double>(), hh.data(), s);
+      for (int p = 0; p < P; p++)
+        if (done[p])
+          for (int j = 0; j <= k; j++) hh[(size_t)j * P + p] = 0.0;
+      upload(std::vector<double>(hh.begin(), hh.begin() + (size_t)(k + 1) * P));
+      hipLaunchKernelGGL(k_bcombo, dim3(nblk(n)), dim3(256), 0, s, n, k + 1, P, Vp, hd.as<double>(),
+                         pid_.as<int32_t>(), w.as<double>());
+    }
+    segdots(1, w.as<double>(), w.as<double>(), hn.data(), s);
+    for (int p = 0; p < P; p++) {
+      hn[p] = std::sqrt(hn[p]);
+      scale[p] = (!done[p] && hn[p] > 0) ? 1.0 / hn[p] : 0.0;
+    }
+    upload(scale);
+    hipLaunchKernelGGL(k_bscale, dim3(nblk(n)), dim3(256), 0, s, n, pid_.as<int32_t>(), hd.as<double>(),
+                       w.as<double>(), 0.0, Vp + (size_t)(k + 1) * n);
+    NNK_HIP(hipGetLastError());
+    bool changed = false;
+    for (int p = 0; p < P; p++) {
+      if (done[p]) continue;
+      double *Hp = H.data() + (size_t)p * (m + 1) * m;
+      double *cp = cs.data() + (size_t)p * m, *sp = sn.data() + (size_t)p * m, *gp = g.data() + (size_t)p * (m + 1);
+      std::vector<double> col(k + 2);
+      for (int j = 0; j <= k; j++) col[j] = h[(size_t)j * P + p] + h2[(size_t)j * P + p];
+      col[k + 1] = hn[p];
+      for (int j = 0; j < k; j++) {
+        const double t = cp[j] * col[j] + sp[j] * col[j + 1];
+        col[j + 1] = -sp[j] * col[j] + cp[j] * col[j + 1];
+        col[j] = t;
+      }
+      const double r = std::hypot(col[k], col[k + 1]);
+      cp[k] = r > 0 ? col[k] / r : 1.0;
+      sp[k] = r > 0 ? col[k + 1] / r : 0.0;
+      col[k] = r;
+      gp[k + 1] = -sp[k] * gp[k];
+      gp[k] = cp[k] * gp[k];
+      for (int j = 0; j <= k; j++) Hp[(size_t)j * m + k] = col[j];
+      kp[p] = k + 1;
+      if (std::fabs(gp[k + 1]) <= tol * bn[p] || hn[p] == 0.0) {
+        done[p] = 1;
+        act[p] = 0;
+        changed = true;
+      }
+    }
+    if (changed) NNK_HIP(hipMemcpyAsync(active_.p, act.data(), sizeof(int32_t) * P, hipMemcpyHostToDevice, s));
+  }
+  // x_p = V y_p
+  std::vector<double> coef((size_t)std::max(k, 1) * P, 0.0);
+  for (int p = 0; p < P; p++) {
+    const int kk = kp[p];
+    const double *Hp = H.data() + (size_t)p * (m + 1) * m, *gp = g.data() + (size_t)p * (m + 1);
+    std::vector<double> yv(kk, 0.0);
+    for (int i = kk - 1; i >= 0; i--) {
+      double t = gp[i];
+      for (int j = i + 1; j < kk; j++) t -= Hp[(size_t)i * m + j] * yv[j];
+      yv[i] = Hp[(size_t)i * m + i] != 0 ? t / Hp[(size_t)i * m + i] : 0.0;
+    }
+    for (int j = 0; j < kk; j++) coef[(size_t)j * P + p] = -yv[j];
+    iters[p] = kk;
+  }
+  NNK_HIP(hipMemsetAsync(x, 0, sizeof(double) * n, s));
+  if (k > 0) {
+    upload(coef);
+    hipLaunchKernelGGL(k_bcombo, dim3(nblk(n)), dim3(256), 0, s, n, k, P, Vp, hd.as<double>(), pid_.as<int32_t>(), x);
+  }
+  // true residuals
+  std::fill(act.begin(), act.end(), 1);
+  NNK_HIP(hipMemcpyAsync(active_.p, act.data(), sizeof(int32_t) * P, hipMemcpyHostToDevice, s));
+  apply(x, w.as<double>(), transpose, s);
+  upload(std::vector<double>(P, 1.0));
+  hipLaunchKernelGGL(k_bscale, dim3(nblk(n)), dim3(256), 0, s, n, pid_.as<int32_t>(), hd.as<double>(), b, -1.0,
+                     w.as<double>());
+  NNK_HIP(hipGetLastError());
+  std::vector<double> rn(P);
+  segdots(1, w.as<double>(), w.as<double>(), rn.data(), s);
+  for (int p = 0; p < P; p++) {
+    relres[p] = bn[p] > 0 ? std::sqrt(rn[p]) / bn[p] : 0.0;
+    flag[p] = relres[p] <= tol ? 0 : 1;
+  }
+}
+
+void Batch::gradient(const double *z, const double *lambda, double *gx, double *gy, double *geps, hipStream_t s) {
+  std::vector<int32_t> act(np_, 1);
+  NNK_HIP(hipMemcpyAsync(active_.p, act.data(), sizeof(int32_t) * np_, hipMemcpyHostToDevice, s));
+  if (qtot_)
+    hipLaunchKernelGGL(k_bgather, dim3(nblk(qtot_)), dim3(256), 0, s, qtot_, nnptr_.as<int64_t>(),
+                       nncol_.as<int32_t>(), nnval_.as<double>(), z, xbase_.as<int64_t>(),
+                       xstride_.as<int64_t>(), g_.as<double>());
+  const double c = 1.0 / (8.0 * kPi * mu_);
+  DBuf ge;
+  ge.alloc(sizeof(double) * ntot_);
+  hipLaunchKernelGGL(k_bcounter_reset, dim3(1), dim3(64), 0, s, counter_.as<int>());
+  {
+    const unsigned grid = (unsigned)std::max(1, std::min(nitems_, 1024));
+    auto launch = [&](auto kern) {
+      hipLaunchKernelGGL(kern, dim3(grid), dim3(256), 0, s, nitems_, items_.as<int2>(), counter_.as<int>(),
+                         poff_.as<int64_t>(), foff_.as<int64_t>(), voff_.as<int64_t>(), nsw_.as<int32_t>(),
+                         pts_.as<double>(), fine_.as<double>(), g_.as<double>(), pbody_.as<int32_t>(),
+                         eps_.as<double>(), active_.as<int32_t>(), c, z, lambda, gx, ge.as<double>());
+    };
+    if (f32_) launch(k_bgradx<true>);
+    else launch(k_bgradx<false>);
+  }
+  if (nitemsT_) {
+    hipLaunchKernelGGL(k_bcounter_reset, dim3(1), dim3(64), 0, s, counter_.as<int>());
+    const unsigned grid = (unsigned)std::max(1, std::min(nitemsT_, 1024));
+    auto launch = [&](auto kern) {
+      hipLaunchKernelGGL(kern, dim3(grid), dim3(256), 0, s, nitemsT_, itemsT_.as<int2>(), counter_.as<int>(),
+                         poff_.as<int64_t>(), foff_.as<int64_t>(), voff_.as<int64_t>(), nsw_.as<int32_t>(),
+                         pts_.as<double>(), fine_.as<double>(), g_.as<double>(), fbody_.as<int32_t>(),
+                         eps_.as<double>(), active_.as<int32_t>(), c, lambda, gy);
+    };
+    if (f32_) launch(k_bgrady<true>);
+    else launch(k_bgrady<false>);
+  }
+  NNK_HIP(hipGetLastError());
+  std::vector<double> he(ntot_);
+  NNK_HIP(hipMemcpyAsync(he.data(), ge.p, sizeof(double) * ntot_, hipMemcpyDeviceToHost, s));
+  NNK_HIP(hipStreamSynchronize(s));
+  for (int p = 0; p < np_; p++) {
+    double t = 0;
+    for (int64_t j = hpoff_[p]; j < hpoff_[p + 1]; j++) t += he[j];
+    geps[p] = t;
+  }
+}
+
+}  // namespace nnk

@@ -7,6 +7,7 @@
 #include <string>
 
 #include "../../include/nnkifmm.h"
+#include "nnk_batch.hpp"
 #include "nnk_fmm.hpp"
 #include "nnk_handles.hpp"
 #include "nnk_mobility.hpp"
@@ -15,6 +16,9 @@
 
 struct nnk_mobility {
   std::unique_ptr<nnk::Mobility> m;
+};
+struct nnk_batch {
+  std::unique_ptr<nnk::Batch> b;
 };
 
 namespace nnk {
@@ -388,5 +392,115 @@ int32_t nnk_fmm_stats(const nnk_fmm *h, int64_t *out, int32_t nstats) {
   if (nstats > nnk::Fmm::kStats + 1) out[nnk::Fmm::kStats + 1] = h->f->rank();
   return NNK_OK;
 }
+
+int32_t nnk_batch_create(int32_t nprob, const int64_t *pt_off, const double *pts, const int64_t *fine_off,
+                         const double *fine, const int64_t *nn_ptr, const int32_t *nn_col, const double *nn_val,
+                         const double *weight, const double *arm, const double *rot, const int32_t *pt_body,
+                         const int64_t *body_off, const double *eps, double mu, int32_t precision,
+                         nnk_batch **out) {
+  return guard([&]() -> int32_t {
+    if (!out) return fail(NNK_EARG, "nnk_batch_create: null output");
+    *out = nullptr;
+    if (nprob <= 0 || !pt_off || !pts || !fine_off || !nn_ptr || !weight || !arm || !rot || !pt_body ||
+        !body_off || !eps)
+      return fail(NNK_EARG, "nnk_batch_create: bad argument");
+    if (fine_off[nprob] > 0 && (!fine || !nn_col || !nn_val))
+      return fail(NNK_EARG, "nnk_batch_create: quadrature points without a nearest map");
+    if (precision != NNK_FP64 && precision != NNK_MIXED)
+      return fail(NNK_EARG, "nnk_batch_create: precision must be NNK_FP64 or NNK_MIXED");
+    auto h = std::make_unique<nnk_batch>();
+    h->b = std::make_unique<nnk::Batch>(nprob, pt_off, pts, fine_off, fine, nn_ptr, nn_col, nn_val, weight, arm,
+                                        rot, pt_body, body_off, eps, mu, precision == NNK_MIXED ? 1 : 0);
+    *out = h.release();
+    return NNK_OK;
+  });
+}
+
+int32_t nnk_batch_size(const nnk_batch *h, int64_t *n) {
+  return guard([&]() -> int32_t {
+    if (!h || !n) return fail(NNK_EARG, "nnk_batch_size: null argument");
+    *n = h->b->size();
+    return NNK_OK;
+  });
+}
+
+static int32_t batch_apply_host(nnk_batch *h, const double *x, double *y, bool transpose) {
+  const int64_t n = h->b->size();
+  nnk::DBuf dx, dy;
+  dx.alloc(sizeof(double) * n);
+  dy.alloc(sizeof(double) * n);
+  NNK_HIP(hipMemcpy(dx.p, x, sizeof(double) * n, hipMemcpyHostToDevice));
+  h->b->apply(dx.as<double>(), dy.as<double>(), transpose, 0);
+  NNK_HIP(hipMemcpy(y, dy.p, sizeof(double) * n, hipMemcpyDeviceToHost));
+  return NNK_OK;
+}
+
+int32_t nnk_batch_apply(nnk_batch *h, const double *x, double *y) {
+  return guard([&]() -> int32_t {
+    if (!h || !x || !y) return fail(NNK_EARG, "nnk_batch_apply: null argument");
+    return batch_apply_host(h, x, y, false);
+  });
+}
+
+int32_t nnk_batch_apply_transpose(nnk_batch *h, const double *x, double *y) {
+  return guard([&]() -> int32_t {
+    if (!h || !x || !y) return fail(NNK_EARG, "nnk_batch_apply_transpose: null argument");
+    return batch_apply_host(h, x, y, true);
+  });
+}
+
+int32_t nnk_batch_apply_device(nnk_batch *h, const double *d_x, double *d_y, void *stream) {
+  return guard([&]() -> int32_t {
+    if (!h || !d_x || !d_y) return fail(NNK_EARG, "nnk_batch_apply_device: null argument");
+    h->b->apply(d_x, d_y, false, (hipStream_t)stream);
+    return NNK_OK;
+  });
+}
+
+int32_t nnk_batch_set_graph(nnk_batch *h, int32_t on) {
+  return guard([&]() -> int32_t {
+    if (!h) return fail(NNK_EARG, "nnk_batch_set_graph: null handle");
+    h->b->set_graph(on != 0);
+    return NNK_OK;
+  });
+}
+
+int32_t nnk_batch_solve(nnk_batch *h, const double *rhs, double tol, int32_t maxit, int32_t transpose, double *x,
+                        int32_t *flag, double *relres, int32_t *iters) {
+  return guard([&]() -> int32_t {
+    if (!h || !rhs || !x || !flag || !relres || !iters || maxit < 1 || !(tol > 0))
+      return fail(NNK_EARG, "nnk_batch_solve: bad argument");
+    const int64_t n = h->b->size();
+    nnk::DBuf db, dx;
+    db.alloc(sizeof(double) * n);
+    dx.alloc(sizeof(double) * n);
+    NNK_HIP(hipMemcpy(db.p, rhs, sizeof(double) * n, hipMemcpyHostToDevice));
+    h->b->gmres(db.as<double>(), dx.as<double>(), tol, maxit, transpose != 0, 0, flag, relres, iters);
+    NNK_HIP(hipMemcpy(x, dx.p, sizeof(double) * n, hipMemcpyDeviceToHost));
+    return NNK_OK;
+  });
+}
+
+int32_t nnk_batch_gradient(nnk_batch *h, const double *z, const double *lambda, double *gx, double *gy,
+                           double *geps) {
+  return guard([&]() -> int32_t {
+    if (!h || !z || !lambda || !gx || !geps || (h->b->nfine() && !gy))
+      return fail(NNK_EARG, "nnk_batch_gradient: null argument");
+    const int64_t n = h->b->size(), N = h->b->npoints(), Q = h->b->nfine();
+    nnk::DBuf dz, dl, dgx, dgy;
+    dz.alloc(sizeof(double) * n);
+    dl.alloc(sizeof(double) * n);
+    dgx.alloc(sizeof(double) * 3 * N);
+    dgy.alloc(sizeof(double) * 3 * std::max<int64_t>(Q, 1));
+    NNK_HIP(hipMemcpy(dz.p, z, sizeof(double) * n, hipMemcpyHostToDevice));
+    NNK_HIP(hipMemcpy(dl.p, lambda, sizeof(double) * n, hipMemcpyHostToDevice));
+    h->b->gradient(dz.as<double>(), dl.as<double>(), dgx.as<double>(), dgy.as<double>(), geps, 0);
+    NNK_HIP(hipMemcpy(gx, dgx.p, sizeof(double) * 3 * N, hipMemcpyDeviceToHost));
+    if (Q) NNK_HIP(hipMemcpy(gy, dgy.p, sizeof(double) * 3 * Q, hipMemcpyDeviceToHost));
+    return NNK_OK;
+  });
+}
+
+void nnk_batch_destroy(nnk_batch *h) { delete h; }
 
 }  // extern "C"

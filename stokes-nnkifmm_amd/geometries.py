@@ -12,6 +12,11 @@ input scripts outside the accelerated path):
                     NEAREST quadrature (the adaptive mobility workload:
                     64 prolate spheroids, aspect 3, semi-major axis 1, in a
                     20^3 box above a 40 x 40 wall, 4x refined quadrature)
+  squirmer_slip     tangential squirming velocity B1 sin(t) + B2 sin(t) cos(t)
+  squirmer_pair     two spherical squirmers (force- and torque-free)
+  batch_bodies      independent single-body systems (spheres and
+                    spheroids, 200-5000 force points log-uniform, 4x
+                    refined quadrature, eps uniform in [0.002, 0.02])
 """
 import numpy as np
 
@@ -93,4 +98,78 @@ def spheroid_suspension(nbody=64, n=18, refine=4, a=1.0 / 3.0, c=1.0, box=20.0, 
         wall = plane_wall(wall_n, box / 2)
         wfine = plane_wall(refine * wall_n, box / 2)
         sw.updateBnd(wall, wfine, nearestMatrix(wall, wfine), np.zeros_like(wall))
+    return sw
+
+
+def batch_bodies(nprob=256, nmin=200, nmax=5000, refine=4, eps_range=(0.002, 0.02), seed=11):
+    """nprob single-body Swimmers systems for the batched mobility solve:
+    a sphere (radius 1) or a prolate spheroid (semi-axes 1/3, 1/3, 1) with
+    6 n^2 force points, n from a log-uniform draw of the point count in
+    [nmin, nmax], and 6 (refine n)^2 quadrature points; random orientation,
+    a unit force in a random direction and a random moment of size 0.1.
+    Returns (systems, eps)."""
+    from .nearest import nearestMatrix
+    from .swimmers import Swimmers
+    rng = np.random.default_rng(seed)
+    out, eps, cache = [], [], {}
+    for _ in range(nprob):
+        npts = np.exp(rng.uniform(np.log(nmin), np.log(nmax)))
+        n = max(2, int(round(np.sqrt(npts / 6.0))))
+        kind = int(rng.integers(2))
+        key = (kind, n)
+        if key not in cache:
+            body = cube_sphere(n) if kind == 0 else prolate_spheroid(n, 1.0 / 3.0, 1.0)
+            fine = cube_sphere(refine * n) if kind == 0 else prolate_spheroid(refine * n, 1.0 / 3.0, 1.0)
+            cache[key] = (body, fine, nearestMatrix(body, fine))
+        body, fine, NN = cache[key]
+        q, r = np.linalg.qr(rng.standard_normal((3, 3)))
+        q = q * np.sign(np.diag(r))
+        if np.linalg.det(q) < 0:
+            q[:, 2] = -q[:, 2]
+        F = rng.standard_normal(3)
+        F /= np.linalg.norm(F)
+        M = rng.standard_normal(3)
+        M *= 0.1 / np.linalg.norm(M)
+        sw = Swimmers()
+        sw.addSwimmer(body, fine, NN, rng.uniform(-5, 5, 3), np.concatenate([q[:, 0], q[:, 1]]),
+                      np.zeros_like(body), F, M)
+        out.append(sw)
+        eps.append(rng.uniform(*eps_range))
+    return out, np.array(eps)
+
+
+def squirmer_slip(points, B1=1.0, B2=0.0, axis=(0.0, 0.0, 1.0)):
+    """Surface slip of a spherical squirmer in its body frame:
+    u = (B1 sin t + B2 sin t cos t) e_t, t the polar angle from `axis`,
+    e_t = (cos t r - e) / sin t the polar unit vector (zero at the poles).
+    A lone squirmer then swims along `axis` at 2 B1 / 3."""
+    e = np.asarray(axis, float)
+    e = e / np.linalg.norm(e)
+    r = np.asarray(points, float)
+    r = r / np.linalg.norm(r, axis=1, keepdims=True)
+    ct = r @ e
+    st = np.sqrt(np.maximum(0.0, 1.0 - ct * ct))
+    et = np.zeros_like(r)
+    ok = st > 1e-12
+    et[ok] = (ct[ok, None] * r[ok] - e) / st[ok, None]
+    return ((B1 * st + B2 * st * ct)[:, None]) * et
+
+
+def squirmer_pair(n=16, refine=4, gap=0.5, B1=1.0, B2=0.0, tilt=0.5):
+    """Two unit spheres (6 n^2 force points, 6 (refine n)^2 quadrature
+    points, nearest map) a surface gap `gap` apart along x, swimming axes in
+    the x-z plane at +/- tilt radians from z; force- and torque-free.
+    Returns a Swimmers system (tree and operator not built)."""
+    from .nearest import nearestMatrix
+    from .swimmers import Swimmers
+    body, fine = cube_sphere(n), cube_sphere(refine * n)
+    NN = nearestMatrix(body, fine)
+    vel = squirmer_slip(body, B1, B2)
+    sw = Swimmers()
+    d = 1.0 + gap / 2.0
+    for sgn in (-1.0, 1.0):
+        a = sgn * tilt
+        b1 = [np.cos(a), 0.0, -np.sin(a)]   # body x axis
+        b2 = [0.0, 1.0, 0.0]                # body y axis; body z = b1 x b2
+        sw.addSwimmer(body, fine, NN, [sgn * d, 0.0, 0.0], b1 + b2, vel, np.zeros(3), np.zeros(3))
     return sw

@@ -1,0 +1,88 @@
+"""Batched single-body mobility problems (GPU).
+
+Each problem's operator [S BT; B 0] is checked against the oracle's FP64
+direct sum with B, BT from their definitions (genMatrixComponets.m) to
+1e-12; the batched GMRES answer is checked by its residual under that
+oracle operator and, for the one-sphere configuration, against the
+single-problem solver (mobilityProblem) of this library; graph replay must
+be bit-identical; the FP32-math form stays within 1e-5 of FP64."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _oracle_apply(oracle, sw, eps, x):
+    pout, fout, NN, prot, frot = sw.getSwimmers()
+    n = len(pout)
+    f = x[:3 * n].reshape(3, n).T
+    U, W = x[3 * n:3 * n + 3], x[3 * n + 3:3 * n + 6]
+    u = oracle.direct(fout, NN @ f, pout, eps) - U - np.cross(W, prot)
+    w = np.asarray(NN.sum(axis=0)).ravel()
+    pm = NN.T @ frot
+    F = (w[:, None] * f).sum(0)
+    M = np.cross(pm, f).sum(0)
+    return np.concatenate([u.T.ravel(), F, M])
+
+
+@pytest.fixture(scope="module")
+def small_batch(nnk):
+    from importlib import import_module
+    geo = import_module("stokes-nnkifmm_amd.geometries")
+    return geo.batch_bodies(nprob=10, nmin=24, nmax=300, seed=3)
+
+
+def test_batched_operator_matches_oracle(oracle, nnk, small_batch):
+    bodies, eps = small_batch
+    bm = nnk.BatchedMobility(bodies, eps, hipGraph=False)
+    x = np.random.default_rng(0).standard_normal(bm.n)
+    y = bm.apply(x)
+    for p, (sw, xp, yp) in enumerate(zip(bodies, bm.split(x), bm.split(y))):
+        ref = _oracle_apply(oracle, sw, eps[p], xp)
+        assert np.linalg.norm(yp - ref) <= 1e-12 * np.linalg.norm(ref), p
+
+
+def test_batched_solve_residual_and_graph(oracle, nnk, small_batch):
+    bodies, eps = small_batch
+    sols = []
+    for graph in (False, True):
+        bm = nnk.BatchedMobility(bodies, eps, hipGraph=graph)
+        b = bm.rhs()
+        sols.append((bm.solve(b, 1e-10, 200), dict(bm.last_solve)))
+    (x0, i0), (x1, i1) = sols
+    np.testing.assert_array_equal(x0, x1)
+    np.testing.assert_array_equal(i0["iter"], i1["iter"])
+    assert (i0["flag"] == 0).all() and (i0["relres"] <= 1e-10).all()
+    for p, (sw, xp, bp) in enumerate(zip(bodies, bm.split(x0), bm.split(b))):
+        r = _oracle_apply(oracle, sw, eps[p], xp) - bp
+        assert np.linalg.norm(r) <= 2e-10 * np.linalg.norm(bp), p
+
+
+def test_batched_matches_single_problem_solver(nnk):
+    from importlib import import_module
+    geo = import_module("stokes-nnkifmm_amd.geometries")
+    mob = import_module("stokes-nnkifmm_amd.mobility")
+    c, fine = geo.cube_sphere(8), geo.cube_sphere(32)
+    NN = nnk.nearestMatrix(c, fine)
+    sw = nnk.Swimmers()
+    sw.addSwimmer(c, fine, NN, [0.3, -0.2, 0.1], [1, 0, 0, 0, 1, 0], np.zeros_like(c), [6 * np.pi, 0, 0],
+                  [0, 0, 0.5])
+    bm = nnk.BatchedMobility([sw, sw], [0.01, 0.01])
+    xb = bm.split(bm.solve(bm.rhs(), 1e-10, 200))
+    sw.genTree("nodeCapacity", 64, "nearest", 1)
+    sw.genKIFMM([0.01, 1.0], "GMRES", 1, "format", 2)
+    B, BT, A1, A2 = nnk.genMatrixComponets(sw)
+    xs = mob.mobility_gmres(sw, B, BT, A1, A2, mob._rhs(sw, lambda t, x: np.zeros_like(x), 0.0), 1e-10, 200,
+                            False)
+    for xp in xb:
+        assert np.linalg.norm(xp - xs) <= 1e-8 * np.linalg.norm(xs)
+    assert abs(xs[-6] - 1.0) < 0.02   # Stokes' law, U = F / (6 pi mu a)
+
+
+def test_batched_f32_close_to_fp64(nnk, small_batch):
+    bodies, eps = small_batch
+    b64 = nnk.BatchedMobility(bodies, eps, precision="fp64")
+    b32 = nnk.BatchedMobility(bodies, eps, precision="mixed")
+    x = np.random.default_rng(1).standard_normal(b64.n)
+    y64, y32 = b64.apply(x), b32.apply(x)
+    assert np.linalg.norm(y32 - y64) <= 1e-5 * np.linalg.norm(y64)
