@@ -26,7 +26,7 @@ class BatchedMobility:
         if precision not in _PREC:
             raise ValueError(f"BatchedMobility: unknown precision {precision!r}")
         eps = np.broadcast_to(np.asarray(eps, float), (len(systems),)).copy()
-        pts, fines, w, pm, rot, pbody, nptr, ncol, nval = [], [], [], [], [], [], [0], [], []
+        pts, fines, w, pm, rot, pbody, nptr, ncol, nval = [], [], [], [], [], [], [np.zeros(1, np.int64)], [], []
         poff, foff, boff = [0], [0], [0]
         self.systems = list(systems)
         for sw in self.systems:
@@ -46,12 +46,12 @@ class BatchedMobility:
                 frot_all = np.zeros((len(fine), 3))
                 frot_all[:len(frot)] = frot
                 arm = NN.T @ frot_all
-                nptr.append(NN.indptr[1:].astype(np.int64) + nptr[-1])
+                nptr.append(NN.indptr[1:].astype(np.int64) + nptr[-1][-1])
                 ncol.append(NN.indices.astype(np.int32) + poff[-1])
                 nval.append(NN.data)
             else:   # without quadrature the force points are their own sources
                 wj, arm, fine = np.ones(n), r.copy(), points
-                nptr.append(np.arange(1, n + 1, dtype=np.int64) + nptr[-1])
+                nptr.append(np.arange(1, n + 1, dtype=np.int64) + nptr[-1][-1])
                 ncol.append(np.arange(n, dtype=np.int32) + poff[-1])
                 nval.append(np.ones(n))
             pts.append(points)
@@ -65,7 +65,7 @@ class BatchedMobility:
             boff.append(boff[-1] + nsw)
         cat = lambda a, d: np.ascontiguousarray(np.concatenate(a), dtype=d)  # noqa: E731
         self._arrs = [np.asarray(poff, np.int64), cat(pts, np.float64), np.asarray(foff, np.int64),
-                      cat(fines, np.float64), np.asarray(np.concatenate([[0]] + nptr), np.int64),
+                      cat(fines, np.float64), np.ascontiguousarray(np.concatenate(nptr), np.int64),
                       cat(ncol, np.int32), cat(nval, np.float64), cat(w, np.float64), cat(pm, np.float64),
                       cat(rot, np.float64), cat(pbody, np.int32), np.asarray(boff, np.int64), eps]
         a = self._arrs

@@ -65,7 +65,10 @@ def test_batched_matches_single_problem_solver(nnk):
     c, fine = geo.cube_sphere(8), geo.cube_sphere(32)
     NN = nnk.nearestMatrix(c, fine)
     sw = nnk.Swimmers()
-    sw.addSwimmer(c, fine, NN, [0.3, -0.2, 0.1], [1, 0, 0, 0, 1, 0], np.zeros_like(c), [6 * np.pi, 0, 0],
+    # centred at the origin: each octant holds 48 <= 64 points, so the
+    # single-problem KIFMM tree is the root and 8 mutually adjacent leaves
+    # and its matvec is the direct sum
+    sw.addSwimmer(c, fine, NN, [0.0, 0.0, 0.0], [1, 0, 0, 0, 1, 0], np.zeros_like(c), [6 * np.pi, 0, 0],
                   [0, 0, 0.5])
     bm = nnk.BatchedMobility([sw, sw], [0.01, 0.01])
     xb = bm.split(bm.solve(bm.rhs(), 1e-10, 200))
