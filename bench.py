@@ -24,6 +24,12 @@ Prints one JSON line.  Besides the throughput it reports:
                 OpenMP) timed on this host on a bounded sample;
   rel_err       ||u - u_direct|| / ||u_direct|| on 4096 sampled targets,
                 u_direct from the GPU direct sum (blockcomputation).
+  mobility solves (skipped with --no-solve), each with wall time,
+  iterations and residuals: one sphere (384 / 6144 points, FP64 GMRES to
+  1e-6); 64 spheroids above a wall (~126k / ~2M points, FP64 and
+  mixed-refined); 256 batched single-body problems (FP64 and FP32-math,
+  HIP-graph replay); the squirmer-pair adjoint (forward, batched transpose
+  GMRES, gradient kernels).
 """
 import argparse
 import ctypes
@@ -53,7 +59,8 @@ def parse():
     ap.add_argument("--svd-tol", type=float, default=1e-8, help="pseudo-inverse cut of the mixed forms")
     ap.add_argument("--forms", default="fp64_res6,fp64_res8,mixed_res6,mixed3_res8",
                     help="comma-separated forms to time; the last one is the headline")
-    ap.add_argument("--no-solve", action="store_true", help="skip the one-sphere mobility solve")
+    ap.add_argument("--no-solve", action="store_true",
+                    help="skip the mobility solves (one sphere, spheroids + wall, batched, adjoint)")
     ap.add_argument("--seed", type=int, default=20240611)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--cpu-n", type=int, default=1 << 16, help="CPU baseline sample size")
@@ -171,6 +178,112 @@ def sphere_solve(nnk):
     return out
 
 
+def spheroid_solve(nnk):
+    """64 prolate spheroids (aspect 3, semi-major axis 1) above a 40 x 40
+    wall in a 20^3 box, 4x refined NEAREST quadrature (~126k force / ~2M
+    quadrature points), eps = 0.005, each body pulled by a unit force:
+    GMRES to 1e-6 with the block preconditioner, FP64 and by iterative
+    refinement around the mixed form (FP64 residuals)."""
+    import torch
+    import importlib as il
+    geo = il.import_module("stokes-nnkifmm_amd.geometries")
+    mob = il.import_module("stokes-nnkifmm_amd.mobility")
+    t0 = time.perf_counter()
+    sw = geo.spheroid_suspension()
+    t_geo = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    sw.genTree("nodeCapacity", 64, "nearest", 1)
+    sw.genKIFMM([0.005, 1.0], "GMRES", 1, "format", 2)
+    B, BT, A1, A2 = nnk.genMatrixComponets(sw)
+    t_setup = time.perf_counter() - t0
+    b = mob._rhs(sw, lambda t, x: np.zeros_like(x), 0.0)
+    out = {"force_points": int(sw.Tree.points.shape[0]), "quadrature_points": int(len(sw.getSwimmerBnd()[1])),
+           "tree_nodes": int(sw.Tree.nodeCount), "setup_s": {"geometry_nearest": round(t_geo, 3),
+                                                             "tree_B": round(t_setup, 3)}}
+    t0 = time.perf_counter()
+    x64 = mob.mobility_gmres(sw, B, BT, A1, A2, b, 1e-6, 400, True)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    info = mob.last_solve
+    out["fp64_precon"] = {"wall_s": round(dt, 3), "iterations": info["iter"], "relres": info["relres"]}
+    t0 = time.perf_counter()
+    xr = mob.mobility_refined(sw, B, BT, A1, A2, b, 1e-6, 400, True, innerTol=1e-3)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    info = mob.last_solve
+    r = nnk.mobilityMatrixMulti(xr, sw, B, BT) - b
+    nb = 6 * sw.swimmerNo()
+    out["mixed_refined_precon"] = {
+        "wall_s": round(dt, 3), "inner_iterations": info["iter"], "outer": info["outer"],
+        "relres_fp64": float(np.linalg.norm(r) / np.linalg.norm(b)),
+        "body_velocity_diff_vs_fp64": float(np.linalg.norm(xr[-nb:] - x64[-nb:]) / np.linalg.norm(x64[-nb:]))}
+    return out
+
+
+def batched_solves(nnk):
+    """256 independent single-body problems (spheres and spheroids, 200-5000
+    force points log-uniform, 4x refined quadrature, eps in [0.002, 0.02]):
+    one batched GMRES to 1e-6, HIP-graph replay of the persistent batched
+    operator, FP64 and FP32-math."""
+    import importlib as il
+    geo = il.import_module("stokes-nnkifmm_amd.geometries")
+    t0 = time.perf_counter()
+    bodies, eps = geo.batch_bodies()
+    out = {"problems": len(bodies), "force_points": int(sum(len(b.Geometries[0]) for b in bodies)),
+           "setup_geometry_s": round(time.perf_counter() - t0, 3)}
+    x64 = None
+    for prec in ("fp64", "mixed"):
+        t0 = time.perf_counter()
+        bm = nnk.BatchedMobility(bodies, eps, precision=prec, hipGraph=True)
+        ts = time.perf_counter() - t0
+        b = bm.rhs()
+        t0 = time.perf_counter()
+        x = bm.solve(b, 1e-6, 300)
+        dt = time.perf_counter() - t0
+        i = bm.last_solve
+        rec = {"setup_s": round(ts, 3), "wall_s": round(dt, 3), "max_iterations": int(i["iter"].max()),
+               "mean_iterations": float(i["iter"].mean()), "unconverged": int(i["flag"].sum()),
+               "max_relres": float(i["relres"].max())}
+        if x64 is None:
+            x64 = x
+        else:
+            rec["rel_diff_vs_fp64"] = float(np.linalg.norm(x - x64) / np.linalg.norm(x64))
+        out[prec] = rec
+    return out
+
+
+def adjoint_pair(nnk):
+    """Squirmer pair (radius 1, gap 0.5, B1 = 1, B2 = -5; 1536 force and
+    24576 quadrature points each, eps = 0.01): forward solve, one batched
+    transpose GMRES for the six translational-velocity functionals and the
+    adjoint gradients with respect to every node position and eps."""
+    import importlib as il
+    geo = il.import_module("stokes-nnkifmm_amd.geometries")
+    t0 = time.perf_counter()
+    sw = geo.squirmer_pair(n=16, refine=4, gap=0.5, B1=1.0, B2=-5.0)
+    bm6 = nnk.BatchedMobility([sw] * 6, [0.01] * 6)
+    ts = time.perf_counter() - t0
+    n = bm6.n // 6
+    N = int(bm6.npts[0])
+    t0 = time.perf_counter()
+    z = bm6.solve(bm6.rhs(), 1e-10, 400)
+    t_fwd = time.perf_counter() - t0
+    U = z[3 * N:3 * N + 6].reshape(3, 2).T
+    e = np.zeros(6 * n)
+    for k in range(6):   # U of body k % 2, component k // 2
+        e[k * n + 3 * N + (k // 2) * 2 + k % 2] = 1.0
+    t0 = time.perf_counter()
+    lam = bm6.solve(e, 1e-10, 400, transpose=True)
+    t_adj = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    gx, gy, ge = bm6.gradient(z, lam)
+    t_grad = time.perf_counter() - t0
+    return {"setup_s": round(ts, 3), "forward_s": round(t_fwd, 3), "transpose_gmres_s": round(t_adj, 3),
+            "gradient_kernels_s": round(t_grad, 3), "U_body0": U[0].tolist(), "U_body1": U[1].tolist(),
+            "dU_deps": ge.tolist(), "max_abs_dU_dx": float(max(np.abs(g).max() for g in gx)),
+            "adjoint_iterations": bm6.last_solve["iter"].tolist()}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -179,7 +292,7 @@ def main():
         sys.exit(2)
     import torch
     if args.copy_mb > 0:
-        n = args.copy_mb << 18   # doubles
+        n = args.copy_mb << 17   # doubles in copy_mb MiB
         a = torch.ones(n, dtype=torch.float64, device="cuda:0")
         b = torch.empty_like(a)
         for _ in range(3):
@@ -255,6 +368,13 @@ def main():
             out[f"speedup_vs_{name}"] = round(forms[name]["ms_per_step"] / head["ms_per_step"], 3)
     if not args.no_solve:
         out["mobility_solve_one_sphere"] = sphere_solve(nnk)
+        print("# sphere solve done", file=sys.stderr, flush=True)
+        out["mobility_solve_spheroids_wall"] = spheroid_solve(nnk)
+        print("# spheroid solve done", file=sys.stderr, flush=True)
+        out["batched_solves"] = batched_solves(nnk)
+        print("# batched solves done", file=sys.stderr, flush=True)
+        out["adjoint_squirmer_pair"] = adjoint_pair(nnk)
+        print("# adjoint done", file=sys.stderr, flush=True)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args, eps, mu)
     print(json.dumps(out), flush=True)
