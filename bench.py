@@ -62,6 +62,8 @@ def parse():
     ap.add_argument("--no-solve", action="store_true",
                     help="skip the mobility solves (one sphere, spheroids + wall, batched, adjoint)")
     ap.add_argument("--seed", type=int, default=20240611)
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default M2L launch shapes (no timing of candidates in the warm-up)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--cpu-n", type=int, default=1 << 16, help="CPU baseline sample size")
     ap.add_argument("--copy-mb", type=int, default=0,
@@ -94,7 +96,8 @@ def run_form(nnk, lib, tree, fd, args, eps, mu, precision, res, svd_tol):
     """Time one form of the matvec; returns (u, record)."""
     import torch
     P = ctypes.POINTER
-    fmm = nnk.KIFMM(tree, [eps, mu], "coronaRes", res, "precision", precision, "svdTol", svd_tol)
+    fmm = nnk.KIFMM(tree, [eps, mu], "coronaRes", res, "precision", precision, "svdTol", svd_tol,
+                    "autotune", 0 if args.no_autotune else 1)
     t0 = time.perf_counter()
     h = fmm._handle()
     torch.cuda.synchronize()
@@ -116,6 +119,9 @@ def run_form(nnk, lib, tree, fd, args, eps, mu, precision, res, svd_tol):
     lib.nnk_fmm_set_profiling(h.h, 0)
     st = np.zeros(14, np.int64)
     lib.nnk_fmm_stats(h.h, st.ctypes.data_as(P(ctypes.c_int64)), 14)
+    spl = np.zeros(16, np.int32)
+    dep = np.zeros(16, np.int32)
+    lib.nnk_fmm_launch_config(h.h, spl.ctypes.data_as(P(ctypes.c_int32)), dep.ctypes.data_as(P(ctypes.c_int32)), 16)
     K, R = int(st[12]), int(st[13])
     ms_step = dt / args.steps * 1e3
     phases = ["strengths", "S2M", "up", "M2L", "L2L", "P2L", "down", "leaf"]
@@ -124,16 +130,18 @@ def run_form(nnk, lib, tree, fd, args, eps, mu, precision, res, svd_tol):
         peak, kern = FP64_MFMA_PEAK_TFLOPS, "k_gemm_gather<160,64,16,1> (M2L, FP64 MFMA)"
     elif precision == "mixed":
         flops = 3 * 2.0 * R * R * st[0]    # three BF16 products per split product
-        peak, kern = BF16_MFMA_PEAK_TFLOPS, "k_m2l_bf16<2,3,1> (M2L, two-part split-BF16 MFMA)"
+        peak, kern = BF16_MFMA_PEAK_TFLOPS, "k_m2l_bf16<2,D,1> (M2L, two-part split-BF16 MFMA)"
     else:
         flops = 6 * 2.0 * R * R * st[0]    # six BF16 products per split product
-        peak, kern = BF16_MFMA_PEAK_TFLOPS, "k_m2l_bf16<3,2,1> (M2L, three-part split-BF16 MFMA)"
+        peak, kern = BF16_MFMA_PEAK_TFLOPS, "k_m2l_bf16<3,D,1> (M2L, three-part split-BF16 MFMA)"
     m2l_ms = ms[3] / args.steps
     achieved = flops / (m2l_ms * 1e-3) / 1e12 if m2l_ms > 0 else 0.0
     rec = {"precision": precision, "coronaRes": res, "svd_tol": svd_tol if precision != "fp64" else None,
            "rank": R if precision != "fp64" else K, "Mpts_s": round(tree.points.shape[0] / (ms_step * 1e-3) / 1e6, 4),
            "ms_per_step": round(ms_step, 3), "setup_s": round(t_setup, 3),
            "phase_ms_per_step": {p: round(float(ms[i] / args.steps), 3) for i, p in enumerate(phases)},
+           "m2l_launch_per_level": {str(l): {"split": int(spl[l]), "depth": int(dep[l])}
+                                    for l in range(16) if spl[l] > 0},
            "m2l": {"kernel": kern, "flops_per_step": flops, "ms_per_step": round(m2l_ms, 3),
                    "launches_per_step": float(cnt[3] / args.steps), "achieved_tflops": round(achieved, 3),
                    "peak_tflops": peak}}

@@ -165,6 +165,17 @@ int32_t nnk_mobility_solve(nnk_mobility *h, const double *rhs, double tol, int32
  *           replayed with one launch; results are bit-identical to on = 0. */
 int32_t nnk_fmm_set_graph(nnk_fmm *h, int32_t on);
 
+/* No counterpart in the reference: with on = 1 the first later apply
+ *           times the M2L launch candidates of every level on its own data
+ *           (split-K factor s/2, s, 2s of the default s; for the mixed forms
+ *           also the register-ring depth 2 or 3) and keeps the fastest.  The
+ *           choice changes summation order only (results stay within
+ *           rounding of each other and deterministic afterwards).
+ *           nnk_fmm_launch_config reports the per-level choice (split,
+ *           depth; 0 where a level has no M2L) for levels 0..nlev-1. */
+int32_t nnk_fmm_set_autotune(nnk_fmm *h, int32_t on);
+int32_t nnk_fmm_launch_config(const nnk_fmm *h, int32_t *split, int32_t *depth, int32_t nlev);
+
 /* No counterpart in the reference: the same for the GMRES operator (A or
  *           A P) of nnk_mobility_solve / nnk_mobility_solve_refined, one
  *           graph per operator form, captured on first use. */

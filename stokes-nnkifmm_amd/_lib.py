@@ -60,6 +60,8 @@ SIGNATURES = {
                                                     ctypes.c_double, ctypes.c_int32, _d, _i32, _d, _i32,
                                                     _i32]),
     "nnk_fmm_set_graph": (ctypes.c_int32, [_vp, ctypes.c_int32]),
+    "nnk_fmm_set_autotune": (ctypes.c_int32, [_vp, ctypes.c_int32]),
+    "nnk_fmm_launch_config": (ctypes.c_int32, [_vp, _i32, _i32, ctypes.c_int32]),
     "nnk_mobility_set_graph": (ctypes.c_int32, [_vp, ctypes.c_int32]),
     "nnk_mobility_destroy": (None, [_vp]),
     "nnk_batch_create": (ctypes.c_int32, [ctypes.c_int32, _i64, _d, _i64, _d, _i64, _i32, _d, _d, _d, _d, _i32,

@@ -300,6 +300,27 @@ int32_t nnk_fmm_set_graph(nnk_fmm *h, int32_t on) {
   });
 }
 
+int32_t nnk_fmm_set_autotune(nnk_fmm *h, int32_t on) {
+  return guard([&]() -> int32_t {
+    if (!h) return fail(NNK_EARG, "nnk_fmm_set_autotune: null handle");
+    h->f->set_autotune(on != 0);
+    return NNK_OK;
+  });
+}
+
+int32_t nnk_fmm_launch_config(const nnk_fmm *h, int32_t *split, int32_t *depth, int32_t nlev) {
+  return guard([&]() -> int32_t {
+    if (!h || !split || !depth || nlev < 0) return fail(NNK_EARG, "nnk_fmm_launch_config: bad argument");
+    for (int l = 0; l < nlev; l++) {
+      int a = 0, b = 0;
+      h->f->launch_config(l, &a, &b);
+      split[l] = a;
+      depth[l] = b;
+    }
+    return NNK_OK;
+  });
+}
+
 int32_t nnk_mobility_set_graph(nnk_mobility *h, int32_t on) {
   return guard([&]() -> int32_t {
     if (!h) return fail(NNK_EARG, "nnk_mobility_set_graph: null handle");

@@ -802,10 +802,8 @@ void Fmm::plan() {
           j++;
         }
         P.m2l_split = split_for((int64_t)nj * (Mpad_ / GBM) * ((mN + 63) / 64), minnb);
-        if (P.m2l_split > 1) {
-          P.m2l_slot.upload(slot);
-          reserve_part((size_t)P.m2l_split * nj * mN * K_);
-        }
+        P.m2l_slot.upload(slot);  // also for the autotuner's split candidates
+        reserve_part((size_t)std::min(64, 2 * P.m2l_split) * nj * mN * K_);
       }
       std::vector<GemmJob> j1, j2;
       add_job(P.dn_solve1, j1, {O.Wdn.as<double>()}, &dn, nullptr, P.n_down);
@@ -940,6 +938,79 @@ void Fmm::launch_leaf(int far, double *d_u, hipStream_t s) {
 }
 
 void Fmm::apply(const double *d_f, double *d_u, hipStream_t s) {
+  if (tune_ && !tuned_) {
+    apply_impl(d_f, d_u, s);  // fills the densities the candidates run on
+    autotune(s);
+    tuned_ = true;
+  }
+  apply_impl(d_f, d_u, s);
+}
+
+void Fmm::launch_config(int level, int *split, int *depth) const {
+  *split = 0;
+  *depth = 0;
+  if (level < 2 || level > maxlev_) return;
+  const LevelPlan &P = plans_[level];
+  if (prec_ >= 1) {
+    if (P.mx_m2l_jobs) { *split = P.mx_split; *depth = P.mx_depth; }
+  } else if (P.n_down) {
+    *split = P.m2l_split;
+  }
+}
+
+void Fmm::autotune(hipStream_t s) {
+  hipEvent_t a, b;
+  NNK_HIP(hipEventCreate(&a));
+  NNK_HIP(hipEventCreate(&b));
+  const bool prof = prof_;
+  prof_ = false;
+  try {
+    for (int l = 2; l <= maxlev_; l++) {
+      LevelPlan &P = plans_[l];
+      const bool mixed = prec_ >= 1;
+      if (mixed ? !P.mx_m2l_jobs : !P.n_down) continue;
+      const int s0 = mixed ? P.mx_split : P.m2l_split;
+      const size_t per_split = mixed ? (size_t)P.mx_m2l_jobs * P.mx_m2l_maxN * RP_
+                                     : (size_t)P.m2l.njobs * P.m2l.maxN * K_;
+      std::vector<int> splits = {s0};
+      if (s0 > 1) splits.push_back(std::max(1, s0 / 2));
+      if ((size_t)std::min(64, 2 * s0) * per_split <= part_elems_) splits.push_back(std::min(64, 2 * s0));
+      std::vector<int> depths = mixed ? std::vector<int>{2, 3} : std::vector<int>{0};
+      float best = 1e30f;
+      int bs = s0, bd = mixed ? P.mx_depth : 0;
+      for (int sp : splits) {
+        if (sp > 1 && !(mixed ? P.mx_slot.p : P.m2l_slot.p)) continue;
+        for (int dp : depths) {
+          auto run = [&]() {
+            if (mixed) launch_m2l_mixed(l, dp, sp, s);
+            else gemm(P.m2l, uppot_.as<double>(), K_, chk_.as<double>(), K_, 0, s, 1, sp, &P.m2l_slot);
+          };
+          run();  // warm
+          NNK_HIP(hipEventRecord(a, s));
+          run();
+          run();
+          NNK_HIP(hipEventRecord(b, s));
+          NNK_HIP(hipEventSynchronize(b));
+          float ms = 0;
+          NNK_HIP(hipEventElapsedTime(&ms, a, b));
+          if (ms < best) { best = ms; bs = sp; bd = dp; }
+        }
+      }
+      if (mixed) { P.mx_split = bs; P.mx_depth = bd; }
+      else P.m2l_split = bs;
+    }
+  } catch (...) {
+    prof_ = prof;
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    throw;
+  }
+  prof_ = prof;
+  NNK_HIP(hipEventDestroy(a));
+  NNK_HIP(hipEventDestroy(b));
+}
+
+void Fmm::apply_impl(const double *d_f, double *d_u, hipStream_t s) {
   if (prec_ >= 1) { apply_mixed(d_f, d_u, s); return; }
   mark(0, true, s);
   strengths(d_f, s);

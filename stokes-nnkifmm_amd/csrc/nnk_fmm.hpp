@@ -55,7 +55,7 @@ struct LevelPlan {
   // mixed form
   GemmBatch mx_m2m, mx_upP, mx_l2l, mx_xP, mx_downU, mx_wV;
   DBuf mx_m2l;                // GemmJobBf16[8]
-  int mx_split = 1;
+  int mx_split = 1, mx_depth = 3;  // split-K factor and register-ring depth of the M2L launch
   DBuf mx_slot;
   std::vector<DBuf> mx_owned;
   int mx_m2l_jobs = 0, mx_m2l_maxN = 0;
@@ -100,6 +100,10 @@ class Fmm {
   // the same matvec replayed from a HIP graph captured on first use (one
   // launch per call instead of one per kernel); bit-identical to apply
   void apply_graph(const double *d_f, double *d_u, hipStream_t s);
+  // autotune the M2L launch shapes on the first apply (off by default)
+  void set_autotune(bool on) { tune_ = on; }
+  // per level: M2L split-K factor and (mixed forms) register-ring depth
+  void launch_config(int level, int *split, int *depth) const;
   // leaf-local product (reducedPass.m) on device pointers
   void reduced_apply(const double *d_f, double *d_u, hipStream_t s);
   // leaf-block solve (ReducedSolve.m): f = blockdiag(K_leaf)^-1 v, with the
@@ -141,6 +145,12 @@ class Fmm {
   void build_mixed();
   void plan_mixed();
   void apply_mixed(const double *d_f, double *d_u, hipStream_t s);
+  void apply_impl(const double *d_f, double *d_u, hipStream_t s);
+  void launch_m2l_mixed(int l, int depth, int split, hipStream_t s);
+  // time the M2L launch candidates of every level on the current densities
+  // (register-ring depth 2/3 for the mixed forms, split-K factor s/2, s, 2s)
+  // and keep the fastest
+  void autotune(hipStream_t s);
   void launch_surf(const int32_t *nodes, int n, const int64_t *lptr, const int32_t *lidx, int down,
                    int accumulate, double *chk, hipStream_t s);
   void launch_leaf(int far, double *d_u, hipStream_t s);
@@ -183,6 +193,7 @@ class Fmm {
   int red_maxn_ = 0;
   int64_t stats_[kStats] = {0};
   bool prof_ = false;
+  bool tune_ = false, tuned_ = false;
   GraphOp gop_;
   struct Mark { int phase; hipEvent_t a, b; };
   std::vector<Mark> marks_;

@@ -355,16 +355,42 @@ void Fmm::plan_mixed() {
         j++;
       }
       P.mx_split = split_for((int64_t)nj * (RPm_ / MBM) * ((mN + MBN - 1) / MBN), minnb);
-      if (P.mx_split > 1) {
-        P.mx_slot.upload(slot);
-        reserve_part((size_t)P.mx_split * nj * mN * RP_);
-      }
+      P.mx_depth = np_ == 3 ? 2 : 3;
+      P.mx_slot.upload(slot);  // also for the autotuner's split candidates
+      reserve_part((size_t)std::min(64, 2 * P.mx_split) * nj * mN * RP_);
     }
     P.mx_l2l.njobs = (int)lj.size();
     if (!lj.empty()) P.mx_l2l.jobs.upload(lj);
   }
   n_upall_ = (int)upall.size();
   upall_.upload(upall);
+}
+
+// M2L of level l in the mixed form with register-ring depth `depth` (2 or 3)
+// and split-K factor `split` (the part buffer must hold it)
+void Fmm::launch_m2l_mixed(int l, int depth, int split, hipStream_t s) {
+  LevelPlan &P = plans_[l];
+  if (!P.mx_m2l_jobs) return;
+  const int nrb = RPm_ / MBM, nct = (P.mx_m2l_maxN + MBN - 1) / MBN;
+  const int nj = P.mx_m2l_jobs, mN = P.mx_m2l_maxN;
+  if (split > 1 && (size_t)split * nj * mN * RP_ > part_elems_) throw Error(NNK_ESTATE, "mixed M2L: split buffer too small");
+  const dim3 grid((unsigned)(nj * split * nrb * nct));
+  const int zrow = tree_.count;
+  double *d = dhat_.as<double>();
+  double *pp = split > 1 ? part_.as<double>() : nullptr;
+  auto launch = [&](auto kern) {
+    hipLaunchKernelGGL(kern, grid, dim3(512), 0, s, R_, KCm_, RT_, chat_x_.as<uint16_t>(), zrow, d, RP_,
+                       P.mx_m2l.as<GemmJobBf16>(), nj, nrb, split, pp, mN);
+  };
+  if (np_ == 3) {
+    if (depth == 3) launch(k_m2l_bf16<3, 3, 1>);
+    else launch(k_m2l_bf16<3, 2, 1>);
+  } else {
+    if (depth == 2) launch(k_m2l_bf16<2, 2, 1>);
+    else launch(k_m2l_bf16<2, 3, 1>);
+  }
+  NNK_HIP(hipGetLastError());
+  if (split > 1) split_reduce(P.mx_slot, nj * mN, R_, RP_, split, d, s);
 }
 
 void Fmm::apply_mixed(const double *d_f, double *d_u, hipStream_t s) {
@@ -397,18 +423,7 @@ void Fmm::apply_mixed(const double *d_f, double *d_u, hipStream_t s) {
     LevelPlan &P = plans_[l];
     if (P.mx_m2l_jobs) {
       mark(3, true, s);
-      const int nrb = RPm_ / MBM, nct = (P.mx_m2l_maxN + MBN - 1) / MBN, sp = P.mx_split;
-      const dim3 grid((unsigned)(P.mx_m2l_jobs * sp * nrb * nct));
-      const int zrow = tree_.count;
-      double *pp = sp > 1 ? part_.as<double>() : nullptr;
-      if (np_ == 3)
-        hipLaunchKernelGGL((k_m2l_bf16<3, 2, 1>), grid, dim3(512), 0, s, R, KCm_, RT_, chat_x_.as<uint16_t>(),
-                           zrow, d, RP_, P.mx_m2l.as<GemmJobBf16>(), P.mx_m2l_jobs, nrb, sp, pp, P.mx_m2l_maxN);
-      else
-        hipLaunchKernelGGL((k_m2l_bf16<2, 3, 1>), grid, dim3(512), 0, s, R, KCm_, RT_, chat_x_.as<uint16_t>(),
-                           zrow, d, RP_, P.mx_m2l.as<GemmJobBf16>(), P.mx_m2l_jobs, nrb, sp, pp, P.mx_m2l_maxN);
-      NNK_HIP(hipGetLastError());
-      if (sp > 1) split_reduce(P.mx_slot, P.mx_m2l_jobs * P.mx_m2l_maxN, R, RP_, sp, d, s);
+      launch_m2l_mixed(l, P.mx_depth, P.mx_split, s);
       mark(3, false, s);
     }
     mark(4, true, s);

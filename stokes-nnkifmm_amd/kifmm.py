@@ -15,7 +15,7 @@ from ._lib import c64, check, lib, ptr
 from .octree import parse_options
 
 _OPTS = {"GPU": 0, "parThreads": 0, "blockSize": 0.1, "GMRES": 0, "coronaRes": 6, "coronaShells": 1,
-         "format": 1, "precision": "fp64", "svdTol": 1e-8, "hipGraph": 0}
+         "format": 1, "precision": "fp64", "svdTol": 1e-8, "hipGraph": 0, "autotune": 0}
 _PREC = {"fp64": 0, "mixed": 1, "mixed3": 2}
 
 
@@ -24,13 +24,15 @@ def _is_torch(x):
 
 
 class _Handle:
-    def __init__(self, tree, eps, mu, res, shells, precision, svd_tol=1e-8, graph=False):
+    def __init__(self, tree, eps, mu, res, shells, precision, svd_tol=1e-8, graph=False, autotune=False):
         h = ctypes.c_void_p()
         check(lib().nnk_fmm_create(tree.handle, float(eps), float(mu), int(res), int(shells),
                                    _PREC[precision], float(svd_tol), ctypes.byref(h)))
         self.h = h
         if graph:
             check(lib().nnk_fmm_set_graph(h, 1))
+        if autotune:
+            check(lib().nnk_fmm_set_autotune(h, 1))
         self.tree = tree  # the tree must outlive the handle
 
     def __del__(self):
@@ -51,7 +53,8 @@ class KIFMM:
     the reference's numbers, 'mixed' or 'mixed3': two- or three-part
     split-BF16 M2L), 'svdTol' (the mixed forms' pseudo-inverse cut) and
     'hipGraph' (1: replay each matvec, and the mobility solve's GMRES
-    operator, from a captured HIP graph)."""
+    operator, from a captured HIP graph) and 'autotune' (1: pick the M2L
+    launch shapes by timing candidates on the first matvec)."""
 
     def __init__(self, tree, kernelPar, *args, **kwargs):
         self.tree = tree
@@ -70,11 +73,11 @@ class KIFMM:
         res = a["coronaRes"] if res is None else res
         shells = a["coronaShells"] if shells is None else shells
         key = (int(res), int(shells), a["precision"], float(a["kernelPar"][0]), float(a["kernelPar"][1]),
-               float(a["svdTol"]), bool(a["hipGraph"]))
+               float(a["svdTol"]), bool(a["hipGraph"]), bool(a["autotune"]))
         h = self._handles.get(key)
         if h is None:
             h = _Handle(self.tree, a["kernelPar"][0], a["kernelPar"][1], res, shells, a["precision"],
-                        a["svdTol"], bool(a["hipGraph"]))
+                        a["svdTol"], bool(a["hipGraph"]), bool(a["autotune"]))
             self._handles[key] = h
         return h
 

@@ -48,3 +48,30 @@ def test_mobility_solve_graph_matches_eager(oracle, nnk, precon):
     (x0, i0), (x1, i1) = out
     assert i0["iter"] == i1["iter"] and i0["flag"] == i1["flag"] == 0
     np.testing.assert_array_equal(x0, x1)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed3"])
+def test_autotuned_launch_shapes(nnk, precision):
+    """The autotuner only reorders sums: tuned results agree with the
+    default launch shapes to rounding, are deterministic once tuned, and
+    the chosen per-level shapes are reported."""
+    import ctypes
+    rng = np.random.default_rng(12)
+    p = rng.random((60000, 3))
+    t = nnk.OcTree(p, p, "nodeCapacity", 32)
+    q = rng.standard_normal(p.shape)
+    u0 = nnk.KIFMM(t, [1e-3, 1.0], "precision", precision).computeVel(q)
+    f = nnk.KIFMM(t, [1e-3, 1.0], "precision", precision, "autotune", 1)
+    u1 = f.computeVel(q)
+    u2 = f.computeVel(q)
+    np.testing.assert_array_equal(u1, u2)
+    tol = 1e-12 if precision == "fp64" else 1e-6
+    assert np.linalg.norm(u1 - u0) <= tol * np.linalg.norm(u0)
+    split = np.zeros(16, np.int32)
+    depth = np.zeros(16, np.int32)
+    i32 = ctypes.POINTER(ctypes.c_int32)
+    assert nnk.lib().nnk_fmm_launch_config(f._handle().h, split.ctypes.data_as(i32), depth.ctypes.data_as(i32),
+                                           16) == 0
+    assert (split[2:] >= 0).all() and split[2] >= 1
+    if precision != "fp64":
+        assert set(depth[depth > 0]) <= {2, 3}
