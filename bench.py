@@ -11,8 +11,11 @@ Forms timed (all reported under "forms"):
                exact check-to-equivalent inverses are ill-conditioned)
   mixed_res6   truncated-SVD pseudo-inverses, two-part split-BF16 M2L on the
                matrix cores, FP32-math direct sums
-  mixed3_res8  the same with three-part split-BF16 M2L at coronaRes 8: the
-               first form below 1e-6 relative error, the headline `value`
+  mixed3_res8  the same with three-part split-BF16 M2L at coronaRes 8 (below
+               1e-6 relative error)
+  mixedh_res8  two-part split-FP16 M2L on power-of-two scaled operands at
+               coronaRes 8: the same accuracy with half the MFMAs, the
+               headline `value`
 
 Prints one JSON line.  Besides the throughput it reports:
   roofline      the headline form's M2L translation GEMM (the dominant
@@ -57,7 +60,7 @@ def parse():
     ap.add_argument("--n", type=int, default=1 << 20)
     ap.add_argument("--res", type=int, default=6, help="corona resolution of the CPU baseline")
     ap.add_argument("--svd-tol", type=float, default=1e-8, help="pseudo-inverse cut of the mixed forms")
-    ap.add_argument("--forms", default="fp64_res6,fp64_res8,mixed_res6,mixed3_res8",
+    ap.add_argument("--forms", default="fp64_res6,fp64_res8,mixed_res6,mixed3_res8,mixedh_res8",
                     help="comma-separated forms to time; the last one is the headline")
     ap.add_argument("--no-solve", action="store_true",
                     help="skip the mobility solves (one sphere, spheroids + wall, batched, adjoint)")
@@ -130,10 +133,13 @@ def run_form(nnk, lib, tree, fd, args, eps, mu, precision, res, svd_tol):
         peak, kern = FP64_MFMA_PEAK_TFLOPS, "k_gemm_gather<160,64,16,1> (M2L, FP64 MFMA)"
     elif precision == "mixed":
         flops = 3 * 2.0 * R * R * st[0]    # three BF16 products per split product
-        peak, kern = BF16_MFMA_PEAK_TFLOPS, "k_m2l_bf16<2,D,1> (M2L, two-part split-BF16 MFMA)"
-    else:
+        peak, kern = BF16_MFMA_PEAK_TFLOPS, "k_m2l_split<2,D,1,false> (M2L, two-part split-BF16 MFMA)"
+    elif precision == "mixed3":
         flops = 6 * 2.0 * R * R * st[0]    # six BF16 products per split product
-        peak, kern = BF16_MFMA_PEAK_TFLOPS, "k_m2l_bf16<3,D,1> (M2L, three-part split-BF16 MFMA)"
+        peak, kern = BF16_MFMA_PEAK_TFLOPS, "k_m2l_split<3,D,1,false> (M2L, three-part split-BF16 MFMA)"
+    else:
+        flops = 3 * 2.0 * R * R * st[0]    # three FP16 products per split product
+        peak, kern = BF16_MFMA_PEAK_TFLOPS, "k_m2l_split<2,D,1,true> (M2L, two-part split-FP16 MFMA)"
     m2l_ms = ms[3] / args.steps
     achieved = flops / (m2l_ms * 1e-3) / 1e12 if m2l_ms > 0 else 0.0
     rec = {"precision": precision, "coronaRes": res, "svd_tol": svd_tol if precision != "fp64" else None,
@@ -325,7 +331,7 @@ def main():
     names = [f for f in args.forms.split(",") if f]
     for name in names:
         prec, res = name.split("_res")
-        if prec not in ("fp64", "mixed", "mixed3"):
+        if prec not in ("fp64", "mixed", "mixed3", "mixedh"):
             raise SystemExit(f"unknown form {name}")
         u, rec = run_form(nnk, lib, tree, fd, args, eps, mu, prec, int(res),
                           0.0 if prec == "fp64" else args.svd_tol)
@@ -358,7 +364,8 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"mixed": "bf16x2-split-mfma/f32/f64", "mixed3": "bf16x3-split-mfma/f32/f64"}.get(
+        "dtype": {"mixed": "bf16x2-split-mfma/f32/f64", "mixed3": "bf16x3-split-mfma/f32/f64",
+                  "mixedh": "fp16x2-split-mfma/f32/f64"}.get(
             head["precision"], "f64"),
         "data": "synthetic",
         "config": {"workload": f"KIFMM matvec, N={N} uniform points in the unit cube, forces N(0,1), "

@@ -47,8 +47,11 @@ enum { NNK_LIST_U = 0, NNK_LIST_V = 1, NNK_LIST_W = 2, NNK_LIST_X = 3 };
  * cores with operands split in two BF16 parts (3 products, ~16 bits) and
  * FP32 accumulation promoted to FP64 per translation, FP32-math direct sums
  * with FP64 accumulation.
- * NNK_MIXED3: as NNK_MIXED with three BF16 parts (6 products, ~24 bits). */
-enum { NNK_FP64 = 0, NNK_MIXED = 1, NNK_MIXED3 = 2 };
+ * NNK_MIXED3: as NNK_MIXED with three BF16 parts (6 products, ~24 bits).
+ * NNK_MIXED_H: as NNK_MIXED with two FP16 parts (3 products, ~22 bits) on
+ * operands scaled by powers of two (per operator row; per matvec for the
+ * coefficients). */
+enum { NNK_FP64 = 0, NNK_MIXED = 1, NNK_MIXED3 = 2, NNK_MIXED_H = 3 };
 
 /* Library version (major*10000 + minor*100 + patch). */
 int32_t nnk_version(void);
@@ -92,7 +95,7 @@ void nnk_tree_destroy(nnk_tree *t);
  *           Classes/KIFMM.m:46-61.  Builds the device-resident operator:
  *           per-level check-to-equivalent factors and translation
  *           operators, point data in leaf order.  precision is NNK_FP64,
- *           NNK_MIXED or NNK_MIXED3; svd_tol (mixed forms only, <= 0 for the
+ *           NNK_MIXED, NNK_MIXED3 or NNK_MIXED_H; svd_tol (mixed forms only, <= 0 for the
  *           default 1e-8)
  *           is the relative singular-value cut of the pseudo-inverses.  The
  *           tree must stay alive while the handle exists. */

@@ -146,7 +146,7 @@ int32_t nnk_fmm_create(const nnk_tree *t, double eps, double mu, int32_t res, in
   return guard([&]() -> int32_t {
     if (!t || !out) return fail(NNK_EARG, "nnk_fmm_create: null argument");
     *out = nullptr;
-    if (precision != NNK_FP64 && precision != NNK_MIXED && precision != NNK_MIXED3)
+    if (precision != NNK_FP64 && precision != NNK_MIXED && precision != NNK_MIXED3 && precision != NNK_MIXED_H)
       return fail(NNK_EARG, "nnk_fmm_create: unknown precision");
     auto f = std::make_unique<nnk::Fmm>(t->t, eps, mu, res, shells, precision,
                                         svd_tol > 0 ? svd_tol : 1e-8);

@@ -413,7 +413,7 @@ Fmm::Fmm(std::shared_ptr<const Tree> t, double eps, double mu, int res, int shel
     : tp_(std::move(t)), tree_(*tp_), eps_(eps), mu_(mu), res_(res), shells_(shells),
       prec_(precision), tau_(tau) {
   const Tree &T = tree_;
-  if (precision < 0 || precision > 2) throw Error(NNK_EARG, "unknown precision");
+  if (precision < 0 || precision > 3) throw Error(NNK_EARG, "unknown precision");
   if (precision >= 1 && !(tau > 0 && tau < 1)) throw Error(NNK_EARG, "truncation tolerance must be in (0,1)");
   if (res < 2 || res > 16 || shells < 1 || 2 * shells > res)
     throw Error(NNK_EARG, "coronaRes must be in [2,16] and 1 <= coronaShells <= res/2");

@@ -73,8 +73,10 @@ struct MixedOps {
   DBuf Pup, Vr, Pdn, Ur;      // FP64 At layout: R x K, K x R, R x K, K x R
   DBuf VrT, UrT;              // R rows of length K (precompute inputs)
   DBuf Mc, Lc;                // 8 each, FP64 At (R x R)
-  DBuf Cf;                    // 316 M2L operators, NP bf16 parts in MFMA fragment order
+  DBuf Cf;                    // 316 M2L operators, NP 16-bit parts in MFMA fragment order
+  DBuf Uf;                    // FP16 form: per operator row, the unscale factor
   std::vector<const uint16_t *> cop;  // per offset slot (343)
+  std::vector<const float *> uop;
 };
 
 struct LevelOps {
@@ -89,7 +91,8 @@ class Fmm {
  public:
   // precision 0: FP64 form (reference parity); 1: mixed form (truncated SVD
   // at relative tolerance tau, two-part split-BF16 M2L, FP32-math direct
-  // sums); 2: as 1 with three-part split-BF16 M2L
+  // sums); 2: as 1 with three-part split-BF16 M2L; 3: as 1 with two-part
+  // split-FP16 M2L on power-of-two scaled operands
   Fmm(std::shared_ptr<const Tree> t, double eps, double mu, int res, int shells, int precision = 0,
       double tau = 1e-10);
   ~Fmm();
@@ -182,7 +185,9 @@ class Fmm {
   std::vector<std::vector<double>> svdU_, svdS_, svdV_;  // host, per level
   std::vector<MixedOps> mops_;
   DBuf chat_, dhat_, chat_x_, upall_;
-  int np_ = 2, KCm_ = 0, RT_ = 0;   // mixed M2L: bf16 parts, 32-deep k chunks, 16-row tiles
+  int np_ = 2, KCm_ = 0, RT_ = 0;   // mixed M2L: 16-bit parts, 32-deep k chunks, 16-row tiles
+  bool f16_ = false;                // FP16 parts (with scaling) instead of BF16
+  DBuf amax_, xun_;                 // FP16 form: max |coefficient|, coefficient unscale factor
   DBuf part_;                       // split-K partial tiles
   size_t part_elems_ = 0;
   int n_upall_ = 0;

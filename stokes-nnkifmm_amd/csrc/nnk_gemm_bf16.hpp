@@ -1,15 +1,20 @@
-// Split-BF16 gathered-column block GEMM on the BF16 matrix cores
-// (v_mfma_f32_16x16x32_bf16), for the mixed-precision M2L:
+// Split-precision gathered-column block GEMM on the 16-bit matrix cores
+// (v_mfma_f32_16x16x32_bf16 / _f16), for the mixed-precision M2L:
 //
 //   D[:, cidx[n]] = sum_b A_b X[:, gidx[b][n]]      (D in FP64)
 //
-// Every operand is held as NP BF16 parts, v ~ p0 + p1 (+ p2), each part the
-// rounding of what the previous ones leave (16 or 24 significant bits).
-// Products keep the terms down to the order of the last part:
+// Every operand is held as NP 16-bit parts, v ~ p0 + p1 (+ p2), each part
+// the rounding of what the previous ones leave.  Products keep the terms
+// down to the order of the last part:
 //   NP = 2:  p1*q0 + p0*q1 + p0*q0                      (3 MFMAs)
 //   NP = 3:  p2*q0 + p1*q1 + p0*q2 + p1*q0 + p0*q1 + p0*q0   (6 MFMAs)
 // accumulated in FP32 over one operator block (K = R) and then added into
 // FP64 accumulators, so rounding does not grow with the 189-block sum.
+// BF16 parts carry 8 bits each (16 or 24 in total).  FP16 parts (F16, NP =
+// 2) carry 11 bits each (22 in total) but need their operands in range: each
+// operator row is stored scaled by a power of two (its unscale factor per
+// row in U_b, applied when the block's FP32 sums are promoted) and the
+// coefficients by one power of two per matvec (*xun, applied at the end).
 //
 // Storage is in MFMA fragment order, so that every global load, LDS store
 // and LDS load is a contiguous 1 KB per wavefront:
@@ -31,12 +36,14 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 
 #include "nnk_fmm.hpp"
 
 namespace nnk {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 // global-address-space views (pointers read from memory are otherwise
@@ -46,24 +53,28 @@ typedef const int32_t __attribute__((address_space(1))) *gi32p;
 
 struct GemmJobBf16 {
   const uint16_t *const *A;  // nb operators in fragment order
+  const float *const *U;     // F16 form: per operator, the unscale factor of each row
   const int32_t *gidx;       // nb x N gather slots (-1 = none)
   const int32_t *cidx;       // N output slots
   int nb, N;
   const int32_t *tptr, *tlist;  // per 128-column tile: its active blocks, in order
 };
 
-template <int NP, int D, int TAG>
-__global__ __launch_bounds__(512, 1) void k_m2l_bf16(const int M, const int KC, const int RT,
-                                                     const uint16_t *__restrict__ X, const int zrow,
-                                                     double *__restrict__ C, const int ldc,
-                                                     const GemmJobBf16 *__restrict__ jobs, const int njobs,
-                                                     const int nrb, const int nsplit, double *__restrict__ P,
-                                                     const int maxN) {
+template <int NP, int D, int TAG, bool F16>
+__global__ __launch_bounds__(512, 1) void k_m2l_split(const int M, const int KC, const int RT,
+                                                      const uint16_t *__restrict__ X, const int zrow,
+                                                      double *__restrict__ C, const int ldc,
+                                                      const GemmJobBf16 *__restrict__ jobs, const int njobs,
+                                                      const int nrb, const int nsplit, double *__restrict__ P,
+                                                      const int maxN, const double *__restrict__ xun) {
+  static_assert(!F16 || NP == 2, "FP16 parts: two");
   constexpr int BN = 128, TM = 4, TN = 2, TILE = 512;
+  typedef typename std::conditional<F16, f16x8, bf16x8>::type op8;
   __shared__ __attribute__((aligned(16))) uint16_t As[2][NP][8][TILE];
   __shared__ __attribute__((aligned(16))) uint16_t Xs[2][NP][8][TILE];
   __shared__ int act[512];
   __shared__ const uint16_t *aptr[512];
+  __shared__ const float *uptr[F16 ? 512 : 1];
 
   const int bid = blockIdx.x;
   const int jb = bid % njobs;
@@ -85,6 +96,7 @@ __global__ __launch_bounds__(512, 1) void k_m2l_bf16(const int M, const int KC, 
     const int b = J.tlist[t0 + i];
     act[i] = b;
     aptr[i] = J.A[b];
+    if constexpr (F16) uptr[i] = J.U[b];
   }
   __syncthreads();
   const int a0 = (int)((int64_t)nact * sp / nsplit), a1 = (int)((int64_t)nact * (sp + 1) / nsplit);
@@ -148,14 +160,18 @@ __global__ __launch_bounds__(512, 1) void k_m2l_bf16(const int M, const int KC, 
       for (int r = 0; r < 4; r++) acc64[i][j][r] = 0.0;
     }
 
+  auto mfma = [](const op8 &a, const op8 &b, const f32x4 &c) -> f32x4 {
+    if constexpr (F16) return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+    else return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+  };
   auto compute = [&](int s) {
-    bf16x8 a[NP][TM], x[NP][TN];
+    op8 a[NP][TM], x[NP][TN];
 #pragma unroll
     for (int p = 0; p < NP; p++) {
 #pragma unroll
-      for (int i = 0; i < TM; i++) a[p][i] = *reinterpret_cast<const bf16x8 *>(&As[s][p][wm * 4 + i][lane * 8]);
+      for (int i = 0; i < TM; i++) a[p][i] = *reinterpret_cast<const op8 *>(&As[s][p][wm * 4 + i][lane * 8]);
 #pragma unroll
-      for (int j = 0; j < TN; j++) x[p][j] = *reinterpret_cast<const bf16x8 *>(&Xs[s][p][wn * 2 + j][lane * 8]);
+      for (int j = 0; j < TN; j++) x[p][j] = *reinterpret_cast<const op8 *>(&Xs[s][p][wn * 2 + j][lane * 8]);
     }
 #pragma unroll
     for (int i = 0; i < TM; i++)
@@ -163,23 +179,30 @@ __global__ __launch_bounds__(512, 1) void k_m2l_bf16(const int M, const int KC, 
       for (int j = 0; j < TN; j++) {
         f32x4 t = acc[i][j];
         if constexpr (NP == 3) {
-          t = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2][i], x[0][j], t, 0, 0, 0);
-          t = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1][i], x[1][j], t, 0, 0, 0);
-          t = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][i], x[2][j], t, 0, 0, 0);
+          t = mfma(a[2][i], x[0][j], t);
+          t = mfma(a[1][i], x[1][j], t);
+          t = mfma(a[0][i], x[2][j], t);
         }
-        t = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1][i], x[0][j], t, 0, 0, 0);
-        t = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][i], x[1][j], t, 0, 0, 0);
-        t = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][i], x[0][j], t, 0, 0, 0);
+        t = mfma(a[1][i], x[0][j], t);
+        t = mfma(a[0][i], x[1][j], t);
+        t = mfma(a[0][i], x[0][j], t);
         acc[i][j] = t;
       }
   };
-  auto promote = [&]() {
+  const int mrow0 = rb * 128 + wm * 64 + 4 * lg;  // this lane's first output row
+  auto promote = [&](int bi) {
+    f32x4 un[TM];
+#pragma unroll
+    for (int i = 0; i < TM; i++) {
+      if constexpr (F16) un[i] = *reinterpret_cast<const f32x4 *>(uptr[bi] + mrow0 + 16 * i);
+      else un[i] = f32x4{1, 1, 1, 1};
+    }
 #pragma unroll
     for (int i = 0; i < TM; i++)
 #pragma unroll
       for (int j = 0; j < TN; j++) {
 #pragma unroll
-        for (int r = 0; r < 4; r++) acc64[i][j][r] += (double)acc[i][j][r];
+        for (int r = 0; r < 4; r++) acc64[i][j][r] += (double)acc[i][j][r] * (double)un[i][r];
         acc[i][j] = f32x4{0, 0, 0, 0};
       }
   };
@@ -199,7 +222,7 @@ __global__ __launch_bounds__(512, 1) void k_m2l_bf16(const int M, const int KC, 
         fetch(cc + D, R[d]);
         if (cc < total) {
           compute(cc & 1);
-          if ((cc + 1) % KC == 0) promote();  // end of an operator block
+          if ((cc + 1) % KC == 0) promote(a0 + cc / KC);  // end of an operator block
         }
         stash(R[(d + 1) % D], (cc + 1) & 1);
         __syncthreads();
@@ -208,6 +231,7 @@ __global__ __launch_bounds__(512, 1) void k_m2l_bf16(const int M, const int KC, 
   }
 
   const int m0 = rb * 128 + wm * 64;
+  const double xs = F16 ? *xun : 1.0;
 #pragma unroll
   for (int i = 0; i < TM; i++)
 #pragma unroll
@@ -219,7 +243,7 @@ __global__ __launch_bounds__(512, 1) void k_m2l_bf16(const int M, const int KC, 
 #pragma unroll
       for (int r = 0; r < 4; r++) {
         const int row = m0 + i * 16 + 4 * lg + r;
-        if (row < M) out[row] = acc64[i][j][r];
+        if (row < M) out[row] = acc64[i][j][r] * xs;
       }
     }
 }

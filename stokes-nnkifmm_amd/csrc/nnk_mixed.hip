@@ -47,7 +47,101 @@ __device__ __forceinline__ uint16_t split_part(double v, int np, int p) {
   return h;
 }
 
+__device__ __forceinline__ uint16_t f2h(double v) {
+  const _Float16 h = (_Float16)(float)v;
+  return __builtin_bit_cast(uint16_t, h);
+}
+__device__ __forceinline__ double h2d(uint16_t b) { return (double)(float)__builtin_bit_cast(_Float16, b); }
+
+// FP16 parts: v ~ h0 + h1
+__device__ __forceinline__ uint16_t split_part_h(double v, int p) {
+  const uint16_t h0 = f2h(v);
+  return p == 0 ? h0 : f2h(v - h2d(h0));
+}
+
+// 2^k with k = floor(log2(a)), a > 0 (exact power of two)
+__device__ __forceinline__ double pow2_floor(double a) {
+  int e;
+  (void)frexp(a, &e);  // a = m 2^e, m in [0.5, 1)
+  return ldexp(1.0, e - 1);
+}
+
 }  // namespace
+
+// max |X[rows[i]][k]| over k < R into *amax (bit pattern of a non-negative
+// double, so unsigned max orders it)
+__global__ void k_absmax(const int32_t *__restrict__ rows, int nrows, const double *__restrict__ X, int ldx, int R,
+                         unsigned long long *amax) {
+  __shared__ double red[256];
+  double m = 0.0;
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < (int64_t)nrows * R;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    const int i = (int)(idx / R), k = (int)(idx % R);
+    m = fmax(m, fabs(X[(int64_t)rows[i] * ldx + k]));
+  }
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + st]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicMax(amax, (unsigned long long)__double_as_longlong(red[0]));
+}
+
+__global__ void k_amax_reset(unsigned long long *amax) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *amax = 0ull;
+}
+
+// FP16 coefficient parts with one power-of-two scale per matvec: x 2^14 /
+// 2^floor(log2 max|X|); *xun receives the inverse
+__global__ void k_split_rows_h(const int32_t *__restrict__ rows, int nrows, const double *__restrict__ X, int ldx,
+                               int R, int KC, const unsigned long long *__restrict__ amax, uint16_t *__restrict__ out,
+                               double *__restrict__ xun) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const double a = __longlong_as_double((long long)*amax);
+  const double sc = a > 0 ? 16384.0 / pow2_floor(a) : 1.0;
+  if (idx == 0) *xun = 1.0 / sc;
+  const int kk = KC * 32;
+  if (idx >= (int64_t)nrows * kk) return;
+  const int i = (int)(idx / kk), k = (int)(idx % kk);
+  const int64_t r = rows[i];
+  const double v = k < R ? X[r * ldx + k] * sc : 0.0;
+  const int kc = k >> 5, e = k & 31;
+  for (int p = 0; p < 2; p++) out[((r * KC + kc) * 2 + p) * 32 + e] = split_part_h(v, p);
+}
+
+// per operator row: unscale factor 2^floor(log2 max_n |C_j[m][n]|) / 2^14
+// (0 for rows beyond R or all-zero rows)
+__global__ void k_op_rowscale(const double *__restrict__ CT, int ldct, int R, int nops, int RPm,
+                              float *__restrict__ un) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (int64_t)nops * RPm) return;
+  const int j = (int)(idx / RPm), m = (int)(idx % RPm);
+  double a = 0.0;
+  if (m < R)
+    for (int n = 0; n < R; n++) a = fmax(a, fabs(CT[((size_t)j * R + n) * ldct + m]));
+  un[idx] = a > 0 ? (float)(pow2_floor(a) / 16384.0) : 0.0f;
+}
+
+// FP16 operator parts in fragment order (as k_split_op), rows scaled by
+// the inverse of their unscale factor
+__global__ void k_split_op_h(const double *__restrict__ CT, int ldct, int R, int nops, int KC, int RT,
+                             const float *__restrict__ un, uint16_t *__restrict__ out) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t per = (int64_t)KC * 2 * RT * 512;
+  if (idx >= per * nops) return;
+  const int j = (int)(idx / per);
+  int64_t rem = idx % per;
+  const int e = (int)(rem & 7), q = (int)((rem >> 3) & 63);
+  rem >>= 9;
+  const int t = (int)(rem % RT);
+  rem /= RT;
+  const int p = (int)(rem % 2), kc = (int)(rem / 2);
+  const int m = 16 * t + (q & 15), k = 32 * kc + 8 * (q >> 4) + e;
+  const float u = un[(size_t)j * RT * 16 + m];
+  const double v = (m < R && k < R && u > 0) ? CT[((size_t)j * R + k) * ldct + m] / (double)u : 0.0;
+  out[idx] = split_part_h(v, p);
+}
 
 // rows[i] of X (length R, stride ldx) -> parts in [node][KC][np][32] order
 // (zero beyond R)
@@ -96,6 +190,7 @@ void Fmm::build_mixed() {
   }
   if (R_ == 0) R_ = 32;
   np_ = prec_ == 2 ? 3 : 2;
+  f16_ = prec_ == 3;
   RP_ = (R_ + 31) / 32 * 32;
   RPm_ = (R_ + MBM - 1) / MBM * MBM;
   KCm_ = RP_ / 32;
@@ -176,12 +271,25 @@ void Fmm::build_mixed() {
       const size_t per = (size_t)KCm_ * np_ * RT_ * 512;
       X.Cf.alloc(sizeof(uint16_t) * per * no);
       const int64_t tot = (int64_t)per * no;
-      hipLaunchKernelGGL(k_split_op, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, 0, CT.as<double>(),
-                         MpadR_, R, no, KCm_, np_, RT_, X.Cf.as<uint16_t>());
+      if (f16_) {
+        X.Uf.alloc(sizeof(float) * (size_t)RPm_ * no);
+        const int64_t nr = (int64_t)RPm_ * no;
+        hipLaunchKernelGGL(k_op_rowscale, dim3((unsigned)((nr + 255) / 256)), dim3(256), 0, 0, CT.as<double>(),
+                           MpadR_, R, no, RPm_, X.Uf.as<float>());
+        hipLaunchKernelGGL(k_split_op_h, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, 0, CT.as<double>(),
+                           MpadR_, R, no, KCm_, RT_, X.Uf.as<float>(), X.Cf.as<uint16_t>());
+      } else {
+        hipLaunchKernelGGL(k_split_op, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, 0, CT.as<double>(),
+                           MpadR_, R, no, KCm_, np_, RT_, X.Cf.as<uint16_t>());
+      }
       NNK_HIP(hipGetLastError());
       NNK_HIP(hipDeviceSynchronize());
       X.cop.assign(343, nullptr);
-      for (int j = 0; j < no; j++) X.cop[slots[j]] = X.Cf.as<uint16_t>() + per * j;
+      X.uop.assign(343, nullptr);
+      for (int j = 0; j < no; j++) {
+        X.cop[slots[j]] = X.Cf.as<uint16_t>() + per * j;
+        if (f16_) X.uop[slots[j]] = X.Uf.as<float>() + (size_t)RPm_ * j;
+      }
     }
     // ---- M2M (this level is the child level; parent level l-1 >= 2) ----
     if (l >= 3) {
@@ -225,6 +333,12 @@ void Fmm::build_mixed() {
   chat_.zero();
   dhat_.zero();
   chat_x_.zero();
+  amax_.alloc(sizeof(unsigned long long));
+  xun_.alloc(sizeof(double));
+  {
+    const double one = 1.0;
+    NNK_HIP(hipMemcpy(xun_.p, &one, sizeof(double), hipMemcpyHostToDevice));
+  }
   NNK_HIP(hipDeviceSynchronize());
 }
 
@@ -291,6 +405,7 @@ void Fmm::plan_mixed() {
       if (H.cls[a].empty()) continue;
       const int g[3] = {(a >> 2) & 1, (a >> 1) & 1, a & 1};
       std::vector<const uint16_t *> ops;
+      std::vector<const float *> uops;
       for (int ox = -2 - g[0]; ox <= 3 - g[0]; ox++)
         for (int oy = -2 - g[1]; oy <= 3 - g[1]; oy++)
           for (int oz = -2 - g[2]; oz <= 3 - g[2]; oz++) {
@@ -298,11 +413,15 @@ void Fmm::plan_mixed() {
             const int sl = (ox + 3) * 49 + (oy + 3) * 7 + (oz + 3);
             if (!X.cop[sl]) throw Error(NNK_ESTATE, "mixed M2L: missing translation operator");
             ops.push_back(X.cop[sl]);
+            uops.push_back(f16_ ? X.uop[sl] : nullptr);
           }
       GemmJobBf16 J{};
       P.mx_owned.emplace_back();
       P.mx_owned.back().upload(ops);
       J.A = P.mx_owned.back().as<const uint16_t *>();
+      P.mx_owned.emplace_back();
+      P.mx_owned.back().upload(uops);
+      J.U = P.mx_owned.back().as<const float *>();
       P.mx_owned.emplace_back();
       P.mx_owned.back().upload(H.gv[a]);
       J.gidx = P.mx_owned.back().as<int32_t>();
@@ -355,7 +474,7 @@ void Fmm::plan_mixed() {
         j++;
       }
       P.mx_split = split_for((int64_t)nj * (RPm_ / MBM) * ((mN + MBN - 1) / MBN), minnb);
-      P.mx_depth = np_ == 3 ? 2 : 3;
+      P.mx_depth = (np_ == 3 && !f16_) ? 2 : 3;
       P.mx_slot.upload(slot);  // also for the autotuner's split candidates
       reserve_part((size_t)std::min(64, 2 * P.mx_split) * nj * mN * RP_);
     }
@@ -380,14 +499,17 @@ void Fmm::launch_m2l_mixed(int l, int depth, int split, hipStream_t s) {
   double *pp = split > 1 ? part_.as<double>() : nullptr;
   auto launch = [&](auto kern) {
     hipLaunchKernelGGL(kern, grid, dim3(512), 0, s, R_, KCm_, RT_, chat_x_.as<uint16_t>(), zrow, d, RP_,
-                       P.mx_m2l.as<GemmJobBf16>(), nj, nrb, split, pp, mN);
+                       P.mx_m2l.as<GemmJobBf16>(), nj, nrb, split, pp, mN, xun_.as<double>());
   };
-  if (np_ == 3) {
-    if (depth == 3) launch(k_m2l_bf16<3, 3, 1>);
-    else launch(k_m2l_bf16<3, 2, 1>);
+  if (f16_) {
+    if (depth == 2) launch(k_m2l_split<2, 2, 1, true>);
+    else launch(k_m2l_split<2, 3, 1, true>);
+  } else if (np_ == 3) {
+    if (depth == 3) launch(k_m2l_split<3, 3, 1, false>);
+    else launch(k_m2l_split<3, 2, 1, false>);
   } else {
-    if (depth == 2) launch(k_m2l_bf16<2, 2, 1>);
-    else launch(k_m2l_bf16<2, 3, 1>);
+    if (depth == 2) launch(k_m2l_split<2, 2, 1, false>);
+    else launch(k_m2l_split<2, 3, 1, false>);
   }
   NNK_HIP(hipGetLastError());
   if (split > 1) split_reduce(P.mx_slot, nj * mN, R_, RP_, split, d, s);
@@ -412,8 +534,17 @@ void Fmm::apply_mixed(const double *d_f, double *d_u, hipStream_t s) {
   }
   if (n_upall_) {
     const int64_t tot = (int64_t)n_upall_ * KCm_ * 32;
-    hipLaunchKernelGGL(k_split_rows, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s,
-                       upall_.as<int32_t>(), n_upall_, c, RP_, R, KCm_, np_, chat_x_.as<uint16_t>());
+    if (f16_) {
+      hipLaunchKernelGGL(k_amax_reset, dim3(1), dim3(64), 0, s, amax_.as<unsigned long long>());
+      hipLaunchKernelGGL(k_absmax, dim3(1024), dim3(256), 0, s, upall_.as<int32_t>(), n_upall_, c, RP_, R,
+                         amax_.as<unsigned long long>());
+      hipLaunchKernelGGL(k_split_rows_h, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s,
+                         upall_.as<int32_t>(), n_upall_, c, RP_, R, KCm_, amax_.as<unsigned long long>(),
+                         chat_x_.as<uint16_t>(), xun_.as<double>());
+    } else {
+      hipLaunchKernelGGL(k_split_rows, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s,
+                         upall_.as<int32_t>(), n_upall_, c, RP_, R, KCm_, np_, chat_x_.as<uint16_t>());
+    }
     NNK_HIP(hipGetLastError());
   }
   for (int l = maxlev_; l >= 2; l--)

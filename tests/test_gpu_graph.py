@@ -13,7 +13,7 @@ from geometry import blobs
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("precision", ["fp64", "mixed", "mixedh"])
 def test_matvec_graph_replays_match_eager(nnk, precision):
     import torch
     rng = np.random.default_rng(11)
@@ -50,7 +50,7 @@ def test_mobility_solve_graph_matches_eager(oracle, nnk, precon):
     np.testing.assert_array_equal(x0, x1)
 
 
-@pytest.mark.parametrize("precision", ["fp64", "mixed3"])
+@pytest.mark.parametrize("precision", ["fp64", "mixed3", "mixedh"])
 def test_autotuned_launch_shapes(nnk, precision):
     """The autotuner only reorders sums: tuned results agree with the
     default launch shapes to rounding, are deterministic once tuned, and

@@ -42,17 +42,33 @@ def test_mixed_against_direct_and_fp64(oracle, nnk, case):
     assert rel(um, u64) < 5e-5
 
 
-def test_mixed3_below_1e6_at_res8(oracle, nnk):
-    """The three-part form at 8 points per corona edge stays below 1e-6
-    against the FP64 direct sum (the two-part form stops near 3e-6)."""
+@pytest.mark.parametrize("precision", ["mixed3", "mixedh"])
+def test_below_1e6_at_res8(oracle, nnk, precision):
+    """The three-part BF16 and the two-part FP16 forms at 8 points per corona
+    edge stay below 1e-6 against the FP64 direct sum (the two-part BF16
+    form stops near 3e-6)."""
     rng = np.random.default_rng(8)
     p = rng.random((30000, 3))
     q = rng.standard_normal(p.shape)
     tree = nnk.OcTree(p, p, "nodeCapacity", 64)
-    u3 = nnk.KIFMM(tree, [1e-3, 1.0], "coronaRes", 8, "precision", "mixed3").computeVel(q)
+    u3 = nnk.KIFMM(tree, [1e-3, 1.0], "coronaRes", 8, "precision", precision).computeVel(q)
     idx = rng.choice(len(p), 2000, replace=False)
     ud = oracle.direct(p, q, p[idx], 1e-3)
     assert rel(u3[idx], ud) < 1e-6
+
+
+def test_mixedh_scaling_is_input_invariant(nnk):
+    """The FP16 form scales its coefficients by one power of two per matvec:
+    scaling the input by 2^k (k = -40, 40) scales the output exactly, and a
+    zero input gives a zero output."""
+    rng = np.random.default_rng(9)
+    p = rng.random((20000, 3))
+    q = rng.standard_normal(p.shape)
+    f = nnk.KIFMM(nnk.OcTree(p, p, "nodeCapacity", 64), [1e-3, 1.0], "precision", "mixedh")
+    u = f.computeVel(q)
+    for k in (-40, 40):
+        np.testing.assert_array_equal(f.computeVel(q * 2.0 ** k), u * 2.0 ** k)
+    assert not f.computeVel(np.zeros_like(q)).any()
 
 
 def test_mixed_nearest_sphere_pair(oracle, nnk):
