@@ -207,7 +207,7 @@ def spheroid_solve(nnk):
     t_geo = time.perf_counter() - t0
     t0 = time.perf_counter()
     sw.genTree("nodeCapacity", 64, "nearest", 1)
-    sw.genKIFMM([0.005, 1.0], "GMRES", 1, "format", 2)
+    sw.genKIFMM([0.005, 1.0], "GMRES", 1, "format", 2, "autotune", 1)
     B, BT, A1, A2 = nnk.genMatrixComponets(sw)
     t_setup = time.perf_counter() - t0
     b = mob._rhs(sw, lambda t, x: np.zeros_like(x), 0.0)

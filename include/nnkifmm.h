@@ -187,13 +187,15 @@ int32_t nnk_mobility_set_graph(nnk_mobility *h, int32_t on);
 /* No counterpart in the reference (which solves in FP64 only): the same
  *           solve by iterative refinement.  Outer residuals use the FP64
  *           operator of nnk_mobility_create; each correction is a GMRES solve
- *           on `inner` (a KIFMM handle on the same tree, usually
- *           NNK_MIXED) to relative tolerance inner_tol.  Stops when the true
- *           FP64 relative residual of the unpreconditioned system is <= tol
- *           or maxit inner iterations are used.  *iters = inner iterations,
- *           *outer = refinement steps. */
-int32_t nnk_mobility_solve_refined(nnk_mobility *h, nnk_fmm *inner, const double *rhs, double tol,
-                                   int32_t maxit, double inner_tol, int32_t precon, double *x,
+ *           on `inner` (a KIFMM handle on the same tree, usually a mixed
+ *           form) to relative tolerance inner_tol, preconditioned (precon
+ *           != 0) with `inner_sparse` as the coronaRes-4 operator if given
+ *           (else the FP64 one).  Stops when the true FP64 relative residual
+ *           of the unpreconditioned system is <= tol or maxit inner
+ *           iterations are used.  *iters = inner iterations, *outer =
+ *           refinement steps. */
+int32_t nnk_mobility_solve_refined(nnk_mobility *h, nnk_fmm *inner, nnk_fmm *inner_sparse, const double *rhs,
+                                   double tol, int32_t maxit, double inner_tol, int32_t precon, double *x,
                                    int32_t *flag, double *relres, int32_t *iters, int32_t *outer);
 
 void nnk_mobility_destroy(nnk_mobility *h);

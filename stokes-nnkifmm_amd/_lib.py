@@ -56,7 +56,7 @@ SIGNATURES = {
     "nnk_mobility_precon_device": (ctypes.c_int32, [_vp, _vp, _vp, _vp]),
     "nnk_mobility_solve": (ctypes.c_int32, [_vp, _d, ctypes.c_double, ctypes.c_int32, ctypes.c_int32, _d,
                                             _i32, _d, _i32, _d]),
-    "nnk_mobility_solve_refined": (ctypes.c_int32, [_vp, _vp, _d, ctypes.c_double, ctypes.c_int32,
+    "nnk_mobility_solve_refined": (ctypes.c_int32, [_vp, _vp, _vp, _d, ctypes.c_double, ctypes.c_int32,
                                                     ctypes.c_double, ctypes.c_int32, _d, _i32, _d, _i32,
                                                     _i32]),
     "nnk_fmm_set_graph": (ctypes.c_int32, [_vp, ctypes.c_int32]),

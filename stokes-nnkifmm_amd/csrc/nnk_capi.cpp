@@ -329,20 +329,21 @@ int32_t nnk_mobility_set_graph(nnk_mobility *h, int32_t on) {
   });
 }
 
-int32_t nnk_mobility_solve_refined(nnk_mobility *h, nnk_fmm *inner, const double *rhs, double tol,
-                                   int32_t maxit, double inner_tol, int32_t precon, double *x,
+int32_t nnk_mobility_solve_refined(nnk_mobility *h, nnk_fmm *inner, nnk_fmm *inner_sparse, const double *rhs,
+                                   double tol, int32_t maxit, double inner_tol, int32_t precon, double *x,
                                    int32_t *flag, double *relres, int32_t *iters, int32_t *outer) {
   return guard([&]() -> int32_t {
     if (!h || !inner || !rhs || !x || maxit < 1 || !(tol > 0) || !(inner_tol > 0))
       return fail(NNK_EARG, "nnk_mobility_solve_refined: bad argument");
-    if (inner->npot != h->m->points() || inner->ntgt != h->m->points())
+    if (inner->npot != h->m->points() || inner->ntgt != h->m->points() ||
+        (inner_sparse && (inner_sparse->npot != h->m->points() || inner_sparse->ntgt != h->m->points())))
       return fail(NNK_EARG, "nnk_mobility_solve_refined: inner operator is not on the mobility tree");
     const int64_t n = h->m->size();
     nnk::DBuf db, dx;
     db.alloc(sizeof(double) * n);
     dx.alloc(sizeof(double) * n);
     NNK_HIP(hipMemcpy(db.p, rhs, sizeof(double) * n, hipMemcpyHostToDevice));
-    h->m->set_inner(inner->f.get());
+    h->m->set_inner(inner->f.get(), inner_sparse ? inner_sparse->f.get() : nullptr);
     int nout = 0;
     nnk::GmresResult R =
         h->m->refine(db.as<double>(), dx.as<double>(), tol, maxit, inner_tol, precon != 0, 0, nullptr, &nout);

@@ -97,25 +97,44 @@ struct SurfArgs {
 
 // Check potential on a node surface from the sources of a set of leaves
 // (S2M: the leaf itself onto its down surface; P2L: the X list onto the up
-// surface).  One block per node; sources staged in LDS.
+// surface).  One 512-thread block per node; sources staged in LDS 512 at a
+// time.  The nq surface points are taken in chunks of c (chosen to keep the
+// most lanes busy), each point shared by nsub = 512 / c lanes that split the
+// sources; their partial sums are added in a fixed order.
+constexpr int kSurfThreads = 512;
+
 template <bool F32>
-__global__ __launch_bounds__(256) void k_surf_eval(SurfArgs a) {
+__global__ __launch_bounds__(kSurfThreads) void k_surf_eval(SurfArgs a) {
+  constexpr int T = kSurfThreads;
   const int n = a.nodes[blockIdx.x];
-  __shared__ double sp[256 * 6];
+  __shared__ double sp[T * 6];
+  __shared__ double red[T * 3];
   const double *box = a.corners + 6 * (size_t)(n - 1);
   const int tid = threadIdx.x;
   int64_t lb = 0, le = 1;
   if (a.lptr) { lb = a.lptr[n]; le = a.lptr[n + 1]; }
-  for (int p0 = 0; p0 < a.nq; p0 += 256) {
-    const int p = p0 + tid;
+  // chunk size: the c = ceil(nq / k), k = 1..8, that keeps the most lanes busy
+  int csz = min(a.nq, T), best = -1;
+  for (int k = 1; k <= 8; k++) {
+    const int c = (a.nq + k - 1) / k;
+    if (c > T) continue;
+    const int used = c * (T / c);
+    if (used > best) { best = used; csz = c; }
+  }
+  for (int p0 = 0; p0 < a.nq; p0 += csz) {
+    const int ntc = min(csz, a.nq - p0);
+    const int nsub = T / ntc;
+    const bool act = tid < ntc * nsub;
+    const int ti = tid % ntc, sub = tid / ntc;
+    const int p = p0 + ti;
     double tx = 0, ty = 0, tz = 0;
-    if (p < a.nq) surf_point(box, a.down, a.surf_idx[p], a.n1, tx, ty, tz);
+    if (act) surf_point(box, a.down, a.surf_idx[p], a.n1, tx, ty, tz);
     double ux = 0, uy = 0, uz = 0;
     for (int64_t e = lb; e < le; e++) {
       const int leaf = a.lptr ? a.lidx[e] : n;
       const int sb = a.src_ptr[leaf], se = a.src_ptr[leaf + 1];
-      for (int s0 = sb; s0 < se; s0 += 256) {
-        const int cnt = min(256, se - s0);
+      for (int s0 = sb; s0 < se; s0 += T) {
+        const int cnt = min(T, se - s0);
         if (tid < cnt) {
           const int s = s0 + tid;
           for (int d = 0; d < 3; d++) {
@@ -124,21 +143,31 @@ __global__ __launch_bounds__(256) void k_surf_eval(SurfArgs a) {
           }
         }
         __syncthreads();
-        if (p < a.nq)
-          for (int j = 0; j < cnt; j++)
+        if (act)
+          for (int j = sub; j < cnt; j += nsub)
             stk<F32>(tx, ty, tz, sp[6 * j], sp[6 * j + 1], sp[6 * j + 2], sp[6 * j + 3],
                      sp[6 * j + 4], sp[6 * j + 5], a.e2, ux, uy, uz);
         __syncthreads();
       }
     }
-    if (p < a.nq) {
-      double *o = a.chk + (size_t)(n - 1) * a.K + 3 * p;
+    if (act) {
+      red[3 * tid] = ux;
+      red[3 * tid + 1] = uy;
+      red[3 * tid + 2] = uz;
+    }
+    __syncthreads();
+    if (tid < ntc) {
+      double v[3] = {0, 0, 0};
+      for (int q = 0; q < nsub; q++)
+        for (int d = 0; d < 3; d++) v[d] += red[3 * (tid + q * ntc) + d];
+      double *o = a.chk + (size_t)(n - 1) * a.K + 3 * (p0 + tid);
       if (a.accumulate) {
-        o[0] += a.c * ux; o[1] += a.c * uy; o[2] += a.c * uz;
+        o[0] += a.c * v[0]; o[1] += a.c * v[1]; o[2] += a.c * v[2];
       } else {
-        o[0] = a.c * ux; o[1] = a.c * uy; o[2] = a.c * uz;
+        o[0] = a.c * v[0]; o[1] = a.c * v[1]; o[2] = a.c * v[2];
       }
     }
+    __syncthreads();
   }
 }
 
@@ -906,8 +935,8 @@ void Fmm::launch_surf(const int32_t *nodes, int n, const int64_t *lptr, const in
   a.e2 = eps_ * eps_;
   a.c = 1.0 / (8.0 * kPi * mu_);
   a.chk = chk;
-  if (prec_ >= 1) hipLaunchKernelGGL(k_surf_eval<true>, dim3(n), dim3(256), 0, s, a);
-  else hipLaunchKernelGGL(k_surf_eval<false>, dim3(n), dim3(256), 0, s, a);
+  if (prec_ >= 1) hipLaunchKernelGGL(k_surf_eval<true>, dim3(n), dim3(kSurfThreads), 0, s, a);
+  else hipLaunchKernelGGL(k_surf_eval<false>, dim3(n), dim3(kSurfThreads), 0, s, a);
   NNK_HIP(hipGetLastError());
 }
 

@@ -266,7 +266,7 @@ void Mobility::precon(const double *x, double *y, hipStream_t s) {
   reduced(x, y, s);                                        // y_f = R^-1 f
   small_solve(A1lu_, A1piv_, x + n3_, s6a_.as<double>(), 1.0, s);   // A1 \ UO
   Bt_.mul(s6a_.as<double>(), t3a_.as<double>(), false, 1.0, s);     // B^T (.)
-  kifmm(F4_, t3a_.as<double>(), t3b_.as<double>(), false, s);       // S4 (.)
+  kifmm((Fcur_ && F4i_) ? F4i_ : F4_, t3a_.as<double>(), t3b_.as<double>(), false, s);  // S4 (.)
   BTt_.mul(t3b_.as<double>(), s6b_.as<double>(), false, 1.0, s);    // BT^T (.)
   small_solve(A2lu_, A2piv_, s6b_.as<double>(), y + n3_, -1.0, s);  // -A2 \ (.)
   BT_.mul(y + n3_, t3a_.as<double>(), false, 1.0, s);               // BT out

@@ -44,9 +44,12 @@ class Mobility {
   // iteration count in iter and the outer count in outer.
   GmresResult refine(const double *b, double *x, double tol, int maxit, double inner_tol, bool use_precon,
                      hipStream_t s, std::vector<double> *resvec, int *outer);
-  void set_inner(Fmm *inner) {
-    if (inner != Fi_) { gops_[2].reset(); gops_[3].reset(); }
+  // inner operators of the refinement: the matvec and (optional) the
+  // coronaRes-4 operator of the preconditioner used inside the inner solves
+  void set_inner(Fmm *inner, Fmm *inner_sparse = nullptr) {
+    if (inner != Fi_ || inner_sparse != F4i_) { gops_[2].reset(); gops_[3].reset(); }
     Fi_ = inner;
+    F4i_ = inner_sparse;
   }
   // replay the GMRES operator (A or A P) from HIP graphs, one per
   // (preconditioned, inner operator) combination, captured on first use
@@ -61,7 +64,7 @@ class Mobility {
                    hipStream_t s);
   double dot(const double *a, const double *b, hipStream_t s);
 
-  Fmm *F_, *F4_, *Fi_ = nullptr;
+  Fmm *F_, *F4_, *Fi_ = nullptr, *F4i_ = nullptr;
   Fmm *Fcur_ = nullptr;  // operator used by apply(): F_ or Fi_
   int64_t nb6_, N_ = 0, n3_ = 0, n_ = 0;
   DevCsr B_, Bt_, BT_, BTt_;

@@ -147,21 +147,26 @@ def mobility_gmres(swimmers, B, BT, A1, A2, rhs, tol, maxIter, precon):
     return x
 
 
-def mobility_refined(swimmers, B, BT, A1, A2, rhs, tol, maxIter, precon, innerTol=1e-4, svdTol=1e-8):
+def mobility_refined(swimmers, B, BT, A1, A2, rhs, tol, maxIter, precon, innerTol=1e-4, svdTol=1e-8,
+                     innerPrecision="mixed"):
     """The solve of mobility_gmres by iterative refinement (no counterpart in
     the reference): residuals with the FP64 operator, corrections by GMRES on
-    the mixed-precision KIFMM of the same tree to innerTol.  The answer meets
-    tol on the unpreconditioned FP64 system.  Records flag, relres, iter
-    (inner iterations) and outer in mobility.last_solve."""
+    the mixed-precision KIFMM of the same tree to innerTol (preconditioned
+    with mixed-precision coronaRes-4 operators too).  The answer meets tol on
+    the unpreconditioned FP64 system.  Records flag, relres, iter (inner
+    iterations) and outer in mobility.last_solve."""
     from .kifmm import _Handle
     m = _mobility(swimmers, B, BT, A1, A2, precon=bool(precon))
     fmm = swimmers.FMM
     a = fmm.arguments
-    key = ("mixed", float(svdTol))
+    key = (innerPrecision, float(svdTol), bool(precon))
     inner = getattr(m, "inner", {}).get(key)
     if inner is None:
-        inner = _Handle(fmm.tree, a["kernelPar"][0], a["kernelPar"][1], a["coronaRes"], a["coronaShells"],
-                        "mixed", svdTol)
+        tune = bool(a.get("autotune"))
+        inner = (_Handle(fmm.tree, a["kernelPar"][0], a["kernelPar"][1], a["coronaRes"], a["coronaShells"],
+                         innerPrecision, svdTol, autotune=tune),
+                 _Handle(fmm.tree, a["kernelPar"][0], a["kernelPar"][1], 4, a["coronaShells"],
+                         innerPrecision, svdTol, autotune=tune) if precon else None)
         m.inner = {key: inner}
     b = c64(rhs).ravel()
     x = np.empty_like(b)
@@ -169,7 +174,8 @@ def mobility_refined(swimmers, B, BT, A1, A2, rhs, tol, maxIter, precon, innerTo
     it = np.zeros(1, np.int32)
     outer = np.zeros(1, np.int32)
     rel = np.zeros(1)
-    check(lib().nnk_mobility_solve_refined(m.h, inner.h, ptr(b), float(tol), int(maxIter), float(innerTol),
+    check(lib().nnk_mobility_solve_refined(m.h, inner[0].h, inner[1].h if inner[1] else None, ptr(b), float(tol),
+                                           int(maxIter), float(innerTol),
                                            int(bool(precon)), ptr(x), ptr(flag, _lib._i32), ptr(rel),
                                            ptr(it, _lib._i32), ptr(outer, _lib._i32)))
     last_solve.clear()
