@@ -14,8 +14,12 @@ Forms timed (all reported under "forms"):
   mixed3_res8  the same with three-part split-BF16 M2L at coronaRes 8 (below
                1e-6 relative error)
   mixedh_res8  two-part split-FP16 M2L on power-of-two scaled operands at
-               coronaRes 8: the same accuracy with half the MFMAs, the
-               headline `value`
+               coronaRes 8: the same accuracy with half the MFMAs
+  mixedh_res9_tol3e-7
+               the same at coronaRes 9 with the pseudo-inverses cut at
+               3e-7 (rank 477 instead of 605 at 8 / 1e-8, below one
+               128-row tile boundary): the fastest form below 1e-6 in a
+               sweep of corona resolution and cut, the headline `value`
 
 Prints one JSON line.  Besides the throughput it reports:
   roofline      the headline form's M2L translation GEMM (the dominant
@@ -60,8 +64,9 @@ def parse():
     ap.add_argument("--n", type=int, default=1 << 20)
     ap.add_argument("--res", type=int, default=6, help="corona resolution of the CPU baseline")
     ap.add_argument("--svd-tol", type=float, default=1e-8, help="pseudo-inverse cut of the mixed forms")
-    ap.add_argument("--forms", default="fp64_res6,fp64_res8,mixed_res6,mixed3_res8,mixedh_res8",
-                    help="comma-separated forms to time; the last one is the headline")
+    ap.add_argument("--forms", default="fp64_res6,fp64_res8,mixed_res6,mixed3_res8,mixedh_res8,mixedh_res9_tol3e-7",
+                    help="comma-separated forms <precision>_res<r>[_tol<svd tol>] to time; the last one is the "
+                         "headline")
     ap.add_argument("--no-solve", action="store_true",
                     help="skip the mobility solves (one sphere, spheroids + wall, batched, adjoint)")
     ap.add_argument("--seed", type=int, default=20240611)
@@ -330,11 +335,12 @@ def main():
     forms = {}
     names = [f for f in args.forms.split(",") if f]
     for name in names:
-        prec, res = name.split("_res")
-        if prec not in ("fp64", "mixed", "mixed3", "mixedh"):
+        parts = name.split("_")
+        prec, res = parts[0], int(parts[1][3:])
+        tol = float(parts[2][3:]) if len(parts) > 2 else args.svd_tol
+        if prec not in ("fp64", "mixed", "mixed3", "mixedh") or not parts[1].startswith("res"):
             raise SystemExit(f"unknown form {name}")
-        u, rec = run_form(nnk, lib, tree, fd, args, eps, mu, prec, int(res),
-                          0.0 if prec == "fp64" else args.svd_tol)
+        u, rec = run_form(nnk, lib, tree, fd, args, eps, mu, prec, res, 0.0 if prec == "fp64" else tol)
         uo = u.cpu().numpy()[idx]
         rec["rel_err_vs_direct"] = float(np.linalg.norm(uo - ud) / np.linalg.norm(ud))
         forms[name] = rec

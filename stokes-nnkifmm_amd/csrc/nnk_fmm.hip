@@ -1004,7 +1004,8 @@ void Fmm::autotune(hipStream_t s) {
       std::vector<int> splits = {s0};
       if (s0 > 1) splits.push_back(std::max(1, s0 / 2));
       if ((size_t)std::min(64, 2 * s0) * per_split <= part_elems_) splits.push_back(std::min(64, 2 * s0));
-      std::vector<int> depths = mixed ? std::vector<int>{2, 3} : std::vector<int>{0};
+      std::vector<int> depths = !mixed ? std::vector<int>{0} : np_ == 3 && !f16_ ? std::vector<int>{2}
+                                                                                  : std::vector<int>{2, 3};
       float best = 1e30f;
       int bs = s0, bd = mixed ? P.mx_depth : 0;
       for (int sp : splits) {

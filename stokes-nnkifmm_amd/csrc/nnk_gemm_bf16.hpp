@@ -190,19 +190,22 @@ __global__ __launch_bounds__(512, 1) void k_m2l_split(const int M, const int KC,
       }
   };
   const int mrow0 = rb * 128 + wm * 64 + 4 * lg;  // this lane's first output row
-  auto promote = [&](int bi) {
-    f32x4 un[TM];
+  // row unscale factors of block bi (FP16 form), loaded before the block's
+  // last chunk is multiplied so that the MFMAs cover their latency
+  auto unscale = [&](int bi, f32x4(&un)[TM]) {
 #pragma unroll
-    for (int i = 0; i < TM; i++) {
-      if constexpr (F16) un[i] = *reinterpret_cast<const f32x4 *>(uptr[bi] + mrow0 + 16 * i);
-      else un[i] = f32x4{1, 1, 1, 1};
-    }
+    for (int i = 0; i < TM; i++) un[i] = *reinterpret_cast<const f32x4 *>(uptr[bi] + mrow0 + 16 * i);
+  };
+  auto promote = [&](const f32x4 *un) {
 #pragma unroll
     for (int i = 0; i < TM; i++)
 #pragma unroll
       for (int j = 0; j < TN; j++) {
 #pragma unroll
-        for (int r = 0; r < 4; r++) acc64[i][j][r] += (double)acc[i][j][r] * (double)un[i][r];
+        for (int r = 0; r < 4; r++) {
+          if constexpr (F16) acc64[i][j][r] += (double)acc[i][j][r] * (double)un[i][r];
+          else acc64[i][j][r] += (double)acc[i][j][r];
+        }
         acc[i][j] = f32x4{0, 0, 0, 0};
       }
   };
@@ -221,8 +224,19 @@ __global__ __launch_bounds__(512, 1) void k_m2l_split(const int M, const int KC,
         const int cc = c + d;
         fetch(cc + D, R[d]);
         if (cc < total) {
-          compute(cc & 1);
-          if ((cc + 1) % KC == 0) promote(a0 + cc / KC);  // end of an operator block
+          if ((cc + 1) % KC == 0) {  // last chunk of an operator block
+            if constexpr (F16) {
+              f32x4 un[TM];
+              unscale(a0 + cc / KC, un);
+              compute(cc & 1);
+              promote(un);
+            } else {
+              compute(cc & 1);
+              promote(nullptr);
+            }
+          } else {
+            compute(cc & 1);
+          }
         }
         stash(R[(d + 1) % D], (cc + 1) & 1);
         __syncthreads();

@@ -505,8 +505,7 @@ void Fmm::launch_m2l_mixed(int l, int depth, int split, hipStream_t s) {
     if (depth == 2) launch(k_m2l_split<2, 2, 1, true>);
     else launch(k_m2l_split<2, 3, 1, true>);
   } else if (np_ == 3) {
-    if (depth == 3) launch(k_m2l_split<3, 3, 1, false>);
-    else launch(k_m2l_split<3, 2, 1, false>);
+    launch(k_m2l_split<3, 2, 1, false>);  // a third register stage would spill
   } else {
     if (depth == 2) launch(k_m2l_split<2, 2, 1, false>);
     else launch(k_m2l_split<2, 3, 1, false>);
