@@ -137,7 +137,13 @@ int32_t nnk_nearest_matrix(const double *coarse, int64_t n, const double *fine, 
  *           operator and fmm_sparse its coronaRes-4 twin (computeVelSparse,
  *           KIFMM.m:107-155; may be null if no preconditioner is used).  Both
  *           handles must outlive the mobility handle.  Vectors are
- *           [f (3N, all x, all y, all z); U (3 nsw); Omega (3 nsw)]. */
+ *           [f (3N, all x, all y, all z); U (3 nsw); Omega (3 nsw)].
+ *           nb6 = 0 (no bodies; B, BT empty, A1 and A2 unused) gives the
+ *           resistance operator S alone: nnk_mobility_solve then is
+ *           gmres(@(x) fmm.computeVel(x), vel, [], tol, maxit)
+ *           (FlowField.m:17, DirectMethodComparison.m:28,
+ *           Rigid_Resistance.m:24), preconditioned by the leaf-block solve
+ *           if precon != 0. */
 int32_t nnk_mobility_create(nnk_fmm *fmm, nnk_fmm *fmm_sparse, int64_t nb6,
                             const int64_t *B_rowptr, const int32_t *B_col, const double *B_val,
                             const int64_t *BT_rowptr, const int32_t *BT_col, const double *BT_val,

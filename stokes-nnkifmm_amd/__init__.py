@@ -11,6 +11,7 @@ from .mobility import (genMatrixComponets, mobility_gmres, mobility_refined, mob
                        mobilityPrecon,
                        mobilityProblem, mobilityProblemGravity)
 from .nearest import nearestMatrix  # noqa: F401
+from .resistance import Gen_Resistance, Rigid_Resistance, resistanceSolve, velocityModes  # noqa: F401
 from .octree import OcTree  # noqa: F401
 from .swimmers import Swimmers  # noqa: F401
 

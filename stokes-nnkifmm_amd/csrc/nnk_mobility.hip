@@ -195,7 +195,7 @@ Mobility::Mobility(Fmm *fmm, Fmm *fmm_sparse, int64_t nb6, const int64_t *Brp, c
   N_ = F_->npot();
   if (F_->ntgt() != N_ || (F4_ && (F4_->npot() != N_ || F4_->ntgt() != N_)))
     throw Error(NNK_EARG, "mobility: operators must map the N force points to themselves");
-  if (nb6 <= 0 || nb6 % 6) throw Error(NNK_EARG, "mobility: 6*nsw must be a positive multiple of 6");
+  if (nb6 < 0 || nb6 % 6) throw Error(NNK_EARG, "mobility: 6*nsw must be a non-negative multiple of 6");
   n3_ = 3 * N_;
   n_ = n3_ + nb6_;
   std::vector<int64_t> rp(Brp, Brp + nb6 + 1);
