@@ -65,6 +65,16 @@ const char *nnk_last_error(void);
 int32_t nnk_tree_create(const double *pot, int64_t npot, const double *tgt, int64_t ntgt,
                         int32_t node_capacity, int32_t max_depth, nnk_tree **out);
 
+/* No counterpart in the reference: nnk_tree_create with the root cube fixed
+ *           to [cube[0] cube[0] cube[0] cube[1] cube[1] cube[1]] instead of
+ *           the points' bounding cube (OcTree.m:97-98); every point must lie
+ *           in it.  Subdivision is unchanged.  For time stepping: with a fixed
+ *           root the level widths, and so the per-level operators, stay the
+ *           same from one right-hand side to the next (nnk_op_cache_limit). */
+int32_t nnk_tree_create_in_cube(const double *pot, int64_t npot, const double *tgt, int64_t ntgt,
+                                int32_t node_capacity, int32_t max_depth, const double *cube,
+                                nnk_tree **out);
+
 /* Replaces: OcTree(..., 'nearest', 1, 'finePoints', F, 'NN', NN)
  *           Classes/OcTree.m:115-128.  NN is Q x npot in CSC form (MATLAB
  *           sparse jc/ir/pr with 0-based row indices). */
@@ -184,6 +194,23 @@ int32_t nnk_fmm_set_graph(nnk_fmm *h, int32_t on);
  *           depth; 0 where a level has no M2L) for levels 0..nlev-1. */
 int32_t nnk_fmm_set_autotune(nnk_fmm *h, int32_t on);
 int32_t nnk_fmm_launch_config(const nnk_fmm *h, int32_t *split, int32_t *depth, int32_t nlev);
+
+/* Replaces nothing; serves the time-stepping callers of mobilityProblem
+ *           (ode45 in Data Generation/Squiremers.m:47-54,
+ *           SquiremersGrav.m:55-67), which rebuild the KIFMM every right-hand
+ *           side (mobilityProblem.m:26-35).  The per-level operators (SVD
+ *           factors of the check-to-equivalent solves, M2M, L2L, M2L, and the
+ *           compressed mixed-form operators) depend only on (device, eps, mu,
+ *           coronaRes, shells, box width, level[, precision, svd_tol, rank]),
+ *           so handles whose root cube is unchanged share them.  Handles alive
+ *           at the same time always share; nnk_op_cache_limit keeps up to
+ *           `bytes` of operators whose last handle is gone (least recently
+ *           used dropped first; 0, the default, keeps none).  Shared
+ *           operators are built by the same deterministic code, so results
+ *           are bit-identical with or without reuse.  nnk_op_cache_stats
+ *           returns hits, misses, bytes kept and entries kept. */
+int32_t nnk_op_cache_limit(int64_t bytes);
+int32_t nnk_op_cache_stats(int64_t *hits, int64_t *misses, int64_t *bytes, int64_t *entries);
 
 /* No counterpart in the reference: the same for the GMRES operator (A or
  *           A P) of nnk_mobility_solve / nnk_mobility_solve_refined, one

@@ -57,6 +57,8 @@ def lib():
         L.or_lu_solve.argtypes = [_d, ctypes.c_int, _d]
         L.or_tree_build.argtypes = [_d, ctypes.c_int64, _d, ctypes.c_int64, ctypes.c_int, ctypes.c_int,
                                     ctypes.POINTER(ctypes.c_void_p)]
+        L.or_tree_build_cube.argtypes = [_d, ctypes.c_int64, _d, ctypes.c_int64, ctypes.c_int, ctypes.c_int, _d,
+                                         ctypes.POINTER(ctypes.c_void_p)]
         L.or_tree_free.argtypes = [ctypes.c_void_p]
         for f in ("or_tree_npts", "or_tree_npot", "or_tree_ntgt"):
             getattr(L, f).argtypes = [ctypes.c_void_p]
@@ -149,12 +151,13 @@ def nearest_matrix(coarse, fine):
 class Tree:
     """Oracle octree (OcTree.m semantics); ids are 1-based like MATLAB."""
 
-    def __init__(self, pot, tgt, nodeCapacity=200, maxDepth=21, finePoints=None, NN=None):
+    def __init__(self, pot, tgt, nodeCapacity=200, maxDepth=21, finePoints=None, NN=None, rootCube=None):
         pot = _c(pot).reshape(-1, 3)
         tgt = _c(tgt).reshape(-1, 3)
         h = ctypes.c_void_p()
-        rc = lib().or_tree_build(_p(pot), len(pot), _p(tgt), len(tgt), int(nodeCapacity), int(maxDepth),
-                                 ctypes.byref(h))
+        cube = None if rootCube is None else _c(rootCube).ravel()
+        rc = lib().or_tree_build_cube(_p(pot), len(pot), _p(tgt), len(tgt), int(nodeCapacity), int(maxDepth),
+                                      None if cube is None else _p(cube), ctypes.byref(h))
         if rc != 0:
             raise RuntimeError(lib().or_last_error().decode())
         self._h = h

@@ -544,8 +544,20 @@ static int64_t ht_find_or_insert(htab *h, const double *p, int64_t idx, int inse
 
 void or_tree_free(otree *t);
 
+/* cube: NULL for OcTree.m's root [m m m M M M] (min/max over all
+   coordinates, OcTree.m:97-98), or a fixed root [cube[0] x3, cube[1] x3]
+   that must contain every point (an extension for time stepping, where a
+   fixed root keeps the level widths, and so the operators, unchanged). */
+int or_tree_build_cube(const double *pot, int64_t npot, const double *tgt, int64_t ntgt,
+                       int cap, int max_depth, const double *cube, otree **out);
+
 int or_tree_build(const double *pot, int64_t npot, const double *tgt, int64_t ntgt,
                   int cap, int max_depth, otree **out) {
+  return or_tree_build_cube(pot, npot, tgt, ntgt, cap, max_depth, NULL, out);
+}
+
+int or_tree_build_cube(const double *pot, int64_t npot, const double *tgt, int64_t ntgt,
+                       int cap, int max_depth, const double *cube, otree **out) {
   *out = NULL;
   if (npot + ntgt < 2) { snprintf(g_err, sizeof g_err, "need at least two points"); return OR_EARG; }
   otree *t = calloc(1, sizeof(otree));
@@ -582,6 +594,15 @@ int or_tree_build(const double *pot, int64_t npot, const double *tgt, int64_t nt
   for (int64_t i = 0; i < 3 * np; i++) {
     if (t->points[i] < m) m = t->points[i];
     if (t->points[i] > M) M = t->points[i];
+  }
+  if (cube) {
+    if (!(cube[0] <= m && M <= cube[1])) {
+      snprintf(g_err, sizeof g_err, "root cube does not contain every point");
+      or_tree_free(t);
+      return OR_EARG;
+    }
+    m = cube[0];
+    M = cube[1];
   }
   t->max_depth = max_depth;
   t->cap = cap < np - 1 ? cap : (int32_t)(np - 1);

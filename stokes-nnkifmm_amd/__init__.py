@@ -14,6 +14,8 @@ from .nearest import nearestMatrix  # noqa: F401
 from .resistance import Gen_Resistance, Rigid_Resistance, resistanceSolve, velocityModes  # noqa: F401
 from .octree import OcTree  # noqa: F401
 from .swimmers import Swimmers  # noqa: F401
+from .timestepping import (ode45, odeset, operatorCache, operatorCacheStats, trajectoryOutput,  # noqa: F401
+                           writeTrajectory)
 
 
 def library_path():

@@ -24,6 +24,8 @@ SIGNATURES = {
     "nnk_last_error": (ctypes.c_char_p, []),
     "nnk_tree_create": (ctypes.c_int32, [_d, ctypes.c_int64, _d, ctypes.c_int64, ctypes.c_int32,
                                          ctypes.c_int32, _h]),
+    "nnk_tree_create_in_cube": (ctypes.c_int32, [_d, ctypes.c_int64, _d, ctypes.c_int64, ctypes.c_int32,
+                                                 ctypes.c_int32, _d, _h]),
     "nnk_tree_set_nearest": (ctypes.c_int32, [_vp, _d, ctypes.c_int64, _i64, _i64, _d]),
     "nnk_tree_sizes": (ctypes.c_int32, [_vp, _i64, _i64, _i64, _i32, _i32]),
     "nnk_tree_export": (ctypes.c_int32, [_vp, _d, _u8, _u8, _i32, _d, _i32, _i32, _i32]),
@@ -62,6 +64,8 @@ SIGNATURES = {
     "nnk_fmm_set_graph": (ctypes.c_int32, [_vp, ctypes.c_int32]),
     "nnk_fmm_set_autotune": (ctypes.c_int32, [_vp, ctypes.c_int32]),
     "nnk_fmm_launch_config": (ctypes.c_int32, [_vp, _i32, _i32, ctypes.c_int32]),
+    "nnk_op_cache_limit": (ctypes.c_int32, [ctypes.c_int64]),
+    "nnk_op_cache_stats": (ctypes.c_int32, [_i64, _i64, _i64, _i64]),
     "nnk_mobility_set_graph": (ctypes.c_int32, [_vp, ctypes.c_int32]),
     "nnk_mobility_destroy": (None, [_vp]),
     "nnk_batch_create": (ctypes.c_int32, [ctypes.c_int32, _i64, _d, _i64, _d, _i64, _i32, _d, _d, _d, _d, _i32,

@@ -73,14 +73,20 @@ const char *nnk_last_error(void) { return g_err.c_str(); }
 
 int32_t nnk_tree_create(const double *pot, int64_t npot, const double *tgt, int64_t ntgt,
                         int32_t node_capacity, int32_t max_depth, nnk_tree **out) {
+  return nnk_tree_create_in_cube(pot, npot, tgt, ntgt, node_capacity, max_depth, nullptr, out);
+}
+
+int32_t nnk_tree_create_in_cube(const double *pot, int64_t npot, const double *tgt, int64_t ntgt,
+                                int32_t node_capacity, int32_t max_depth, const double *cube, nnk_tree **out) {
   return guard([&]() -> int32_t {
     if (!out) return fail(NNK_EARG, "nnk_tree_create: null out");
     *out = nullptr;
     if ((npot && !pot) || (ntgt && !tgt)) return fail(NNK_EARG, "nnk_tree_create: null points");
     if (node_capacity < 1) return fail(NNK_EARG, "nodeCapacity must be >= 1");
+    if (cube && !(cube[0] < cube[1])) return fail(NNK_EARG, "nnk_tree_create_in_cube: empty root cube");
     auto t = std::make_shared<nnk::Tree>();
     std::string err;
-    int rc = nnk::build_tree(pot, npot, tgt, ntgt, node_capacity, max_depth, *t, err);
+    int rc = nnk::build_tree(pot, npot, tgt, ntgt, node_capacity, max_depth, *t, err, cube);
     if (rc) return fail(rc == nnk::NNK_EMAXDEPTH ? NNK_EMAXDEPTH : NNK_EARG, err);
     *out = new nnk_tree{t};
     return NNK_OK;
@@ -304,6 +310,27 @@ int32_t nnk_fmm_set_autotune(nnk_fmm *h, int32_t on) {
   return guard([&]() -> int32_t {
     if (!h) return fail(NNK_EARG, "nnk_fmm_set_autotune: null handle");
     h->f->set_autotune(on != 0);
+    return NNK_OK;
+  });
+}
+
+int32_t nnk_op_cache_limit(int64_t bytes) {
+  return guard([&]() -> int32_t {
+    if (bytes < 0) return fail(NNK_EARG, "nnk_op_cache_limit: negative byte limit");
+    nnk::OpCache::get().set_limit(bytes);
+    return NNK_OK;
+  });
+}
+
+int32_t nnk_op_cache_stats(int64_t *hits, int64_t *misses, int64_t *bytes, int64_t *entries) {
+  return guard([&]() -> int32_t {
+    if (!hits || !misses || !bytes || !entries) return fail(NNK_EARG, "nnk_op_cache_stats: null output");
+    int64_t o[4];
+    nnk::OpCache::get().stats(o);
+    *hits = o[0];
+    *misses = o[1];
+    *bytes = o[2];
+    *entries = o[3];
     return NNK_OK;
   });
 }

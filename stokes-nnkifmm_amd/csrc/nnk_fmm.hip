@@ -23,6 +23,8 @@
 #include <cstring>
 #include <cstdlib>
 #include <map>
+#include <mutex>
+#include <tuple>
 
 #include "nnk_fmm.hpp"
 #include "nnk_gemm.hpp"
@@ -554,13 +556,21 @@ Fmm::Fmm(std::shared_ptr<const Tree> t, double eps, double mu, int res, int shel
   chk_.zero();
   uppot_.zero();
   downpot_.zero();
-  ops_.resize(maxlev_ + 1);
-  if (prec_ >= 1) {
-    svdU_.resize(maxlev_ + 1);
-    svdS_.resize(maxlev_ + 1);
-    svdV_.resize(maxlev_ + 1);
+  ops_.assign(maxlev_ + 1, nullptr);
+  {
+    OpCache &C = OpCache::get();
+    for (int l = 2; l <= maxlev_; l++) {
+      // FP64 form: operators with T.  Mixed forms: any entry (T unused).
+      auto hit = C.find(key(0, l));
+      if (!hit && prec_ >= 1) hit = C.find(key(1, l));
+      if (hit) {
+        ops_[l] = std::static_pointer_cast<const LevelOps>(hit);
+      } else {
+        ops_[l] = build_ops(l, prec_ == 0);
+        C.put(key(prec_ == 0 ? 0 : 1, l), ops_[l], ops_[l]->bytes());
+      }
+    }
   }
-  for (int l = 2; l <= maxlev_; l++) build_ops(l);
   plan();
   if (prec_ >= 1) {
     build_mixed();
@@ -569,8 +579,136 @@ Fmm::Fmm(std::shared_ptr<const Tree> t, double eps, double mu, int res, int shel
   NNK_HIP(hipDeviceSynchronize());
 }
 
-void Fmm::build_ops(int l) {
-  LevelOps &O = ops_[l];
+size_t LevelOps::bytes() const {
+  return Wup.bytes + Vup.bytes + Wdn.bytes + Udn.bytes + M2M.bytes + L2L.bytes + T.bytes +
+         sizeof(double) * (svdU.size() + svdS.size() + svdV.size());
+}
+
+size_t MixedOps::bytes() const {
+  return Pup.bytes + Vr.bytes + Pdn.bytes + Ur.bytes + VrT.bytes + UrT.bytes + Mc.bytes + Lc.bytes + Cf.bytes +
+         Uf.bytes;
+}
+
+bool OpKey::operator<(const OpKey &o) const {
+  return std::tie(dev, kind, res, shells, level, rank, eps, mu, w, tau) <
+         std::tie(o.dev, o.kind, o.res, o.shells, o.level, o.rank, o.eps, o.mu, o.w, o.tau);
+}
+
+static std::mutex g_cache_mu;
+
+// never destroyed: kept operators must not be freed by a static destructor
+// after the HIP runtime has shut down (the process exit reclaims them)
+OpCache &OpCache::get() {
+  static OpCache *c = new OpCache;
+  return *c;
+}
+
+std::shared_ptr<const void> OpCache::find(const OpKey &k) {
+  std::lock_guard<std::mutex> g(g_cache_mu);
+  auto it = map_.find(k);
+  std::shared_ptr<const void> v;
+  if (it != map_.end()) v = it->second.live.lock();
+  if (v) {
+    hits_++;
+    it->second.tick = ++tick_;
+  } else {
+    misses_++;
+    if (it != map_.end()) map_.erase(it);
+  }
+  return v;
+}
+
+void OpCache::put(const OpKey &k, std::shared_ptr<const void> v, size_t bytes) {
+  std::lock_guard<std::mutex> g(g_cache_mu);
+  Ent &e = map_[k];
+  e.live = v;
+  e.kept = limit_ > 0 ? v : nullptr;
+  e.bytes = bytes;
+  e.tick = ++tick_;
+  trim();
+}
+
+// drop kept references, least recently used first, until the entries
+// without a live owner outside the cache fit in the limit; forget entries
+// that nothing owns any more
+void OpCache::trim() {
+  for (;;) {
+    int64_t held = 0;
+    auto lru = map_.end();
+    for (auto it = map_.begin(); it != map_.end();) {
+      if (it->second.live.expired()) { it = map_.erase(it); continue; }
+      if (it->second.kept && it->second.kept.use_count() == 1) {
+        held += (int64_t)it->second.bytes;
+        if (lru == map_.end() || it->second.tick < lru->second.tick) lru = it;
+      }
+      ++it;
+    }
+    if (held <= limit_ || lru == map_.end()) return;
+    lru->second.kept.reset();
+  }
+}
+
+void OpCache::set_limit(int64_t bytes) {
+  std::lock_guard<std::mutex> g(g_cache_mu);
+  limit_ = std::max<int64_t>(0, bytes);
+  if (limit_ == 0)
+    for (auto &kv : map_) kv.second.kept.reset();
+  else
+    for (auto &kv : map_) if (!kv.second.kept) kv.second.kept = kv.second.live.lock();
+  trim();
+}
+
+void OpCache::stats(int64_t *out) {
+  std::lock_guard<std::mutex> g(g_cache_mu);
+  trim();
+  int64_t held = 0, n = 0;
+  for (auto &kv : map_)
+    if (kv.second.kept) { held += (int64_t)kv.second.bytes; n++; }
+  out[0] = hits_;
+  out[1] = misses_;
+  out[2] = held;
+  out[3] = n;
+}
+
+void OpCache::clear() {
+  std::lock_guard<std::mutex> g(g_cache_mu);
+  map_.clear();
+  hits_ = misses_ = 0;
+}
+
+OpKey Fmm::key(int kind, int l) const {
+  int dev = 0;
+  NNK_HIP(hipGetDevice(&dev));
+  OpKey k{};
+  k.dev = dev;
+  k.kind = kind;
+  k.res = res_;
+  k.shells = shells_;
+  k.level = l;
+  k.rank = kind >= 2 ? R_ : 0;
+  k.eps = eps_;
+  k.mu = mu_;
+  k.w = std::ldexp(w0_, -l);
+  k.tau = kind >= 2 ? tau_ : 0.0;
+  return k;
+}
+
+// k_form_At of one source/target surface pair into dst (At layout)
+static void form_op(const std::vector<double> &src, const std::vector<double> &tgt, int nq, double e2, double c,
+                    double *dst, int Mpad) {
+  DBuf ds, dt;
+  ds.upload(src);
+  dt.upload(tgt);
+  int tot = nq * nq;
+  hipLaunchKernelGGL(k_form_At, dim3((tot + 255) / 256), dim3(256), 0, 0, ds.as<double>(), nq, dt.as<double>(), nq,
+                     e2, c, dst, Mpad);
+  NNK_HIP(hipGetLastError());
+  NNK_HIP(hipDeviceSynchronize());
+}
+
+std::shared_ptr<const LevelOps> Fmm::build_ops(int l, bool with_T) {
+  auto op = std::make_shared<LevelOps>();
+  LevelOps &O = *op;
   const auto codes = surface_codes(res_, shells_);
   const double w = std::ldexp(w0_, -l);
   const double e2 = eps_ * eps_, c = 1.0 / (8.0 * kPi * mu_);
@@ -592,11 +730,6 @@ void Fmm::build_ops(int l) {
             A[(size_t)(3 * p + a) * n + 3 * q + b] = (r[a] * r[b] + (a == b ? r2 + 2 * e2 : 0.0)) * invs;
       }
     svd_jacobi(A.data(), n, U.data(), S.data(), V.data());
-    if (prec_ >= 1) {
-      svdU_[l] = U;
-      svdS_[l] = S;
-      svdV_[l] = V;
-    }
     // up:   x = V (S^-1 U^T b);  down (K(down->up) = A^T): x = U (S^-1 V^T b)
     std::vector<double> Wup(mat, 0.0), Vup(mat, 0.0), Wdn(mat, 0.0), Udn(mat, 0.0);
     for (int k = 0; k < n; k++)
@@ -610,18 +743,11 @@ void Fmm::build_ops(int l) {
     O.Vup.upload(Vup);
     O.Wdn.upload(Wdn);
     O.Udn.upload(Udn);
+    O.svdU = std::move(U);
+    O.svdS = std::move(S);
+    O.svdV = std::move(V);
   }
   // ---- kernel-matrix operators formed on the device ----
-  auto form = [&](const std::vector<double> &src, const std::vector<double> &tgt, double *dst) {
-    DBuf ds, dt;
-    ds.upload(src);
-    dt.upload(tgt);
-    int tot = nq_ * nq_;
-    hipLaunchKernelGGL(k_form_At, dim3((tot + 255) / 256), dim3(256), 0, 0, ds.as<double>(), nq_,
-                       dt.as<double>(), nq_, e2, c, dst, Mpad_);
-    NNK_HIP(hipGetLastError());
-    NNK_HIP(hipDeviceSynchronize());
-  };
   O.M2M.alloc(sizeof(double) * mat * 8);
   O.L2L.alloc(sizeof(double) * mat * 8);
   O.M2M.zero();
@@ -633,29 +759,38 @@ void Fmm::build_ops(int l) {
     for (int k = 0; k < 8; k++) {
       double lo[3] = {((k >> 2) & 1) * w, ((k >> 1) & 1) * w, (k & 1) * w};
       auto cup = surface_pts(codes, res_, lo, w, false);
-      form(cup, pdn, O.M2M.as<double>() + k * mat);  // child up -> parent down
-      form(pdn, cup, O.L2L.as<double>() + k * mat);  // parent down -> child up
+      form_op(cup, pdn, nq_, e2, c, O.M2M.as<double>() + k * mat, Mpad_);  // child up -> parent down
+      form_op(pdn, cup, nq_, e2, c, O.L2L.as<double>() + k * mat, Mpad_);  // parent down -> child up
     }
   }
-  O.T.alloc(sizeof(double) * mat * 316);
-  O.T.zero();
+  if (with_T) form_T(l, O.T, O.tslot);
+  else O.tslot.assign(343, nullptr);
+  return op;
+}
+
+void Fmm::form_T(int l, DBuf &T, std::vector<double *> &tslot) {
+  const auto codes = surface_codes(res_, shells_);
+  const double w = std::ldexp(w0_, -l);
+  const double e2 = eps_ * eps_, c = 1.0 / (8.0 * kPi * mu_);
+  const size_t mat = (size_t)Kpad_ * Mpad_;
+  const double zero[3] = {0, 0, 0};
+  T.alloc(sizeof(double) * mat * 316);
+  T.zero();
   NNK_HIP(hipDeviceSynchronize());
-  O.tslot.assign(343, nullptr);
-  {
-    auto tup = surface_pts(codes, res_, zero, w, false);
-    int k = 0;
-    for (int ox = -3; ox <= 3; ox++)
-      for (int oy = -3; oy <= 3; oy++)
-        for (int oz = -3; oz <= 3; oz++) {
-          if (std::max({std::abs(ox), std::abs(oy), std::abs(oz)}) < 2) continue;
-          double lo[3] = {ox * w, oy * w, oz * w};
-          auto sup = surface_pts(codes, res_, lo, w, false);
-          double *dst = O.T.as<double>() + (size_t)k * mat;
-          form(sup, tup, dst);
-          O.tslot[offset_slot(ox, oy, oz)] = dst;
-          k++;
-        }
-  }
+  tslot.assign(343, nullptr);
+  auto tup = surface_pts(codes, res_, zero, w, false);
+  int k = 0;
+  for (int ox = -3; ox <= 3; ox++)
+    for (int oy = -3; oy <= 3; oy++)
+      for (int oz = -3; oz <= 3; oz++) {
+        if (std::max({std::abs(ox), std::abs(oy), std::abs(oz)}) < 2) continue;
+        double lo[3] = {ox * w, oy * w, oz * w};
+        auto sup = surface_pts(codes, res_, lo, w, false);
+        double *dst = T.as<double>() + (size_t)k * mat;
+        form_op(sup, tup, nq_, e2, c, dst, Mpad_);
+        tslot[offset_slot(ox, oy, oz)] = dst;
+        k++;
+      }
 }
 
 void Fmm::plan() {
@@ -716,7 +851,7 @@ void Fmm::plan() {
   };
   for (int l = 2; l <= maxlev_; l++) {
     LevelPlan &P = plans_[l];
-    LevelOps &O = ops_[l];
+    const LevelOps &O = *ops_[l];
     P.level = l;
     // ---- upward ----
     std::vector<int32_t> up, nl;
@@ -731,7 +866,7 @@ void Fmm::plan() {
       // M2M: the children (level l+1) carry class-k operators of level l+1
       std::vector<const double *> A(8);
       const size_t mat = (size_t)Kpad_ * Mpad_;
-      for (int k = 0; k < 8; k++) A[k] = ops_[l + 1].M2M.as<double>() + k * mat;
+      for (int k = 0; k < 8; k++) A[k] = ops_[l + 1]->M2M.as<double>() + k * mat;
       std::vector<int32_t> g(8 * nl.size());
       for (size_t j = 0; j < nl.size(); j++)
         for (int k = 0; k < 8; k++) {

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <array>
+#include <map>
 #include <memory>
 #include <vector>
 
@@ -68,23 +69,63 @@ struct HostLevel {
   std::vector<int32_t> cls[8], gv[8], gp[8];              // gv: 189 x |cls|
 };
 
-// compressed (truncated-SVD) operators of one level for the mixed form
+// compressed (truncated-SVD) operators of one level for the mixed form;
+// read-only once built (shared between Fmm objects through the cache)
 struct MixedOps {
   DBuf Pup, Vr, Pdn, Ur;      // FP64 At layout: R x K, K x R, R x K, K x R
   DBuf VrT, UrT;              // R rows of length K (precompute inputs)
-  DBuf Mc, Lc;                // 8 each, FP64 At (R x R)
+  DBuf Mc, Lc;                // 8 each, FP64 At (R x R); absent at level 2
   DBuf Cf;                    // 316 M2L operators, NP 16-bit parts in MFMA fragment order
   DBuf Uf;                    // FP16 form: per operator row, the unscale factor
   std::vector<const uint16_t *> cop;  // per offset slot (343)
   std::vector<const float *> uop;
+  size_t bytes() const;
 };
 
+// operators of one level (all boxes of a level share them); read-only once
+// built (shared between Fmm objects through the cache)
 struct LevelOps {
   DBuf Wup, Vup, Wdn, Udn;    // check-to-equivalent factors, At layout
   DBuf M2M, L2L;              // 8 operators each (this level's child class)
-  DBuf T;                     // 316 translation operators
-  DBuf Tptr;                  // device pointer table for T (343 slots)
+  DBuf T;                     // 316 translation operators (absent in the mixed forms)
   std::vector<double *> tslot;  // host copy: slot -> device pointer or null
+  std::vector<double> svdU, svdS, svdV;  // host SVD of K(up -> down), row-major
+  size_t bytes() const;
+};
+
+// Process-wide cache of per-level operators.  The operators of a level
+// depend only on (device, eps, mu, corona resolution, shells, box width,
+// level) and, for the mixed forms, on (precision, SVD cut, common rank), so
+// trees whose root cube is unchanged (time stepping, repeated solves) reuse
+// them instead of repeating the SVDs and kernel-matrix formation.  Live
+// entries are always shared; up to `limit` bytes of entries whose last
+// owner is gone are kept (least recently used first out).
+struct OpKey {
+  int dev, kind, res, shells, level, rank;  // kind: 0 FP64 (with T), 1 no T, 2+p mixed precision p
+  double eps, mu, w, tau;
+  bool operator<(const OpKey &o) const;
+};
+class OpCache {
+ public:
+  static OpCache &get();
+  std::shared_ptr<const void> find(const OpKey &k);
+  void put(const OpKey &k, std::shared_ptr<const void> v, size_t bytes);
+  void set_limit(int64_t bytes);
+  // hits, misses, bytes retained (beyond live owners), entries retained
+  void stats(int64_t *out4);
+  void clear();
+
+ private:
+  void trim();
+  struct Ent {
+    std::weak_ptr<const void> live;
+    std::shared_ptr<const void> kept;
+    size_t bytes = 0;
+    uint64_t tick = 0;
+  };
+  std::map<OpKey, Ent> map_;
+  int64_t limit_ = 0, hits_ = 0, misses_ = 0;
+  uint64_t tick_ = 0;
 };
 
 class Fmm {
@@ -135,7 +176,10 @@ class Fmm {
 
  private:
   void plan();
-  void build_ops(int level);
+  std::shared_ptr<const LevelOps> build_ops(int level, bool with_T);
+  // the 316 translation operators of a level into T (tslot: 343 pointers)
+  void form_T(int level, DBuf &T, std::vector<double *> &tslot);
+  OpKey key(int kind, int level) const;
   void gemm(const GemmBatch &b, const double *X, int ldx, double *C, int ldc, int beta, hipStream_t s,
             int tag = 0, int nsplit = 1, const DBuf *slot = nullptr);
   // split-K factor for a launch of `tiles` workgroups over blocks of nb
@@ -176,14 +220,13 @@ class Fmm {
   int n_pleaves_ = 0, n_tleaves_ = 0;
   // work buffers
   DBuf chk_, uppot_, downpot_, tmp_;
-  std::vector<LevelOps> ops_;
+  std::vector<std::shared_ptr<const LevelOps>> ops_;
   std::vector<LevelPlan> plans_;
   std::vector<HostLevel> hl_;
   int prec_ = 0;
   double tau_ = 1e-10;
   int R_ = 0, RP_ = 0, RPm_ = 0, MpadR_ = 0, KpadR_ = 0;
-  std::vector<std::vector<double>> svdU_, svdS_, svdV_;  // host, per level
-  std::vector<MixedOps> mops_;
+  std::vector<std::shared_ptr<const MixedOps>> mops_;
   DBuf chat_, dhat_, chat_x_, upall_;
   int np_ = 2, KCm_ = 0, RT_ = 0;   // mixed M2L: 16-bit parts, 32-deep k chunks, 16-row tiles
   bool f16_ = false;                // FP16 parts (with scaling) instead of BF16

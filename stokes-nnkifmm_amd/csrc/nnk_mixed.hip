@@ -183,7 +183,7 @@ void Fmm::build_mixed() {
   const int K = K_;
   R_ = 0;
   for (int l = 2; l <= maxlev_; l++) {
-    const auto &S = svdS_[l];
+    const auto &S = ops_[l]->svdS;
     int r = 0;
     while (r < K && S[r] >= tau_ * S[0]) r++;
     R_ = std::max(R_, r);
@@ -198,13 +198,21 @@ void Fmm::build_mixed() {
   MpadR_ = (R_ + 159) / 160 * 160;
   KpadR_ = (R_ + 15) / 16 * 16;
   const int R = R_;
-  mops_.resize(maxlev_ + 1);
   const size_t matKR = (size_t)Kpad_ * MpadR_;   // R x K operators (Pup, Pdn)
   const size_t matRK = (size_t)KpadR_ * Mpad_;   // K x R operators (Vr, Ur)
   const size_t matRR = (size_t)KpadR_ * MpadR_;  // R x R operators
+  OpCache &cache = OpCache::get();
+  mops_.assign(maxlev_ + 1, nullptr);
+  std::vector<std::shared_ptr<MixedOps>> fresh(maxlev_ + 1);
   for (int l = 2; l <= maxlev_; l++) {
-    MixedOps &X = mops_[l];
-    const auto &U = svdU_[l], &S = svdS_[l], &V = svdV_[l];
+    if (auto hit = cache.find(key(2 + prec_, l))) {
+      mops_[l] = std::static_pointer_cast<const MixedOps>(hit);
+      continue;
+    }
+    fresh[l] = std::make_shared<MixedOps>();
+    mops_[l] = fresh[l];
+    MixedOps &X = *fresh[l];
+    const auto &U = ops_[l]->svdU, &S = ops_[l]->svdS, &V = ops_[l]->svdV;
     std::vector<double> Pup(matKR, 0.0), Pdn(matKR, 0.0), Vr(matRK, 0.0), Ur(matRK, 0.0),
         VrT((size_t)R * K), UrT((size_t)R * K);
     for (int k = 0; k < K; k++)
@@ -250,13 +258,17 @@ void Fmm::build_mixed() {
     return v;
   };
   for (int l = 2; l <= maxlev_; l++) {
-    LevelOps &O = ops_[l];
-    MixedOps &X = mops_[l];
-    // ---- M2L: C_o = Pdn T_o V_R ----
+    if (!fresh[l]) continue;
+    const LevelOps &O = *ops_[l];
+    MixedOps &X = *fresh[l];
+    // ---- M2L: C_o = Pdn T_o V_R, with T_o formed for this level only ----
+    DBuf Ttmp;
+    std::vector<double *> tslot = O.tslot;
+    if (!O.T.p) form_T(l, Ttmp, tslot);
     std::vector<const double *> Ts;
     std::vector<int> slots;
     for (int sl = 0; sl < 343; sl++)
-      if (O.tslot[sl]) { Ts.push_back(O.tslot[sl]); slots.push_back(sl); }
+      if (tslot[sl]) { Ts.push_back(tslot[sl]); slots.push_back(sl); }
     const int no = (int)Ts.size();
     {
       DBuf Y, CT;
@@ -304,14 +316,14 @@ void Fmm::build_mixed() {
       X.Mc.alloc(sizeof(double) * matRR * 8);
       X.Mc.zero();
       auto outs = offs_ids(8, KpadR_);
-      std::vector<const double *> P(8, mops_[l - 1].Pup.as<double>());
+      std::vector<const double *> P(8, mops_[l - 1]->Pup.as<double>());
       GemmBatch b2 = batch(8, P, R, &ids, &outs);
       gemm_dims(b2, Y.as<double>(), K, X.Mc.as<double>(), MpadR_, 0, 0, R, K, MpadR_, Kpad_);
       // ---- L2L: Lc_a = Pdn(l) L2L_a U_R(l-1) ----
       std::vector<const double *> Ls(8);
       for (int k = 0; k < 8; k++) Ls[k] = O.L2L.as<double>() + k * mat;
       GemmBatch b3 = batch(8, Ls, R, nullptr, &ids);
-      gemm_dims(b3, mops_[l - 1].UrT.as<double>(), K, Y.as<double>(), K, 0, 0, K, K, Mpad_, Kpad_);
+      gemm_dims(b3, mops_[l - 1]->UrT.as<double>(), K, Y.as<double>(), K, 0, 0, K, K, Mpad_, Kpad_);
       X.Lc.alloc(sizeof(double) * matRR * 8);
       X.Lc.zero();
       std::vector<const double *> P2(8, X.Pdn.as<double>());
@@ -319,11 +331,7 @@ void Fmm::build_mixed() {
       gemm_dims(b4, Y.as<double>(), K, X.Lc.as<double>(), MpadR_, 0, 0, R, K, MpadR_, Kpad_);
       NNK_HIP(hipDeviceSynchronize());
     }
-  }
-  // the FP64 translation operators are no longer needed in this form
-  for (int l = 2; l <= maxlev_; l++) {
-    ops_[l].T.release();
-    ops_[l].tslot.assign(343, nullptr);
+    cache.put(key(2 + prec_, l), fresh[l], X.bytes());
   }
   const int nc = tree_.count;
   chat_.alloc(sizeof(double) * (size_t)nc * RP_);
@@ -367,14 +375,14 @@ void Fmm::plan_mixed() {
   for (int l = 2; l <= maxlev_; l++) {
     LevelPlan &P = plans_[l];
     HostLevel &H = hl_[l];
-    MixedOps &X = mops_[l];
+    const MixedOps &X = *mops_[l];
     upall.insert(upall.end(), H.up_leaves.begin(), H.up_leaves.end());
     upall.insert(upall.end(), H.up_nonleaves.begin(), H.up_nonleaves.end());
     if (!H.up_nonleaves.empty()) {
       const int N = (int)H.up_nonleaves.size();
       std::vector<GemmJob> jobs(1);
       std::vector<const double *> A(8);
-      for (int k = 0; k < 8; k++) A[k] = mops_[l + 1].Mc.as<double>() + k * matRR;
+      for (int k = 0; k < 8; k++) A[k] = mops_[l + 1]->Mc.as<double>() + k * matRR;
       P.mx_m2m.owned.emplace_back();
       P.mx_m2m.owned.back().upload(A);
       jobs[0].A = P.mx_m2m.owned.back().as<const double *>();

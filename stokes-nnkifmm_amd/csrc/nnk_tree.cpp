@@ -281,7 +281,7 @@ void build_lists(Tree &t) {
 }  // namespace
 
 int build_tree(const double *pot, int64_t npot, const double *tgt, int64_t ntgt, int capacity,
-               int max_depth, Tree &t, std::string &err) {
+               int max_depth, Tree &t, std::string &err, const double *cube) {
   t = Tree();
   if (npot < 0 || ntgt < 0 || npot + ntgt < 2) {
     err = "OcTree needs at least two points";
@@ -321,6 +321,14 @@ int build_tree(const double *pot, int64_t npot, const double *tgt, int64_t ntgt,
   for (int64_t i = 0; i < 3 * np; i++) {
     m = std::min(m, t.points[i]);
     M = std::max(M, t.points[i]);
+  }
+  if (cube) {
+    if (!(cube[0] <= m && M <= cube[1])) {
+      err = "OcTree: the root cube does not contain every point";
+      return NNK_EARG;
+    }
+    m = cube[0];
+    M = cube[1];
   }
   t.max_depth = max_depth;
   t.capacity = (int32_t)std::min<int64_t>(capacity, np - 1);

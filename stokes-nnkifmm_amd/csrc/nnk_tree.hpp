@@ -38,9 +38,11 @@ struct Tree {
   const double *box(int n) const { return &corners[6 * (n - 1)]; }
 };
 
-// Build; returns 0 or an error code with a message in err.
+// Build; returns 0 or an error code with a message in err.  cube: null for
+// the bounding cube of the points (OcTree.m), or a fixed root
+// [cube[0] x3, cube[1] x3] containing every point.
 int build_tree(const double *pot, int64_t npot, const double *tgt, int64_t ntgt,
-               int capacity, int max_depth, Tree &t, std::string &err);
+               int capacity, int max_depth, Tree &t, std::string &err, const double *cube = nullptr);
 int set_nearest(Tree &t, const double *fine, int64_t nfine, const int64_t *colptr,
                 const int64_t *row, const double *val, std::string &err);
 

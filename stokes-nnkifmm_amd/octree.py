@@ -12,7 +12,7 @@ from . import _lib
 from ._lib import c64, check, lib, ptr
 
 _OPTS = {"parThreads": 0, "nodeCapacity": 200, "maxDepth": 21, "finePoints": None, "nearest": 0,
-         "NN": None, "blockSize": 0.2}
+         "NN": None, "blockSize": 0.2, "rootCube": None}
 
 
 def parse_options(defaults, args, kwargs, where):
@@ -55,15 +55,25 @@ class Interactions:
 
 class OcTree:
     """OcTree(potPoints, targetPoints, 'nodeCapacity', c, 'maxDepth', d,
-    'nearest', 0|1, 'finePoints', F, 'NN', NN, ...)   Classes/OcTree.m:77-129"""
+    'nearest', 0|1, 'finePoints', F, 'NN', NN, ...)   Classes/OcTree.m:77-129
+
+    One option beyond the reference: 'rootCube', [lo, hi] fixes the root box
+    instead of taking the points' bounding cube."""
 
     def __init__(self, potPoints, targetPoints, *args, **kwargs):
         a = parse_options(_OPTS, args, kwargs, "OcTree")
         pot = c64(potPoints).reshape(-1, 3)
         tgt = c64(targetPoints).reshape(-1, 3)
         h = ctypes.c_void_p()
-        check(lib().nnk_tree_create(ptr(pot), len(pot), ptr(tgt), len(tgt), int(a["nodeCapacity"]),
-                                    int(a["maxDepth"]), ctypes.byref(h)))
+        if a["rootCube"] is None:
+            check(lib().nnk_tree_create(ptr(pot), len(pot), ptr(tgt), len(tgt), int(a["nodeCapacity"]),
+                                        int(a["maxDepth"]), ctypes.byref(h)))
+        else:  # extension: fixed root [lo lo lo hi hi hi] (time stepping reuses its operators)
+            cube = c64(a["rootCube"]).ravel()
+            if cube.size != 2:
+                raise ValueError("OcTree: rootCube must be [lo, hi]")
+            check(lib().nnk_tree_create_in_cube(ptr(pot), len(pot), ptr(tgt), len(tgt), int(a["nodeCapacity"]),
+                                                int(a["maxDepth"]), ptr(cube), ctypes.byref(h)))
         self._h = h
         npts, npot, ntgt = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
         cnt, cap = ctypes.c_int32(), ctypes.c_int32()

@@ -85,3 +85,34 @@ def test_library_exports_header_symbols(nnk):
     sig = import_module("stokes-nnkifmm_amd._lib").SIGNATURES
     assert names == set(sig), names ^ set(sig)
     assert lib.nnk_version() >= 100
+
+
+def test_root_cube_option(oracle, nnk):
+    """'rootCube' (an extension: fixed root box for time stepping).  With the
+    points' own bounding cube it reproduces the default tree; with a larger
+    cube the product tree equals the oracle's and every node has width
+    (hi - lo) / 2^level and contains its points."""
+    rng = np.random.default_rng(5)
+    p = rng.random((3000, 3)) * [0.7, 0.9, 0.5] + 0.1
+    m, M = p.min(), p.max()
+    a = nnk.OcTree(p, p, "nodeCapacity", 24)
+    b = nnk.OcTree(p, p, "nodeCapacity", 24, "rootCube", [m, M])
+    for attr in ("points", "pointIndex", "nodeCorners", "nodeLevel", "nodeParents", "nodeChildren"):
+        np.testing.assert_array_equal(getattr(a, attr), getattr(b, attr), err_msg=attr)
+    lo, hi = -1.0, 2.0
+    c = nnk.OcTree(p, p, "nodeCapacity", 24, "rootCube", [lo, hi])
+    d = oracle.Tree(p, p, nodeCapacity=24, rootCube=[lo, hi])
+    assert c.nodeCount == d.nodeCount
+    for attr in ("points", "pointIndex", "nodeCorners", "nodeLevel", "nodeParents", "nodeChildren"):
+        np.testing.assert_array_equal(getattr(c, attr), getattr(d, attr), err_msg=attr)
+    for kind in range(4):
+        for n in range(1, c.nodeCount + 1):
+            np.testing.assert_array_equal(c.interactions(n, kind + 1), d.interactions(n, kind))
+    w = c.nodeCorners[:, 3:] - c.nodeCorners[:, :3]
+    np.testing.assert_allclose(w, ((hi - lo) / 2.0 ** c.nodeLevel)[:, None] * np.ones(3), rtol=1e-14)
+    box = c.nodeCorners[c.pointIndex - 1]
+    assert np.all(c.points >= box[:, :3]) and np.all(c.points <= box[:, 3:])
+    with pytest.raises(nnk.NNKError, match="root cube"):
+        nnk.OcTree(p, p, "rootCube", [0.2, 2.0])
+    with pytest.raises(RuntimeError, match="root cube"):
+        oracle.Tree(p, p, rootCube=[0.2, 2.0])
