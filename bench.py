@@ -303,6 +303,57 @@ def adjoint_pair(nnk):
             "adjoint_iterations": bm6.last_solve["iter"].tolist()}
 
 
+def time_stepping_squirmers(nnk):
+    """Data Generation/Squiremers.m: 10 squirmers (radius 0.5, B1 = 2) in a
+    closed box of six 10 x 10 walls (spacing 0.3), eps = 0.1, nodeCapacity
+    500, GMRES tol 1e-4 (at most 50 iterations), preconditioned; ode45
+    (RelTol 1e-3) over t in [0, 0.3].  Every right-hand side rebuilds the
+    tree, the KIFMM handles and B/BT as the reference does; the per-level
+    operators come from the operator cache when it is on (the walls fix the
+    root box, so every rebuild hits)."""
+    import gc
+    import importlib as il
+
+    import torch
+    geo = il.import_module("stokes-nnkifmm_amd.geometries")
+    t0 = time.perf_counter()
+    sw, y0 = geo.squirmer_box(nsw=10, seed=1)
+    t_geo = time.perf_counter() - t0
+    calls = [0]
+
+    def rhs(t, x):
+        calls[0] += 1
+        return nnk.mobilityProblem(sw, lambda t, p: np.zeros_like(p), t, x, [0.1, 1.0],
+                                   ["nodeCapacity", 500, "nearest", 1], 20, 0, 1, 1e-4, 50, [1e-5, 100])
+
+    def timed(k):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(k):
+            r = rhs(0.0, y0)
+        return (time.perf_counter() - t0) / k, r
+    nnk.operatorCache(0)
+    gc.collect()
+    cold, r_cold = timed(3)
+    nnk.operatorCache(8 << 30)
+    rhs(0.0, y0)
+    warm, r_warm = timed(3)
+    assert np.array_equal(r_cold, r_warm)
+    calls[0] = 0
+    t0 = time.perf_counter()
+    t, y = nnk.ode45(rhs, [0.0, 0.3], y0)
+    t_ode = time.perf_counter() - t0
+    hits, misses, kept, _ = nnk.operatorCacheStats()
+    nnk.operatorCache(0)
+    disp = np.linalg.norm((y[-1] - y[0]).reshape(-1, 9)[:, :3], axis=1)
+    return {"bodies": 10, "force_points": int(len(sw.getSwimmerBnd()[0])), "setup_geometry_s": round(t_geo, 3),
+            "rhs_cold_s": round(cold, 4), "rhs_warm_s": round(warm, 4), "rhs_speedup": round(cold / warm, 2),
+            "ode45": {"t_end": 0.3, "wall_s": round(t_ode, 3), "rhs_evaluations": calls[0],
+                      "output_points": int(len(t)), "mean_displacement": float(disp.mean())},
+            "cache": {"hits": hits, "misses": misses, "bytes_kept": kept},
+            "bit_identical_cold_vs_warm": True}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -396,6 +447,8 @@ def main():
         print("# batched solves done", file=sys.stderr, flush=True)
         out["adjoint_squirmer_pair"] = adjoint_pair(nnk)
         print("# adjoint done", file=sys.stderr, flush=True)
+        out["time_stepping_squirmers"] = time_stepping_squirmers(nnk)
+        print("# time stepping done", file=sys.stderr, flush=True)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args, eps, mu)
     print(json.dumps(out), flush=True)

@@ -17,6 +17,11 @@ input scripts outside the accelerated path):
   batch_bodies      independent single-body systems (spheres and
                     spheroids, 200-5000 force points log-uniform, 4x
                     refined quadrature, eps uniform in [0.002, 0.02])
+  wall, prolateSpheroid, Rotate, unique_rows
+                    the reference's generators (Geometries/wall.m,
+                    prolateSpheroid.m, Rotate.m; unique(...,'rows','stable'))
+  squirmer_box      the suspension of Data Generation/Squiremers.m: squirmers
+                    in a closed box of six walls
 """
 import numpy as np
 
@@ -173,3 +178,105 @@ def squirmer_pair(n=16, refine=4, gap=0.5, B1=1.0, B2=0.0, tilt=0.5):
         b2 = [0.0, 1.0, 0.0]                # body y axis; body z = b1 x b2
         sw.addSwimmer(body, fine, NN, [sgn * d, 0.0, 0.0], b1 + b2, vel, np.zeros(3), np.zeros(3))
     return sw
+
+
+def _linspace(a, b, n):
+    """MATLAB linspace(a, b, n) (n >= 1; the last point is b)."""
+    n = int(n)
+    if n <= 1:
+        return np.array([float(b)])
+    return np.linspace(float(a), float(b), n)
+
+
+def wall(centre, scale, axis, h):
+    """Geometries/wall.m: a ceil(scale/h) grid on the plane through centre
+    normal to `axis` (1 = x, 2 = y, 3 = z), in meshgrid column-major order."""
+    c = np.asarray(centre, float).ravel()
+    sc = np.asarray(scale, float).ravel()
+    res = np.ceil(sc / h).astype(int)
+    ax = int(axis)
+    if ax not in (1, 2, 3):
+        raise ValueError("wall: axis must be 1, 2 or 3")
+    if ax == 1:
+        u, v = (1, 2)
+    elif ax == 2:
+        u, v = (0, 2)
+    else:
+        u, v = (0, 1)
+    gu = _linspace(c[u] - sc[0] / 2, c[u] + sc[0] / 2, res[0])
+    gv = _linspace(c[v] - sc[1] / 2, c[v] + sc[1] / 2, res[1])
+    U, V = np.meshgrid(gu, gv)                # rows follow gv, columns gu
+    p = np.empty((U.size, 3))
+    p[:, ax - 1] = c[ax - 1]
+    p[:, u] = U.ravel(order="F")
+    p[:, v] = V.ravel(order="F")
+    return p
+
+
+def prolateSpheroid(a, c, h):
+    """Geometries/prolateSpheroid.m: the two poles, then rings of latitude
+    v(i) = pi (i-1)/(vN-1) with ceil(2 pi a sqrt(1 - (z/c)^2) / h) points."""
+    vN = int(np.ceil(np.pi * (3 * (a + c) - np.sqrt((3 * a + c) * (a + 3 * c))) / (2 * h)))
+    v = _linspace(0, np.pi, vN)
+    pts = [np.array([[0.0, 0.0, c], [0.0, 0.0, -c]])]
+    for i in range(1, vN - 1):
+        z = c * np.cos(v[i])
+        uN = int(np.ceil(2 * np.pi * a * np.sqrt(1 - (z / c) ** 2) / h))
+        u = _linspace(0, 2 * np.pi, uN + 1)[1:]
+        pts.append(np.stack([a * np.cos(u) * np.sin(v[i]), a * np.sin(u) * np.sin(v[i]), np.full(uN, z)], 1))
+    return np.concatenate(pts)
+
+
+def Rotate(points, alpha, beta, gamma, centre):
+    """Geometries/Rotate.m: rotate about centre by Rx(alpha) Ry(beta) Rz(gamma)
+    (the matrices as written there)."""
+    ca, sa, cb, sb, cg, sg = np.cos(alpha), np.sin(alpha), np.cos(beta), np.sin(beta), np.cos(gamma), np.sin(gamma)
+    Rx = np.array([[1, 0, 0], [0, ca, sa], [0, -sa, ca]])
+    Ry = np.array([[cb, 0, -sb], [0, 1, 0], [sb, 0, cb]])
+    Rz = np.array([[cg, sg, 0], [-sg, cg, 0], [0, 0, 1]])
+    c = np.asarray(centre, float).ravel()
+    return ((Rx @ Ry @ Rz) @ (np.asarray(points, float) - c).T).T + c
+
+
+def unique_rows(p):
+    """unique(p, 'rows', 'stable'): first occurrence of each row, in order."""
+    p = np.ascontiguousarray(p, float)
+    _, first = np.unique(p, axis=0, return_index=True)
+    return p[np.sort(first)]
+
+
+def squirmer_box(nsw=10, seed=0, a=0.5, h_coarse=0.25, h_fine=0.13, box=10.0, wall_h=0.3, B1=2.0,
+                 min_dist=1.5):
+    """The suspension of Data Generation/Squiremers.m: `nsw` spherical
+    squirmers (prolateSpheroid(a, a, h) bodies, coarse h_coarse, fine h_fine,
+    nearest map, slip B1 sin(theta) e_theta about the body z axis) at seeded
+    positions in [-4, 4]^3 at least min_dist apart with random orientations,
+    inside a closed box of six box x box walls with spacing wall_h (the
+    boundary is its own quadrature, NN = identity).  Returns (Swimmers,
+    initial condition [x0; b1; b2] per body)."""
+    from .nearest import nearestMatrix
+    from .swimmers import Swimmers
+    rng = np.random.default_rng(seed)
+    hb = box / 2
+    bnd = np.concatenate([wall([hb, 0, 0], [box, box], 1, wall_h), wall([-hb, 0, 0], [box, box], 1, wall_h),
+                          wall([0, hb, 0], [box, box], 2, wall_h), wall([0, -hb, 0], [box, box], 2, wall_h),
+                          wall([0, 0, hb], [box, box], 3, wall_h), wall([0, 0, -hb], [box, box], 3, wall_h)])
+    bnd = unique_rows(bnd)
+    sw = Swimmers()
+    sw.updateBnd(bnd, bnd, nearestMatrix(bnd, bnd), np.zeros_like(bnd))
+    coarse = prolateSpheroid(a, a, h_coarse)
+    fine = prolateSpheroid(a, a, h_fine)
+    NN = nearestMatrix(coarse, fine)
+    vel = squirmer_slip(coarse, B1=B1)
+    centres = []
+    while len(centres) < nsw:
+        x = -4 + 8 * rng.random(3)
+        if all(np.linalg.norm(x - y) >= min_dist for y in centres):
+            centres.append(x)
+    y0 = []
+    for x in centres:
+        basis = Rotate(np.eye(3), *(2 * np.pi * rng.random(3)), [0, 0, 0])
+        b1b2 = basis.ravel(order="F")[:6]
+        sw.addSwimmer(coarse, fine, NN, x, b1b2, vel, [0, 0, 0], [0, 0, 0])
+        y0.append(np.concatenate([x, b1b2]))
+    return sw, np.concatenate(y0)
