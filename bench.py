@@ -219,6 +219,15 @@ def spheroid_solve(nnk):
     out = {"force_points": int(sw.Tree.points.shape[0]), "quadrature_points": int(len(sw.getSwimmerBnd()[1])),
            "tree_nodes": int(sw.Tree.nodeCount), "setup_s": {"geometry_nearest": round(t_geo, 3),
                                                              "tree_B": round(t_setup, 3)}}
+    # nearestMatrix over the whole suspension (every coarse point against
+    # every fine point, 2.6e11 pairs by brute force); the grid search is exact
+    pts, fine_all, _ = sw.getSwimmerBnd()
+    t0 = time.perf_counter()
+    NNg = nnk.nearestMatrix(pts, fine_all)
+    t_nn = time.perf_counter() - t0
+    out["nearest_matrix_whole_system"] = {
+        "coarse": int(len(pts)), "fine": int(len(fine_all)), "wall_s": round(t_nn, 3), "nnz": int(NNg.nnz),
+        "pairs_brute_force": float(len(pts)) * float(len(fine_all))}
     t0 = time.perf_counter()
     x64 = mob.mobility_gmres(sw, B, BT, A1, A2, b, 1e-6, 400, True)
     torch.cuda.synchronize()

@@ -18,10 +18,24 @@ def rel(a, b):
     return np.linalg.norm(a - b) / np.linalg.norm(b)
 
 
-def test_nearest_matrix_bit_exact(oracle, nnk):
+def _nearest_cases():
     rng = np.random.default_rng(0)
-    for coarse, fine in ((cube_sphere(8), cube_sphere(32)), (rng.random((500, 3)), rng.random((4000, 3))),
-                         (np.array([[0.0, 0, 0], [1, 0, 0], [0, 1, 0]]), np.array([[.5, 0, 0], [.5, .5, 0]]))):
+    yield cube_sphere(8), cube_sphere(32)
+    yield rng.random((500, 3)), rng.random((4000, 3))
+    yield np.array([[0.0, 0, 0], [1, 0, 0], [0, 1, 0]]), np.array([[.5, 0, 0], [.5, .5, 0]])
+    g = np.stack(np.meshgrid(*[np.arange(8.0)] * 3, indexing="ij"), -1).reshape(-1, 3)
+    yield g, g + 0.5                                           # 8-way ties at cell centres
+    yield g, rng.integers(-2, 17, (3000, 3)) / 2.0             # lattice ties, points outside the grid
+    yield np.c_[rng.random((2000, 2)), np.zeros(2000)], rng.random((3000, 3)) * 3 - 1   # flat coarse set
+    yield np.zeros((5, 3)), rng.random((50, 3))                 # coincident coarse points
+    s = cube_sphere(20)
+    yield s, np.r_[np.zeros((1, 3)), 1.3 * s[:500], 0.2 * rng.standard_normal((500, 3))]
+    yield rng.random((3000, 3)) * 1e6 + 1e7, rng.random((2000, 3)) * 1e6 + 1e7
+    yield cube_sphere(32, 1.0, (0, 0, 0)), cube_sphere(128)    # 6144 x 98304
+
+
+def test_nearest_matrix_bit_exact(oracle, nnk):
+    for coarse, fine in _nearest_cases():
         NN = nnk.nearestMatrix(coarse, fine)
         r, c, w = oracle.nearest_matrix(coarse, fine)
         ref = sp.csc_matrix((w, (r, c)), shape=(len(fine), len(coarse)))
