@@ -8,6 +8,9 @@
 //    splits when a node holds more than capacity points, recursing into
 //    the 8 new children before the next sibling (DFS creation order)  (:165-178)
 //  * centre = (lo+hi)/2, child k = 1 + 4[x>c] + 2[y>c] + [z>c]          (:191-221)
+// The subdivision runs on the GPU (nnk_tree_gpu.hip: path keys, one radix
+// sort, level-by-level node creation, preorder ids); the de-duplication
+// and the lists run here.
 //  * closed-box adjacency                       (Tools/Tree/nodeintersection.m)
 //  * U, V, W lists and the W coarsening, X = find(Wpadded == i)         (:265-366)
 // The lists are found by descending from the root through the nodes whose
@@ -65,91 +68,6 @@ inline bool isect(const double *c1, const double *c2) {
   return !(c2[0] > c1[3] || c2[3] < c1[0] || c2[1] > c1[4] || c2[4] < c1[1] ||
            c2[2] > c1[5] || c2[5] < c1[2]);
 }
-
-struct Builder {
-  Tree &t;
-  std::vector<int64_t> perm;               // point rows, grouped by node
-  std::vector<int64_t> beg, end;           // per node [id-1]
-  std::string &err;
-  Builder(Tree &tt, std::string &e) : t(tt), err(e) {}
-
-  void add_nodes(int k) {
-    size_t need = (size_t)t.count + k;
-    if (t.level.size() < need) {
-      size_t na = std::max<size_t>(need, 2 * t.level.size());
-      t.corners.resize(6 * na);
-      t.level.resize(na);
-      t.parent.resize(na);
-      t.children.resize(8 * na);
-      beg.resize(na);
-      end.resize(na);
-    }
-  }
-
-  void divide_node(int node) {
-    int base = t.count;
-    add_nodes(8);
-    const double *c = &t.corners[6 * (node - 1)];
-    double lo[3] = {c[0], c[1], c[2]}, hi[3] = {c[3], c[4], c[5]}, ce[3];
-    for (int d = 0; d < 3; d++) ce[d] = (lo[d] + hi[d]) / 2.0;
-    const double co[9] = {lo[0], lo[1], lo[2], ce[0], ce[1], ce[2], hi[0], hi[1], hi[2]};
-    static const int rows[8][6] = {
-        {0, 1, 2, 3, 4, 5}, {0, 1, 5, 3, 4, 8}, {0, 4, 2, 3, 7, 5}, {0, 4, 5, 3, 7, 8},
-        {3, 1, 2, 6, 4, 5}, {3, 1, 5, 6, 4, 8}, {3, 4, 2, 6, 7, 5}, {3, 4, 5, 6, 7, 8}};
-    for (int k = 0; k < 8; k++) {
-      int id = base + k + 1;
-      for (int e = 0; e < 6; e++) t.corners[6 * (id - 1) + e] = co[rows[k][e]];
-      t.level[id - 1] = t.level[node - 1] + 1;
-      t.parent[id - 1] = node;
-      std::fill(&t.children[8 * (id - 1)], &t.children[8 * id], 0);
-      t.children[8 * (node - 1) + k] = id;
-    }
-    // counting partition of this node's points into the 8 children
-    int64_t b = beg[node - 1], e = end[node - 1];
-    int64_t cnt[8] = {0};
-    std::vector<uint8_t> loc(e - b);
-    for (int64_t i = b; i < e; i++) {
-      const double *p = &t.points[3 * perm[i]];
-      int k = 4 * (p[0] > ce[0]) + 2 * (p[1] > ce[1]) + (p[2] > ce[2]);
-      loc[i - b] = (uint8_t)k;
-      cnt[k]++;
-    }
-    int64_t off[8];
-    int64_t run = b;
-    for (int k = 0; k < 8; k++) {
-      off[k] = run;
-      beg[base + k] = run;
-      run += cnt[k];
-      end[base + k] = run;
-    }
-    std::vector<int64_t> tmp(perm.begin() + b, perm.begin() + e);
-    for (int64_t i = 0; i < e - b; i++) {
-      int k = loc[i];
-      perm[off[k]++] = tmp[i];
-      t.point_index[tmp[i]] = base + k + 1;
-    }
-    t.count = base + 8;
-  }
-
-  int divide(const int *nodes, int nn) {
-    for (int i = 0; i < nn; i++) {
-      int node = nodes[i];
-      if (t.level[node - 1] + 1 >= t.max_depth) {
-        err = "Max level reached: Increase Max Depth";
-        return NNK_EMAXDEPTH;
-      }
-      int old = t.count;
-      if (end[node - 1] - beg[node - 1] > t.capacity) {
-        divide_node(node);
-        int kids[8];
-        for (int k = 0; k < 8; k++) kids[k] = old + k + 1;
-        int rc = divide(kids, 8);
-        if (rc) return rc;
-      }
-    }
-    return NNK_OK;
-  }
-};
 
 // Visit every node whose closed box meets q and whose level <= maxlev.
 template <class F>
@@ -332,27 +250,8 @@ int build_tree(const double *pot, int64_t npot, const double *tgt, int64_t ntgt,
   }
   t.max_depth = max_depth;
   t.capacity = (int32_t)std::min<int64_t>(capacity, np - 1);
-  Builder b(t, err);
-  t.count = 0;
-  b.add_nodes(1);
-  t.count = 1;
-  const double rc[6] = {m, m, m, M, M, M};
-  std::memcpy(&t.corners[0], rc, sizeof rc);
-  t.level[0] = 0;
-  t.parent[0] = 0;
-  std::fill(&t.children[0], &t.children[8], 0);
-  t.point_index.assign(np, 1);
-  b.perm.resize(np);
-  for (int64_t i = 0; i < np; i++) b.perm[i] = i;
-  b.beg[0] = 0;
-  b.end[0] = np;
-  int root = 1;
-  int rcode = b.divide(&root, 1);
+  int rcode = subdivide_gpu(t, m, M, err);
   if (rcode) return rcode;
-  t.corners.resize(6 * t.count);
-  t.level.resize(t.count);
-  t.parent.resize(t.count);
-  t.children.resize(8 * t.count);
   t.max_level = 0;
   for (int n = 0; n < t.count; n++) t.max_level = std::max(t.max_level, t.level[n]);
   t.pot_leaf.assign(t.count + 1, 0);

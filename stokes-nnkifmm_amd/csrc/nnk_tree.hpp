@@ -43,6 +43,10 @@ struct Tree {
 // [cube[0] x3, cube[1] x3] containing every point.
 int build_tree(const double *pot, int64_t npot, const double *tgt, int64_t ntgt,
                int capacity, int max_depth, Tree &t, std::string &err, const double *cube = nullptr);
+// GPU subdivision of the root cube [m m m M M M] over t.points with
+// t.capacity and t.max_depth: fills count, corners, level, parent, children
+// and point_index (nnk_tree_gpu.hip)
+int subdivide_gpu(Tree &t, double m, double M, std::string &err);
 int set_nearest(Tree &t, const double *fine, int64_t nfine, const int64_t *colptr,
                 const int64_t *row, const double *val, std::string &err);
 

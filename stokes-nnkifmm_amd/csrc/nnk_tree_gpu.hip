@@ -1,0 +1,309 @@
+// Octree subdivision on the GPU with the reference's node numbering
+// (Classes/OcTree.m:155-222).
+//
+// A point's path through the tree depends only on the root cube: at each
+// level its box is split at the centre (lo+hi)/2 and it goes to child
+// 1 + 4[x>cx] + 2[y>cy] + [z>cz] (:191-221), the child box being copies of
+// the parent's lo/centre/hi.  So every point's path key (3 bits per level,
+// most significant first) is computed independently, with the same
+// floating-point operations as the reference, and one stable radix sort
+// groups the points of every node of every level into a contiguous range
+// (points of a node stay in ascending row order).
+//
+// The nodes are then created level by level (breadth first): a node splits
+// when it holds more than `capacity` points, and its 8 children (all of
+// them, also empty ones, :191-221) find their point ranges by binary search
+// in the sorted keys.  The reference numbers nodes in the order Divide
+// creates them: a splitting node allocates 8 consecutive ids, and Divide
+// recurses into each child before the next sibling (:165-178).  A node's
+// children therefore get ids 2 + 8 r .. 9 + 8 r, r = the number of splitting
+// nodes before it in preorder; r is found with one bottom-up pass (number of
+// splitting nodes per subtree) and one top-down pass.
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "nnk_common.hpp"
+#include "nnk_tree.hpp"
+
+namespace nnk {
+
+namespace {
+
+constexpr int kKeyLevels = 21;   // 63 bits of path key
+
+__host__ __device__ inline int key_shift(int level) { return 3 * (kKeyLevels - level); }
+
+__global__ void k_tree_keys(const double *__restrict__ pts, int64_t np, double m, double M, int nlev,
+                            uint64_t *__restrict__ key, int32_t *__restrict__ row) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  double lo[3] = {m, m, m}, hi[3] = {M, M, M};
+  const double p[3] = {pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]};
+  uint64_t k = 0;
+  for (int l = 1; l <= nlev; l++) {
+    int d = 0;
+    for (int a = 0; a < 3; a++) {
+      const double c = (lo[a] + hi[a]) / 2.0;
+      const bool up = p[a] > c;
+      d = 2 * d + (up ? 1 : 0);
+      if (up) lo[a] = c;
+      else hi[a] = c;
+    }
+    k |= (uint64_t)d << key_shift(l);
+  }
+  key[i] = k;
+  row[i] = (int32_t)i;
+}
+
+__global__ void k_split_flags(int64_t n, const int64_t *__restrict__ beg, const int64_t *__restrict__ end,
+                              int64_t cap, int32_t *__restrict__ split) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) split[i] = (end[i] - beg[i] > cap) ? 1 : 0;
+}
+
+// first index in [lo, hi) with key >= x
+__device__ inline int64_t lower_bound(const uint64_t *__restrict__ key, int64_t lo, int64_t hi, uint64_t x) {
+  while (lo < hi) {
+    const int64_t mid = lo + (hi - lo) / 2;
+    if (key[mid] < x) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// children (level l+1) of the splitting nodes of level l
+__global__ void k_make_children(int64_t n, int l, const int32_t *__restrict__ split,
+                                const int32_t *__restrict__ scan, const int64_t *__restrict__ beg,
+                                const int64_t *__restrict__ end, const uint64_t *__restrict__ pfx,
+                                const double *__restrict__ corners, const uint64_t *__restrict__ key,
+                                int64_t *__restrict__ cbeg, int64_t *__restrict__ cend,
+                                uint64_t *__restrict__ cpfx, double *__restrict__ ccorners) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || !split[i]) return;
+  const int sh = key_shift(l + 1);
+  const double *c = corners + 6 * i;
+  const double co[9] = {c[0], c[1], c[2], (c[0] + c[3]) / 2.0, (c[1] + c[4]) / 2.0, (c[2] + c[5]) / 2.0,
+                        c[3], c[4], c[5]};
+  // lo/hi rows of the 8 children in the 3x3 table [lo; centre; hi]
+  const int rows[8][6] = {{0, 1, 2, 3, 4, 5}, {0, 1, 5, 3, 4, 8}, {0, 4, 2, 3, 7, 5}, {0, 4, 5, 3, 7, 8},
+                          {3, 1, 2, 6, 4, 5}, {3, 1, 5, 6, 4, 8}, {3, 4, 2, 6, 7, 5}, {3, 4, 5, 6, 7, 8}};
+  int64_t b = beg[i];
+  for (int k = 0; k < 8; k++) {
+    const int64_t ch = 8 * (int64_t)scan[i] + k;
+    const uint64_t p = pfx[i] | ((uint64_t)k << sh);
+    const int64_t e = k == 7 ? end[i] : lower_bound(key, b, end[i], pfx[i] | ((uint64_t)(k + 1) << sh));
+    cbeg[ch] = b;
+    cend[ch] = e;
+    cpfx[ch] = p;
+    for (int a = 0; a < 6; a++) ccorners[6 * ch + a] = co[rows[k][a]];
+    b = e;
+  }
+}
+
+// number of splitting nodes in each subtree (children level done first)
+__global__ void k_subtree_splits(int64_t n, const int32_t *__restrict__ split, const int32_t *__restrict__ scan,
+                                 const int64_t *__restrict__ sub_next, int64_t *__restrict__ sub) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int64_t s = 0;
+  if (split[i]) {
+    s = 1;
+    for (int k = 0; k < 8; k++) s += sub_next[8 * (int64_t)scan[i] + k];
+  }
+  sub[i] = s;
+}
+
+// preorder rank among splitting nodes and ids of the children
+__global__ void k_child_ids(int64_t n, const int32_t *__restrict__ split, const int32_t *__restrict__ scan,
+                            const int64_t *__restrict__ rank, const int32_t *__restrict__ id,
+                            const int64_t *__restrict__ sub_next, int64_t *__restrict__ crank,
+                            int32_t *__restrict__ cid, int32_t *__restrict__ cparent) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || !split[i]) return;
+  int64_t acc = rank[i] + 1;
+  for (int k = 0; k < 8; k++) {
+    const int64_t c = 8 * (int64_t)scan[i] + k;
+    crank[c] = acc;
+    cid[c] = (int32_t)(2 + 8 * rank[i] + k);
+    cparent[c] = id[i];
+    acc += sub_next[c];
+  }
+}
+
+// node arrays in id order, and the leaf id of every point
+__global__ void k_emit_nodes(int64_t n, int l, const int32_t *__restrict__ split, const int32_t *__restrict__ scan,
+                             const int32_t *__restrict__ id, const int32_t *__restrict__ parent,
+                             const double *__restrict__ corners, const int64_t *__restrict__ beg,
+                             const int64_t *__restrict__ end, const int32_t *__restrict__ cid,
+                             const int32_t *__restrict__ rows, double *__restrict__ o_corners,
+                             int32_t *__restrict__ o_level, int32_t *__restrict__ o_parent,
+                             int32_t *__restrict__ o_children, int32_t *__restrict__ o_pindex) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int64_t d = id[i] - 1;
+  for (int a = 0; a < 6; a++) o_corners[6 * d + a] = corners[6 * i + a];
+  o_level[d] = l;
+  o_parent[d] = parent[i];
+  for (int k = 0; k < 8; k++) o_children[8 * d + k] = split[i] ? cid[8 * (int64_t)scan[i] + k] : 0;
+  if (!split[i])
+    for (int64_t p = beg[i]; p < end[i]; p++) o_pindex[rows[p]] = id[i];
+}
+
+struct Level {
+  int64_t n = 0, nsplit = 0;
+  DBuf beg, end, pfx, corners, split, scan, sub, rank, id, parent;
+};
+
+inline unsigned blocks(int64_t n) { return (unsigned)((n + 255) / 256); }
+
+}  // namespace
+
+int subdivide_gpu(Tree &t, double m, double M, std::string &err) {
+  const int64_t np = t.npts;
+  // levels a path key can describe; Divide errors before a node is
+  // created at level max_depth - 1 (OcTree.m:166-168)
+  const int nlev = std::max(0, std::min(kKeyLevels, t.max_depth - 1));
+  DBuf dpts, key0, key1, row0, row1, tmp;
+  dpts.alloc(sizeof(double) * 3 * np);
+  NNK_HIP(hipMemcpy(dpts.p, t.points.data(), sizeof(double) * 3 * np, hipMemcpyHostToDevice));
+  key0.alloc(sizeof(uint64_t) * np);
+  key1.alloc(sizeof(uint64_t) * np);
+  row0.alloc(sizeof(int32_t) * np);
+  row1.alloc(sizeof(int32_t) * np);
+  hipLaunchKernelGGL(k_tree_keys, dim3(blocks(np)), dim3(256), 0, 0, dpts.as<double>(), np, m, M, nlev,
+                     key0.as<uint64_t>(), row0.as<int32_t>());
+  NNK_HIP(hipGetLastError());
+  {
+    size_t tb = 0;
+    NNK_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, key0.as<uint64_t>(), key1.as<uint64_t>(),
+                                               row0.as<int32_t>(), row1.as<int32_t>(), (int)np, 0, 63));
+    tmp.alloc(tb);
+    NNK_HIP(hipcub::DeviceRadixSort::SortPairs(tmp.p, tb, key0.as<uint64_t>(), key1.as<uint64_t>(),
+                                               row0.as<int32_t>(), row1.as<int32_t>(), (int)np, 0, 63));
+  }
+  const uint64_t *key = key1.as<uint64_t>();
+  // ---- levels, breadth first ----
+  std::vector<Level> L(1);
+  {
+    Level &r = L[0];
+    r.n = 1;
+    const int64_t b = 0;
+    const uint64_t z = 0;
+    const double rc[6] = {m, m, m, M, M, M};
+    r.beg.upload(std::vector<int64_t>{b});
+    r.end.upload(std::vector<int64_t>{np});
+    r.pfx.upload(std::vector<uint64_t>{z});
+    r.corners.upload(std::vector<double>(rc, rc + 6));
+  }
+  if (t.max_depth <= 1) {
+    err = "Max level reached: Increase Max Depth";
+    return NNK_EMAXDEPTH;
+  }
+  for (int l = 0;; l++) {
+    Level &P = L[l];
+    P.split.alloc(sizeof(int32_t) * P.n);
+    P.scan.alloc(sizeof(int32_t) * P.n);
+    hipLaunchKernelGGL(k_split_flags, dim3(blocks(P.n)), dim3(256), 0, 0, P.n, P.beg.as<int64_t>(),
+                       P.end.as<int64_t>(), (int64_t)t.capacity, P.split.as<int32_t>());
+    NNK_HIP(hipGetLastError());
+    size_t tb = 0;
+    NNK_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, P.split.as<int32_t>(), P.scan.as<int32_t>(), (int)P.n));
+    DBuf stmp;
+    stmp.alloc(tb);
+    NNK_HIP(hipcub::DeviceScan::ExclusiveSum(stmp.p, tb, P.split.as<int32_t>(), P.scan.as<int32_t>(), (int)P.n));
+    int32_t last[2];
+    NNK_HIP(hipMemcpy(&last[0], P.split.as<int32_t>() + P.n - 1, 4, hipMemcpyDeviceToHost));
+    NNK_HIP(hipMemcpy(&last[1], P.scan.as<int32_t>() + P.n - 1, 4, hipMemcpyDeviceToHost));
+    P.nsplit = (int64_t)last[0] + last[1];
+    if (P.nsplit == 0) break;
+    // the children are at level l+1, and Divide is called on each of them
+    if (l + 2 >= t.max_depth) {
+      err = "Max level reached: Increase Max Depth";
+      return NNK_EMAXDEPTH;
+    }
+    if (l + 1 > nlev) {
+      err = "OcTree: more than 21 levels are not supported";
+      return NNK_EMAXDEPTH;
+    }
+    L.emplace_back();
+    Level &C = L[l + 1];
+    Level &Pp = L[l];   // L may have moved
+    C.n = 8 * Pp.nsplit;
+    if (C.n + 1 > INT32_MAX) { err = "OcTree: too many nodes"; return NNK_EARG; }
+    C.beg.alloc(sizeof(int64_t) * C.n);
+    C.end.alloc(sizeof(int64_t) * C.n);
+    C.pfx.alloc(sizeof(uint64_t) * C.n);
+    C.corners.alloc(sizeof(double) * 6 * C.n);
+    hipLaunchKernelGGL(k_make_children, dim3(blocks(Pp.n)), dim3(256), 0, 0, Pp.n, l, Pp.split.as<int32_t>(),
+                       Pp.scan.as<int32_t>(), Pp.beg.as<int64_t>(), Pp.end.as<int64_t>(), Pp.pfx.as<uint64_t>(),
+                       Pp.corners.as<double>(), key, C.beg.as<int64_t>(), C.end.as<int64_t>(),
+                       C.pfx.as<uint64_t>(), C.corners.as<double>());
+    NNK_HIP(hipGetLastError());
+  }
+  const int nl = (int)L.size();
+  // ---- subtree split counts (bottom up), ranks and ids (top down) ----
+  for (int l = nl - 1; l >= 0; l--) {
+    Level &P = L[l];
+    P.sub.alloc(sizeof(int64_t) * P.n);
+    const int64_t *next = l + 1 < nl ? L[l + 1].sub.as<int64_t>() : nullptr;
+    if (!next) {  // deepest level: nothing splits
+      NNK_HIP(hipMemset(P.sub.p, 0, sizeof(int64_t) * P.n));
+    } else {
+      hipLaunchKernelGGL(k_subtree_splits, dim3(blocks(P.n)), dim3(256), 0, 0, P.n, P.split.as<int32_t>(),
+                         P.scan.as<int32_t>(), next, P.sub.as<int64_t>());
+      NNK_HIP(hipGetLastError());
+    }
+  }
+  int64_t total = 1;
+  for (int l = 0; l < nl; l++) total += 8 * L[l].nsplit;
+  {
+    Level &r = L[0];
+    r.rank.upload(std::vector<int64_t>{0});
+    r.id.upload(std::vector<int32_t>{1});
+    r.parent.upload(std::vector<int32_t>{0});
+  }
+  for (int l = 0; l + 1 < nl; l++) {
+    Level &P = L[l], &C = L[l + 1];
+    C.rank.alloc(sizeof(int64_t) * C.n);
+    C.id.alloc(sizeof(int32_t) * C.n);
+    C.parent.alloc(sizeof(int32_t) * C.n);
+    hipLaunchKernelGGL(k_child_ids, dim3(blocks(P.n)), dim3(256), 0, 0, P.n, P.split.as<int32_t>(),
+                       P.scan.as<int32_t>(), P.rank.as<int64_t>(), P.id.as<int32_t>(), C.sub.as<int64_t>(),
+                       C.rank.as<int64_t>(), C.id.as<int32_t>(), C.parent.as<int32_t>());
+    NNK_HIP(hipGetLastError());
+  }
+  // ---- node arrays in id order, leaf id per point ----
+  DBuf oc, olev, opar, ochi, opi;
+  oc.alloc(sizeof(double) * 6 * total);
+  olev.alloc(sizeof(int32_t) * total);
+  opar.alloc(sizeof(int32_t) * total);
+  ochi.alloc(sizeof(int32_t) * 8 * total);
+  opi.alloc(sizeof(int32_t) * np);
+  for (int l = 0; l < nl; l++) {
+    Level &P = L[l];
+    const int32_t *cid = l + 1 < nl ? L[l + 1].id.as<int32_t>() : nullptr;
+    hipLaunchKernelGGL(k_emit_nodes, dim3(blocks(P.n)), dim3(256), 0, 0, P.n, l, P.split.as<int32_t>(),
+                       P.scan.as<int32_t>(), P.id.as<int32_t>(), P.parent.as<int32_t>(), P.corners.as<double>(),
+                       P.beg.as<int64_t>(), P.end.as<int64_t>(), cid, row1.as<int32_t>(), oc.as<double>(),
+                       olev.as<int32_t>(), opar.as<int32_t>(), ochi.as<int32_t>(), opi.as<int32_t>());
+    NNK_HIP(hipGetLastError());
+  }
+  t.count = (int32_t)total;
+  t.corners.resize(6 * total);
+  t.level.resize(total);
+  t.parent.resize(total);
+  t.children.resize(8 * total);
+  t.point_index.resize(np);
+  NNK_HIP(hipMemcpy(t.corners.data(), oc.p, sizeof(double) * 6 * total, hipMemcpyDeviceToHost));
+  NNK_HIP(hipMemcpy(t.level.data(), olev.p, sizeof(int32_t) * total, hipMemcpyDeviceToHost));
+  NNK_HIP(hipMemcpy(t.parent.data(), opar.p, sizeof(int32_t) * total, hipMemcpyDeviceToHost));
+  NNK_HIP(hipMemcpy(t.children.data(), ochi.p, sizeof(int32_t) * 8 * total, hipMemcpyDeviceToHost));
+  NNK_HIP(hipMemcpy(t.point_index.data(), opi.p, sizeof(int32_t) * np, hipMemcpyDeviceToHost));
+  return NNK_OK;
+}
+
+}  // namespace nnk

@@ -1,12 +1,10 @@
-"""libnnkifmm host side against the oracle (CPU; no GPU call is made).
+"""The octree (subdivision on the GPU) and its interaction lists against
+the oracle (GPU).
 
 The tree and its interaction lists must be bit-exact: same ids, corners,
-point-to-leaf map and list contents in the same order.  Also: error paths
-and the exported C ABI.
+point-to-leaf map and list contents in the same order.  Also: error paths.
 """
-import ctypes
 import os
-import re
 
 import numpy as np
 import pytest
@@ -14,6 +12,7 @@ import pytest
 from geometry import blobs, cube_sphere, grid16
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+pytestmark = pytest.mark.gpu
 
 
 def _cases():
@@ -71,20 +70,6 @@ def test_options_name_value_and_keywords(nnk):
     assert a.nodeCount == b.nodeCount and a.arguments["nodeCapacity"] == 20
     np.testing.assert_array_equal(a.getPotentialPoints(), p)
     np.testing.assert_array_equal(a.getTargetPoints(), p)
-
-
-def test_library_exports_header_symbols(nnk):
-    with open(os.path.join(ROOT, "include", "nnkifmm.h")) as fh:
-        hdr = fh.read()
-    names = set(re.findall(r"\b(nnk_[a-z_]+)\s*\(", hdr))
-    assert len(names) >= 20
-    lib = ctypes.CDLL(nnk.library_path())
-    missing = [n for n in sorted(names) if not hasattr(lib, n)]
-    assert not missing, missing
-    from importlib import import_module
-    sig = import_module("stokes-nnkifmm_amd._lib").SIGNATURES
-    assert names == set(sig), names ^ set(sig)
-    assert lib.nnk_version() >= 100
 
 
 def test_root_cube_option(oracle, nnk):
