@@ -82,6 +82,11 @@ int32_t nnk_tree_set_nearest(nnk_tree *t, const double *fine, int64_t nfine,
                              const int64_t *nn_colptr, const int64_t *nn_rowidx,
                              const double *nn_val);
 
+/* No counterpart in the reference (its OcTree is timed with tic/toc): the
+ *           tree's build time in ms by phase: 0 de-duplication of the points,
+ *           1 subdivision (GPU), 2 per-leaf point lists, 3 U/V/W/X lists. */
+int32_t nnk_tree_build_times(const nnk_tree *t, double *ms4);
+
 /* Replaces: properties points/nodeCount/nodeCapacity (Classes/OcTree.m:59-74). */
 int32_t nnk_tree_sizes(const nnk_tree *t, int64_t *npts, int64_t *npot, int64_t *ntgt,
                        int32_t *node_count, int32_t *node_capacity);

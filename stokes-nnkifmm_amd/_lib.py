@@ -27,6 +27,7 @@ SIGNATURES = {
     "nnk_tree_create_in_cube": (ctypes.c_int32, [_d, ctypes.c_int64, _d, ctypes.c_int64, ctypes.c_int32,
                                                  ctypes.c_int32, _d, _h]),
     "nnk_tree_set_nearest": (ctypes.c_int32, [_vp, _d, ctypes.c_int64, _i64, _i64, _d]),
+    "nnk_tree_build_times": (ctypes.c_int32, [_vp, _d]),
     "nnk_tree_sizes": (ctypes.c_int32, [_vp, _i64, _i64, _i64, _i32, _i32]),
     "nnk_tree_export": (ctypes.c_int32, [_vp, _d, _u8, _u8, _i32, _d, _i32, _i32, _i32]),
     "nnk_tree_list_size": (ctypes.c_int32, [_vp, ctypes.c_int32, _i64]),

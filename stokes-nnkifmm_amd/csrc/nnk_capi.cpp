@@ -115,6 +115,12 @@ int32_t nnk_tree_sizes(const nnk_tree *t, int64_t *npts, int64_t *npot, int64_t 
   return NNK_OK;
 }
 
+int32_t nnk_tree_build_times(const nnk_tree *t, double *ms4) {
+  if (!t || !ms4) return fail(NNK_EARG, "nnk_tree_build_times: null argument");
+  for (int k = 0; k < 4; k++) ms4[k] = t->t->build_ms[k];
+  return NNK_OK;
+}
+
 int32_t nnk_tree_export(const nnk_tree *h, double *points, uint8_t *is_pot, uint8_t *is_tgt,
                         int32_t *point_index, double *corners, int32_t *level, int32_t *parent,
                         int32_t *children) {

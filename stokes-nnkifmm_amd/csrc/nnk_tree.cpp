@@ -2,15 +2,16 @@
 //
 // Semantics follow the reference exactly (Classes/OcTree.m):
 //  * points = unique([pot; tgt],'rows','stable'); masks by ismember   (:80-82)
+//    (on the GPU: three stable radix sorts by z, y, x, runs of equal rows)
 //  * root cube [m m m M M M] from the min/max over all coordinates      (:97-98)
 //  * capacity = min(nodeCapacity, nPts-1)                              (:105)
 //  * Divide visits nodes in order, errors when level+1 >= maxDepth,
 //    splits when a node holds more than capacity points, recursing into
 //    the 8 new children before the next sibling (DFS creation order)  (:165-178)
 //  * centre = (lo+hi)/2, child k = 1 + 4[x>c] + 2[y>c] + [z>c]          (:191-221)
-// The subdivision runs on the GPU (nnk_tree_gpu.hip: path keys, one radix
-// sort, level-by-level node creation, preorder ids); the de-duplication
-// and the lists run here.
+// The de-duplication and the subdivision run on the GPU (nnk_tree_gpu.hip:
+// path keys, one radix sort, level-by-level node creation, preorder ids);
+// the lists run here.
 //  * closed-box adjacency                       (Tools/Tree/nodeintersection.m)
 //  * U, V, W lists and the W coarsening, X = find(Wpadded == i)         (:265-366)
 // The lists are found by descending from the root through the nodes whose
@@ -22,47 +23,12 @@
 #include "nnk_tree.hpp"
 
 #include <algorithm>
+#include <chrono>
 #include <cstring>
 
 namespace nnk {
 
 namespace {
-
-struct RowHash {
-  std::vector<int64_t> slot;
-  uint64_t mask = 0;
-  const double *base = nullptr;
-  static uint64_t h(const double *p) {
-    uint64_t x = 0x243f6a8885a308d3ull;
-    for (int d = 0; d < 3; d++) {
-      double v = p[d] == 0.0 ? 0.0 : p[d];  // -0 and 0 compare equal
-      uint64_t b;
-      std::memcpy(&b, &v, 8);
-      x ^= b;
-      x *= 0x9e3779b97f4a7c15ull;
-      x ^= x >> 29;
-    }
-    return x;
-  }
-  void init(int64_t n, const double *b) {
-    uint64_t cap = 16;
-    while (cap < (uint64_t)(2 * n + 16)) cap <<= 1;
-    slot.assign(cap, -1);
-    mask = cap - 1;
-    base = b;
-  }
-  // index of an equal row already stored, or -1 (then inserts idx if asked)
-  int64_t find(const double *p, int64_t idx, bool insert) {
-    uint64_t s = h(p) & mask;
-    while (slot[s] >= 0) {
-      const double *q = base + 3 * slot[s];
-      if (q[0] == p[0] && q[1] == p[1] && q[2] == p[2]) return slot[s];
-      s = (s + 1) & mask;
-    }
-    if (insert) slot[s] = idx;
-    return -1;
-  }
-};
 
 inline bool isect(const double *c1, const double *c2) {
   return !(c2[0] > c1[3] || c2[3] < c1[0] || c2[1] > c1[4] || c2[4] < c1[1] ||
@@ -201,30 +167,21 @@ void build_lists(Tree &t) {
 int build_tree(const double *pot, int64_t npot, const double *tgt, int64_t ntgt, int capacity,
                int max_depth, Tree &t, std::string &err, const double *cube) {
   t = Tree();
+  using clk = std::chrono::steady_clock;
+  auto ms = [](clk::time_point a, clk::time_point b) {
+    return std::chrono::duration<double, std::milli>(b - a).count();
+  };
+  const auto t0 = clk::now();
   if (npot < 0 || ntgt < 0 || npot + ntgt < 2) {
     err = "OcTree needs at least two points";
     return NNK_EARG;
   }
-  int64_t ntot = npot + ntgt;
   for (int64_t i = 0; i < 3 * npot; i++)
     if (pot[i] != pot[i]) { err = "OcTree: NaN in potential points"; return NNK_EARG; }
   for (int64_t i = 0; i < 3 * ntgt; i++)
     if (tgt[i] != tgt[i]) { err = "OcTree: NaN in target points"; return NNK_EARG; }
-  t.points.resize(3 * ntot);
-  RowHash h;
-  h.init(ntot, t.points.data());
-  int64_t np = 0;
-  for (int64_t i = 0; i < ntot; i++) {
-    const double *p = i < npot ? pot + 3 * i : tgt + 3 * (i - npot);
-    std::memcpy(&t.points[3 * np], p, 24);
-    if (h.find(&t.points[3 * np], np, true) < 0) np++;
-  }
-  t.points.resize(3 * np);
-  t.npts = np;
-  t.is_pot.assign(np, 0);
-  t.is_tgt.assign(np, 0);
-  for (int64_t i = 0; i < npot; i++) t.is_pot[h.find(pot + 3 * i, -1, false)] = 1;
-  for (int64_t i = 0; i < ntgt; i++) t.is_tgt[h.find(tgt + 3 * i, -1, false)] = 1;
+  dedup_gpu(pot, npot, tgt, ntgt, t);
+  const int64_t np = t.npts;
   t.pot_ord.assign(np, -1);
   t.tgt_ord.assign(np, -1);
   for (int64_t i = 0; i < np; i++) {
@@ -250,8 +207,10 @@ int build_tree(const double *pot, int64_t npot, const double *tgt, int64_t ntgt,
   }
   t.max_depth = max_depth;
   t.capacity = (int32_t)std::min<int64_t>(capacity, np - 1);
+  const auto t1 = clk::now();
   int rcode = subdivide_gpu(t, m, M, err);
   if (rcode) return rcode;
+  const auto t2 = clk::now();
   t.max_level = 0;
   for (int n = 0; n < t.count; n++) t.max_level = std::max(t.max_level, t.level[n]);
   t.pot_leaf.assign(t.count + 1, 0);
@@ -271,7 +230,13 @@ int build_tree(const double *pot, int64_t npot, const double *tgt, int64_t ntgt,
       t.pt_rows[t.pt_ptr[n] + fill[n]++] = i;
     }
   }
+  const auto t3 = clk::now();
   build_lists(t);
+  const auto t4 = clk::now();
+  t.build_ms[0] = ms(t0, t1);
+  t.build_ms[1] = ms(t1, t2);
+  t.build_ms[2] = ms(t2, t3);
+  t.build_ms[3] = ms(t3, t4);
   return NNK_OK;
 }
 

@@ -33,6 +33,9 @@ struct Tree {
   std::vector<double> fine;
   std::vector<int64_t> nn_colptr, nn_row;
   std::vector<double> nn_val;
+  // build time in ms: 0 de-duplication, 1 subdivision, 2 point lists,
+  // 3 interaction lists
+  double build_ms[4] = {0, 0, 0, 0};
 
   bool is_leaf(int n) const { return children[8 * (n - 1)] == 0; }
   const double *box(int n) const { return &corners[6 * (n - 1)]; }
@@ -47,6 +50,9 @@ int build_tree(const double *pot, int64_t npot, const double *tgt, int64_t ntgt,
 // t.capacity and t.max_depth: fills count, corners, level, parent, children
 // and point_index (nnk_tree_gpu.hip)
 int subdivide_gpu(Tree &t, double m, double M, std::string &err);
+// GPU unique([pot; tgt], 'rows', 'stable') (-0 equals 0) and the ismember
+// masks: fills npts, points, is_pot, is_tgt (nnk_tree_gpu.hip)
+int dedup_gpu(const double *pot, int64_t npot, const double *tgt, int64_t ntgt, Tree &t);
 int set_nearest(Tree &t, const double *fine, int64_t nfine, const int64_t *colptr,
                 const int64_t *row, const double *val, std::string &err);
 

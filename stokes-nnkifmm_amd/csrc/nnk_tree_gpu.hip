@@ -153,6 +153,69 @@ __global__ void k_emit_nodes(int64_t n, int l, const int32_t *__restrict__ split
     for (int64_t p = beg[i]; p < end[i]; p++) o_pindex[rows[p]] = id[i];
 }
 
+// ---- unique([pot; tgt], 'rows', 'stable') and the ismember masks ----
+
+// order-preserving integer image of a double (-0 and 0 map alike)
+__device__ inline uint64_t sortable(double v) {
+  v = v == 0.0 ? 0.0 : v;
+  uint64_t b;
+  __builtin_memcpy(&b, &v, 8);
+  return (b >> 63) ? ~b : (b | (1ull << 63));
+}
+
+__global__ void k_row_key(int64_t n, const double *__restrict__ R, const int32_t *__restrict__ idx, int coord,
+                          uint64_t *__restrict__ key, int32_t *__restrict__ val) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const int32_t i = idx ? idx[j] : (int32_t)j;
+  key[j] = sortable(R[3 * (int64_t)i + coord]);
+  val[j] = i;
+}
+
+// heads of runs of equal rows in sorted order
+__global__ void k_row_heads(int64_t n, const double *__restrict__ R, const int32_t *__restrict__ ord,
+                            int32_t *__restrict__ head) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  int h = 1;
+  if (j > 0) {
+    const double *a = R + 3 * (int64_t)ord[j], *b = R + 3 * (int64_t)ord[j - 1];
+    h = !(a[0] == b[0] && a[1] == b[1] && a[2] == b[2]);
+  }
+  head[j] = h;
+}
+
+// rep[i]: first (smallest) row index equal to row i; gscan = inclusive scan of heads
+__global__ void k_row_rep(int64_t n, const int32_t *__restrict__ ord, const int32_t *__restrict__ head,
+                          const int32_t *__restrict__ gscan, int32_t *__restrict__ leader, int pass,
+                          int32_t *__restrict__ rep) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const int32_t g = gscan[j] - 1;
+  if (pass == 0) {
+    if (head[j]) leader[g] = ord[j];
+  } else {
+    rep[ord[j]] = leader[g];
+  }
+}
+
+__global__ void k_row_keep(int64_t n, const int32_t *__restrict__ rep, int32_t *__restrict__ keep) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) keep[i] = rep[i] == (int32_t)i ? 1 : 0;
+}
+
+__global__ void k_row_emit(int64_t n, int64_t npot, const double *__restrict__ R, const int32_t *__restrict__ rep,
+                           const int32_t *__restrict__ keep, const int32_t *__restrict__ uidx,
+                           double *__restrict__ pts, uint8_t *__restrict__ is_pot, uint8_t *__restrict__ is_tgt) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (keep[i])
+    for (int d = 0; d < 3; d++) pts[3 * (int64_t)uidx[i] + d] = R[3 * i + d];
+  const int32_t u = uidx[rep[i]];
+  if (i < npot) is_pot[u] = 1;
+  else is_tgt[u] = 1;
+}
+
 struct Level {
   int64_t n = 0, nsplit = 0;
   DBuf beg, end, pfx, corners, split, scan, sub, rank, id, parent;
@@ -161,6 +224,77 @@ struct Level {
 inline unsigned blocks(int64_t n) { return (unsigned)((n + 255) / 256); }
 
 }  // namespace
+
+int dedup_gpu(const double *pot, int64_t npot, const double *tgt, int64_t ntgt, Tree &t) {
+  const int64_t n = npot + ntgt;
+  if (n > INT32_MAX) throw Error(NNK_EARG, "OcTree: too many points");
+  DBuf R, ka, kb, va, vb, tmp, head, gscan, leader, rep, keep, uidx, pts, ip, it;
+  R.alloc(sizeof(double) * 3 * n);
+  if (npot) NNK_HIP(hipMemcpy(R.p, pot, sizeof(double) * 3 * npot, hipMemcpyHostToDevice));
+  if (ntgt)
+    NNK_HIP(hipMemcpy(R.as<double>() + 3 * npot, tgt, sizeof(double) * 3 * ntgt, hipMemcpyHostToDevice));
+  ka.alloc(sizeof(uint64_t) * n);
+  kb.alloc(sizeof(uint64_t) * n);
+  va.alloc(sizeof(int32_t) * n);
+  vb.alloc(sizeof(int32_t) * n);
+  size_t tb = 0;
+  NNK_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, ka.as<uint64_t>(), kb.as<uint64_t>(), va.as<int32_t>(),
+                                             vb.as<int32_t>(), (int)n, 0, 64));
+  tmp.alloc(tb);
+  // lexicographic (x, y, z) order, equal rows in index order: stable sorts
+  // by z, then y, then x
+  const int32_t *ord = nullptr;
+  for (int coord = 2; coord >= 0; coord--) {
+    hipLaunchKernelGGL(k_row_key, dim3(blocks(n)), dim3(256), 0, 0, n, R.as<double>(), ord, coord,
+                       ka.as<uint64_t>(), va.as<int32_t>());
+    NNK_HIP(hipGetLastError());
+    NNK_HIP(hipcub::DeviceRadixSort::SortPairs(tmp.p, tb, ka.as<uint64_t>(), kb.as<uint64_t>(), va.as<int32_t>(),
+                                               vb.as<int32_t>(), (int)n, 0, 64));
+    std::swap(va, vb);   // va: current order
+    ord = va.as<int32_t>();
+  }
+  head.alloc(sizeof(int32_t) * n);
+  gscan.alloc(sizeof(int32_t) * n);
+  hipLaunchKernelGGL(k_row_heads, dim3(blocks(n)), dim3(256), 0, 0, n, R.as<double>(), ord, head.as<int32_t>());
+  NNK_HIP(hipGetLastError());
+  size_t sb = 0;
+  NNK_HIP(hipcub::DeviceScan::InclusiveSum(nullptr, sb, head.as<int32_t>(), gscan.as<int32_t>(), (int)n));
+  DBuf stmp;
+  stmp.alloc(sb);
+  NNK_HIP(hipcub::DeviceScan::InclusiveSum(stmp.p, sb, head.as<int32_t>(), gscan.as<int32_t>(), (int)n));
+  leader.alloc(sizeof(int32_t) * n);
+  rep.alloc(sizeof(int32_t) * n);
+  for (int pass = 0; pass < 2; pass++) {
+    hipLaunchKernelGGL(k_row_rep, dim3(blocks(n)), dim3(256), 0, 0, n, ord, head.as<int32_t>(), gscan.as<int32_t>(),
+                       leader.as<int32_t>(), pass, rep.as<int32_t>());
+    NNK_HIP(hipGetLastError());
+  }
+  keep.alloc(sizeof(int32_t) * n);
+  uidx.alloc(sizeof(int32_t) * n);
+  hipLaunchKernelGGL(k_row_keep, dim3(blocks(n)), dim3(256), 0, 0, n, rep.as<int32_t>(), keep.as<int32_t>());
+  NNK_HIP(hipGetLastError());
+  NNK_HIP(hipcub::DeviceScan::ExclusiveSum(stmp.p, sb, keep.as<int32_t>(), uidx.as<int32_t>(), (int)n));
+  int32_t last[2];
+  NNK_HIP(hipMemcpy(&last[0], keep.as<int32_t>() + n - 1, 4, hipMemcpyDeviceToHost));
+  NNK_HIP(hipMemcpy(&last[1], uidx.as<int32_t>() + n - 1, 4, hipMemcpyDeviceToHost));
+  const int64_t np = (int64_t)last[0] + last[1];
+  pts.alloc(sizeof(double) * 3 * np);
+  ip.alloc(np);
+  it.alloc(np);
+  NNK_HIP(hipMemset(ip.p, 0, np));
+  NNK_HIP(hipMemset(it.p, 0, np));
+  hipLaunchKernelGGL(k_row_emit, dim3(blocks(n)), dim3(256), 0, 0, n, npot, R.as<double>(), rep.as<int32_t>(),
+                     keep.as<int32_t>(), uidx.as<int32_t>(), pts.as<double>(), ip.as<uint8_t>(), it.as<uint8_t>());
+  NNK_HIP(hipGetLastError());
+  t.npts = np;
+  t.points.resize(3 * np);
+  t.is_pot.resize(np);
+  t.is_tgt.resize(np);
+  NNK_HIP(hipMemcpy(t.points.data(), pts.p, sizeof(double) * 3 * np, hipMemcpyDeviceToHost));
+  NNK_HIP(hipMemcpy(t.is_pot.data(), ip.p, np, hipMemcpyDeviceToHost));
+  NNK_HIP(hipMemcpy(t.is_tgt.data(), it.p, np, hipMemcpyDeviceToHost));
+  return NNK_OK;
+}
 
 int subdivide_gpu(Tree &t, double m, double M, std::string &err) {
   const int64_t np = t.npts;

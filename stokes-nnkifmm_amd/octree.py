@@ -75,6 +75,8 @@ class OcTree:
             check(lib().nnk_tree_create_in_cube(ptr(pot), len(pot), ptr(tgt), len(tgt), int(a["nodeCapacity"]),
                                                 int(a["maxDepth"]), ptr(cube), ctypes.byref(h)))
         self._h = h
+        self.buildTimes = np.zeros(4)   # ms: de-duplication, subdivision, point lists, U/V/W/X lists
+        check(lib().nnk_tree_build_times(h, ptr(self.buildTimes)))
         npts, npot, ntgt = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
         cnt, cap = ctypes.c_int32(), ctypes.c_int32()
         check(lib().nnk_tree_sizes(h, ctypes.byref(npts), ctypes.byref(npot), ctypes.byref(ntgt),
