@@ -53,6 +53,8 @@ int subdivide_gpu(Tree &t, double m, double M, std::string &err);
 // GPU unique([pot; tgt], 'rows', 'stable') (-0 equals 0) and the ismember
 // masks: fills npts, points, is_pot, is_tgt (nnk_tree_gpu.hip)
 int dedup_gpu(const double *pot, int64_t npot, const double *tgt, int64_t ntgt, Tree &t);
+// GPU U/V/W/X lists of a subdivided tree: fills lptr, lidx (nnk_tree_gpu.hip)
+void build_lists_gpu(Tree &t);
 int set_nearest(Tree &t, const double *fine, int64_t nfine, const int64_t *colptr,
                 const int64_t *row, const double *val, std::string &err);
 

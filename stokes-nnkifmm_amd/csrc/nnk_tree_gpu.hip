@@ -19,6 +19,10 @@
 // children therefore get ids 2 + 8 r .. 9 + 8 r, r = the number of splitting
 // nodes before it in preorder; r is found with one bottom-up pass (number of
 // splitting nodes per subtree) and one top-down pass.
+//
+// The interaction lists are found per node by walks over the tree (no
+// stack: siblings have consecutive ids), counted, scanned, filled and
+// sorted with segmented sorts; X is the inverse of W by one radix sort.
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 
@@ -216,6 +220,154 @@ __global__ void k_row_emit(int64_t n, int64_t npot, const double *__restrict__ R
   else is_tgt[u] = 1;
 }
 
+// ---- interaction lists (OcTree.m:265-366), one thread per node ----
+
+struct TreeView {
+  const double *corners;      // 6 per node [id-1]
+  const int32_t *level, *parent, *children;
+  const uint8_t *pot_leaf, *tgt_leaf, *complete;   // [id]
+};
+
+__device__ inline bool isect(const double *c1, const double *c2) {
+  return !(c2[0] > c1[3] || c2[3] < c1[0] || c2[1] > c1[4] || c2[4] < c1[1] || c2[2] > c1[5] || c2[5] < c1[2]);
+}
+__device__ inline bool is_leaf(const TreeView &T, int n) { return T.children[8 * (n - 1)] == 0; }
+__device__ inline const double *box(const TreeView &T, int n) { return T.corners + 6 * (int64_t)(n - 1); }
+
+// Preorder walk of the nodes strictly below `top`, children in order, with
+// no stack: siblings have consecutive ids, so the walk moves to the next
+// sibling or climbs through the parent.  enter(n) returns whether to visit
+// n's children.
+template <class F>
+__device__ void walk_below(const TreeView &T, int top, F &&enter) {
+  if (is_leaf(T, top)) return;
+  int n = T.children[8 * (top - 1)];
+  for (;;) {
+    if (enter(n) && !is_leaf(T, n)) {
+      n = T.children[8 * (n - 1)];
+      continue;
+    }
+    for (;;) {
+      const int p = T.parent[n - 1];
+      if (n - T.children[8 * (p - 1)] < 7) {
+        n++;
+        break;
+      }
+      n = p;
+      if (n == top) return;
+    }
+  }
+}
+
+// every node (root included) whose closed box meets q, descending below
+// nodes of level < maxlev (the host reference walk of the lists)
+template <class F>
+__device__ void descend(const TreeView &T, const double *q, int maxlev, F &&visit) {
+  if (!isect(q, box(T, 1))) return;
+  visit(1);
+  if (T.level[0] >= maxlev) return;
+  walk_below(T, 1, [&](int n) {
+    if (!isect(q, box(T, n))) return false;
+    visit(n);
+    return T.level[n - 1] < maxlev;
+  });
+}
+
+// U, V, W of node i (FILL = false: sizes only).  U: potential leaves
+// adjacent to i (:323-327); V: children, not adjacent to i, of the non-leaf
+// same-level neighbours of the parent (:330-339); W: below the non-leaf
+// same-level neighbours of i, the nodes not adjacent to i whose parent is
+// adjacent and whose subtree is complete, else their children (:343-360);
+// `last` is the merged non-leaf of smallest level, then largest id.
+template <bool FILL>
+__global__ void k_lists(TreeView T, int nc, int32_t *__restrict__ cnt, const int64_t *__restrict__ ptrU,
+                        const int64_t *__restrict__ ptrV, const int64_t *__restrict__ ptrW,
+                        int32_t *__restrict__ idxU, int32_t *__restrict__ idxV, int32_t *__restrict__ idxW,
+                        int32_t *__restrict__ lastW) {
+  const int i = 2 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (i > nc) return;
+  const double *bi = box(T, i);
+  const int li = T.level[i - 1];
+  const bool tl = T.tgt_leaf[i];
+  if (tl) {
+    int64_t o = FILL ? ptrU[i] : 0;
+    int c = 0;
+    descend(T, bi, 1 << 30, [&](int n) {
+      if (is_leaf(T, n) && T.pot_leaf[n]) {
+        if (FILL) idxU[o + c] = n;
+        c++;
+      }
+    });
+    if (!FILL) cnt[0 * (nc + 2) + i + 1] = c;
+  }
+  {
+    const int P = T.parent[i - 1];
+    const int lp = T.level[P - 1];
+    int64_t o = FILL ? ptrV[i] : 0;
+    int c = 0;
+    descend(T, box(T, P), lp, [&](int n) {
+      if (n == P || T.level[n - 1] != lp || is_leaf(T, n)) return;
+      for (int k = 0; k < 8; k++) {
+        const int ch = T.children[8 * (n - 1) + k];
+        if (!isect(bi, box(T, ch))) {
+          if (FILL) idxV[o + c] = ch;
+          c++;
+        }
+      }
+    });
+    if (!FILL) cnt[1 * (nc + 2) + i + 1] = c;
+  }
+  if (tl) {
+    int64_t o = FILL ? ptrW[i] : 0;
+    int c = 0, last = 0;
+    descend(T, bi, li, [&](int st) {
+      if (T.level[st - 1] != li || is_leaf(T, st)) return;
+      walk_below(T, st, [&](int y) {
+        const bool leaf = is_leaf(T, y);
+        if (isect(bi, box(T, y))) return !leaf;
+        if (T.complete[y]) {
+          if (FILL) idxW[o + c] = y;
+          c++;
+          if (!leaf && (!last || T.level[y - 1] < T.level[last - 1] ||
+                        (T.level[y - 1] == T.level[last - 1] && y > last)))
+            last = y;
+          return false;
+        }
+        return !leaf;
+      });
+    });
+    if (!FILL) cnt[2 * (nc + 2) + i + 1] = c;
+    else lastW[i] = last;
+  }
+}
+
+// W(i) sorted, then its last merged node moved to the end
+__global__ void k_w_last(int nc, const int64_t *__restrict__ ptrW, int32_t *__restrict__ idxW,
+                         const int32_t *__restrict__ lastW) {
+  const int i = 2 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (i > nc || !lastW[i]) return;
+  const int64_t b = ptrW[i], e = ptrW[i + 1];
+  int64_t p = b;
+  while (p < e && idxW[p] != lastW[i]) p++;
+  if (p == e) return;
+  for (; p + 1 < e; p++) idxW[p] = idxW[p + 1];
+  idxW[e - 1] = lastW[i];
+}
+
+// X(y) = { j : y in W(j) }, ordered by the position k of y in W(j), then j
+// (find on the padded W matrix, :279-288): keys (y, k), values j, in j order
+__global__ void k_x_pairs(int nc, const int64_t *__restrict__ ptrW, const int32_t *__restrict__ idxW,
+                          uint64_t *__restrict__ key, int32_t *__restrict__ val, int32_t *__restrict__ cntX) {
+  const int j = 2 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (j > nc) return;
+  for (int64_t e = ptrW[j]; e < ptrW[j + 1]; e++) {
+    const int32_t y = idxW[e];
+    key[e] = ((uint64_t)(uint32_t)y << 32) | (uint64_t)(e - ptrW[j]);
+    val[e] = j;
+    atomicAdd(&cntX[y + 1], 1);
+  }
+}
+
 struct Level {
   int64_t n = 0, nsplit = 0;
   DBuf beg, end, pfx, corners, split, scan, sub, rank, id, parent;
@@ -224,6 +376,115 @@ struct Level {
 inline unsigned blocks(int64_t n) { return (unsigned)((n + 255) / 256); }
 
 }  // namespace
+
+void build_lists_gpu(Tree &t) {
+  const int nc = t.count;
+  for (int q = 0; q < 4; q++) {
+    t.lptr[q].assign(nc + 2, 0);
+    t.lidx[q].clear();
+  }
+  if (nc < 2) return;
+  // complete(x): a potential leaf, or a non-leaf whose 8 children are
+  // complete -- exactly the nodes the W coarsening ends up merging
+  std::vector<uint8_t> complete(nc + 1, 0);
+  {
+    std::vector<std::vector<int>> bylev(t.max_level + 1);
+    for (int n = 1; n <= nc; n++) bylev[t.level[n - 1]].push_back(n);
+    for (int l = t.max_level; l >= 0; l--)
+      for (int n : bylev[l]) {
+        if (t.is_leaf(n)) complete[n] = t.pot_leaf[n];
+        else {
+          bool all = true;
+          for (int k = 0; k < 8; k++) all = all && complete[t.children[8 * (n - 1) + k]];
+          complete[n] = all;
+        }
+      }
+  }
+  DBuf dc, dl, dp, dch, dpl, dtl, dcomp, cnt, ptr, tmp;
+  dc.upload(t.corners);
+  dl.upload(t.level);
+  dp.upload(t.parent);
+  dch.upload(t.children);
+  dpl.upload(t.pot_leaf);
+  dtl.upload(t.tgt_leaf);
+  dcomp.upload(complete);
+  TreeView T{dc.as<double>(), dl.as<int32_t>(), dp.as<int32_t>(), dch.as<int32_t>(),
+              dpl.as<uint8_t>(), dtl.as<uint8_t>(), dcomp.as<uint8_t>()};
+  const int64_t stride = nc + 2;
+  cnt.alloc(sizeof(int32_t) * 3 * stride);
+  NNK_HIP(hipMemset(cnt.p, 0, sizeof(int32_t) * 3 * stride));
+  const unsigned nb = (unsigned)((nc - 1 + 127) / 128);
+  hipLaunchKernelGGL(k_lists<false>, dim3(nb), dim3(128), 0, 0, T, nc, cnt.as<int32_t>(), nullptr, nullptr,
+                     nullptr, nullptr, nullptr, nullptr, nullptr);
+  NNK_HIP(hipGetLastError());
+  // ptr[q][n] = sum of sizes of nodes < n (CSR over ids, count+2 entries)
+  ptr.alloc(sizeof(int64_t) * 4 * stride);
+  size_t tb = 0;
+  NNK_HIP(hipcub::DeviceScan::InclusiveSum(nullptr, tb, cnt.as<int32_t>(), ptr.as<int64_t>(), (int)stride));
+  tmp.alloc(tb);
+  int64_t tot[4];
+  for (int q = 0; q < 3; q++) {
+    NNK_HIP(hipcub::DeviceScan::InclusiveSum(tmp.p, tb, cnt.as<int32_t>() + q * stride,
+                                             ptr.as<int64_t>() + q * stride, (int)stride));
+    NNK_HIP(hipMemcpy(&tot[q], ptr.as<int64_t>() + q * stride + nc + 1, 8, hipMemcpyDeviceToHost));
+  }
+  DBuf idx[4], lastW, sorted;
+  for (int q = 0; q < 3; q++) idx[q].alloc(sizeof(int32_t) * std::max<int64_t>(tot[q], 1));
+  lastW.alloc(sizeof(int32_t) * (nc + 1));
+  NNK_HIP(hipMemset(lastW.p, 0, sizeof(int32_t) * (nc + 1)));
+  const int64_t *pU = ptr.as<int64_t>(), *pV = pU + stride, *pW = pV + stride;
+  hipLaunchKernelGGL(k_lists<true>, dim3(nb), dim3(128), 0, 0, T, nc, nullptr, pU, pV, pW, idx[0].as<int32_t>(),
+                     idx[1].as<int32_t>(), idx[2].as<int32_t>(), lastW.as<int32_t>());
+  NNK_HIP(hipGetLastError());
+  // sort every list (segments = nodes)
+  for (int q = 0; q < 3; q++) {
+    if (!tot[q]) continue;
+    const int64_t *pq = pU + q * stride;
+    sorted.alloc(sizeof(int32_t) * tot[q]);
+    size_t sb = 0;
+    NNK_HIP(hipcub::DeviceSegmentedSort::SortKeys(nullptr, sb, idx[q].as<int32_t>(), sorted.as<int32_t>(),
+                                                  (int)tot[q], (int)(nc + 1), pq, pq + 1));
+    DBuf st;
+    st.alloc(sb);
+    NNK_HIP(hipcub::DeviceSegmentedSort::SortKeys(st.p, sb, idx[q].as<int32_t>(), sorted.as<int32_t>(),
+                                                  (int)tot[q], (int)(nc + 1), pq, pq + 1));
+    std::swap(idx[q], sorted);
+  }
+  if (tot[2]) {
+    hipLaunchKernelGGL(k_w_last, dim3(nb), dim3(128), 0, 0, nc, pW, idx[2].as<int32_t>(), lastW.as<int32_t>());
+    NNK_HIP(hipGetLastError());
+  }
+  // X from W
+  DBuf cntX, ptrX;
+  cntX.alloc(sizeof(int32_t) * stride);
+  NNK_HIP(hipMemset(cntX.p, 0, sizeof(int32_t) * stride));
+  ptrX.alloc(sizeof(int64_t) * stride);
+  tot[3] = tot[2];
+  idx[3].alloc(sizeof(int32_t) * std::max<int64_t>(tot[3], 1));
+  if (tot[2]) {
+    DBuf k0, k1, v0;
+    k0.alloc(sizeof(uint64_t) * tot[2]);
+    k1.alloc(sizeof(uint64_t) * tot[2]);
+    v0.alloc(sizeof(int32_t) * tot[2]);
+    hipLaunchKernelGGL(k_x_pairs, dim3(nb), dim3(128), 0, 0, nc, pW, idx[2].as<int32_t>(), k0.as<uint64_t>(),
+                       v0.as<int32_t>(), cntX.as<int32_t>());
+    NNK_HIP(hipGetLastError());
+    size_t sb = 0;
+    NNK_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, sb, k0.as<uint64_t>(), k1.as<uint64_t>(), v0.as<int32_t>(),
+                                               idx[3].as<int32_t>(), (int)tot[2], 0, 64));
+    DBuf st;
+    st.alloc(sb);
+    NNK_HIP(hipcub::DeviceRadixSort::SortPairs(st.p, sb, k0.as<uint64_t>(), k1.as<uint64_t>(), v0.as<int32_t>(),
+                                               idx[3].as<int32_t>(), (int)tot[2], 0, 64));
+  }
+  NNK_HIP(hipcub::DeviceScan::InclusiveSum(tmp.p, tb, cntX.as<int32_t>(), ptrX.as<int64_t>(), (int)stride));
+  for (int q = 0; q < 4; q++) {
+    const int64_t *pq = q < 3 ? pU + q * stride : ptrX.as<int64_t>();
+    NNK_HIP(hipMemcpy(t.lptr[q].data(), pq, sizeof(int64_t) * stride, hipMemcpyDeviceToHost));
+    t.lidx[q].resize(tot[q]);
+    if (tot[q]) NNK_HIP(hipMemcpy(t.lidx[q].data(), idx[q].p, sizeof(int32_t) * tot[q], hipMemcpyDeviceToHost));
+  }
+}
 
 int dedup_gpu(const double *pot, int64_t npot, const double *tgt, int64_t ntgt, Tree &t) {
   const int64_t n = npot + ntgt;
