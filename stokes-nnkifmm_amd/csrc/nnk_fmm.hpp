@@ -60,6 +60,10 @@ struct LevelPlan {
   DBuf mx_slot;
   std::vector<DBuf> mx_owned;
   int mx_m2l_jobs = 0, mx_m2l_maxN = 0;
+  // FP16 form: the same split-FP16 GEMM for the down expansion of the
+  // target leaves, over this level's down coefficients (dn_rows)
+  DBuf hx_dn, dn_rows;
+  int hx_dn_n = 0, hx_dn_maxN = 0, n_dn_rows = 0;
 };
 
 // host copies of a level's node lists (slots = id-1), shared by both forms
@@ -77,6 +81,11 @@ struct MixedOps {
   DBuf Mc, Lc;                // 8 each, FP64 At (R x R); absent at level 2
   DBuf Cf;                    // 316 M2L operators, NP 16-bit parts in MFMA fragment order
   DBuf Uf;                    // FP16 form: per operator row, the unscale factor
+  // FP16 form: the down expansion U_R (K x R) as FP16 parts in fragment
+  // order, with its row unscale factors.  (The up expansion S^-1 U^T, M2M
+  // and L2L stay FP64: their S^-1 amplifies the rounding of 22-bit inputs
+  // beyond the 1e-6 target.)
+  DBuf UrF, UrU;
   std::vector<const uint16_t *> cop;  // per offset slot (343)
   std::vector<const float *> uop;
   size_t bytes() const;
@@ -231,6 +240,14 @@ class Fmm {
   int np_ = 2, KCm_ = 0, RT_ = 0;   // mixed M2L: 16-bit parts, 32-deep k chunks, 16-row tiles
   bool f16_ = false;                // FP16 parts (with scaling) instead of BF16
   DBuf amax_, xun_;                 // FP16 form: max |coefficient|, coefficient unscale factor
+  // FP16 form: split down coefficients ([node][KC][2][32], zero row at nc)
+  // and their unscale factor per level
+  DBuf dhat_x_, dun_lv_;
+  int RTk_ = 0;                     // K rows in 16-row tiles (whole 128-row blocks)
+  void launch_h16(const DBuf &jobs, int njobs, int maxN, int M, int KC, const uint16_t *X, double *C, int ldc,
+                  const double *xun, int beta, hipStream_t s);
+  void split_h(const DBuf &rows, int nrows, const double *X, int ldx, int len, int KC, uint16_t *out,
+               double *xun, hipStream_t s);
   DBuf part_;                       // split-K partial tiles
   size_t part_elems_ = 0;
   int n_upall_ = 0;

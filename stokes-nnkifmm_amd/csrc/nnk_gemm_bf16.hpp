@@ -21,7 +21,8 @@
 //   A_b:  [KC][NP][RT][64 lanes][8]   lane q of row tile t holds
 //                                     A[16t + q%16][32kc + 8(q/16) + e]
 //   X:    [node][KC][NP][32]          (the 64-byte k-slice of one node)
-// A gather slot of -1 reads node `zrow`, an all-zero row.
+// A gather slot of -1 reads node `zrow`, an all-zero row.  With beta = 1
+// (and no split) the tile is added to D instead of stored.
 //
 // Work decomposition: one 512-thread workgroup owns a 128 x 128 tile of D
 // (8 waves as 2 x 4, 64 x 32 each).  The flattened grid is walked
@@ -66,7 +67,8 @@ __global__ __launch_bounds__(512, 1) void k_m2l_split(const int M, const int KC,
                                                       double *__restrict__ C, const int ldc,
                                                       const GemmJobBf16 *__restrict__ jobs, const int njobs,
                                                       const int nrb, const int nsplit, double *__restrict__ P,
-                                                      const int maxN, const double *__restrict__ xun) {
+                                                      const int maxN, const double *__restrict__ xun,
+                                                      const int beta) {
   static_assert(!F16 || NP == 2, "FP16 parts: two");
   constexpr int BN = 128, TM = 4, TN = 2, TILE = 512;
   typedef typename std::conditional<F16, f16x8, bf16x8>::type op8;
@@ -257,7 +259,7 @@ __global__ __launch_bounds__(512, 1) void k_m2l_split(const int M, const int KC,
 #pragma unroll
       for (int r = 0; r < 4; r++) {
         const int row = m0 + i * 16 + 4 * lg + r;
-        if (row < M) out[row] = acc64[i][j][r] * xs;
+        if (row < M) out[row] = (beta && nsplit == 1 ? out[row] : 0.0) + acc64[i][j][r] * xs;
       }
     }
 }

@@ -11,8 +11,10 @@
 //   M2L_o <- Pdn(l)   K(up at offset o -> up)    V_R(l)
 // formed once on the device with the FP64 GEMM.  M2L (the dominant work)
 // runs as split-BF16 MFMA GEMMs (two or three BF16 parts per operand) with
-// FP32 accumulation promoted to FP64 per translation; M2M, L2L and the
-// expansions to densities stay FP64; the
+// FP32 accumulation promoted to FP64 per translation; M2M, L2L and the up
+// expansion stay FP64 (their S^-1 factor amplifies input rounding), the
+// down expansion U_R d (orthonormal U_R) runs split-FP16 in the FP16 form;
+// the
 // direct sums (S2M, P2L, L2T, P2P, M2T) evaluate the kernel in FP32 from
 // FP64 separations and accumulate in FP64.
 #include <hip/hip_runtime.h>
@@ -143,6 +145,37 @@ __global__ void k_split_op_h(const double *__restrict__ CT, int ldct, int R, int
   out[idx] = split_part_h(v, p);
 }
 
+// the same for rectangular operators: A_j(m, n) = A[j * ostride + n * ld + m]
+// (m < Mr rows, n < Kc columns)
+__global__ void k_op_rowscale_g(const double *__restrict__ A, int64_t ostride, int ld, int Mr, int Kc, int nops,
+                                int RPm, float *__restrict__ un) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (int64_t)nops * RPm) return;
+  const int j = (int)(idx / RPm), m = (int)(idx % RPm);
+  double a = 0.0;
+  if (m < Mr)
+    for (int n = 0; n < Kc; n++) a = fmax(a, fabs(A[j * ostride + (int64_t)n * ld + m]));
+  un[idx] = a > 0 ? (float)(pow2_floor(a) / 16384.0) : 0.0f;
+}
+
+__global__ void k_split_op_hg(const double *__restrict__ A, int64_t ostride, int ld, int Mr, int Kc, int nops,
+                              int KC, int RT, const float *__restrict__ un, uint16_t *__restrict__ out) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t per = (int64_t)KC * 2 * RT * 512;
+  if (idx >= per * nops) return;
+  const int j = (int)(idx / per);
+  int64_t rem = idx % per;
+  const int e = (int)(rem & 7), q = (int)((rem >> 3) & 63);
+  rem >>= 9;
+  const int t = (int)(rem % RT);
+  rem /= RT;
+  const int p = (int)(rem % 2), kc = (int)(rem / 2);
+  const int m = 16 * t + (q & 15), k = 32 * kc + 8 * (q >> 4) + e;
+  const float u = un[(size_t)j * RT * 16 + m];
+  const double v = (m < Mr && k < Kc && u > 0) ? A[j * ostride + (int64_t)k * ld + m] / (double)u : 0.0;
+  out[idx] = split_part_h(v, p);
+}
+
 // rows[i] of X (length R, stride ldx) -> parts in [node][KC][np][32] order
 // (zero beyond R)
 __global__ void k_split_rows(const int32_t *__restrict__ rows, int nrows, const double *__restrict__ X,
@@ -197,6 +230,7 @@ void Fmm::build_mixed() {
   RT_ = RPm_ / 16;
   MpadR_ = (R_ + 159) / 160 * 160;
   KpadR_ = (R_ + 15) / 16 * 16;
+  RTk_ = (K_ + MBM - 1) / MBM * MBM / 16;
   const int R = R_;
   const size_t matKR = (size_t)Kpad_ * MpadR_;   // R x K operators (Pup, Pdn)
   const size_t matRK = (size_t)KpadR_ * Mpad_;   // K x R operators (Vr, Ur)
@@ -331,6 +365,21 @@ void Fmm::build_mixed() {
       gemm_dims(b4, Y.as<double>(), K, X.Lc.as<double>(), MpadR_, 0, 0, R, K, MpadR_, Kpad_);
       NNK_HIP(hipDeviceSynchronize());
     }
+    if (f16_) {  // the down expansion U_R as FP16 parts
+      auto frag = [&](const DBuf &A, int64_t ostride, int ld, int Mr, int Kc, int nops, int KC, int RT, DBuf &F,
+                      DBuf &U) {
+        const int64_t nu = (int64_t)nops * RT * 16, per = (int64_t)KC * 2 * RT * 512;
+        U.alloc(sizeof(float) * nu);
+        F.alloc(sizeof(uint16_t) * per * nops);
+        hipLaunchKernelGGL(k_op_rowscale_g, dim3((unsigned)((nu + 255) / 256)), dim3(256), 0, 0, A.as<double>(),
+                           ostride, ld, Mr, Kc, nops, RT * 16, U.as<float>());
+        hipLaunchKernelGGL(k_split_op_hg, dim3((unsigned)((per * nops + 255) / 256)), dim3(256), 0, 0,
+                           A.as<double>(), ostride, ld, Mr, Kc, nops, KC, RT, U.as<float>(), F.as<uint16_t>());
+        NNK_HIP(hipGetLastError());
+      };
+      frag(X.Ur, 0, Mpad_, K, R, 1, KCm_, RTk_, X.UrF, X.UrU);
+      NNK_HIP(hipDeviceSynchronize());
+    }
     cache.put(key(2 + prec_, l), fresh[l], X.bytes());
   }
   const int nc = tree_.count;
@@ -343,6 +392,13 @@ void Fmm::build_mixed() {
   chat_x_.zero();
   amax_.alloc(sizeof(unsigned long long));
   xun_.alloc(sizeof(double));
+  if (f16_) {
+    dhat_x_.alloc(sizeof(uint16_t) * (size_t)(nc + 1) * KCm_ * 2 * 32);
+    dhat_x_.zero();
+    dun_lv_.alloc(sizeof(double) * (maxlev_ + 2));
+    std::vector<double> ones(maxlev_ + 2, 1.0);
+    NNK_HIP(hipMemcpy(dun_lv_.p, ones.data(), sizeof(double) * ones.size(), hipMemcpyHostToDevice));
+  }
   {
     const double one = 1.0;
     NNK_HIP(hipMemcpy(xun_.p, &one, sizeof(double), hipMemcpyHostToDevice));
@@ -488,6 +544,44 @@ void Fmm::plan_mixed() {
     }
     P.mx_l2l.njobs = (int)lj.size();
     if (!lj.empty()) P.mx_l2l.jobs.upload(lj);
+    if (f16_) {  // split-FP16 down expansion
+      auto hjob = [&](std::vector<GemmJobBf16> &jobs, int &maxN, const std::vector<const uint16_t *> &ops,
+                      const std::vector<const float *> &uops, const std::vector<int32_t> &g,
+                      const std::vector<int32_t> &c) {
+        GemmJobBf16 J{};
+        P.mx_owned.emplace_back();
+        P.mx_owned.back().upload(ops);
+        J.A = P.mx_owned.back().as<const uint16_t *>();
+        P.mx_owned.emplace_back();
+        P.mx_owned.back().upload(uops);
+        J.U = P.mx_owned.back().as<const float *>();
+        P.mx_owned.emplace_back();
+        P.mx_owned.back().upload(g);
+        J.gidx = P.mx_owned.back().as<int32_t>();
+        P.mx_owned.emplace_back();
+        P.mx_owned.back().upload(c);
+        J.cidx = P.mx_owned.back().as<int32_t>();
+        J.nb = (int)ops.size();
+        J.N = (int)c.size();
+        std::vector<int32_t> tp, tl;
+        tile_lists(g, J.nb, J.N, MBN, tp, tl);
+        P.mx_owned.emplace_back();
+        P.mx_owned.back().upload(tp);
+        J.tptr = P.mx_owned.back().as<int32_t>();
+        P.mx_owned.emplace_back();
+        P.mx_owned.back().upload(tl);
+        J.tlist = P.mx_owned.back().as<int32_t>();
+        jobs.push_back(J);
+        maxN = std::max(maxN, J.N);
+      };
+      std::vector<GemmJobBf16> jd;
+      if (!H.tleaves.empty())
+        hjob(jd, P.hx_dn_maxN, {X.UrF.as<uint16_t>()}, {X.UrU.as<float>()}, H.tleaves, H.tleaves);
+      P.hx_dn_n = (int)jd.size();
+      if (!jd.empty()) P.hx_dn.upload(jd);
+      P.n_dn_rows = (int)H.dn.size();
+      P.dn_rows.upload(H.dn);
+    }
   }
   n_upall_ = (int)upall.size();
   upall_.upload(upall);
@@ -507,7 +601,7 @@ void Fmm::launch_m2l_mixed(int l, int depth, int split, hipStream_t s) {
   double *pp = split > 1 ? part_.as<double>() : nullptr;
   auto launch = [&](auto kern) {
     hipLaunchKernelGGL(kern, grid, dim3(512), 0, s, R_, KCm_, RT_, chat_x_.as<uint16_t>(), zrow, d, RP_,
-                       P.mx_m2l.as<GemmJobBf16>(), nj, nrb, split, pp, mN, xun_.as<double>());
+                       P.mx_m2l.as<GemmJobBf16>(), nj, nrb, split, pp, mN, xun_.as<double>(), 0);
   };
   if (f16_) {
     if (depth == 2) launch(k_m2l_split<2, 2, 1, true>);
@@ -520,6 +614,32 @@ void Fmm::launch_m2l_mixed(int l, int depth, int split, hipStream_t s) {
   }
   NNK_HIP(hipGetLastError());
   if (split > 1) split_reduce(P.mx_slot, nj * mN, R_, RP_, split, d, s);
+}
+
+// one launch of the split-FP16 GEMM (no split-K, register-ring depth 2):
+// C[:, cidx] (+)= sum_b A_b X[:, gidx_b] * (*xun)
+void Fmm::launch_h16(const DBuf &jobs, int njobs, int maxN, int M, int KC, const uint16_t *X, double *C, int ldc,
+                     const double *xun, int beta, hipStream_t s) {
+  if (!njobs) return;
+  const int nrb = (M + MBM - 1) / MBM, nct = (maxN + MBN - 1) / MBN;
+  const int RT = nrb * MBM / 16;
+  hipLaunchKernelGGL((k_m2l_split<2, 2, 1, true>), dim3((unsigned)(njobs * nrb * nct)), dim3(512), 0, s, M, KC, RT,
+                     X, tree_.count, C, ldc, jobs.as<GemmJobBf16>(), njobs, nrb, 1, nullptr, maxN, xun, beta);
+  NNK_HIP(hipGetLastError());
+}
+
+// rows of X (length len, stride ldx) -> FP16 parts scaled by one power of
+// two from their max |value|; *xun receives the unscale factor
+void Fmm::split_h(const DBuf &rows, int nrows, const double *X, int ldx, int len, int KC, uint16_t *out,
+                  double *xun, hipStream_t s) {
+  if (!nrows) return;
+  const int64_t tot = (int64_t)nrows * KC * 32;
+  hipLaunchKernelGGL(k_amax_reset, dim3(1), dim3(64), 0, s, amax_.as<unsigned long long>());
+  hipLaunchKernelGGL(k_absmax, dim3(std::min<int64_t>(1024, (tot + 255) / 256)), dim3(256), 0, s, rows.as<int32_t>(),
+                     nrows, X, ldx, len, amax_.as<unsigned long long>());
+  hipLaunchKernelGGL(k_split_rows_h, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, rows.as<int32_t>(), nrows,
+                     X, ldx, len, KC, amax_.as<unsigned long long>(), out, xun);
+  NNK_HIP(hipGetLastError());
 }
 
 void Fmm::apply_mixed(const double *d_f, double *d_u, hipStream_t s) {
@@ -574,7 +694,13 @@ void Fmm::apply_mixed(const double *d_f, double *d_u, hipStream_t s) {
       mark(5, false, s);
     }
     mark(6, true, s);
-    gemm_dims(P.mx_downU, d, RP_, downpot_.as<double>(), K, 0, s, K, R, Mpad_, KpadR_);
+    if (f16_) {  // U_R is orthonormal: the down expansion keeps its accuracy in split FP16
+      split_h(P.dn_rows, P.n_dn_rows, d, RP_, R, KCm_, dhat_x_.as<uint16_t>(), dun_lv_.as<double>() + l, s);
+      launch_h16(P.hx_dn, P.hx_dn_n, P.hx_dn_maxN, K, KCm_, dhat_x_.as<uint16_t>(), downpot_.as<double>(), K,
+                 dun_lv_.as<double>() + l, 0, s);
+    } else {
+      gemm_dims(P.mx_downU, d, RP_, downpot_.as<double>(), K, 0, s, K, R, Mpad_, KpadR_);
+    }
     mark(6, false, s);
   }
   mark(7, true, s);
