@@ -385,6 +385,7 @@ def main():
     rng = np.random.Generator(np.random.PCG64(args.seed))
     pts = rng.random((N, 3))
     f = rng.standard_normal((N, 3))
+    nnk.OcTree(pts[:4096], pts[:4096], "nodeCapacity", 64)   # GPU runtime and code objects loaded
     t0 = time.perf_counter()
     tree = nnk.OcTree(pts, pts, "nodeCapacity", 64)
     t_tree = time.perf_counter() - t0
@@ -441,7 +442,9 @@ def main():
                    "parallelism": "single GPU"},
         "rel_err_vs_direct": head["rel_err_vs_direct"],
         "forms": forms,
-        "setup_s": {"tree": round(t_tree, 3)},
+        "setup_s": {"tree": round(t_tree, 3),
+                    "tree_build_ms": dict(zip(["dedup", "subdivide", "point_lists", "uvwx_lists"],
+                                              [round(float(v), 2) for v in tree.buildTimes]))},
         "roofline": roof,
     }
     for name in names[:-1]:
