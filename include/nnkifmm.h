@@ -299,7 +299,9 @@ int32_t nnk_batch_apply_device(nnk_batch *h, const double *d_x, double *d_y, voi
 int32_t nnk_batch_set_graph(nnk_batch *h, int32_t on);
 /* Unrestarted GMRES from x = 0 on every problem at once, on A (transpose =
  * 0) or A^T (transpose = 1): lockstep iterations, a problem stops when its
- * residual estimate reaches tol.  flag, relres (true relative residual),
+ * residual estimate reaches tol (tol / 2 with FP32-math operators, whose
+ * estimate drifts from the true residual).  flag (0 when the recomputed
+ * relative residual is <= tol, as MATLAB's gmres), relres (that residual),
  * iters: nprob entries each. */
 int32_t nnk_batch_solve(nnk_batch *h, const double *rhs, double tol, int32_t maxit, int32_t transpose,
                         double *x, int32_t *flag, double *relres, int32_t *iters);

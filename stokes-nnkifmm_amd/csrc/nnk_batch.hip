@@ -737,6 +737,10 @@ void Batch::gmres(const double *b, double *x, double tol, int maxit, bool transp
                   int32_t *flag, double *relres, int32_t *iters) {
   const int P = np_;
   const int64_t n = n_;
+  // with FP32-math operators the Arnoldi estimate drifts from the true
+  // residual by the operator's rounding: drive the estimate to tol / 2 so
+  // that the recomputed residual (which sets the flag) meets tol
+  const double tol_est = f32_ ? 0.5 * tol : tol;
   int64_t minlen = INT64_MAX;
   for (int p = 0; p < P; p++) minlen = std::min(minlen, hvoff_[p + 1] - hvoff_[p]);
   const int m = std::max(1, (int)std::min<int64_t>(maxit, minlen));
@@ -815,7 +819,7 @@ void Batch::gmres(const double *b, double *x, double tol, int maxit, bool transp
       gp[k] = cp[k] * gp[k];
       for (int j = 0; j <= k; j++) Hp[(size_t)j * m + k] = col[j];
       kp[p] = k + 1;
-      if (std::fabs(gp[k + 1]) <= tol * bn[p] || hn[p] == 0.0) {
+      if (std::fabs(gp[k + 1]) <= tol_est * bn[p] || hn[p] == 0.0) {
         done[p] = 1;
         act[p] = 0;
         changed = true;

@@ -51,7 +51,9 @@ class Batch {
   void apply(const double *x, double *y, bool transpose, hipStream_t s);
   void set_graph(bool on) { graph_ = on; }
   // unrestarted GMRES from x = 0 per problem on A (or A^T); per-problem
-  // flag, true relative residual and iteration count
+  // flag (0 when the recomputed relative residual is <= tol), that residual
+  // and the iteration count.  With FP32-math operators the Arnoldi
+  // estimate is driven to tol / 2.
   void gmres(const double *b, double *x, double tol, int maxit, bool transpose, hipStream_t s, int32_t *flag,
              double *relres, int32_t *iters);
   // gx: 3 per force point, gy: 3 per quadrature point, geps: per problem

@@ -623,7 +623,8 @@ void Fmm::launch_h16(const DBuf &jobs, int njobs, int maxN, int M, int KC, const
   if (!njobs) return;
   const int nrb = (M + MBM - 1) / MBM, nct = (maxN + MBN - 1) / MBN;
   const int RT = nrb * MBM / 16;
-  hipLaunchKernelGGL((k_m2l_split<2, 2, 1, true>), dim3((unsigned)(njobs * nrb * nct)), dim3(512), 0, s, M, KC, RT,
+  // TAG 2: a kernel symbol of its own, apart from the M2L launches (TAG 1)
+  hipLaunchKernelGGL((k_m2l_split<2, 2, 2, true>), dim3((unsigned)(njobs * nrb * nct)), dim3(512), 0, s, M, KC, RT,
                      X, tree_.count, C, ldc, jobs.as<GemmJobBf16>(), njobs, nrb, 1, nullptr, maxN, xun, beta);
   NNK_HIP(hipGetLastError());
 }
