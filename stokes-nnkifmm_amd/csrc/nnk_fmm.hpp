@@ -253,7 +253,7 @@ class Fmm {
   int n_upall_ = 0;
   void reduced_factor();
   bool red_ready_ = false;
-  DBuf red_blk_, red_off_, red_piv_, red_poff_, red_ord2loc_;
+  DBuf red_blk_, red_off_, red_piv_, red_perm_, red_poff_, red_ord2loc_;
   struct RedArgs red_args() const;
   int red_maxn_ = 0;
   int64_t stats_[kStats] = {0};

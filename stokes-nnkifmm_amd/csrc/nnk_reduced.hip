@@ -30,6 +30,7 @@ struct RedArgs {
   const int64_t *off;       // block offset (doubles) per leaf slot
   double *blk;
   int32_t *piv;             // 3n pivots per leaf at off3[leaf]
+  int32_t *perm;            // the pivots as one permutation (same offsets)
   const int64_t *poff;
   double e2, c;
   int nearest;
@@ -127,8 +128,51 @@ __global__ __launch_bounds__(256) void k_red_lu(RedArgs a) {
   }
 }
 
+// the factors of each leaf to column-major order (in place), so that the
+// substitutions read whole columns contiguously
+__global__ __launch_bounds__(256) void k_red_transpose(RedArgs a) {
+  const int li = blockIdx.x;
+  const int n = a.leaves[li];
+  const int n3 = 3 * (a.tgt_ptr[n + 1] - a.tgt_ptr[n]);
+  double *A = a.blk + a.off[li];
+  const int64_t tot = (int64_t)n3 * n3;
+  for (int64_t idx = threadIdx.x; idx < tot; idx += 256) {
+    const int i = (int)(idx / n3), j = (int)(idx % n3);
+    if (i < j) {
+      const double t = A[(size_t)i * n3 + j];
+      A[(size_t)i * n3 + j] = A[(size_t)j * n3 + i];
+      A[(size_t)j * n3 + i] = t;
+    }
+  }
+}
+
+// the row interchanges of each leaf's LU, applied in order to the identity:
+// perm[i] is the row of the right-hand side that lands in position i
+__global__ void k_red_perm(RedArgs a, int nleaves) {
+  const int li = blockIdx.x * blockDim.x + threadIdx.x;
+  if (li >= nleaves) return;
+  const int n = a.leaves[li];
+  const int n3 = 3 * (a.tgt_ptr[n + 1] - a.tgt_ptr[n]);
+  const int32_t *piv = a.piv + a.poff[li];
+  int32_t *perm = a.perm + a.poff[li];
+  for (int i = 0; i < n3; i++) perm[i] = i;
+  for (int k = 0; k < n3; k++) {
+    const int p = piv[k];
+    const int32_t t = perm[k];
+    perm[k] = perm[p];
+    perm[p] = t;
+  }
+}
+
+constexpr int kRedSolveThreads = 512;
+
 // x = A^-1 v for each leaf (v, x indexed by target ordinal, 3 per point)
-__global__ __launch_bounds__(256) void k_red_solve(RedArgs a, const double *__restrict__ v,
+// with the column-major LU factors (Ac[k * n3 + i] = LU[i][k]).  Blocked
+// substitution: each 64-row diagonal block is solved by wavefront 0 alone,
+// passing the solved values between lanes with shuffles (no barrier inside
+// a block); then all threads of the workgroup update the remaining rows
+// (two barriers per block instead of two per row).
+__global__ __launch_bounds__(kRedSolveThreads) void k_red_solve(RedArgs a, const double *__restrict__ v,
                                                    double *__restrict__ x) {
   extern __shared__ double b[];
   const int li = blockIdx.x;
@@ -136,28 +180,66 @@ __global__ __launch_bounds__(256) void k_red_solve(RedArgs a, const double *__re
   const int tb = a.tgt_ptr[n], nt = a.tgt_ptr[n + 1] - tb;
   const int n3 = 3 * nt;
   const double *A = a.blk + a.off[li];
-  const int32_t *piv = a.piv + a.poff[li];
-  for (int i = threadIdx.x; i < n3; i += 256) b[i] = v[3 * (size_t)a.tgt_out[tb + i / 3] + i % 3];
+  const int32_t *perm = a.perm + a.poff[li];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  for (int i = tid; i < n3; i += kRedSolveThreads) {
+    const int r = perm[i];
+    b[i] = v[3 * (size_t)a.tgt_out[tb + r / 3] + r % 3];
+  }
   __syncthreads();
-  if (threadIdx.x == 0)
-    for (int k = 0; k < n3; k++) {
-      const int p = piv[k];
-      if (p != k) { double t = b[k]; b[k] = b[p]; b[p] = t; }
+  // unit lower triangle, top down
+  for (int kb = 0; kb < n3; kb += 64) {
+    const int ke = min(kb + 64, n3);
+    if (wave == 0) {  // lane l holds row kb + l of the diagonal block
+      const int r = kb + lane;
+      double L[64];
+#pragma unroll
+      for (int kk = 0; kk < 64; kk++) L[kk] = (kb + kk < ke && r < ke) ? A[(size_t)(kb + kk) * n3 + r] : 0.0;
+      double br = r < ke ? b[r] : 0.0;
+#pragma unroll
+      for (int kk = 0; kk < 64; kk++) {
+        const double bk = __shfl(br, kk);
+        if (lane > kk) br -= L[kk] * bk;
+      }
+      if (r < ke) b[r] = br;
     }
-  __syncthreads();
-  for (int k = 0; k < n3; k++) {          // unit lower
-    const double bk = b[k];
-    for (int i = k + 1 + threadIdx.x; i < n3; i += 256) b[i] -= A[(size_t)i * n3 + k] * bk;
+    __syncthreads();
+    for (int i = ke + tid; i < n3; i += kRedSolveThreads) {
+      double t = b[i];
+      for (int k = kb; k < ke; k++) t -= A[(size_t)k * n3 + i] * b[k];
+      b[i] = t;
+    }
     __syncthreads();
   }
-  for (int k = n3 - 1; k >= 0; k--) {     // upper
-    if (threadIdx.x == 0) b[k] /= A[(size_t)k * n3 + k];
+  // upper triangle, bottom up
+  for (int ke = n3; ke > 0; ke -= 64) {
+    const int kb = max(ke - 64, 0);
+    if (wave == 0) {
+      const int r = kb + lane;
+      double U[64];
+#pragma unroll
+      for (int kk = 0; kk < 64; kk++) U[kk] = (kb + kk < ke && r < ke) ? A[(size_t)(kb + kk) * n3 + r] : 0.0;
+      double br = r < ke ? b[r] : 0.0;
+#pragma unroll
+      for (int kk = 63; kk >= 0; kk--) {
+        if (kb + kk >= ke) continue;                    // the same for the whole wavefront
+        const double xk = __shfl(br, kk) / __shfl(U[kk], kk);   // lane kk holds the diagonal
+        if (lane == kk) br = xk;
+        else if (lane < kk) br -= U[kk] * xk;
+      }
+      if (r < ke) {
+        b[r] = br;
+        x[3 * (size_t)a.tgt_out[tb + r / 3] + r % 3] = br;
+      }
+    }
     __syncthreads();
-    const double bk = b[k];
-    for (int i = threadIdx.x; i < k; i += 256) b[i] -= A[(size_t)i * n3 + k] * bk;
+    for (int i = tid; i < kb; i += kRedSolveThreads) {
+      double t = b[i];
+      for (int k = kb; k < ke; k++) t -= A[(size_t)k * n3 + i] * b[k];
+      b[i] = t;
+    }
     __syncthreads();
   }
-  for (int i = threadIdx.x; i < n3; i += 256) x[3 * (size_t)a.tgt_out[tb + i / 3] + i % 3] = b[i];
 }
 
 void Fmm::reduced_factor() {
@@ -190,12 +272,16 @@ void Fmm::reduced_factor() {
   red_ord2loc_.upload(o2l);
   red_blk_.alloc(sizeof(double) * std::max<int64_t>(off.back(), 1));
   red_piv_.alloc(sizeof(int32_t) * std::max<int64_t>(poff.back(), 1));
+  red_perm_.alloc(sizeof(int32_t) * std::max<int64_t>(poff.back(), 1));
   red_poff_ = std::move(dpoff);
   RedArgs a = red_args();
   if (n_tleaves_) {
     hipLaunchKernelGGL(k_red_form, dim3(n_tleaves_), dim3(256), 0, 0, a);
     NNK_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_red_lu, dim3(n_tleaves_), dim3(256), 0, 0, a);
+    NNK_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_red_transpose, dim3(n_tleaves_), dim3(256), 0, 0, a);
+    hipLaunchKernelGGL(k_red_perm, dim3((n_tleaves_ + 63) / 64), dim3(64), 0, 0, a, n_tleaves_);
     NNK_HIP(hipGetLastError());
   }
   NNK_HIP(hipDeviceSynchronize());
@@ -217,6 +303,7 @@ RedArgs Fmm::red_args() const {
   a.off = red_off_.as<int64_t>();
   a.blk = red_blk_.as<double>();
   a.piv = red_piv_.as<int32_t>();
+  a.perm = red_perm_.as<int32_t>();
   a.poff = red_poff_.as<int64_t>();
   a.e2 = eps_ * eps_;
   a.c = 1.0 / (8.0 * kPi * mu_);
@@ -228,7 +315,7 @@ void Fmm::reduced_solve(const double *d_v, double *d_f, hipStream_t s) {
   if (!red_ready_) reduced_factor();
   if (!n_tleaves_) return;
   RedArgs a = red_args();
-  hipLaunchKernelGGL(k_red_solve, dim3(n_tleaves_), dim3(256), sizeof(double) * red_maxn_, s, a, d_v,
+  hipLaunchKernelGGL(k_red_solve, dim3(n_tleaves_), dim3(kRedSolveThreads), sizeof(double) * red_maxn_, s, a, d_v,
                      d_f);
   NNK_HIP(hipGetLastError());
 }
