@@ -130,6 +130,8 @@ def run_form(nnk, lib, tree, fd, args, eps, mu, precision, res, svd_tol):
     spl = np.zeros(16, np.int32)
     dep = np.zeros(16, np.int32)
     lib.nnk_fmm_launch_config(h.h, spl.ctypes.data_as(P(ctypes.c_int32)), dep.ctypes.data_as(P(ctypes.c_int32)), 16)
+    leaf_threads = ctypes.c_int32()
+    lib.nnk_fmm_leaf_threads(h.h, ctypes.byref(leaf_threads))
     K, R = int(st[12]), int(st[13])
     ms_step = dt / args.steps * 1e3
     phases = ["strengths", "S2M", "up", "M2L", "L2L", "P2L", "down", "leaf"]
@@ -153,6 +155,7 @@ def run_form(nnk, lib, tree, fd, args, eps, mu, precision, res, svd_tol):
            "phase_ms_per_step": {p: round(float(ms[i] / args.steps), 3) for i, p in enumerate(phases)},
            "m2l_launch_per_level": {str(l): {"split": int(spl[l]), "depth": int(dep[l])}
                                     for l in range(16) if spl[l] > 0},
+           "leaf_kernel_threads": int(leaf_threads.value),
            "m2l": {"kernel": kern, "flops_per_step": flops, "ms_per_step": round(m2l_ms, 3),
                    "launches_per_step": float(cnt[3] / args.steps), "achieved_tflops": round(achieved, 3),
                    "peak_tflops": peak}}
@@ -252,7 +255,7 @@ def batched_solves(nnk):
     """256 independent single-body problems (spheres and spheroids, 200-5000
     force points log-uniform, 4x refined quadrature, eps in [0.002, 0.02]):
     one batched GMRES to 1e-6, HIP-graph replay of the persistent batched
-    operator, FP64 and FP32-math."""
+    operator with its P2P workgroup size autotuned, FP64 and FP32-math."""
     import importlib as il
     geo = il.import_module("stokes-nnkifmm_amd.geometries")
     t0 = time.perf_counter()
@@ -262,14 +265,15 @@ def batched_solves(nnk):
     x64 = None
     for prec in ("fp64", "mixed"):
         t0 = time.perf_counter()
-        bm = nnk.BatchedMobility(bodies, eps, precision=prec, hipGraph=True)
+        bm = nnk.BatchedMobility(bodies, eps, precision=prec, hipGraph=True, autotune=True)
         ts = time.perf_counter() - t0
         b = bm.rhs()
         t0 = time.perf_counter()
         x = bm.solve(b, 1e-6, 300)
         dt = time.perf_counter() - t0
         i = bm.last_solve
-        rec = {"setup_s": round(ts, 3), "wall_s": round(dt, 3), "max_iterations": int(i["iter"].max()),
+        rec = {"setup_s": round(ts, 3), "wall_s": round(dt, 3), "p2p_waves_autotuned": bm.p2pWaves(),
+               "max_iterations": int(i["iter"].max()),
                "mean_iterations": float(i["iter"].mean()), "unconverged": int(i["flag"].sum()),
                "max_relres": float(i["relres"].max())}
         if x64 is None:

@@ -199,6 +199,12 @@ int32_t nnk_fmm_set_graph(nnk_fmm *h, int32_t on);
  *           depth; 0 where a level has no M2L) for levels 0..nlev-1. */
 int32_t nnk_fmm_set_autotune(nnk_fmm *h, int32_t on);
 int32_t nnk_fmm_launch_config(const nnk_fmm *h, int32_t *split, int32_t *depth, int32_t nlev);
+/* No counterpart in the reference: the workgroup size of the leaf kernel
+ *           (L2T + P2P + M2T per target leaf), 128, 256 or 512 threads; with
+ *           nnk_fmm_set_autotune the first apply times the three and keeps
+ *           the fastest (summation order of the P2P sums changes, not their
+ *           accuracy).  256 without autotuning. */
+int32_t nnk_fmm_leaf_threads(const nnk_fmm *h, int32_t *threads);
 
 /* Replaces nothing; serves the time-stepping callers of mobilityProblem
  *           (ode45 in Data Generation/Squiremers.m:47-54,
@@ -297,6 +303,12 @@ int32_t nnk_batch_apply_transpose(nnk_batch *h, const double *x, double *y);
 int32_t nnk_batch_apply_device(nnk_batch *h, const double *d_x, double *d_y, void *stream);
 /* on = 1: replay the batched operators from HIP graphs captured on first use. */
 int32_t nnk_batch_set_graph(nnk_batch *h, int32_t on);
+/* on = 1: the first operator application times the batched P2P at 1, 2
+ * and 4 wavefronts per workgroup (sources of a 64-target tile split over
+ * the wavefronts) and keeps the fastest; nnk_batch_p2p_waves reports the
+ * choice (4 without autotuning).  Summation order changes, not accuracy. */
+int32_t nnk_batch_set_autotune(nnk_batch *h, int32_t on);
+int32_t nnk_batch_p2p_waves(const nnk_batch *h, int32_t *waves);
 /* Unrestarted GMRES from x = 0 on every problem at once, on A (transpose =
  * 0) or A^T (transpose = 1): lockstep iterations, a problem stops when its
  * residual estimate reaches tol (tol / 2 with FP32-math operators, whose

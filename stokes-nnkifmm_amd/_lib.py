@@ -65,6 +65,7 @@ SIGNATURES = {
     "nnk_fmm_set_graph": (ctypes.c_int32, [_vp, ctypes.c_int32]),
     "nnk_fmm_set_autotune": (ctypes.c_int32, [_vp, ctypes.c_int32]),
     "nnk_fmm_launch_config": (ctypes.c_int32, [_vp, _i32, _i32, ctypes.c_int32]),
+    "nnk_fmm_leaf_threads": (ctypes.c_int32, [_vp, _i32]),
     "nnk_op_cache_limit": (ctypes.c_int32, [ctypes.c_int64]),
     "nnk_op_cache_stats": (ctypes.c_int32, [_i64, _i64, _i64, _i64]),
     "nnk_mobility_set_graph": (ctypes.c_int32, [_vp, ctypes.c_int32]),
@@ -76,6 +77,8 @@ SIGNATURES = {
     "nnk_batch_apply_transpose": (ctypes.c_int32, [_vp, _d, _d]),
     "nnk_batch_apply_device": (ctypes.c_int32, [_vp, _vp, _vp, _vp]),
     "nnk_batch_set_graph": (ctypes.c_int32, [_vp, ctypes.c_int32]),
+    "nnk_batch_set_autotune": (ctypes.c_int32, [_vp, ctypes.c_int32]),
+    "nnk_batch_p2p_waves": (ctypes.c_int32, [_vp, _i32]),
     "nnk_batch_solve": (ctypes.c_int32, [_vp, _d, ctypes.c_double, ctypes.c_int32, ctypes.c_int32, _d, _i32, _d,
                                          _i32]),
     "nnk_batch_gradient": (ctypes.c_int32, [_vp, _d, _d, _d, _d, _d]),

@@ -21,7 +21,7 @@ _PREC = {"fp64": 0, "mixed": 1}
 
 
 class BatchedMobility:
-    def __init__(self, systems, eps, mu=1.0, precision="fp64", hipGraph=True):
+    def __init__(self, systems, eps, mu=1.0, precision="fp64", hipGraph=True, autotune=False):
         import scipy.sparse as sp
         if precision not in _PREC:
             raise ValueError(f"BatchedMobility: unknown precision {precision!r}")
@@ -78,6 +78,8 @@ class BatchedMobility:
         self.h = h
         if hipGraph:
             check(lib().nnk_batch_set_graph(h, 1))
+        if autotune:
+            check(lib().nnk_batch_set_autotune(h, 1))
         n = np.zeros(1, np.int64)
         check(lib().nnk_batch_size(h, ptr(n, _lib._i64)))
         self.n = int(n[0])
@@ -86,6 +88,13 @@ class BatchedMobility:
         self.nsw = np.diff(a[11])
         self.offsets = np.concatenate([[0], np.cumsum(3 * self.npts + 6 * self.nsw)])
         self.last_solve = {}
+
+    def p2pWaves(self):
+        """Wavefronts per workgroup of the batched P2P (autotuned on the
+        first operator application when autotune=True; 4 otherwise)."""
+        v = ctypes.c_int32()
+        check(lib().nnk_batch_p2p_waves(self.h, ctypes.byref(v)))
+        return int(v.value)
 
     def __del__(self):
         if getattr(self, "h", None):

@@ -109,9 +109,11 @@ __global__ void k_bcounter_reset(int *c) {
   __syncthreads();              \
   }
 
-// forward P2P: u_i = c sum_q S(x_i - y_q) g_q, then + BT (U; W) on body points
-template <bool F32>
-__global__ __launch_bounds__(256) void k_bp2p(int nitems, const int2 *__restrict__ items, int *counter,
+// forward P2P: u_i = c sum_q S(x_i - y_q) g_q, then + BT (U; W) on body
+// points.  NW wavefronts per workgroup split the sources of a tile (the
+// autotuner picks 1, 2 or 4); their partial sums are added in wave order.
+template <bool F32, int NW>
+__global__ __launch_bounds__(64 * NW) void k_bp2p(int nitems, const int2 *__restrict__ items, int *counter,
                                               const int64_t *__restrict__ poff, const int64_t *__restrict__ foff,
                                               const int64_t *__restrict__ voff, const int32_t *__restrict__ nsw,
                                               const double *__restrict__ pts, const double *__restrict__ fine,
@@ -119,8 +121,9 @@ __global__ __launch_bounds__(256) void k_bp2p(int nitems, const int2 *__restrict
                                               const int32_t *__restrict__ pbody, const double *__restrict__ eps,
                                               const int32_t *__restrict__ active, double c,
                                               const double *__restrict__ x, double *__restrict__ y) {
-  __shared__ double sp[256 * 6];
-  __shared__ double red[4][kTile * 3];
+  constexpr int T = 64 * NW;
+  __shared__ double sp[T * 6];
+  __shared__ double red[NW][kTile * 3];
   __shared__ int item_s;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   NNK_PERSISTENT_LOOP_BEGIN
@@ -136,8 +139,8 @@ __global__ __launch_bounds__(256) void k_bp2p(int nitems, const int2 *__restrict
   const double e2 = eps[p] * eps[p];
   double ux = 0, uy = 0, uz = 0;
   const int64_t qb = foff[p], qe = foff[p + 1];
-  for (int64_t q0 = qb; q0 < qe; q0 += 256) {
-    const int cnt = (int)min((int64_t)256, qe - q0);
+  for (int64_t q0 = qb; q0 < qe; q0 += T) {
+    const int cnt = (int)min((int64_t)T, qe - q0);
     if (tid < cnt) {
       const int64_t q = q0 + tid;
       for (int d = 0; d < 3; d++) {
@@ -147,7 +150,7 @@ __global__ __launch_bounds__(256) void k_bp2p(int nitems, const int2 *__restrict
     }
     __syncthreads();
     if (lane < nt)
-      for (int j = wv; j < cnt; j += 4)
+      for (int j = wv; j < cnt; j += NW)
         stk<F32>(tx, ty, tz, sp[6 * j], sp[6 * j + 1], sp[6 * j + 2], sp[6 * j + 3], sp[6 * j + 4], sp[6 * j + 5],
                  e2, ux, uy, uz);
     __syncthreads();
@@ -158,8 +161,12 @@ __global__ __launch_bounds__(256) void k_bp2p(int nitems, const int2 *__restrict
   __syncthreads();
   if (wv == 0 && lane < nt) {
     double u[3];
-    for (int d = 0; d < 3; d++)
-      u[d] = c * (((red[0][3 * lane + d] + red[1][3 * lane + d]) + red[2][3 * lane + d]) + red[3][3 * lane + d]);
+    for (int d = 0; d < 3; d++) {
+      double t = red[0][3 * lane + d];
+#pragma unroll
+      for (int w2 = 1; w2 < NW; w2++) t += red[w2][3 * lane + d];
+      u[d] = c * t;
+    }
     const Slots S = slots(p, poff, voff, nsw);
     const int64_t i = i0 + lane, loc = i - pb;
     const int b = pbody[i];
@@ -668,16 +675,25 @@ void Batch::op(const double *x, double *y, hipStream_t s) {
                        nncol_.as<int32_t>(), nnval_.as<double>(), x, xbase_.as<int64_t>(),
                        xstride_.as<int64_t>(), g_.as<double>());
   hipLaunchKernelGGL(k_bcounter_reset, dim3(1), dim3(64), 0, s, counter_.as<int>());
-  const unsigned grid = (unsigned)std::max(1, std::min(nitems_, 1024));
+  // persistent grid: about 256k threads whatever the workgroup size
+  const int NW = p2p_waves_;
+  const unsigned grid = (unsigned)std::max(1, std::min(nitems_, 1024 * 4 / NW));
   const double c = 1.0 / (8.0 * kPi * mu_);
   auto launch = [&](auto kern) {
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(256), 0, s, nitems_, items_.as<int2>(), counter_.as<int>(),
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(64 * NW), 0, s, nitems_, items_.as<int2>(), counter_.as<int>(),
                        poff_.as<int64_t>(), foff_.as<int64_t>(), voff_.as<int64_t>(), nsw_.as<int32_t>(),
                        pts_.as<double>(), fine_.as<double>(), g_.as<double>(), rot_.as<double>(),
                        pbody_.as<int32_t>(), eps_.as<double>(), active_.as<int32_t>(), c, x, y);
   };
-  if (f32_) launch(k_bp2p<true>);
-  else launch(k_bp2p<false>);
+  if (f32_) {
+    if (NW == 1) launch(k_bp2p<true, 1>);
+    else if (NW == 2) launch(k_bp2p<true, 2>);
+    else launch(k_bp2p<true, 4>);
+  } else {
+    if (NW == 1) launch(k_bp2p<false, 1>);
+    else if (NW == 2) launch(k_bp2p<false, 2>);
+    else launch(k_bp2p<false, 4>);
+  }
   if (nbody_)
     hipLaunchKernelGGL(k_bB, dim3(nbody_), dim3(256), 0, s, bprob_.as<int32_t>(), blocal_.as<int32_t>(),
                        bpt_.as<int64_t>(), poff_.as<int64_t>(), voff_.as<int64_t>(), nsw_.as<int32_t>(),
@@ -709,7 +725,50 @@ void Batch::opT(const double *x, double *y, hipStream_t s) {
   NNK_HIP(hipGetLastError());
 }
 
+// time the forward operator with 1, 2 and 4 wavefronts per P2P workgroup
+// (on the caller's vectors; y is overwritten) and keep the fastest
+void Batch::autotune_p2p(const double *x, double *y, hipStream_t s) {
+  // time every problem, then restore the caller's active flags
+  std::vector<int32_t> saved(np_), act(np_, 1);
+  NNK_HIP(hipMemcpyAsync(saved.data(), active_.p, sizeof(int32_t) * np_, hipMemcpyDeviceToHost, s));
+  NNK_HIP(hipStreamSynchronize(s));
+  NNK_HIP(hipMemcpyAsync(active_.p, act.data(), sizeof(int32_t) * np_, hipMemcpyHostToDevice, s));
+  hipEvent_t a, b;
+  NNK_HIP(hipEventCreate(&a));
+  NNK_HIP(hipEventCreate(&b));
+  float best = 1e30f;
+  int bw = p2p_waves_;
+  try {
+    for (int nw : {1, 2, 4}) {
+      p2p_waves_ = nw;
+      op(x, y, s);  // warm
+      NNK_HIP(hipEventRecord(a, s));
+      op(x, y, s);
+      op(x, y, s);
+      NNK_HIP(hipEventRecord(b, s));
+      NNK_HIP(hipEventSynchronize(b));
+      float ms = 0;
+      NNK_HIP(hipEventElapsedTime(&ms, a, b));
+      if (ms < best) { best = ms; bw = nw; }
+    }
+  } catch (...) {
+    p2p_waves_ = bw;
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    throw;
+  }
+  p2p_waves_ = bw;
+  NNK_HIP(hipMemcpyAsync(active_.p, saved.data(), sizeof(int32_t) * np_, hipMemcpyHostToDevice, s));
+  NNK_HIP(hipStreamSynchronize(s));
+  NNK_HIP(hipEventDestroy(a));
+  NNK_HIP(hipEventDestroy(b));
+}
+
 void Batch::apply(const double *x, double *y, bool transpose, hipStream_t s) {
+  if (tune_ && !tuned_) {  // outside any graph capture
+    autotune_p2p(x, y, s);
+    tuned_ = true;
+  }
   if (!graph_) {
     if (transpose) opT(x, y, s);
     else op(x, y, s);

@@ -50,6 +50,9 @@ class Batch {
   // y = A x (transpose: y = A^T x) for every problem (device vectors)
   void apply(const double *x, double *y, bool transpose, hipStream_t s);
   void set_graph(bool on) { graph_ = on; }
+  // time the P2P workgroup size (1, 2, 4 wavefronts) on the first apply
+  void set_autotune(bool on) { tune_ = on; }
+  int p2p_waves() const { return p2p_waves_; }
   // unrestarted GMRES from x = 0 per problem on A (or A^T); per-problem
   // flag (0 when the recomputed relative residual is <= tol), that residual
   // and the iteration count.  With FP32-math operators the Arnoldi
@@ -63,6 +66,7 @@ class Batch {
 
  private:
   void op(const double *x, double *y, hipStream_t s);
+  void autotune_p2p(const double *x, double *y, hipStream_t s);
   void opT(const double *x, double *y, hipStream_t s);
   void segdots(int k, const double *V, const double *w, double *host_out, hipStream_t s);
 
@@ -77,6 +81,8 @@ class Batch {
   DBuf xbase_, xstride_, pid_, pidf_, items_, itemsT_, counter_, g_, v_, active_, part_;
   int nitems_ = 0, nitemsT_ = 0;
   bool graph_ = false;
+  bool tune_ = false, tuned_ = false;
+  int p2p_waves_ = 4;
   GraphOp gop_[2];
 };
 

@@ -354,6 +354,12 @@ int32_t nnk_fmm_launch_config(const nnk_fmm *h, int32_t *split, int32_t *depth, 
   });
 }
 
+int32_t nnk_fmm_leaf_threads(const nnk_fmm *h, int32_t *threads) {
+  if (!h || !threads) return fail(NNK_EARG, "nnk_fmm_leaf_threads: bad argument");
+  *threads = h->f->leaf_threads();
+  return NNK_OK;
+}
+
 int32_t nnk_mobility_set_graph(nnk_mobility *h, int32_t on) {
   return guard([&]() -> int32_t {
     if (!h) return fail(NNK_EARG, "nnk_mobility_set_graph: null handle");
@@ -518,6 +524,20 @@ int32_t nnk_batch_set_graph(nnk_batch *h, int32_t on) {
     h->b->set_graph(on != 0);
     return NNK_OK;
   });
+}
+
+int32_t nnk_batch_set_autotune(nnk_batch *h, int32_t on) {
+  return guard([&]() -> int32_t {
+    if (!h) return fail(NNK_EARG, "nnk_batch_set_autotune: null handle");
+    h->b->set_autotune(on != 0);
+    return NNK_OK;
+  });
+}
+
+int32_t nnk_batch_p2p_waves(const nnk_batch *h, int32_t *waves) {
+  if (!h || !waves) return fail(NNK_EARG, "nnk_batch_p2p_waves: bad argument");
+  *waves = h->b->p2p_waves();
+  return NNK_OK;
 }
 
 int32_t nnk_batch_solve(nnk_batch *h, const double *rhs, double tol, int32_t maxit, int32_t transpose, double *x,

@@ -194,17 +194,18 @@ struct LeafArgs {
 // Velocity at the targets of one leaf: L2T from its down density, P2P from
 // its U-list sources, M2T from its W-list up densities (Downleaf.m:37-40).
 // Each target is shared by nsub lanes that split the sources; partial sums
-// are combined in a fixed order.
-template <bool F32>
-__global__ __launch_bounds__(256) void k_leaf(LeafArgs a) {
+// are combined in a fixed order.  T threads per workgroup (the autotuner
+// picks 128, 256 or 512: more lanes per target, or more targets in flight).
+template <bool F32, int T>
+__global__ __launch_bounds__(T) void k_leaf(LeafArgs a) {
   const int n = a.leaves[blockIdx.x];
-  __shared__ double sp[256 * 6];
-  __shared__ double red[256 * 3];
+  __shared__ double sp[T * 6];
+  __shared__ double red[T * 3];
   const int tid = threadIdx.x;
   const int tb = a.tgt_ptr[n], nt = a.tgt_ptr[n + 1] - tb;
-  for (int t0 = 0; t0 < nt; t0 += 256) {
-    const int ntc = min(256, nt - t0);
-    const int nsub = 256 / ntc;
+  for (int t0 = 0; t0 < nt; t0 += T) {
+    const int ntc = min(T, nt - t0);
+    const int nsub = T / ntc;
     const bool act = tid < ntc * nsub;
     const int ti = tid % ntc, sub = tid / ntc;
     double tx = 0, ty = 0, tz = 0;
@@ -221,8 +222,8 @@ __global__ __launch_bounds__(256) void k_leaf(LeafArgs a) {
     };
     auto surface = [&](int node, int down, const double *dens) {
       const double *box = a.corners + 6 * (size_t)(node - 1);
-      for (int p0 = 0; p0 < a.nq; p0 += 256) {
-        const int cnt = min(256, a.nq - p0);
+      for (int p0 = 0; p0 < a.nq; p0 += T) {
+        const int cnt = min(T, a.nq - p0);
         if (tid < cnt) {
           double x, y, z;
           surf_point(box, down, a.surf_idx[p0 + tid], a.n1, x, y, z);
@@ -237,8 +238,8 @@ __global__ __launch_bounds__(256) void k_leaf(LeafArgs a) {
     };
     auto sources = [&](int leaf) {
       const int sb = a.src_ptr[leaf], se = a.src_ptr[leaf + 1];
-      for (int s0 = sb; s0 < se; s0 += 256) {
-        const int cnt = min(256, se - s0);
+      for (int s0 = sb; s0 < se; s0 += T) {
+        const int cnt = min(T, se - s0);
         if (tid < cnt) {
           const int s = s0 + tid;
           for (int d = 0; d < 3; d++) {
@@ -1096,8 +1097,17 @@ void Fmm::launch_leaf(int far, double *d_u, hipStream_t s) {
   a.c = 1.0 / (8.0 * kPi * mu_);
   a.far = far;
   a.out = d_u;
-  if (prec_ >= 1) hipLaunchKernelGGL(k_leaf<true>, dim3(n_tleaves_), dim3(256), 0, s, a);
-  else hipLaunchKernelGGL(k_leaf<false>, dim3(n_tleaves_), dim3(256), 0, s, a);
+  auto go = [&](auto kern, int threads) { hipLaunchKernelGGL(kern, dim3(n_tleaves_), dim3(threads), 0, s, a); };
+  const int T = far ? leaf_threads_ : 256;
+  if (prec_ >= 1) {
+    if (T == 128) go(k_leaf<true, 128>, 128);
+    else if (T == 512) go(k_leaf<true, 512>, 512);
+    else go(k_leaf<true, 256>, 256);
+  } else {
+    if (T == 128) go(k_leaf<false, 128>, 128);
+    else if (T == 512) go(k_leaf<false, 512>, 512);
+    else go(k_leaf<false, 256>, 256);
+  }
   NNK_HIP(hipGetLastError());
 }
 
@@ -1105,6 +1115,7 @@ void Fmm::apply(const double *d_f, double *d_u, hipStream_t s) {
   if (tune_ && !tuned_) {
     apply_impl(d_f, d_u, s);  // fills the densities the candidates run on
     autotune(s);
+    autotune_leaf(s);
     tuned_ = true;
   }
   apply_impl(d_f, d_u, s);
@@ -1175,6 +1186,41 @@ void Fmm::autotune(hipStream_t s) {
   NNK_HIP(hipEventDestroy(b));
 }
 
+// time the leaf kernel (L2T + P2P + M2T) at 128, 256 and 512 threads per
+// target leaf on the current densities and keep the fastest
+void Fmm::autotune_leaf(hipStream_t s) {
+  if (!n_tleaves_) return;
+  DBuf out;
+  out.alloc(sizeof(double) * 3 * std::max<int64_t>(ntgt_, 1));
+  hipEvent_t a, b;
+  NNK_HIP(hipEventCreate(&a));
+  NNK_HIP(hipEventCreate(&b));
+  float best = 1e30f;
+  int bt = leaf_threads_;
+  try {
+    for (int T : {128, 256, 512}) {
+      leaf_threads_ = T;
+      launch_leaf(1, out.as<double>(), s);  // warm
+      NNK_HIP(hipEventRecord(a, s));
+      launch_leaf(1, out.as<double>(), s);
+      launch_leaf(1, out.as<double>(), s);
+      NNK_HIP(hipEventRecord(b, s));
+      NNK_HIP(hipEventSynchronize(b));
+      float ms = 0;
+      NNK_HIP(hipEventElapsedTime(&ms, a, b));
+      if (ms < best) { best = ms; bt = T; }
+    }
+  } catch (...) {
+    leaf_threads_ = bt;
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    throw;
+  }
+  leaf_threads_ = bt;
+  NNK_HIP(hipEventDestroy(a));
+  NNK_HIP(hipEventDestroy(b));
+}
+
 void Fmm::apply_impl(const double *d_f, double *d_u, hipStream_t s) {
   if (prec_ >= 1) { apply_mixed(d_f, d_u, s); return; }
   mark(0, true, s);
@@ -1223,6 +1269,7 @@ void Fmm::apply_impl(const double *d_f, double *d_u, hipStream_t s) {
 }
 
 void Fmm::apply_graph(const double *d_f, double *d_u, hipStream_t s) {
+  if (tune_ && !tuned_) apply(d_f, d_u, s);  // tune outside the capture (it times and allocates)
   if (!gop_.ready()) {
     const bool prof = prof_;
     prof_ = false;  // no event records inside the graph

@@ -153,10 +153,13 @@ class Fmm {
   // the same matvec replayed from a HIP graph captured on first use (one
   // launch per call instead of one per kernel); bit-identical to apply
   void apply_graph(const double *d_f, double *d_u, hipStream_t s);
-  // autotune the M2L launch shapes on the first apply (off by default)
+  // autotune the M2L launch shapes and the leaf kernel's workgroup size on
+  // the first apply (off by default)
   void set_autotune(bool on) { tune_ = on; }
   // per level: M2L split-K factor and (mixed forms) register-ring depth
   void launch_config(int level, int *split, int *depth) const;
+  // threads per target leaf of the leaf kernel (L2T + P2P + M2T)
+  int leaf_threads() const { return leaf_threads_; }
   // leaf-local product (reducedPass.m) on device pointers
   void reduced_apply(const double *d_f, double *d_u, hipStream_t s);
   // leaf-block solve (ReducedSolve.m): f = blockdiag(K_leaf)^-1 v, with the
@@ -207,6 +210,8 @@ class Fmm {
   // (register-ring depth 2/3 for the mixed forms, split-K factor s/2, s, 2s)
   // and keep the fastest
   void autotune(hipStream_t s);
+  void autotune_leaf(hipStream_t s);
+  int leaf_threads_ = 256;          // threads per target leaf in the leaf kernel
   void launch_surf(const int32_t *nodes, int n, const int64_t *lptr, const int32_t *lidx, int down,
                    int accumulate, double *chk, hipStream_t s);
   void launch_leaf(int far, double *d_u, hipStream_t s);

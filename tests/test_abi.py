@@ -10,7 +10,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 def test_library_exports_header_symbols(nnk):
     with open(os.path.join(ROOT, "include", "nnkifmm.h")) as fh:
         hdr = fh.read()
-    names = set(re.findall(r"\b(nnk_[a-z_]+)\s*\(", hdr))
+    names = set(re.findall(r"\b(nnk_[a-z0-9_]+)\s*\(", hdr))
     assert len(names) >= 20
     lib = ctypes.CDLL(nnk.library_path())
     missing = [n for n in sorted(names) if not hasattr(lib, n)]

@@ -89,3 +89,25 @@ def test_batched_f32_close_to_fp64(nnk, small_batch):
     x = np.random.default_rng(1).standard_normal(b64.n)
     y64, y32 = b64.apply(x), b32.apply(x)
     assert np.linalg.norm(y32 - y64) <= 1e-5 * np.linalg.norm(y64)
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_batched_p2p_autotune(oracle, nnk, small_batch, graph):
+    """The autotuner picks the P2P workgroup size (1, 2 or 4 wavefronts) on
+    the first application; results agree with the default shape to rounding,
+    match the oracle, and stay deterministic once tuned (also when the
+    operator is replayed from a HIP graph)."""
+    bodies, eps = small_batch
+    x = np.random.default_rng(3).standard_normal(nnk.BatchedMobility(bodies, eps, hipGraph=False).n)
+    ref = nnk.BatchedMobility(bodies, eps, hipGraph=False).apply(x)
+    bm = nnk.BatchedMobility(bodies, eps, hipGraph=graph, autotune=True)
+    y1 = bm.apply(x)
+    y2 = bm.apply(x)
+    assert bm.p2pWaves() in (1, 2, 4)
+    np.testing.assert_array_equal(y1, y2)
+    assert np.linalg.norm(y1 - ref) <= 1e-13 * np.linalg.norm(ref)
+    for p, (sw, xp, yp) in enumerate(zip(bodies, bm.split(x), bm.split(y1))):
+        r = _oracle_apply(oracle, sw, eps[p], xp)
+        assert np.linalg.norm(yp - r) <= 1e-12 * np.linalg.norm(r), p
+    z = bm.solve(bm.rhs(), 1e-10, 200)
+    assert (bm.last_solve["flag"] == 0).all() and np.isfinite(z).all()
