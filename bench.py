@@ -266,8 +266,9 @@ def batched_solves(nnk):
     for prec in ("fp64", "mixed"):
         t0 = time.perf_counter()
         bm = nnk.BatchedMobility(bodies, eps, precision=prec, hipGraph=True, autotune=True)
-        ts = time.perf_counter() - t0
         b = bm.rhs()
+        bm.apply(b)   # the autotuner's timing runs and the graph capture belong to set-up
+        ts = time.perf_counter() - t0
         t0 = time.perf_counter()
         x = bm.solve(b, 1e-6, 300)
         dt = time.perf_counter() - t0
