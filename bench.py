@@ -146,7 +146,11 @@ def run_form(nnk, lib, tree, fd, args, eps, mu, precision, res, svd_tol):
         peak, kern = BF16_MFMA_PEAK_TFLOPS, "k_m2l_split<3,D,1,false> (M2L, three-part split-BF16 MFMA)"
     else:
         flops = 3 * 2.0 * R * R * st[0]    # three FP16 products per split product
-        peak, kern = BF16_MFMA_PEAK_TFLOPS, "k_m2l_split<2,D,1,true> (M2L, two-part split-FP16 MFMA)"
+        peak = BF16_MFMA_PEAK_TFLOPS
+        if all(dep[l] == 4 for l in range(16) if spl[l] > 0):
+            kern = "k_m2l_dma<1> (M2L, two-part split-FP16 MFMA, LDS-DMA staged)"
+        else:
+            kern = "k_m2l_split<2,D,1,true> / k_m2l_dma<1> (M2L, two-part split-FP16 MFMA)"
     m2l_ms = ms[3] / args.steps
     achieved = flops / (m2l_ms * 1e-3) / 1e12 if m2l_ms > 0 else 0.0
     rec = {"precision": precision, "coronaRes": res, "svd_tol": svd_tol if precision != "fp64" else None,

@@ -65,6 +65,7 @@ SIGNATURES = {
     "nnk_fmm_set_graph": (ctypes.c_int32, [_vp, ctypes.c_int32]),
     "nnk_fmm_set_autotune": (ctypes.c_int32, [_vp, ctypes.c_int32]),
     "nnk_fmm_launch_config": (ctypes.c_int32, [_vp, _i32, _i32, ctypes.c_int32]),
+    "nnk_fmm_set_launch_config": (ctypes.c_int32, [_vp, _i32, _i32, ctypes.c_int32]),
     "nnk_fmm_leaf_threads": (ctypes.c_int32, [_vp, _i32]),
     "nnk_op_cache_limit": (ctypes.c_int32, [ctypes.c_int64]),
     "nnk_op_cache_stats": (ctypes.c_int32, [_i64, _i64, _i64, _i64]),

@@ -354,6 +354,17 @@ int32_t nnk_fmm_launch_config(const nnk_fmm *h, int32_t *split, int32_t *depth, 
   });
 }
 
+int32_t nnk_fmm_set_launch_config(nnk_fmm *h, const int32_t *split, const int32_t *depth, int32_t nlev) {
+  return guard([&]() -> int32_t {
+    if (!h || !split || !depth || nlev < 0) return fail(NNK_EARG, "nnk_fmm_set_launch_config: bad argument");
+    for (int l = 0; l < nlev; l++) {
+      if (split[l] == 0) continue;
+      h->f->set_launch_config(l, split[l], depth[l]);
+    }
+    return NNK_OK;
+  });
+}
+
 int32_t nnk_fmm_leaf_threads(const nnk_fmm *h, int32_t *threads) {
   if (!h || !threads) return fail(NNK_EARG, "nnk_fmm_leaf_threads: bad argument");
   *threads = h->f->leaf_threads();

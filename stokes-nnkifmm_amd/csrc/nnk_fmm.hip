@@ -1133,6 +1133,27 @@ void Fmm::launch_config(int level, int *split, int *depth) const {
   }
 }
 
+void Fmm::set_launch_config(int level, int split, int depth) {
+  if (level < 2 || level > maxlev_) throw Error(NNK_EARG, "launch config: level without M2L");
+  LevelPlan &P = plans_[level];
+  const bool mixed = prec_ >= 1;
+  if (mixed ? !P.mx_m2l_jobs : !P.n_down) throw Error(NNK_EARG, "launch config: level without M2L");
+  const size_t per_split = mixed ? (size_t)P.mx_m2l_jobs * P.mx_m2l_maxN * RP_ : (size_t)P.m2l.njobs * P.m2l.maxN * K_;
+  if (split < 1 || split > 64 || (split > 1 && ((size_t)split * per_split > part_elems_ ||
+                                                !(mixed ? P.mx_slot.p : P.m2l_slot.p))))
+    throw Error(NNK_EARG, "launch config: split-K factor out of range");
+  if (!mixed) {
+    if (depth != 0) throw Error(NNK_EARG, "launch config: the FP64 form takes depth 0");
+    P.m2l_split = split;
+  } else {
+    const bool ok = (np_ == 3 && !f16_) ? depth == 2 : (depth == 2 || depth == 3 || (depth == 4 && dma_ok(P, split)));
+    if (!ok) throw Error(NNK_EARG, "launch config: depth not available for this form");
+    P.mx_split = split;
+    P.mx_depth = depth;
+  }
+  gop_.reset();
+}
+
 void Fmm::autotune(hipStream_t s) {
   hipEvent_t a, b;
   NNK_HIP(hipEventCreate(&a));
@@ -1151,12 +1172,13 @@ void Fmm::autotune(hipStream_t s) {
       if (s0 > 1) splits.push_back(std::max(1, s0 / 2));
       if ((size_t)std::min(64, 2 * s0) * per_split <= part_elems_) splits.push_back(std::min(64, 2 * s0));
       std::vector<int> depths = !mixed ? std::vector<int>{0} : np_ == 3 && !f16_ ? std::vector<int>{2}
-                                                                                  : std::vector<int>{2, 3};
+                                                                                  : std::vector<int>{2, 3, 4};
       float best = 1e30f;
       int bs = s0, bd = mixed ? P.mx_depth : 0;
       for (int sp : splits) {
         if (sp > 1 && !(mixed ? P.mx_slot.p : P.m2l_slot.p)) continue;
         for (int dp : depths) {
+          if (dp == 4 && !dma_ok(P, sp)) continue;
           auto run = [&]() {
             if (mixed) launch_m2l_mixed(l, dp, sp, s);
             else gemm(P.m2l, uppot_.as<double>(), K_, chk_.as<double>(), K_, 0, s, 1, sp, &P.m2l_slot);

@@ -57,6 +57,8 @@ struct LevelPlan {
   GemmBatch mx_m2m, mx_upP, mx_l2l, mx_xP, mx_downU, mx_wV;
   DBuf mx_m2l;                // GemmJobBf16[8]
   int mx_split = 1, mx_depth = 3;  // split-K factor and register-ring depth of the M2L launch
+                                   // (depth 4: the DMA-staged FP16 kernel)
+  int mx_maxnb = 0;                // most operator blocks of one M2L job
   DBuf mx_slot;
   std::vector<DBuf> mx_owned;
   int mx_m2l_jobs = 0, mx_m2l_maxN = 0;
@@ -158,6 +160,10 @@ class Fmm {
   void set_autotune(bool on) { tune_ = on; }
   // per level: M2L split-K factor and (mixed forms) register-ring depth
   void launch_config(int level, int *split, int *depth) const;
+  // fixes the M2L launch of one level (split-K factor, register-ring depth
+  // or 4 for the DMA-staged FP16 kernel); throws if the level has no M2L
+  // or the form cannot run it.  A captured graph is dropped.
+  void set_launch_config(int level, int split, int depth);
   // threads per target leaf of the leaf kernel (L2T + P2P + M2T)
   int leaf_threads() const { return leaf_threads_; }
   // leaf-local product (reducedPass.m) on device pointers
@@ -206,6 +212,7 @@ class Fmm {
   void apply_mixed(const double *d_f, double *d_u, hipStream_t s);
   void apply_impl(const double *d_f, double *d_u, hipStream_t s);
   void launch_m2l_mixed(int l, int depth, int split, hipStream_t s);
+  bool dma_ok(const LevelPlan &P, int split) const;
   // time the M2L launch candidates of every level on the current densities
   // (register-ring depth 2/3 for the mixed forms, split-K factor s/2, s, 2s)
   // and keep the fastest

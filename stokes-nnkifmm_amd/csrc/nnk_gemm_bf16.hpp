@@ -264,4 +264,225 @@ __global__ __launch_bounds__(512, 1) void k_m2l_split(const int M, const int KC,
     }
 }
 
+// The split-FP16 form (NP = 2) of the same GEMM with its operand tiles
+// staged by the LDS DMA path (global_load_lds_dwordx4: each lane's 16 bytes
+// go straight from memory to LDS, lane q at byte 16q of the wave's 1 KB
+// fragment) instead of through a register ring and LDS stores.
+//
+// Three LDS stages, each one chunk of A and X in fragment order (32 KB),
+// are separate arrays addressed with static names, so the compiler's wait
+// insertion can tell the stage being read from the ones being filled and
+// does not drain the DMA queue before the LDS reads.  At chunk c each wave
+//   waits until its own DMAs of chunk c have landed (at most the 4 of chunk
+//   c+1 still outstanding), meets the others at a barrier (so every wave's
+//   part of chunk c is in LDS and every wave is done with chunk c-1),
+//   issues the DMAs of chunk c+2 into the stage chunk c-1 used, and
+//   multiplies chunk c.
+// The barrier is s_barrier alone: __syncthreads would also wait for all
+// outstanding loads and undo the look-ahead.
+//
+// The row unscale factors of each operator block (powers of two, or zero
+// for an all-zero row; 2^-126 at the smallest) are read once per workgroup
+// into an LDS table of their exponent bytes, 128 rows x (a1-a0) blocks, so
+// that no global load sits in the main loop apart from the DMAs and the
+// gather slots: a load whose result is needed at the end of a block would
+// make the compiler wait for every DMA issued before it.  Requires
+// a1 - a0 <= 256 active blocks per split (checked on the host).
+// Results are bit-identical to k_m2l_split<2, D, TAG, true>: the same MFMAs
+// in the same order, and the unscale exact (a power of two).
+constexpr int kDmaMaxBlocks = 256;
+
+template <int N>
+__device__ __forceinline__ void wait_vmcnt() {
+  // s_waitcnt with vmcnt = N and the other counters at their maximum
+  __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | (7 << 4) | (15 << 8));
+}
+
+template <int TAG>
+__global__ __launch_bounds__(512, 1) void k_m2l_dma(const int M, const int KC, const int RT,
+                                                    const uint16_t *__restrict__ X, const int zrow,
+                                                    double *__restrict__ C, const int ldc,
+                                                    const GemmJobBf16 *__restrict__ jobs, const int njobs,
+                                                    const int nrb, const int nsplit, double *__restrict__ P,
+                                                    const int maxN, const double *__restrict__ xun,
+                                                    const int beta) {
+  constexpr int NP = 2, BN = 128, TM = 4, TN = 2, TILE = 512;
+  typedef const __attribute__((address_space(1))) void *gvp;
+  typedef __attribute__((address_space(3))) void *lvp;
+  typedef const __attribute__((address_space(1))) f32x4 *gf32x4p;
+  typedef uint16_t Stage[2][NP][8][TILE];  // [A or X][part][fragment tile][lane * 8 + e]
+  __shared__ __attribute__((aligned(1024))) Stage St0;
+  __shared__ __attribute__((aligned(1024))) Stage St1;
+  __shared__ __attribute__((aligned(1024))) Stage St2;
+  __shared__ __attribute__((aligned(16))) uint32_t ue[kDmaMaxBlocks * 32];
+  __shared__ int act[512];
+  __shared__ const uint16_t *aptr[512];
+  __shared__ const float *uptr[512];
+
+  const int bid = blockIdx.x;
+  const int jb = bid % njobs;
+  int rr = bid / njobs;
+  const int sp = rr % nsplit;
+  rr /= nsplit;
+  const int rb = rr % nrb, ct = rr / nrb;
+  const GemmJobBf16 J = jobs[jb];
+  const int col0 = ct * BN;
+  if (col0 >= J.N) return;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 2, wn = wave & 3;
+  const int lr = lane & 15, lg = lane >> 4;
+
+  const int t0 = J.tptr[ct];
+  const int nact = J.tptr[ct + 1] - t0;
+  for (int i = tid; i < nact; i += 512) {
+    const int b = J.tlist[t0 + i];
+    act[i] = b;
+    aptr[i] = J.A[b];
+    uptr[i] = J.U[b];
+  }
+  __syncthreads();
+  const int a0 = (int)((int64_t)nact * sp / nsplit), a1 = (int)((int64_t)nact * (sp + 1) / nsplit);
+  const int total = (a1 - a0) * KC;
+
+  // exponent table: word 32 q + w holds the exponent bytes of rows
+  // 4w .. 4w+3 (of this 128-row tile) of active block a0 + q
+  {
+    const int nq = (a1 - a0) * 32;
+    for (int i0 = 0; i0 < nq; i0 += 4 * 512) {
+      f32x4 v[4];
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        const int i = i0 + k * 512 + tid;
+        if (i < nq) v[k] = *(gf32x4p)(uptr[a0 + (i >> 5)] + rb * 128 + (i & 31) * 4);
+      }
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        const int i = i0 + k * 512 + tid;
+        if (i < nq) {
+          uint32_t e4 = 0;
+#pragma unroll
+          for (int r = 0; r < 4; r++) e4 |= ((__float_as_uint(v[k][r]) >> 23) & 255u) << (8 * r);
+          ue[i] = e4;
+        }
+      }
+    }
+    __syncthreads();
+  }
+
+  // this thread's DMA slot: fragment tile ti = its wave, lane slot tq
+  const int ti = wave, tq = lane;
+  const int my_col = col0 + ti * 16 + (tq & 15);
+  const int kof = 8 * (tq >> 4);
+  const size_t aoff = (size_t)(rb * 8 + ti) * TILE + (size_t)tq * 8;
+  const size_t slab = (size_t)RT * TILE;
+  const gi32p gidx = (gi32p)J.gidx;
+  auto gload = [&](int bi) -> int {
+    const int bc = bi < a1 ? bi : a1 - 1;
+    const int cc = my_col < J.N ? my_col : J.N - 1;
+    return gidx[(size_t)act[bc] * J.N + cc];
+  };
+  auto gslot = [&](int raw, int bi) -> int { return (bi < a1 && my_col < J.N && raw >= 0) ? raw : zrow; };
+  int fblk = -1, gcur = zrow, gnext = total > 0 ? gload(a0) : -1;
+
+  // DMAs of chunk c (clamped to the last chunk, so that every step issues
+  // exactly 2 NP of them and the wait counts stay static)
+  auto fetch = [&](int c, Stage &Sd) {
+    const int cq = c < total ? c : total - 1;
+    const int q = cq / KC, kc = cq - q * KC, bi = a0 + q;
+    if (bi != fblk) {
+      fblk = bi;
+      gcur = gslot(gnext, bi);
+      gnext = gload(bi + 1);
+    }
+    const uint16_t *Ab = aptr[bi] + aoff + (size_t)kc * NP * slab;
+    const uint16_t *Xb = X + ((size_t)gcur * KC + kc) * NP * 32 + kof;
+#pragma unroll
+    for (int p = 0; p < NP; p++) {
+      __builtin_amdgcn_global_load_lds((gvp)(Ab + p * slab), (lvp)&Sd[0][p][ti][0], 16, 0, 0);
+      __builtin_amdgcn_global_load_lds((gvp)(Xb + p * 32), (lvp)&Sd[1][p][ti][0], 16, 0, 0);
+    }
+  };
+
+  f32x4 acc[TM][TN];
+  double acc64[TM][TN][4];
+#pragma unroll
+  for (int i = 0; i < TM; i++)
+#pragma unroll
+    for (int j = 0; j < TN; j++) {
+      acc[i][j] = f32x4{0, 0, 0, 0};
+#pragma unroll
+      for (int r = 0; r < 4; r++) acc64[i][j][r] = 0.0;
+    }
+  auto compute = [&](const Stage &Sc) {
+    f16x8 a[NP][TM], x[NP][TN];
+#pragma unroll
+    for (int p = 0; p < NP; p++) {
+#pragma unroll
+      for (int i = 0; i < TM; i++) a[p][i] = *reinterpret_cast<const f16x8 *>(&Sc[0][p][wm * 4 + i][lane * 8]);
+#pragma unroll
+      for (int j = 0; j < TN; j++) x[p][j] = *reinterpret_cast<const f16x8 *>(&Sc[1][p][wn * 2 + j][lane * 8]);
+    }
+#pragma unroll
+    for (int i = 0; i < TM; i++)
+#pragma unroll
+      for (int j = 0; j < TN; j++) {
+        f32x4 t = acc[i][j];
+        t = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[1][i], x[0][j], t, 0, 0, 0);
+        t = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[0][i], x[1][j], t, 0, 0, 0);
+        t = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[0][i], x[0][j], t, 0, 0, 0);
+        acc[i][j] = t;
+      }
+  };
+  // promotion of block q's FP32 sums, each row times its unscale factor
+  auto promote = [&](int q) {
+#pragma unroll
+    for (int i = 0; i < TM; i++) {
+      const uint32_t e4 = ue[q * 32 + wm * 16 + lg + 4 * i];
+#pragma unroll
+      for (int j = 0; j < TN; j++) {
+#pragma unroll
+        for (int r = 0; r < 4; r++)
+          acc64[i][j][r] += (double)acc[i][j][r] * (double)__uint_as_float(((e4 >> (8 * r)) & 255u) << 23);
+        acc[i][j] = f32x4{0, 0, 0, 0};
+      }
+    }
+  };
+  auto step = [&](int cc, const Stage &Sc, Stage &Sd) {
+    wait_vmcnt<2 * NP>();
+    __builtin_amdgcn_s_barrier();
+    fetch(cc + 2, Sd);
+    if (cc < total) {
+      compute(Sc);
+      if ((cc + 1) % KC == 0) promote(cc / KC);
+    }
+  };
+  if (total > 0) {
+    fetch(0, St0);
+    fetch(1, St1);
+    for (int c = 0; c < total; c += 3) {
+      step(c, St0, St2);
+      step(c + 1, St1, St0);
+      step(c + 2, St2, St1);
+    }
+    wait_vmcnt<0>();  // no DMA may land after the workgroup ends
+  }
+
+  const int m0 = rb * 128 + wm * 64;
+  const double xs = *xun;
+#pragma unroll
+  for (int i = 0; i < TM; i++)
+#pragma unroll
+    for (int j = 0; j < TN; j++) {
+      const int col = col0 + wn * 32 + j * 16 + lr;
+      if (col >= J.N) continue;
+      double *out = nsplit > 1 ? P + ((size_t)(sp * njobs + jb) * maxN + col) * ldc
+                               : C + (size_t)J.cidx[col] * ldc;
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int row = m0 + i * 16 + 4 * lg + r;
+        if (row < M) out[row] = (beta && nsplit == 1 ? out[row] : 0.0) + acc64[i][j][r] * xs;
+      }
+    }
+}
+
 }  // namespace nnk

@@ -122,7 +122,9 @@ __global__ void k_op_rowscale(const double *__restrict__ CT, int ldct, int R, in
   double a = 0.0;
   if (m < R)
     for (int n = 0; n < R; n++) a = fmax(a, fabs(CT[((size_t)j * R + n) * ldct + m]));
-  un[idx] = a > 0 ? (float)(pow2_floor(a) / 16384.0) : 0.0f;
+  // a power of two, normal in FP32 (the DMA-staged kernel keeps only its
+  // exponent byte)
+  un[idx] = a > 0 ? (float)fmax(pow2_floor(a) / 16384.0, 0x1p-126) : 0.0f;
 }
 
 // FP16 operator parts in fragment order (as k_split_op), rows scaled by
@@ -155,7 +157,9 @@ __global__ void k_op_rowscale_g(const double *__restrict__ A, int64_t ostride, i
   double a = 0.0;
   if (m < Mr)
     for (int n = 0; n < Kc; n++) a = fmax(a, fabs(A[j * ostride + (int64_t)n * ld + m]));
-  un[idx] = a > 0 ? (float)(pow2_floor(a) / 16384.0) : 0.0f;
+  // a power of two, normal in FP32 (the DMA-staged kernel keeps only its
+  // exponent byte)
+  un[idx] = a > 0 ? (float)fmax(pow2_floor(a) / 16384.0, 0x1p-126) : 0.0f;
 }
 
 __global__ void k_split_op_hg(const double *__restrict__ A, int64_t ostride, int ld, int Mr, int Kc, int nops,
@@ -537,8 +541,10 @@ void Fmm::plan_mixed() {
         for (size_t c = 0; c < H.cls[a].size(); c++) slot[(size_t)j * mN + c] = H.cls[a][c];
         j++;
       }
+      P.mx_maxnb = 0;
+      for (int j = 0; j < nj; j++) P.mx_maxnb = std::max(P.mx_maxnb, mj[j].nb);
       P.mx_split = split_for((int64_t)nj * (RPm_ / MBM) * ((mN + MBN - 1) / MBN), minnb);
-      P.mx_depth = (np_ == 3 && !f16_) ? 2 : 3;
+      P.mx_depth = (np_ == 3 && !f16_) ? 2 : dma_ok(P, P.mx_split) ? 4 : 3;
       P.mx_slot.upload(slot);  // also for the autotuner's split candidates
       reserve_part((size_t)std::min(64, 2 * P.mx_split) * nj * mN * RP_);
     }
@@ -587,8 +593,15 @@ void Fmm::plan_mixed() {
   upall_.upload(upall);
 }
 
-// M2L of level l in the mixed form with register-ring depth `depth` (2 or 3)
-// and split-K factor `split` (the part buffer must hold it)
+// whether the DMA-staged FP16 M2L kernel can run level plan P with split-K
+// factor `split`: its unscale table holds kDmaMaxBlocks blocks per split
+bool Fmm::dma_ok(const LevelPlan &P, int split) const {
+  return f16_ && split >= 1 && (P.mx_maxnb + split - 1) / split <= kDmaMaxBlocks;
+}
+
+// M2L of level l in the mixed form with register-ring depth `depth` (2 or
+// 3; 4 selects the DMA-staged FP16 kernel) and split-K factor `split` (the
+// part buffer must hold it)
 void Fmm::launch_m2l_mixed(int l, int depth, int split, hipStream_t s) {
   LevelPlan &P = plans_[l];
   if (!P.mx_m2l_jobs) return;
@@ -604,7 +617,10 @@ void Fmm::launch_m2l_mixed(int l, int depth, int split, hipStream_t s) {
                        P.mx_m2l.as<GemmJobBf16>(), nj, nrb, split, pp, mN, xun_.as<double>(), 0);
   };
   if (f16_) {
-    if (depth == 2) launch(k_m2l_split<2, 2, 1, true>);
+    if (depth == 4) {
+      if (!dma_ok(P, split)) throw Error(NNK_ESTATE, "mixed M2L: too many operator blocks for the DMA-staged kernel");
+      launch(k_m2l_dma<1>);
+    } else if (depth == 2) launch(k_m2l_split<2, 2, 1, true>);
     else launch(k_m2l_split<2, 3, 1, true>);
   } else if (np_ == 3) {
     launch(k_m2l_split<3, 2, 1, false>);  // a third register stage would spill
