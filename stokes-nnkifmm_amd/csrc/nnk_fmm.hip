@@ -277,8 +277,6 @@ __global__ __launch_bounds__(T) void k_leaf(LeafArgs a) {
   }
 }
 
-constexpr int GBM = 160, GBK = 16;
-
 // ---------------------------------------------------------------------------
 // host side
 
@@ -394,6 +392,17 @@ int Fmm::split_for(int64_t tiles, int nb) {
   return std::max(1, std::min(s, 64));
 }
 
+int Fmm::split_m2m(int64_t tiles, size_t part_elems) {
+  // the levels near the root have few parents: their M2M is a handful of
+  // workgroups each walking all 8 children; split the children over up to
+  // 8 parts (aiming at ~1,024 workgroups), keeping the partial tiles under
+  // 256 MB
+  if (tiles >= 512) return 1;
+  int s = (int)std::min<int64_t>(8, (1024 + tiles - 1) / tiles);
+  while (s > 1 && (size_t)s * part_elems * sizeof(double) > ((size_t)256 << 20)) s--;
+  return std::max(1, s);
+}
+
 void Fmm::reserve_part(size_t elems) {
   if (elems <= part_elems_) return;
   part_.alloc(sizeof(double) * elems);
@@ -407,9 +416,16 @@ void Fmm::split_reduce(const DBuf &slot, int ncols, int M, int ldc, int nsplit, 
 }
 
 void Fmm::gemm_dims(const GemmBatch &b, const double *X, int ldx, double *C, int ldc, int beta,
-                    hipStream_t s, int M, int K, int Mpad, int Kpad) {
+                    hipStream_t s, int M, int K, int Mpad, int Kpad, int nsplit, const DBuf *slot) {
   if (b.njobs == 0 || b.maxN == 0) return;
   if (Mpad % GBM || Kpad % GBK) throw Error(NNK_ESTATE, "gemm_dims: bad padding");
+  if (nsplit > 1) {
+    if (beta != 0 || !slot) throw Error(NNK_ESTATE, "split GEMM needs beta = 0 and a slot list");
+    if ((size_t)nsplit * b.njobs * b.maxN * ldc > part_elems_) throw Error(NNK_ESTATE, "split buffer too small");
+    launch_gemm<64, 0>(Mpad / GBM, M, K, Mpad, Kpad, X, ldx, C, ldc, 0, b, s, nsplit, part_.as<double>());
+    split_reduce(*slot, b.njobs * b.maxN, M, ldc, nsplit, C, s);
+    return;
+  }
   launch_gemm<64, 0>(Mpad / GBM, M, K, Mpad, Kpad, X, ldx, C, ldc, beta, b, s);
 }
 
@@ -878,6 +894,12 @@ void Fmm::plan() {
       add_job(P.m2m, jobs, A, &g, &nl, (int)nl.size());
       finish(P.m2m, jobs);
       hl_[l].m2m_g = g;
+      const int64_t tiles = (int64_t)(Mpad_ / GBM) * (((int64_t)nl.size() + 63) / 64);
+      P.m2m_split = split_m2m(tiles, nl.size() * (size_t)K_);
+      if (P.m2m_split > 1) {
+        P.m2m_slot.upload(nl);
+        reserve_part((size_t)P.m2m_split * nl.size() * K_);
+      }
     }
     hl_[l].up_leaves = up;
     hl_[l].up_nonleaves = nl;
@@ -1261,7 +1283,7 @@ void Fmm::apply_impl(const double *d_f, double *d_u, hipStream_t s) {
   for (int l = maxlev_; l >= 2; l--) {
     LevelPlan &P = plans_[l];
     if (!P.n_up) continue;
-    gemm(P.m2m, up, K_, chk, K_, 0, s);
+    gemm(P.m2m, up, K_, chk, K_, 0, s, 0, P.m2m_split, &P.m2m_slot);
     gemm(P.up_solve1, chk, K_, tmp, K_, 0, s);
     gemm(P.up_solve2, tmp, K_, up, K_, 0, s);
   }

@@ -16,6 +16,9 @@ namespace nnk {
 
 struct RedArgs;
 
+// rows per workgroup tile and k depth per chunk of the FP64 gather GEMM
+constexpr int GBM = 160, GBK = 16;
+
 struct GemmJob {
   const double *const *A;  // nb operator pointers, each At[Kpad][Mpad] (k-major)
   const int32_t *gidx;     // nb x N gather slots (-1 = none), or null = column index
@@ -42,6 +45,8 @@ struct LevelPlan {
   int n_up = 0;
   DBuf up_nodes;              // slots (id-1)
   GemmBatch m2m;              // children -> parent check surface
+  int m2m_split = 1;          // split-K factor of the M2M launch (one or two children per part)
+  DBuf m2m_slot;              // column -> parent slot, for its split reduction
   GemmBatch up_solve1, up_solve2;
   // downward
   int n_down = 0;
@@ -55,6 +60,8 @@ struct LevelPlan {
   DBuf x_nodes;               // nodes with a non-empty X list
   // mixed form
   GemmBatch mx_m2m, mx_upP, mx_l2l, mx_xP, mx_downU, mx_wV;
+  int mx_m2m_split = 1;       // as m2m_split, for the mixed form's M2M
+  DBuf mx_m2m_slot;
   DBuf mx_m2l;                // GemmJobBf16[8]
   int mx_split = 1, mx_depth = 3;  // split-K factor and register-ring depth of the M2L launch
                                    // (depth 4: the DMA-staged FP16 kernel)
@@ -203,10 +210,13 @@ class Fmm {
   // split-K factor for a launch of `tiles` workgroups over blocks of nb
   // operators, and the reduction of its partial tiles into C
   static int split_for(int64_t tiles, int nb);
+  // split-K factor of an M2M launch (8 child blocks) of `tiles` workgroups
+  // whose partial tiles take `part_elems` doubles per split
+  static int split_m2m(int64_t tiles, size_t part_elems);
   void split_reduce(const DBuf &slot, int ncols, int M, int ldc, int nsplit, double *C, hipStream_t s);
   void reserve_part(size_t elems);
   void gemm_dims(const GemmBatch &b, const double *X, int ldx, double *C, int ldc, int beta,
-                 hipStream_t s, int M, int K, int Mpad, int Kpad);
+                 hipStream_t s, int M, int K, int Mpad, int Kpad, int nsplit = 1, const DBuf *slot = nullptr);
   void build_mixed();
   void plan_mixed();
   void apply_mixed(const double *d_f, double *d_u, hipStream_t s);

@@ -457,6 +457,12 @@ void Fmm::plan_mixed() {
       P.mx_m2m.njobs = 1;
       P.mx_m2m.maxN = N;
       P.mx_m2m.jobs.upload(jobs);
+      const int64_t tiles = (int64_t)(MpadR_ / GBM) * (((int64_t)N + 63) / 64);
+      P.mx_m2m_split = split_m2m(tiles, (size_t)N * RP_);
+      if (P.mx_m2m_split > 1) {
+        P.mx_m2m_slot.upload(H.up_nonleaves);
+        reserve_part((size_t)P.mx_m2m_split * N * RP_);
+      }
     }
     job1(P.mx_upP, X.Pup.as<double>(), H.up_leaves, H.up_leaves);
     job1(P.mx_wV, X.Vr.as<double>(), H.wnodes, H.wnodes);
@@ -673,7 +679,7 @@ void Fmm::apply_mixed(const double *d_f, double *d_u, hipStream_t s) {
   mark(2, true, s);
   for (int l = maxlev_; l >= 2; l--) {
     LevelPlan &P = plans_[l];
-    gemm_dims(P.mx_m2m, c, RP_, c, RP_, 0, s, R, R, MpadR_, KpadR_);
+    gemm_dims(P.mx_m2m, c, RP_, c, RP_, 0, s, R, R, MpadR_, KpadR_, P.mx_m2m_split, &P.mx_m2m_slot);
     gemm_dims(P.mx_upP, chk, K, c, RP_, 0, s, R, K, MpadR_, Kpad_);
   }
   if (n_upall_) {
