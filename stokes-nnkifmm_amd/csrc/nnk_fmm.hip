@@ -100,9 +100,13 @@ struct SurfArgs {
 // Check potential on a node surface from the sources of a set of leaves
 // (S2M: the leaf itself onto its down surface; P2L: the X list onto the up
 // surface).  One 512-thread block per node; sources staged in LDS 512 at a
-// time.  The nq surface points are taken in chunks of c (chosen to keep the
-// most lanes busy), each point shared by nsub = 512 / c lanes that split the
-// sources; their partial sums are added in a fixed order.
+// time.  The nq surface points are taken in chunks of c, each point shared
+// by nsub = 512 / c lanes that split the sources; their partial sums are
+// added in a fixed order.  Every chunk restages the sources (a global-load
+// latency and two barriers), so c is chosen by a cost model in units of
+// one pair per lane: chunks x (sources per lane + 24).  A leaf's ~32
+// sources onto 386 points (res 9) then take one chunk of 386 lanes rather
+// than seven chunks of 56 points x 9 lanes.
 constexpr int kSurfThreads = 512;
 
 template <bool F32>
@@ -115,13 +119,20 @@ __global__ __launch_bounds__(kSurfThreads) void k_surf_eval(SurfArgs a) {
   const int tid = threadIdx.x;
   int64_t lb = 0, le = 1;
   if (a.lptr) { lb = a.lptr[n]; le = a.lptr[n + 1]; }
-  // chunk size: the c = ceil(nq / k), k = 1..8, that keeps the most lanes busy
-  int csz = min(a.nq, T), best = -1;
+  // chunk size: the c = ceil(nq / k), k = 1..8, of least modelled cost
+  int nsrc = 0;
+  for (int64_t e = lb; e < le; e++) {
+    const int leaf = a.lptr ? a.lidx[e] : n;
+    nsrc += a.src_ptr[leaf + 1] - a.src_ptr[leaf];
+  }
+  int csz = min(a.nq, T);
+  int64_t best = -1;
   for (int k = 1; k <= 8; k++) {
     const int c = (a.nq + k - 1) / k;
     if (c > T) continue;
-    const int used = c * (T / c);
-    if (used > best) { best = used; csz = c; }
+    const int nsub = T / c;
+    const int64_t cost = (int64_t)((a.nq + c - 1) / c) * ((nsrc + nsub - 1) / nsub + 24);
+    if (best < 0 || cost < best) { best = cost; csz = c; }
   }
   for (int p0 = 0; p0 < a.nq; p0 += csz) {
     const int ntc = min(csz, a.nq - p0);
@@ -196,13 +207,35 @@ struct LeafArgs {
 // Each target is shared by nsub lanes that split the sources; partial sums
 // are combined in a fixed order.  T threads per workgroup (the autotuner
 // picks 128, 256 or 512: more lanes per target, or more targets in flight).
+// The sources of the U-list leaves are staged as one concatenated list, T
+// at a time (a leaf holds ~32: staging leaf by leaf would pay a load
+// latency and two barriers per leaf); lists longer than kLeafMaxU leaves
+// are staged leaf by leaf.
+constexpr int kLeafMaxU = 512;
+
 template <bool F32, int T>
 __global__ __launch_bounds__(T) void k_leaf(LeafArgs a) {
   const int n = a.leaves[blockIdx.x];
   __shared__ double sp[T * 6];
   __shared__ double red[T * 3];
+  __shared__ int upre[kLeafMaxU + 1];  // source-count prefix over the U list
   const int tid = threadIdx.x;
   const int tb = a.tgt_ptr[n], nt = a.tgt_ptr[n + 1] - tb;
+  const int64_t u0 = a.far ? a.uptr[n] : 0, u1 = a.far ? a.uptr[n + 1] : 0;
+  const int nu = (int)(u1 - u0);
+  const bool ucat = a.far && nu <= kLeafMaxU;
+  if (ucat) {
+    for (int e = tid; e < nu; e += T) {
+      const int lf = a.uidx[u0 + e];
+      upre[e + 1] = a.src_ptr[lf + 1] - a.src_ptr[lf];
+    }
+    __syncthreads();
+    if (tid == 0) {
+      upre[0] = 0;
+      for (int e = 1; e <= nu; e++) upre[e] += upre[e - 1];
+    }
+    __syncthreads();
+  }
   for (int t0 = 0; t0 < nt; t0 += T) {
     const int ntc = min(T, nt - t0);
     const int nsub = T / ntc;
@@ -252,10 +285,37 @@ __global__ __launch_bounds__(T) void k_leaf(LeafArgs a) {
         __syncthreads();
       }
     };
+    // the U-list sources as one list: entry g lies in leaf e with
+    // upre[e] <= g < upre[e + 1] (binary search; empty leaves are skipped)
+    auto usources = [&]() {
+      const int tot = upre[nu];
+      for (int g0 = 0; g0 < tot; g0 += T) {
+        const int cnt = min(T, tot - g0);
+        if (tid < cnt) {
+          const int g = g0 + tid;
+          int lo = 0, hi = nu;
+          while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (upre[mid] <= g) lo = mid;
+            else hi = mid;
+          }
+          const int s = a.src_ptr[a.uidx[u0 + lo]] + (g - upre[lo]);
+          for (int d = 0; d < 3; d++) {
+            sp[6 * tid + d] = a.src_pos[3 * (size_t)s + d];
+            sp[6 * tid + 3 + d] = a.src_q[3 * (size_t)s + d];
+          }
+        }
+        __syncthreads();
+        consume(cnt);
+        __syncthreads();
+      }
+    };
     if (a.far) {
       surface(n, 1, a.downpot);
       for (int64_t e = a.wptr[n]; e < a.wptr[n + 1]; e++) surface(a.widx[e], 0, a.uppot);
-      for (int64_t e = a.uptr[n]; e < a.uptr[n + 1]; e++) sources(a.uidx[e]);
+      if (ucat) usources();
+      else
+        for (int64_t e = u0; e < u1; e++) sources(a.uidx[e]);
     } else {
       sources(n);
     }
