@@ -113,7 +113,7 @@ template <bool F32>
 __global__ __launch_bounds__(kSurfThreads) void k_surf_eval(SurfArgs a) {
   constexpr int T = kSurfThreads;
   const int n = a.nodes[blockIdx.x];
-  __shared__ double sp[T * 6];
+  __shared__ SrcStage<F32, T> st;
   __shared__ double red[T * 3];
   const double *box = a.corners + 6 * (size_t)(n - 1);
   const int tid = threadIdx.x;
@@ -149,17 +149,13 @@ __global__ __launch_bounds__(kSurfThreads) void k_surf_eval(SurfArgs a) {
       for (int s0 = sb; s0 < se; s0 += T) {
         const int cnt = min(T, se - s0);
         if (tid < cnt) {
-          const int s = s0 + tid;
-          for (int d = 0; d < 3; d++) {
-            sp[6 * tid + d] = a.src_pos[3 * (size_t)s + d];
-            sp[6 * tid + 3 + d] = a.src_q[3 * (size_t)s + d];
-          }
+          const size_t s = (size_t)(s0 + tid);
+          st.put(tid, a.src_pos[3 * s], a.src_pos[3 * s + 1], a.src_pos[3 * s + 2], a.src_q[3 * s],
+                 a.src_q[3 * s + 1], a.src_q[3 * s + 2]);
         }
         __syncthreads();
         if (act)
-          for (int j = sub; j < cnt; j += nsub)
-            stk<F32>(tx, ty, tz, sp[6 * j], sp[6 * j + 1], sp[6 * j + 2], sp[6 * j + 3],
-                     sp[6 * j + 4], sp[6 * j + 5], a.e2, ux, uy, uz);
+          for (int j = sub; j < cnt; j += nsub) st.acc(j, tx, ty, tz, a.e2, ux, uy, uz);
         __syncthreads();
       }
     }
@@ -216,7 +212,7 @@ constexpr int kLeafMaxU = 512;
 template <bool F32, int T>
 __global__ __launch_bounds__(T) void k_leaf(LeafArgs a) {
   const int n = a.leaves[blockIdx.x];
-  __shared__ double sp[T * 6];
+  __shared__ SrcStage<F32, T> st;
   __shared__ double red[T * 3];
   __shared__ int upre[kLeafMaxU + 1];  // source-count prefix over the U list
   const int tid = threadIdx.x;
@@ -249,9 +245,7 @@ __global__ __launch_bounds__(T) void k_leaf(LeafArgs a) {
     double ux = 0, uy = 0, uz = 0;
     auto consume = [&](int cnt) {
       if (act)
-        for (int j = sub; j < cnt; j += nsub)
-          stk<F32>(tx, ty, tz, sp[6 * j], sp[6 * j + 1], sp[6 * j + 2], sp[6 * j + 3],
-                   sp[6 * j + 4], sp[6 * j + 5], a.e2, ux, uy, uz);
+        for (int j = sub; j < cnt; j += nsub) st.acc(j, tx, ty, tz, a.e2, ux, uy, uz);
     };
     auto surface = [&](int node, int down, const double *dens) {
       const double *box = a.corners + 6 * (size_t)(node - 1);
@@ -260,9 +254,8 @@ __global__ __launch_bounds__(T) void k_leaf(LeafArgs a) {
         if (tid < cnt) {
           double x, y, z;
           surf_point(box, down, a.surf_idx[p0 + tid], a.n1, x, y, z);
-          sp[6 * tid] = x; sp[6 * tid + 1] = y; sp[6 * tid + 2] = z;
           const double *q = dens + (size_t)(node - 1) * a.K + 3 * (p0 + tid);
-          sp[6 * tid + 3] = q[0]; sp[6 * tid + 4] = q[1]; sp[6 * tid + 5] = q[2];
+          st.put(tid, x, y, z, q[0], q[1], q[2]);
         }
         __syncthreads();
         consume(cnt);
@@ -274,11 +267,9 @@ __global__ __launch_bounds__(T) void k_leaf(LeafArgs a) {
       for (int s0 = sb; s0 < se; s0 += T) {
         const int cnt = min(T, se - s0);
         if (tid < cnt) {
-          const int s = s0 + tid;
-          for (int d = 0; d < 3; d++) {
-            sp[6 * tid + d] = a.src_pos[3 * (size_t)s + d];
-            sp[6 * tid + 3 + d] = a.src_q[3 * (size_t)s + d];
-          }
+          const size_t s = (size_t)(s0 + tid);
+          st.put(tid, a.src_pos[3 * s], a.src_pos[3 * s + 1], a.src_pos[3 * s + 2], a.src_q[3 * s],
+                 a.src_q[3 * s + 1], a.src_q[3 * s + 2]);
         }
         __syncthreads();
         consume(cnt);
@@ -299,11 +290,9 @@ __global__ __launch_bounds__(T) void k_leaf(LeafArgs a) {
             if (upre[mid] <= g) lo = mid;
             else hi = mid;
           }
-          const int s = a.src_ptr[a.uidx[u0 + lo]] + (g - upre[lo]);
-          for (int d = 0; d < 3; d++) {
-            sp[6 * tid + d] = a.src_pos[3 * (size_t)s + d];
-            sp[6 * tid + 3 + d] = a.src_q[3 * (size_t)s + d];
-          }
+          const size_t s = (size_t)(a.src_ptr[a.uidx[u0 + lo]] + (g - upre[lo]));
+          st.put(tid, a.src_pos[3 * s], a.src_pos[3 * s + 1], a.src_pos[3 * s + 2], a.src_q[3 * s],
+                 a.src_q[3 * s + 1], a.src_q[3 * s + 2]);
         }
         __syncthreads();
         consume(cnt);

@@ -56,11 +56,12 @@ __device__ __forceinline__ void stokeslet_acc(double tx, double ty, double tz, d
 
 // Mixed form: the separation r = t - s is formed in FP64 (so near pairs keep
 // their digits), the kernel is evaluated in FP32 with the hardware reciprocal
-// square root, and the result is accumulated in FP64.
-__device__ __forceinline__ void stokeslet_acc_f32(double tx, double ty, double tz, double sx,
-                                                  double sy, double sz, double qx, double qy,
-                                                  double qz, double e2, double &ux, double &uy,
-                                                  double &uz) {
+// square root, and the result is accumulated in FP64.  The strength comes in
+// FP32 (rounded once, where the source is staged).
+__device__ __forceinline__ void stokeslet_acc_f32q(double tx, double ty, double tz, double sx,
+                                                   double sy, double sz, float fx, float fy,
+                                                   float fz, double e2, double &ux, double &uy,
+                                                   double &uz) {
   const float rx = (float)(tx - sx), ry = (float)(ty - sy), rz = (float)(tz - sz);
   const float e2f = (float)e2;
   const float r2 = rx * rx + ry * ry + rz * rz;
@@ -68,12 +69,48 @@ __device__ __forceinline__ void stokeslet_acc_f32(double tx, double ty, double t
   const float inv = __builtin_amdgcn_rsqf(re);
   const float inv3 = inv * inv * inv;
   const float iso = (re + e2f) * inv3;
-  const float fx = (float)qx, fy = (float)qy, fz = (float)qz;
   const float rq = (rx * fx + ry * fy + rz * fz) * inv3;
   ux += (double)(iso * fx + rq * rx);
   uy += (double)(iso * fy + rq * ry);
   uz += (double)(iso * fz + rq * rz);
 }
+
+__device__ __forceinline__ void stokeslet_acc_f32(double tx, double ty, double tz, double sx,
+                                                  double sy, double sz, double qx, double qy,
+                                                  double qz, double e2, double &ux, double &uy,
+                                                  double &uz) {
+  stokeslet_acc_f32q(tx, ty, tz, sx, sy, sz, (float)qx, (float)qy, (float)qz, e2, ux, uy, uz);
+}
+
+// LDS stage of up to T sources: positions in FP64; strengths in FP64, or,
+// for the FP32-math form, rounded to FP32 once when staged (the values the
+// pair routine would otherwise round per pair).  acc(j, ...) adds source j.
+template <bool F32, int T> struct SrcStage;
+template <int T> struct SrcStage<false, T> {
+  double p[3 * T], q[3 * T];
+  __device__ __forceinline__ void put(int i, double x, double y, double z, double qx, double qy, double qz) {
+    p[3 * i] = x; p[3 * i + 1] = y; p[3 * i + 2] = z;
+    q[3 * i] = qx; q[3 * i + 1] = qy; q[3 * i + 2] = qz;
+  }
+  __device__ __forceinline__ void acc(int j, double tx, double ty, double tz, double e2, double &ux,
+                                      double &uy, double &uz) const {
+    stokeslet_acc(tx, ty, tz, p[3 * j], p[3 * j + 1], p[3 * j + 2], q[3 * j], q[3 * j + 1], q[3 * j + 2], e2,
+                  ux, uy, uz);
+  }
+};
+template <int T> struct SrcStage<true, T> {
+  double p[3 * T];
+  float q[4 * T];
+  __device__ __forceinline__ void put(int i, double x, double y, double z, double qx, double qy, double qz) {
+    p[3 * i] = x; p[3 * i + 1] = y; p[3 * i + 2] = z;
+    q[4 * i] = (float)qx; q[4 * i + 1] = (float)qy; q[4 * i + 2] = (float)qz;
+  }
+  __device__ __forceinline__ void acc(int j, double tx, double ty, double tz, double e2, double &ux,
+                                      double &uy, double &uz) const {
+    stokeslet_acc_f32q(tx, ty, tz, p[3 * j], p[3 * j + 1], p[3 * j + 2], q[4 * j], q[4 * j + 1], q[4 * j + 2],
+                       e2, ux, uy, uz);
+  }
+};
 
 template <bool F32>
 __device__ __forceinline__ void stk(double tx, double ty, double tz, double sx, double sy, double sz,
