@@ -206,7 +206,7 @@ int32_t nnk_fmm_launch_config(const nnk_fmm *h, int32_t *split, int32_t *depth, 
  *           split: split-K factor 1..64 within the reserved part buffer;
  *           depth: 0 for the FP64 form; 2 or 3 (register-ring depth) for
  *           the mixed forms, 2 only for "mixed3"; 4 selects the DMA-staged
- *           kernel of the "mixedh" form (at most 256 operator blocks per
+ *           kernel of the "mixedh" form (at most 320 operator blocks per
  *           split-K part).  Results are bit-identical across the depths of
  *           one split.  NNK_EARG for a level without M2L or a value the
  *           form cannot run; a captured graph is recaptured on next use. */

@@ -287,10 +287,11 @@ __global__ __launch_bounds__(512, 1) void k_m2l_split(const int M, const int KC,
 // that no global load sits in the main loop apart from the DMAs and the
 // gather slots: a load whose result is needed at the end of a block would
 // make the compiler wait for every DMA issued before it.  Requires
-// a1 - a0 <= 256 active blocks per split (checked on the host).
+// a1 - a0 <= 320 active blocks per split (checked on the host; the 316
+// offsets of a level fit without a split).
 // Results are bit-identical to k_m2l_split<2, D, TAG, true>: the same MFMAs
 // in the same order, and the unscale exact (a power of two).
-constexpr int kDmaMaxBlocks = 256;
+constexpr int kDmaMaxBlocks = 320;
 
 template <int N>
 __device__ __forceinline__ void wait_vmcnt() {

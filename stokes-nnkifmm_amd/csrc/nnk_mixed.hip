@@ -475,20 +475,62 @@ void Fmm::plan_mixed() {
     // split-BF16 M2L jobs (one per child class) and FP64 L2L jobs
     std::vector<GemmJobBf16> mj;
     std::vector<GemmJob> lj;
-    for (int a = 0; a < 8; a++) {
-      if (H.cls[a].empty()) continue;
+    // the V-list offsets of child class a, in its gather table's block order
+    auto class_offsets = [&](int a) {
       const int g[3] = {(a >> 2) & 1, (a >> 1) & 1, a & 1};
-      std::vector<const uint16_t *> ops;
-      std::vector<const float *> uops;
+      std::vector<int> sls;
       for (int ox = -2 - g[0]; ox <= 3 - g[0]; ox++)
         for (int oy = -2 - g[1]; oy <= 3 - g[1]; oy++)
           for (int oz = -2 - g[2]; oz <= 3 - g[2]; oz++) {
             if (std::max({std::abs(ox), std::abs(oy), std::abs(oz)}) < 2) continue;
             const int sl = (ox + 3) * 49 + (oy + 3) * 7 + (oz + 3);
             if (!X.cop[sl]) throw Error(NNK_ESTATE, "mixed M2L: missing translation operator");
-            ops.push_back(X.cop[sl]);
-            uops.push_back(f16_ ? X.uop[sl] : nullptr);
+            sls.push_back(sl);
           }
+      return sls;
+    };
+    // Levels with few nodes leave most of a 128-column tile empty in every
+    // class job.  One job over all 316 offsets and every node of the level
+    // (a column's offsets outside its class gather nothing) then walks fewer
+    // (tile, operator block) pairs; it is used where it does.
+    int64_t class_pairs = 0;
+    for (int a = 0; a < 8; a++) {
+      if (H.cls[a].empty()) continue;
+      std::vector<int32_t> tp, tl;
+      tile_lists(H.gv[a], (int)class_offsets(a).size(), (int)H.cls[a].size(), MBN, tp, tl);
+      class_pairs += (int64_t)tl.size();
+    }
+    std::vector<int> moff(343, -1), msl;
+    for (int sl = 0; sl < 343; sl++) {
+      const int ox = sl / 49 - 3, oy = (sl / 7) % 7 - 3, oz = sl % 7 - 3;
+      if (std::max({std::abs(ox), std::abs(oy), std::abs(oz)}) < 2) continue;
+      moff[sl] = (int)msl.size();
+      msl.push_back(sl);
+    }
+    std::vector<int32_t> mcols, mg, mtp, mtl;
+    for (int a = 0; a < 8; a++) mcols.insert(mcols.end(), H.cls[a].begin(), H.cls[a].end());
+    const int Nall = (int)mcols.size();
+    if (Nall > 0) {
+      mg.assign(msl.size() * (size_t)Nall, -1);
+      for (int a = 0, off = 0; a < 8; a++) {
+        const int Na = (int)H.cls[a].size();
+        if (!Na) continue;
+        const std::vector<int> sls = class_offsets(a);
+        for (size_t b = 0; b < sls.size(); b++)
+          for (int c = 0; c < Na; c++) mg[(size_t)moff[sls[b]] * Nall + off + c] = H.gv[a][b * Na + c];
+        off += Na;
+      }
+      tile_lists(mg, (int)msl.size(), Nall, MBN, mtp, mtl);
+    }
+    const bool merged = Nall > 0 && (int64_t)mtl.size() < class_pairs;
+    if (merged) {
+      std::vector<const uint16_t *> ops;
+      std::vector<const float *> uops;
+      for (int sl : msl) {
+        if (!X.cop[sl]) throw Error(NNK_ESTATE, "mixed M2L: missing translation operator");
+        ops.push_back(X.cop[sl]);
+        uops.push_back(f16_ ? X.uop[sl] : nullptr);
+      }
       GemmJobBf16 J{};
       P.mx_owned.emplace_back();
       P.mx_owned.back().upload(ops);
@@ -497,25 +539,59 @@ void Fmm::plan_mixed() {
       P.mx_owned.back().upload(uops);
       J.U = P.mx_owned.back().as<const float *>();
       P.mx_owned.emplace_back();
-      P.mx_owned.back().upload(H.gv[a]);
+      P.mx_owned.back().upload(mg);
       J.gidx = P.mx_owned.back().as<int32_t>();
       P.mx_owned.emplace_back();
-      P.mx_owned.back().upload(H.cls[a]);
+      P.mx_owned.back().upload(mcols);
       J.cidx = P.mx_owned.back().as<int32_t>();
       J.nb = (int)ops.size();
-      J.N = (int)H.cls[a].size();
-      {
-        std::vector<int32_t> tp, tl;
-        tile_lists(H.gv[a], J.nb, J.N, MBN, tp, tl);
-        P.mx_owned.emplace_back();
-        P.mx_owned.back().upload(tp);
-        J.tptr = P.mx_owned.back().as<int32_t>();
-        P.mx_owned.emplace_back();
-        P.mx_owned.back().upload(tl);
-        J.tlist = P.mx_owned.back().as<int32_t>();
-      }
+      J.N = Nall;
+      P.mx_owned.emplace_back();
+      P.mx_owned.back().upload(mtp);
+      J.tptr = P.mx_owned.back().as<int32_t>();
+      P.mx_owned.emplace_back();
+      P.mx_owned.back().upload(mtl);
+      J.tlist = P.mx_owned.back().as<int32_t>();
       mj.push_back(J);
-      P.mx_m2l_maxN = std::max(P.mx_m2l_maxN, J.N);
+      P.mx_m2l_maxN = Nall;
+    }
+    for (int a = 0; a < 8; a++) {
+      if (H.cls[a].empty()) continue;
+      if (!merged) {
+        std::vector<const uint16_t *> ops;
+        std::vector<const float *> uops;
+        for (int sl : class_offsets(a)) {
+          ops.push_back(X.cop[sl]);
+          uops.push_back(f16_ ? X.uop[sl] : nullptr);
+        }
+        GemmJobBf16 J{};
+        P.mx_owned.emplace_back();
+        P.mx_owned.back().upload(ops);
+        J.A = P.mx_owned.back().as<const uint16_t *>();
+        P.mx_owned.emplace_back();
+        P.mx_owned.back().upload(uops);
+        J.U = P.mx_owned.back().as<const float *>();
+        P.mx_owned.emplace_back();
+        P.mx_owned.back().upload(H.gv[a]);
+        J.gidx = P.mx_owned.back().as<int32_t>();
+        P.mx_owned.emplace_back();
+        P.mx_owned.back().upload(H.cls[a]);
+        J.cidx = P.mx_owned.back().as<int32_t>();
+        J.nb = (int)ops.size();
+        J.N = (int)H.cls[a].size();
+        {
+          std::vector<int32_t> tp, tl;
+          tile_lists(H.gv[a], J.nb, J.N, MBN, tp, tl);
+          P.mx_owned.emplace_back();
+          P.mx_owned.back().upload(tp);
+          J.tptr = P.mx_owned.back().as<int32_t>();
+          P.mx_owned.emplace_back();
+          P.mx_owned.back().upload(tl);
+          J.tlist = P.mx_owned.back().as<int32_t>();
+        }
+        mj.push_back(J);
+        P.mx_m2l_maxN = std::max(P.mx_m2l_maxN, J.N);
+      }
       if (l >= 3) {
         GemmJob L{};
         P.mx_l2l.owned.emplace_back();
@@ -542,10 +618,14 @@ void Fmm::plan_mixed() {
       for (int j = 0; j < nj; j++) {
         minnb = std::min(minnb, mj[j].nb);
       }
-      for (int j = 0, a = 0; a < 8; a++) {
-        if (H.cls[a].empty()) continue;
-        for (size_t c = 0; c < H.cls[a].size(); c++) slot[(size_t)j * mN + c] = H.cls[a][c];
-        j++;
+      if (merged) {
+        for (int c = 0; c < Nall; c++) slot[c] = mcols[c];
+      } else {
+        for (int j = 0, a = 0; a < 8; a++) {
+          if (H.cls[a].empty()) continue;
+          for (size_t c = 0; c < H.cls[a].size(); c++) slot[(size_t)j * mN + c] = H.cls[a][c];
+          j++;
+        }
       }
       P.mx_maxnb = 0;
       for (int j = 0; j < nj; j++) P.mx_maxnb = std::max(P.mx_maxnb, mj[j].nb);
