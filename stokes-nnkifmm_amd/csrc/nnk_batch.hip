@@ -122,7 +122,7 @@ __global__ __launch_bounds__(64 * NW) void k_bp2p(int nitems, const int2 *__rest
                                               const int32_t *__restrict__ active, double c,
                                               const double *__restrict__ x, double *__restrict__ y) {
   constexpr int T = 64 * NW;
-  __shared__ double sp[T * 6];
+  __shared__ SrcStage<F32, T> st;
   __shared__ double red[NW][kTile * 3];
   __shared__ int item_s;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
@@ -143,16 +143,11 @@ __global__ __launch_bounds__(64 * NW) void k_bp2p(int nitems, const int2 *__rest
     const int cnt = (int)min((int64_t)T, qe - q0);
     if (tid < cnt) {
       const int64_t q = q0 + tid;
-      for (int d = 0; d < 3; d++) {
-        sp[6 * tid + d] = fine[3 * q + d];
-        sp[6 * tid + 3 + d] = g[3 * q + d];
-      }
+      st.put(tid, fine[3 * q], fine[3 * q + 1], fine[3 * q + 2], g[3 * q], g[3 * q + 1], g[3 * q + 2]);
     }
     __syncthreads();
     if (lane < nt)
-      for (int j = wv; j < cnt; j += NW)
-        stk<F32>(tx, ty, tz, sp[6 * j], sp[6 * j + 1], sp[6 * j + 2], sp[6 * j + 3], sp[6 * j + 4], sp[6 * j + 5],
-                 e2, ux, uy, uz);
+      for (int j = wv; j < cnt; j += NW) st.acc(j, tx, ty, tz, e2, ux, uy, uz);
     __syncthreads();
   }
   red[wv][3 * lane] = ux;
@@ -226,7 +221,7 @@ __global__ __launch_bounds__(256) void k_bp2pT(int nitems, const int2 *__restric
                                                const double *__restrict__ fine, const double *__restrict__ eps,
                                                const int32_t *__restrict__ active, double c,
                                                const double *__restrict__ x, double *__restrict__ v) {
-  __shared__ double sp[256 * 6];
+  __shared__ SrcStage<F32, 256> st;
   __shared__ double red[4][kTile * 3];
   __shared__ int item_s;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
@@ -247,16 +242,12 @@ __global__ __launch_bounds__(256) void k_bp2pT(int nitems, const int2 *__restric
     const int cnt = (int)min((int64_t)256, Np - s0);
     if (tid < cnt) {
       const int64_t i = pb + s0 + tid;
-      for (int d = 0; d < 3; d++) {
-        sp[6 * tid + d] = pts[3 * i + d];
-        sp[6 * tid + 3 + d] = x[base + d * Np + s0 + tid];
-      }
+      st.put(tid, pts[3 * i], pts[3 * i + 1], pts[3 * i + 2], x[base + s0 + tid], x[base + Np + s0 + tid],
+             x[base + 2 * Np + s0 + tid]);
     }
     __syncthreads();
     if (lane < nt)
-      for (int j = wv; j < cnt; j += 4)
-        stk<F32>(tx, ty, tz, sp[6 * j], sp[6 * j + 1], sp[6 * j + 2], sp[6 * j + 3], sp[6 * j + 4], sp[6 * j + 5],
-                 e2, ux, uy, uz);
+      for (int j = wv; j < cnt; j += 4) st.acc(j, tx, ty, tz, e2, ux, uy, uz);
     __syncthreads();
   }
   red[wv][3 * lane] = ux;
