@@ -138,19 +138,15 @@ def run_form(nnk, lib, tree, fd, args, eps, mu, precision, res, svd_tol):
     if precision == "fp64":
         flops = 2.0 * K * K * st[0]
         peak, kern = FP64_MFMA_PEAK_TFLOPS, "k_gemm_gather<160,64,16,1> (M2L, FP64 MFMA)"
-    elif precision == "mixed":
-        flops = 3 * 2.0 * R * R * st[0]    # three BF16 products per split product
-        peak, kern = BF16_MFMA_PEAK_TFLOPS, "k_m2l_split<2,D,1,false> (M2L, two-part split-BF16 MFMA)"
-    elif precision == "mixed3":
-        flops = 6 * 2.0 * R * R * st[0]    # six BF16 products per split product
-        peak, kern = BF16_MFMA_PEAK_TFLOPS, "k_m2l_split<3,D,1,false> (M2L, three-part split-BF16 MFMA)"
     else:
-        flops = 3 * 2.0 * R * R * st[0]    # three FP16 products per split product
+        np_, f16, what = {"mixed": (2, "false", "two-part split-BF16"), "mixed3": (3, "false", "three-part split-BF16"),
+                          "mixedh": (2, "true", "two-part split-FP16")}[precision]
+        flops = (6 if np_ == 3 else 3) * 2.0 * R * R * st[0]   # 16-bit products per split product
         peak = BF16_MFMA_PEAK_TFLOPS
-        if all(dep[l] == 4 for l in range(16) if spl[l] > 0):
-            kern = "k_m2l_dma<1> (M2L, two-part split-FP16 MFMA, LDS-DMA staged)"
-        else:
-            kern = "k_m2l_split<2,D,1,true> / k_m2l_dma<1> (M2L, two-part split-FP16 MFMA)"
+        used = {int(dep[l]) for l in range(16) if spl[l] > 0}
+        names = ([f"k_m2l_split<{np_},D,1,{f16}>"] if used - {4} else []) + \
+                ([f"k_m2l_dma<{np_},1,{f16}>"] if 4 in used else [])
+        kern = " / ".join(names) + f" (M2L, {what} MFMA{', LDS-DMA staged' if used == {4} else ''})"
     m2l_ms = ms[3] / args.steps
     achieved = flops / (m2l_ms * 1e-3) / 1e12 if m2l_ms > 0 else 0.0
     rec = {"precision": precision, "coronaRes": res, "svd_tol": svd_tol if precision != "fp64" else None,

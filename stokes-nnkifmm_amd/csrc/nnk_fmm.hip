@@ -1217,7 +1217,7 @@ void Fmm::set_launch_config(int level, int split, int depth) {
     if (depth != 0) throw Error(NNK_EARG, "launch config: the FP64 form takes depth 0");
     P.m2l_split = split;
   } else {
-    const bool ok = (np_ == 3 && !f16_) ? depth == 2 : (depth == 2 || depth == 3 || (depth == 4 && dma_ok(P, split)));
+    const bool ok = depth == 2 || (depth == 3 && !(np_ == 3 && !f16_)) || (depth == 4 && dma_ok(P, split));
     if (!ok) throw Error(NNK_EARG, "launch config: depth not available for this form");
     P.mx_split = split;
     P.mx_depth = depth;
@@ -1242,7 +1242,7 @@ void Fmm::autotune(hipStream_t s) {
       std::vector<int> splits = {s0};
       if (s0 > 1) splits.push_back(std::max(1, s0 / 2));
       if ((size_t)std::min(64, 2 * s0) * per_split <= part_elems_) splits.push_back(std::min(64, 2 * s0));
-      std::vector<int> depths = !mixed ? std::vector<int>{0} : np_ == 3 && !f16_ ? std::vector<int>{2}
+      std::vector<int> depths = !mixed ? std::vector<int>{0} : np_ == 3 && !f16_ ? std::vector<int>{2, 4}
                                                                                   : std::vector<int>{2, 3, 4};
       float best = 1e30f;
       int bs = s0, bd = mixed ? P.mx_depth : 0;

@@ -64,7 +64,7 @@ struct LevelPlan {
   DBuf mx_m2m_slot;
   DBuf mx_m2l;                // GemmJobBf16[8]
   int mx_split = 1, mx_depth = 3;  // split-K factor and register-ring depth of the M2L launch
-                                   // (depth 4: the DMA-staged FP16 kernel)
+                                   // (depth 4: the DMA-staged kernel)
   int mx_maxnb = 0;                // most operator blocks of one M2L job
   DBuf mx_slot;
   std::vector<DBuf> mx_owned;

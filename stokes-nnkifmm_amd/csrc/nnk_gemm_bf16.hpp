@@ -264,33 +264,34 @@ __global__ __launch_bounds__(512, 1) void k_m2l_split(const int M, const int KC,
     }
 }
 
-// The split-FP16 form (NP = 2) of the same GEMM with its operand tiles
-// staged by the LDS DMA path (global_load_lds_dwordx4: each lane's 16 bytes
+// The same GEMM with its operand tiles staged by the LDS DMA path (global_load_lds_dwordx4: each lane's 16 bytes
 // go straight from memory to LDS, lane q at byte 16q of the wave's 1 KB
 // fragment) instead of through a register ring and LDS stores.
 //
-// Three LDS stages, each one chunk of A and X in fragment order (32 KB),
+// Three LDS stages, each one chunk of A and X in fragment order (16 NP KB),
 // are separate arrays addressed with static names, so the compiler's wait
 // insertion can tell the stage being read from the ones being filled and
 // does not drain the DMA queue before the LDS reads.  At chunk c each wave
-//   waits until its own DMAs of chunk c have landed (at most the 4 of chunk
-//   c+1 still outstanding), meets the others at a barrier (so every wave's
+//   waits until its own DMAs of chunk c have landed (at most the 2 NP of
+//   chunk c+1 still outstanding), meets the others at a barrier (so every wave's
 //   part of chunk c is in LDS and every wave is done with chunk c-1),
 //   issues the DMAs of chunk c+2 into the stage chunk c-1 used, and
 //   multiplies chunk c.
 // The barrier is s_barrier alone: __syncthreads would also wait for all
 // outstanding loads and undo the look-ahead.
 //
-// The row unscale factors of each operator block (powers of two, or zero
-// for an all-zero row; 2^-126 at the smallest) are read once per workgroup
+// FP16 parts (F16, NP = 2): the row unscale factors of each operator block
+// (powers of two, or zero for an all-zero row; 2^-126 at the smallest) are
+// read once per workgroup
 // into an LDS table of their exponent bytes, 128 rows x (a1-a0) blocks, so
 // that no global load sits in the main loop apart from the DMAs and the
 // gather slots: a load whose result is needed at the end of a block would
 // make the compiler wait for every DMA issued before it.  Requires
 // a1 - a0 <= 320 active blocks per split (checked on the host; the 316
 // offsets of a level fit without a split).
-// Results are bit-identical to k_m2l_split<2, D, TAG, true>: the same MFMAs
-// in the same order, and the unscale exact (a power of two).
+// BF16 parts (NP = 2 or 3) have no unscale table; NP = 3 takes 144 KB of
+// stages.  Results are bit-identical to k_m2l_split<NP, D, TAG, F16>: the
+// same MFMAs in the same order, and the unscale exact (a power of two).
 constexpr int kDmaMaxBlocks = 320;
 
 template <int N>
@@ -299,7 +300,7 @@ __device__ __forceinline__ void wait_vmcnt() {
   __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | (7 << 4) | (15 << 8));
 }
 
-template <int TAG>
+template <int NP, int TAG, bool F16>
 __global__ __launch_bounds__(512, 1) void k_m2l_dma(const int M, const int KC, const int RT,
                                                     const uint16_t *__restrict__ X, const int zrow,
                                                     double *__restrict__ C, const int ldc,
@@ -307,7 +308,9 @@ __global__ __launch_bounds__(512, 1) void k_m2l_dma(const int M, const int KC, c
                                                     const int nrb, const int nsplit, double *__restrict__ P,
                                                     const int maxN, const double *__restrict__ xun,
                                                     const int beta) {
-  constexpr int NP = 2, BN = 128, TM = 4, TN = 2, TILE = 512;
+  static_assert(!F16 || NP == 2, "FP16 parts: two");
+  constexpr int BN = 128, TM = 4, TN = 2, TILE = 512;
+  typedef typename std::conditional<F16, f16x8, bf16x8>::type op8;
   typedef const __attribute__((address_space(1))) void *gvp;
   typedef __attribute__((address_space(3))) void *lvp;
   typedef const __attribute__((address_space(1))) f32x4 *gf32x4p;
@@ -315,10 +318,10 @@ __global__ __launch_bounds__(512, 1) void k_m2l_dma(const int M, const int KC, c
   __shared__ __attribute__((aligned(1024))) Stage St0;
   __shared__ __attribute__((aligned(1024))) Stage St1;
   __shared__ __attribute__((aligned(1024))) Stage St2;
-  __shared__ __attribute__((aligned(16))) uint32_t ue[kDmaMaxBlocks * 32];
+  __shared__ __attribute__((aligned(16))) uint32_t ue[F16 ? kDmaMaxBlocks * 32 : 1];
   __shared__ int act[512];
   __shared__ const uint16_t *aptr[512];
-  __shared__ const float *uptr[512];
+  __shared__ const float *uptr[F16 ? 512 : 1];
 
   const int bid = blockIdx.x;
   const int jb = bid % njobs;
@@ -339,7 +342,7 @@ __global__ __launch_bounds__(512, 1) void k_m2l_dma(const int M, const int KC, c
     const int b = J.tlist[t0 + i];
     act[i] = b;
     aptr[i] = J.A[b];
-    uptr[i] = J.U[b];
+    if constexpr (F16) uptr[i] = J.U[b];
   }
   __syncthreads();
   const int a0 = (int)((int64_t)nact * sp / nsplit), a1 = (int)((int64_t)nact * (sp + 1) / nsplit);
@@ -347,7 +350,7 @@ __global__ __launch_bounds__(512, 1) void k_m2l_dma(const int M, const int KC, c
 
   // exponent table: word 32 q + w holds the exponent bytes of rows
   // 4w .. 4w+3 (of this 128-row tile) of active block a0 + q
-  {
+  if constexpr (F16) {
     const int nq = (a1 - a0) * 32;
     for (int i0 = 0; i0 < nq; i0 += 4 * 512) {
       f32x4 v[4];
@@ -414,36 +417,50 @@ __global__ __launch_bounds__(512, 1) void k_m2l_dma(const int M, const int KC, c
 #pragma unroll
       for (int r = 0; r < 4; r++) acc64[i][j][r] = 0.0;
     }
+  auto mfma = [](const op8 &a, const op8 &b, const f32x4 &c) -> f32x4 {
+    if constexpr (F16) return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+    else return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+  };
   auto compute = [&](const Stage &Sc) {
-    f16x8 a[NP][TM], x[NP][TN];
+    op8 a[NP][TM], x[NP][TN];
 #pragma unroll
     for (int p = 0; p < NP; p++) {
 #pragma unroll
-      for (int i = 0; i < TM; i++) a[p][i] = *reinterpret_cast<const f16x8 *>(&Sc[0][p][wm * 4 + i][lane * 8]);
+      for (int i = 0; i < TM; i++) a[p][i] = *reinterpret_cast<const op8 *>(&Sc[0][p][wm * 4 + i][lane * 8]);
 #pragma unroll
-      for (int j = 0; j < TN; j++) x[p][j] = *reinterpret_cast<const f16x8 *>(&Sc[1][p][wn * 2 + j][lane * 8]);
+      for (int j = 0; j < TN; j++) x[p][j] = *reinterpret_cast<const op8 *>(&Sc[1][p][wn * 2 + j][lane * 8]);
     }
 #pragma unroll
     for (int i = 0; i < TM; i++)
 #pragma unroll
       for (int j = 0; j < TN; j++) {
         f32x4 t = acc[i][j];
-        t = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[1][i], x[0][j], t, 0, 0, 0);
-        t = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[0][i], x[1][j], t, 0, 0, 0);
-        t = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[0][i], x[0][j], t, 0, 0, 0);
+        if constexpr (NP == 3) {
+          t = mfma(a[2][i], x[0][j], t);
+          t = mfma(a[1][i], x[1][j], t);
+          t = mfma(a[0][i], x[2][j], t);
+        }
+        t = mfma(a[1][i], x[0][j], t);
+        t = mfma(a[0][i], x[1][j], t);
+        t = mfma(a[0][i], x[0][j], t);
         acc[i][j] = t;
       }
   };
-  // promotion of block q's FP32 sums, each row times its unscale factor
+  // promotion of block q's FP32 sums (FP16 parts: each row times its
+  // unscale factor)
   auto promote = [&](int q) {
 #pragma unroll
     for (int i = 0; i < TM; i++) {
-      const uint32_t e4 = ue[q * 32 + wm * 16 + lg + 4 * i];
+      const uint32_t e4 = F16 ? ue[q * 32 + wm * 16 + lg + 4 * i] : 0u;
 #pragma unroll
       for (int j = 0; j < TN; j++) {
 #pragma unroll
-        for (int r = 0; r < 4; r++)
-          acc64[i][j][r] += (double)acc[i][j][r] * (double)__uint_as_float(((e4 >> (8 * r)) & 255u) << 23);
+        for (int r = 0; r < 4; r++) {
+          if constexpr (F16)
+            acc64[i][j][r] += (double)acc[i][j][r] * (double)__uint_as_float(((e4 >> (8 * r)) & 255u) << 23);
+          else
+            acc64[i][j][r] += (double)acc[i][j][r];
+        }
         acc[i][j] = f32x4{0, 0, 0, 0};
       }
     }
@@ -469,7 +486,7 @@ __global__ __launch_bounds__(512, 1) void k_m2l_dma(const int M, const int KC, c
   }
 
   const int m0 = rb * 128 + wm * 64;
-  const double xs = *xun;
+  const double xs = F16 ? *xun : 1.0;
 #pragma unroll
   for (int i = 0; i < TM; i++)
 #pragma unroll

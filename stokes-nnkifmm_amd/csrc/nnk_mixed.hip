@@ -630,7 +630,7 @@ void Fmm::plan_mixed() {
       P.mx_maxnb = 0;
       for (int j = 0; j < nj; j++) P.mx_maxnb = std::max(P.mx_maxnb, mj[j].nb);
       P.mx_split = split_for((int64_t)nj * (RPm_ / MBM) * ((mN + MBN - 1) / MBN), minnb);
-      P.mx_depth = (np_ == 3 && !f16_) ? 2 : dma_ok(P, P.mx_split) ? 4 : 3;
+      P.mx_depth = dma_ok(P, P.mx_split) ? 4 : (np_ == 3 && !f16_) ? 2 : 3;
       P.mx_slot.upload(slot);  // also for the autotuner's split candidates
       reserve_part((size_t)std::min(64, 2 * P.mx_split) * nj * mN * RP_);
     }
@@ -682,12 +682,13 @@ void Fmm::plan_mixed() {
 // whether the DMA-staged FP16 M2L kernel can run level plan P with split-K
 // factor `split`: its unscale table holds kDmaMaxBlocks blocks per split
 bool Fmm::dma_ok(const LevelPlan &P, int split) const {
-  return f16_ && split >= 1 && (P.mx_maxnb + split - 1) / split <= kDmaMaxBlocks;
+  if (split < 1) return false;
+  return !f16_ || (P.mx_maxnb + split - 1) / split <= kDmaMaxBlocks;
 }
 
 // M2L of level l in the mixed form with register-ring depth `depth` (2 or
-// 3; 4 selects the DMA-staged FP16 kernel) and split-K factor `split` (the
-// part buffer must hold it)
+// 3; 4 selects the DMA-staged kernel) and split-K factor `split` (the part
+// buffer must hold it)
 void Fmm::launch_m2l_mixed(int l, int depth, int split, hipStream_t s) {
   LevelPlan &P = plans_[l];
   if (!P.mx_m2l_jobs) return;
@@ -702,16 +703,18 @@ void Fmm::launch_m2l_mixed(int l, int depth, int split, hipStream_t s) {
     hipLaunchKernelGGL(kern, grid, dim3(512), 0, s, R_, KCm_, RT_, chat_x_.as<uint16_t>(), zrow, d, RP_,
                        P.mx_m2l.as<GemmJobBf16>(), nj, nrb, split, pp, mN, xun_.as<double>(), 0);
   };
+  if (depth == 4 && !dma_ok(P, split))
+    throw Error(NNK_ESTATE, "mixed M2L: too many operator blocks for the DMA-staged kernel");
   if (f16_) {
-    if (depth == 4) {
-      if (!dma_ok(P, split)) throw Error(NNK_ESTATE, "mixed M2L: too many operator blocks for the DMA-staged kernel");
-      launch(k_m2l_dma<1>);
-    } else if (depth == 2) launch(k_m2l_split<2, 2, 1, true>);
+    if (depth == 4) launch(k_m2l_dma<2, 1, true>);
+    else if (depth == 2) launch(k_m2l_split<2, 2, 1, true>);
     else launch(k_m2l_split<2, 3, 1, true>);
   } else if (np_ == 3) {
-    launch(k_m2l_split<3, 2, 1, false>);  // a third register stage would spill
+    if (depth == 4) launch(k_m2l_dma<3, 1, false>);
+    else launch(k_m2l_split<3, 2, 1, false>);  // a third register stage would spill
   } else {
-    if (depth == 2) launch(k_m2l_split<2, 2, 1, false>);
+    if (depth == 4) launch(k_m2l_dma<2, 1, false>);
+    else if (depth == 2) launch(k_m2l_split<2, 2, 1, false>);
     else launch(k_m2l_split<2, 3, 1, false>);
   }
   NNK_HIP(hipGetLastError());

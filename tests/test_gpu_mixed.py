@@ -116,17 +116,19 @@ def test_refined_solve_reaches_fp64_accuracy(oracle, nnk, precon):
     assert rel(xr, x64) < 1e-7
 
 
-@pytest.mark.parametrize("split", [1, 2])
-def test_mixedh_m2l_kernels_agree_bitwise(nnk, split):
-    """The FP16 M2L of every level run by the register-ring kernel (depth 2
-    and 3) and by the DMA-staged kernel (depth 4): the same products in the
-    same order, so the matvecs are bit-identical for one split-K factor.
-    Operator rows that are all zero (unscale 0) and rows scaled by widely
-    different powers of two come with the clustered points."""
+@pytest.mark.parametrize("precision,split", [("mixedh", 1), ("mixedh", 2), ("mixed", 1), ("mixed3", 2)])
+def test_mixedh_m2l_kernels_agree_bitwise(nnk, precision, split):
+    """The split-precision M2L of every level run by the register-ring
+    kernel (depth 2 and, but for the three-part form, 3) and by the
+    DMA-staged kernel (depth 4): the same products in the same order, so
+    the matvecs are bit-identical for one split-K factor.  For the FP16
+    form, operator rows that are all zero (unscale 0) and rows scaled by
+    widely different powers of two come with the clustered points."""
     rng = np.random.default_rng(44)
     p = np.concatenate([rng.random((30000, 3)), 1e-3 * rng.random((3000, 3)) + 0.5])
     q = rng.standard_normal(p.shape)
-    f = nnk.KIFMM(nnk.OcTree(p, p, "nodeCapacity", 48), [1e-3, 1.0], "precision", "mixedh")
+    f = nnk.KIFMM(nnk.OcTree(p, p, "nodeCapacity", 48), [1e-3, 1.0], "precision", precision)
+    depths = (2, 4) if precision == "mixed3" else (2, 3, 4)
     lib, h = nnk.lib(), f._handle().h
     i32 = ctypes.POINTER(ctypes.c_int32)
     s0 = np.zeros(16, np.int32)
@@ -135,7 +137,7 @@ def test_mixedh_m2l_kernels_agree_bitwise(nnk, split):
     levels = np.flatnonzero(s0)
     assert len(levels) >= 2 and set(d0[levels]) <= {2, 3, 4}
     out = {}
-    for depth in (2, 3, 4):
+    for depth in depths:
         s = np.where(s0 > 0, split, 0).astype(np.int32)
         d = np.where(s0 > 0, depth, 0).astype(np.int32)
         assert lib.nnk_fmm_set_launch_config(h, s.ctypes.data_as(i32), d.ctypes.data_as(i32), 16) == 0
@@ -144,8 +146,8 @@ def test_mixedh_m2l_kernels_agree_bitwise(nnk, split):
         lib.nnk_fmm_launch_config(h, chk_s.ctypes.data_as(i32), chk_d.ctypes.data_as(i32), 16)
         assert (chk_s[levels] == split).all() and (chk_d[levels] == depth).all()
         out[depth] = f.computeVel(q)
-    np.testing.assert_array_equal(out[2], out[3])
-    np.testing.assert_array_equal(out[3], out[4])
+    for depth in depths[1:]:
+        np.testing.assert_array_equal(out[depth], out[2])
 
 
 def test_launch_config_rejects_what_a_form_cannot_run(nnk):
@@ -168,12 +170,12 @@ def test_launch_config_rejects_what_a_form_cannot_run(nnk):
         return int(np.flatnonzero(s)[0])
 
     f64 = nnk.KIFMM(t, [1e-3, 1.0])
-    fb = nnk.KIFMM(t, [1e-3, 1.0], "precision", "mixed")
+    fb = nnk.KIFMM(t, [1e-3, 1.0], "precision", "mixed3")
     fh = nnk.KIFMM(t, [1e-3, 1.0], "precision", "mixedh")
     u64, ub, uh = f64.computeVel(p), fb.computeVel(p), fh.computeVel(p)
     l64, lb, lh = first_m2l_level(f64), first_m2l_level(fb), first_m2l_level(fh)
     assert setcfg(f64, l64, 1, 3) != 0          # the FP64 form has no ring depth
-    assert setcfg(fb, lb, 1, 4) != 0            # DMA staging is FP16 only
+    assert setcfg(fb, lb, 1, 3) != 0            # three parts: no third register stage
     assert setcfg(fh, lh, 1, 5) != 0
     assert setcfg(fh, lh, 0, 4) == 0            # split 0: level left as it is
     assert setcfg(fh, lh, 65, 3) != 0
