@@ -510,7 +510,12 @@ void Fmm::plan_mixed() {
     std::vector<int32_t> mcols, mg, mtp, mtl;
     for (int a = 0; a < 8; a++) mcols.insert(mcols.end(), H.cls[a].begin(), H.cls[a].end());
     const int Nall = (int)mcols.size();
-    if (Nall > 0) {
+    // with n nodes per class the class jobs walk ~8 ceil(n/128) 189 pairs
+    // and the merged job ~ceil(8n/128) 316: merging only pays below ~128
+    // nodes per class, so larger levels neither build its 316 x N table
+    // nor consider it
+    const bool mergeable = Nall > 0 && Nall <= 8 * MBN;
+    if (mergeable) {
       mg.assign(msl.size() * (size_t)Nall, -1);
       for (int a = 0, off = 0; a < 8; a++) {
         const int Na = (int)H.cls[a].size();
@@ -522,7 +527,9 @@ void Fmm::plan_mixed() {
       }
       tile_lists(mg, (int)msl.size(), Nall, MBN, mtp, mtl);
     }
-    const bool merged = Nall > 0 && (int64_t)mtl.size() < class_pairs;
+    const bool merged = mergeable && (int64_t)mtl.size() < class_pairs;
+    if (merged && (mg.size() != msl.size() * (size_t)Nall || mtp.size() != (size_t)(Nall + MBN - 1) / MBN + 1))
+      throw Error(NNK_ESTATE, "mixed M2L: merged gather table incomplete");
     if (merged) {
       std::vector<const uint16_t *> ops;
       std::vector<const float *> uops;
