@@ -13,14 +13,21 @@ Parity pins (what checks each oracle function independently of itself):
   tree              hand-worked trees, invariants, root+8 for the sphere case
                     (tests/test_oracle_tree.py)
   lists             complete uniform trees: |U|=27 and |V|=189 in the interior,
-                    per-level V totals, U/V symmetry, X = W^T, a brute-force
-                    set computation on small adaptive trees
+                    per-level V totals; U/V symmetry, X = W^T and exactly-once
+                    coverage of every (target leaf, source leaf) pair on
+                    adaptive trees; W and X lists (contents and order: the
+                    re-sorted rest + last merged node of the W coarsening,
+                    X's column-major find) of two adaptive trees worked out
+                    by hand (tests/hand_trees.py)
   nearestMatrix     brute force over scipy cKDTree, row sums = 1
   fmm matvec        exact agreement with the direct sum when every leaf is
                     adjacent to every other (root + 8 leaves); error vs direct
                     sum decreasing with coronaRes
   reduced pass/solve block-diagonal direct sum; residual of the solve
-  mobility/gmres    6 pi mu a U drag on a sphere; numpy.linalg.solve
+  mobility/gmres    6 pi mu a U drag on a sphere; numpy.linalg.solve;
+                    precon and saddle-point operator against dense matrices
+                    written from genMatrixComponets/ReducedSolve/MobilityPrecon
+                    on the one-sphere tree (tests/test_oracle_mobility.py)
 """
 import ctypes
 import os

@@ -10,7 +10,9 @@ NEAREST mode w_j = sum_q NN(q,j) and p_j = sum_q NN(q,j) rfine_q.
 Vectors: [f (3N: all x, all y, all z); U (3 nsw); Omega (3 nsw)].
 GMRES: Saad & Schultz (1986) with modified Gram-Schmidt, x0 = 0.
 Parity pins: Stokes drag on a sphere (tests/test_gpu_mobility.py), numpy
-solves of small dense systems (tests/test_oracle_mobility.py).
+solves of small dense systems, and precon / apply / A1 / A2 against dense
+matrices written from genMatrixComponets.m, ReducedSolve.m and
+MobilityPrecon.m on the one-sphere tree (tests/test_oracle_mobility.py).
 """
 import numpy as np
 

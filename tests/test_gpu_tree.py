@@ -101,3 +101,14 @@ def test_root_cube_option(oracle, nnk):
         nnk.OcTree(p, p, "rootCube", [0.2, 2.0])
     with pytest.raises(RuntimeError, match="root cube"):
         oracle.Tree(p, p, rootCube=[0.2, 2.0])
+
+
+@pytest.mark.parametrize("case", ["A", "B"])
+def test_hand_worked_w_and_x_lists(nnk, case):
+    """The GPU list build against W/X lists worked out by hand from
+    OcTree.m:279-288,343-360 (derivations in tests/hand_trees.py)."""
+    import hand_trees
+    make, expect = hand_trees.CASES[case]
+    p = make()
+    t = nnk.OcTree(p, p, "nodeCapacity", 1)
+    hand_trees.check(lambda n, k: t.interactions(n, k + 1), t.nodeCount, expect)

@@ -185,3 +185,15 @@ def test_list_symmetry_and_coverage(oracle, case):
                     cnt[x] += 1
             a = tr.nodeParents[a - 1]
         assert set(cnt) == pl and set(cnt.values()) == {1}, (i, collections.Counter(cnt.values()))
+
+
+@pytest.mark.parametrize("case", ["A", "B"])
+def test_hand_worked_w_and_x_lists(oracle, case):
+    """W coarsening order (re-sorted rest + last merged node, levels bottom
+    up) and X's column-major find, against lists worked out by hand
+    (tests/hand_trees.py gives the derivations)."""
+    import hand_trees
+    make, expect = hand_trees.CASES[case]
+    p = make()
+    t = oracle.Tree(p, p, nodeCapacity=1)
+    hand_trees.check(t.interactions, t.nodeCount, expect)
