@@ -41,7 +41,7 @@ def build(verbose=False):
     os.makedirs(LIBDIR, exist_ok=True)
     objdir = os.path.join(LIBDIR, "obj")
     os.makedirs(objdir, exist_ok=True)
-    objs = []
+    objs, cmds = [], []
     for src in SOURCES:
         path = os.path.join(CSRC, src)
         obj = os.path.join(objdir, os.path.splitext(src)[0] + ".o")
@@ -50,17 +50,22 @@ def build(verbose=False):
             cmd[1:1] = ["-x", "hip"]
         if verbose:
             print(" ".join(cmd))
-        _run(cmd)
+        cmds.append(cmd)
         objs.append(obj)
+    # the translation units compile independently: a few at a time
+    from concurrent.futures import ThreadPoolExecutor
+    with ThreadPoolExecutor(max_workers=max(1, min(8, os.cpu_count() or 1))) as ex:
+        list(ex.map(_run, cmds))
     tmp = LIB + ".tmp"
     cmd = [HIPCC, "-shared", "-fopenmp", "--offload-arch=gfx950", "-o", tmp] + objs + [
         "-Wl,-rpath,/opt/rocm/llvm/lib", "-Wl,-rpath,/opt/rocm/lib"]
     _run(cmd)
     os.replace(tmp, LIB)
-    # test-only companion library (GEMM / internal-array probes for tests/)
-    st = os.path.join(CSRC, "selftest", "nnk_selftest.hip")
+    # test-only companion library (GEMM / internal-array probes, host-only
+    # planner checks; its source lives with the tests)
+    st = os.path.join(os.path.dirname(HERE), "tests", "native", "nnk_selftest.hip")
     sto = os.path.join(objdir, "nnk_selftest.o")
-    _run([HIPCC, "-x", "hip"] + COMMON + DEVICE_FLAGS + ["-c", st, "-o", sto])
+    _run([HIPCC, "-x", "hip"] + COMMON + DEVICE_FLAGS + ["-I" + CSRC, "-c", st, "-o", sto])
     tmp2 = SELFTEST + ".tmp"
     _run([HIPCC, "-shared", "--offload-arch=gfx950", "-o", tmp2, sto, "-L" + LIBDIR, "-lnnkifmm",
           "-Wl,-rpath,$ORIGIN", "-Wl,-rpath,/opt/rocm/lib"])

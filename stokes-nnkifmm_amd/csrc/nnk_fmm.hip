@@ -432,6 +432,67 @@ void tile_lists(const std::vector<int32_t> &gidx, int nb, int N, int BN, std::ve
   }
 }
 
+void check_gather_job(const std::vector<int32_t> &gidx, int nb, int N, int BN, const std::vector<int32_t> &tptr,
+                      const std::vector<int32_t> &tlist, int64_t count, const char *what) {
+  auto fail = [&](const char *why) { throw Error(NNK_ESTATE, std::string(what) + ": " + why); };
+  if (nb < 0 || N < 0 || BN <= 0 || gidx.size() != (size_t)nb * (size_t)N) fail("gather table size");
+  for (int32_t g : gidx)
+    if (g < -1 || g >= count) fail("gather index out of range");
+  const size_t nct = ((size_t)N + BN - 1) / BN;
+  if (tptr.size() != nct + 1 || tptr[0] != 0 || (size_t)tptr[nct] != tlist.size()) fail("tile pointers");
+  for (size_t ct = 0; ct < nct; ct++)
+    if (tptr[ct + 1] < tptr[ct]) fail("tile pointers not monotone");
+  for (int32_t b : tlist)
+    if (b < 0 || b >= nb) fail("tile-list block out of range");
+}
+
+std::vector<int> class_offset_slots(int a) {
+  const int g[3] = {(a >> 2) & 1, (a >> 1) & 1, a & 1};
+  std::vector<int> sls;
+  for (int ox = -2 - g[0]; ox <= 3 - g[0]; ox++)
+    for (int oy = -2 - g[1]; oy <= 3 - g[1]; oy++)
+      for (int oz = -2 - g[2]; oz <= 3 - g[2]; oz++) {
+        if (std::max({std::abs(ox), std::abs(oy), std::abs(oz)}) < 2) continue;
+        sls.push_back(offset_slot(ox, oy, oz));
+      }
+  return sls;
+}
+
+void plan_m2l_level(const std::vector<int32_t> *cls, const std::vector<int32_t> *gv, int64_t count, int BN,
+                    M2lLevelPlan &P) {
+  P = M2lLevelPlan{};
+  for (int a = 0; a < 8; a++) {
+    if (cls[a].empty()) continue;
+    const int nb = (int)class_offset_slots(a).size(), Na = (int)cls[a].size();
+    tile_lists(gv[a], nb, Na, BN, P.ctp[a], P.ctl[a]);
+    check_gather_job(gv[a], nb, Na, BN, P.ctp[a], P.ctl[a], count, "M2L class job");
+    P.class_pairs += (int64_t)P.ctl[a].size();
+  }
+  std::vector<int> moff(343, -1);
+  for (int sl = 0; sl < 343; sl++) {
+    const int ox = sl / 49 - 3, oy = (sl / 7) % 7 - 3, oz = sl % 7 - 3;
+    if (std::max({std::abs(ox), std::abs(oy), std::abs(oz)}) < 2) continue;
+    moff[sl] = (int)P.msl.size();
+    P.msl.push_back(sl);
+  }
+  for (int a = 0; a < 8; a++) P.mcols.insert(P.mcols.end(), cls[a].begin(), cls[a].end());
+  const int Nall = (int)P.mcols.size();
+  if (Nall == 0 || Nall > 8 * BN) return;
+  P.mg.assign(P.msl.size() * (size_t)Nall, -1);
+  for (int a = 0, off = 0; a < 8; a++) {
+    const int Na = (int)cls[a].size();
+    if (!Na) continue;
+    const std::vector<int> sls = class_offset_slots(a);
+    for (size_t b = 0; b < sls.size(); b++)
+      for (int c = 0; c < Na; c++) P.mg[(size_t)moff[sls[b]] * Nall + off + c] = gv[a][b * Na + c];
+    off += Na;
+  }
+  tile_lists(P.mg, (int)P.msl.size(), Nall, BN, P.mtp, P.mtl);
+  P.merged_pairs = (int64_t)P.mtl.size();
+  P.merged = P.merged_pairs < P.class_pairs;
+  if (P.merged) check_gather_job(P.mg, (int)P.msl.size(), Nall, BN, P.mtp, P.mtl, count, "merged M2L job");
+}
+
 int Fmm::split_for(int64_t tiles, int nb) {
   // aim at ~4 workgroups per CU, with at least 4 operator blocks per split
   const int64_t want = 1024;
@@ -1012,6 +1073,7 @@ void Fmm::plan() {
         {  // active blocks per 64-column tile of the M2L launch
           std::vector<int32_t> tp, tl;
           tile_lists(g, nb, N, 64, tp, tl);
+          check_gather_job(g, nb, N, 64, tp, tl, T.count, "M2L job");
           P.m2l.owned.emplace_back(make_dbuf(tp));
           jobs.back().tptr = P.m2l.owned.back().as<int32_t>();
           P.m2l.owned.emplace_back(make_dbuf(tl));

@@ -33,6 +33,36 @@ struct GemmJob {
 void tile_lists(const std::vector<int32_t> &gidx, int nb, int N, int BN, std::vector<int32_t> &ptr,
                 std::vector<int32_t> &list);
 
+// Refuses (NNK_ESTATE) a gather job whose tables a kernel could index out
+// of bounds: the gather table must be nb x N with entries in [-1, count),
+// the tile pointers must cover ceil(N / BN) tiles monotonically, and every
+// tile-list entry must be a block in [0, nb).
+void check_gather_job(const std::vector<int32_t> &gidx, int nb, int N, int BN, const std::vector<int32_t> &tptr,
+                      const std::vector<int32_t> &tlist, int64_t count, const char *what);
+
+// The V-list offsets a child of class a (bits x, y, z) can see, as slots
+// (ox+3)*49 + (oy+3)*7 + (oz+3) of the 7^3 offset table, in the order of
+// that class's gather-table blocks (189 of them).
+std::vector<int> class_offset_slots(int a);
+
+// Host plan of one level's split-precision M2L: per child class a job over
+// its 189 offsets (gather table gv[a], 189 x |cls[a]|), or one merged job
+// over all 316 offsets and every node of the level, whichever walks fewer
+// (column tile, operator block) pairs.  The merged table (316 x N) is built
+// only for levels of at most 8 BN nodes: with n nodes per class the class
+// jobs walk ~8 ceil(n/BN) 189 pairs and the merged job ~ceil(8n/BN) 316,
+// so larger levels never gain from it.  Every table chosen is checked with
+// check_gather_job (count = node count).
+struct M2lLevelPlan {
+  bool merged = false;
+  int64_t class_pairs = 0, merged_pairs = -1;   // -1: merged table not built
+  std::vector<int32_t> ctp[8], ctl[8];          // class jobs' tile lists
+  std::vector<int> msl;                         // merged job: the 316 slots in block order
+  std::vector<int32_t> mcols, mg, mtp, mtl;     // merged job: columns, gather table, tile lists
+};
+void plan_m2l_level(const std::vector<int32_t> *cls, const std::vector<int32_t> *gv, int64_t count, int BN,
+                    M2lLevelPlan &out);
+
 struct GemmBatch {
   DBuf jobs;                 // GemmJob[n]
   std::vector<DBuf> owned;   // index/pointer arrays referenced by the jobs

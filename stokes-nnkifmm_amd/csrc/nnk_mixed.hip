@@ -477,59 +477,20 @@ void Fmm::plan_mixed() {
     std::vector<GemmJob> lj;
     // the V-list offsets of child class a, in its gather table's block order
     auto class_offsets = [&](int a) {
-      const int g[3] = {(a >> 2) & 1, (a >> 1) & 1, a & 1};
-      std::vector<int> sls;
-      for (int ox = -2 - g[0]; ox <= 3 - g[0]; ox++)
-        for (int oy = -2 - g[1]; oy <= 3 - g[1]; oy++)
-          for (int oz = -2 - g[2]; oz <= 3 - g[2]; oz++) {
-            if (std::max({std::abs(ox), std::abs(oy), std::abs(oz)}) < 2) continue;
-            const int sl = (ox + 3) * 49 + (oy + 3) * 7 + (oz + 3);
-            if (!X.cop[sl]) throw Error(NNK_ESTATE, "mixed M2L: missing translation operator");
-            sls.push_back(sl);
-          }
+      std::vector<int> sls = class_offset_slots(a);
+      for (int sl : sls)
+        if (!X.cop[sl]) throw Error(NNK_ESTATE, "mixed M2L: missing translation operator");
       return sls;
     };
     // Levels with few nodes leave most of a 128-column tile empty in every
-    // class job.  One job over all 316 offsets and every node of the level
-    // (a column's offsets outside its class gather nothing) then walks fewer
-    // (tile, operator block) pairs; it is used where it does.
-    int64_t class_pairs = 0;
-    for (int a = 0; a < 8; a++) {
-      if (H.cls[a].empty()) continue;
-      std::vector<int32_t> tp, tl;
-      tile_lists(H.gv[a], (int)class_offsets(a).size(), (int)H.cls[a].size(), MBN, tp, tl);
-      class_pairs += (int64_t)tl.size();
-    }
-    std::vector<int> moff(343, -1), msl;
-    for (int sl = 0; sl < 343; sl++) {
-      const int ox = sl / 49 - 3, oy = (sl / 7) % 7 - 3, oz = sl % 7 - 3;
-      if (std::max({std::abs(ox), std::abs(oy), std::abs(oz)}) < 2) continue;
-      moff[sl] = (int)msl.size();
-      msl.push_back(sl);
-    }
-    std::vector<int32_t> mcols, mg, mtp, mtl;
-    for (int a = 0; a < 8; a++) mcols.insert(mcols.end(), H.cls[a].begin(), H.cls[a].end());
+    // class job; one job over all 316 offsets then walks fewer (tile,
+    // operator block) pairs (plan_m2l_level decides and checks the tables).
+    M2lLevelPlan mp;
+    plan_m2l_level(H.cls, H.gv, tree_.count, MBN, mp);
+    const bool merged = mp.merged;
+    const std::vector<int> &msl = mp.msl;
+    const std::vector<int32_t> &mcols = mp.mcols;
     const int Nall = (int)mcols.size();
-    // with n nodes per class the class jobs walk ~8 ceil(n/128) 189 pairs
-    // and the merged job ~ceil(8n/128) 316: merging only pays below ~128
-    // nodes per class, so larger levels neither build its 316 x N table
-    // nor consider it
-    const bool mergeable = Nall > 0 && Nall <= 8 * MBN;
-    if (mergeable) {
-      mg.assign(msl.size() * (size_t)Nall, -1);
-      for (int a = 0, off = 0; a < 8; a++) {
-        const int Na = (int)H.cls[a].size();
-        if (!Na) continue;
-        const std::vector<int> sls = class_offsets(a);
-        for (size_t b = 0; b < sls.size(); b++)
-          for (int c = 0; c < Na; c++) mg[(size_t)moff[sls[b]] * Nall + off + c] = H.gv[a][b * Na + c];
-        off += Na;
-      }
-      tile_lists(mg, (int)msl.size(), Nall, MBN, mtp, mtl);
-    }
-    const bool merged = mergeable && (int64_t)mtl.size() < class_pairs;
-    if (merged && (mg.size() != msl.size() * (size_t)Nall || mtp.size() != (size_t)(Nall + MBN - 1) / MBN + 1))
-      throw Error(NNK_ESTATE, "mixed M2L: merged gather table incomplete");
     if (merged) {
       std::vector<const uint16_t *> ops;
       std::vector<const float *> uops;
@@ -546,7 +507,7 @@ void Fmm::plan_mixed() {
       P.mx_owned.back().upload(uops);
       J.U = P.mx_owned.back().as<const float *>();
       P.mx_owned.emplace_back();
-      P.mx_owned.back().upload(mg);
+      P.mx_owned.back().upload(mp.mg);
       J.gidx = P.mx_owned.back().as<int32_t>();
       P.mx_owned.emplace_back();
       P.mx_owned.back().upload(mcols);
@@ -554,10 +515,10 @@ void Fmm::plan_mixed() {
       J.nb = (int)ops.size();
       J.N = Nall;
       P.mx_owned.emplace_back();
-      P.mx_owned.back().upload(mtp);
+      P.mx_owned.back().upload(mp.mtp);
       J.tptr = P.mx_owned.back().as<int32_t>();
       P.mx_owned.emplace_back();
-      P.mx_owned.back().upload(mtl);
+      P.mx_owned.back().upload(mp.mtl);
       J.tlist = P.mx_owned.back().as<int32_t>();
       mj.push_back(J);
       P.mx_m2l_maxN = Nall;
@@ -586,16 +547,12 @@ void Fmm::plan_mixed() {
         J.cidx = P.mx_owned.back().as<int32_t>();
         J.nb = (int)ops.size();
         J.N = (int)H.cls[a].size();
-        {
-          std::vector<int32_t> tp, tl;
-          tile_lists(H.gv[a], J.nb, J.N, MBN, tp, tl);
-          P.mx_owned.emplace_back();
-          P.mx_owned.back().upload(tp);
-          J.tptr = P.mx_owned.back().as<int32_t>();
-          P.mx_owned.emplace_back();
-          P.mx_owned.back().upload(tl);
-          J.tlist = P.mx_owned.back().as<int32_t>();
-        }
+        P.mx_owned.emplace_back();
+        P.mx_owned.back().upload(mp.ctp[a]);
+        J.tptr = P.mx_owned.back().as<int32_t>();
+        P.mx_owned.emplace_back();
+        P.mx_owned.back().upload(mp.ctl[a]);
+        J.tlist = P.mx_owned.back().as<int32_t>();
         mj.push_back(J);
         P.mx_m2l_maxN = std::max(P.mx_m2l_maxN, J.N);
       }
@@ -664,6 +621,7 @@ void Fmm::plan_mixed() {
         J.N = (int)c.size();
         std::vector<int32_t> tp, tl;
         tile_lists(g, J.nb, J.N, MBN, tp, tl);
+        check_gather_job(g, J.nb, J.N, MBN, tp, tl, tree_.count, "FP16 down expansion job");
         P.mx_owned.emplace_back();
         P.mx_owned.back().upload(tp);
         J.tptr = P.mx_owned.back().as<int32_t>();

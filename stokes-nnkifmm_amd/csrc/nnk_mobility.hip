@@ -96,34 +96,6 @@ __global__ void k_subtract_combo(int64_t n, int k, const double *__restrict__ V,
   w[i] -= s;
 }
 
-// small dense LU solve x = A^-1 b (factors from the host: row-major LU and
-// pivots), one block
-__global__ void k_small_lu_solve(int n, const double *__restrict__ LU, const int32_t *__restrict__ piv,
-                                 const double *__restrict__ b, double *__restrict__ x, double scale) {
-  extern __shared__ double y[];
-  for (int i = threadIdx.x; i < n; i += blockDim.x) y[i] = b[i];
-  __syncthreads();
-  if (threadIdx.x == 0)
-    for (int k = 0; k < n; k++) {
-      int p = piv[k];
-      if (p != k) { double t = y[k]; y[k] = y[p]; y[p] = t; }
-    }
-  __syncthreads();
-  for (int k = 0; k < n; k++) {
-    double yk = y[k];
-    for (int i = k + 1 + threadIdx.x; i < n; i += blockDim.x) y[i] -= LU[(size_t)i * n + k] * yk;
-    __syncthreads();
-  }
-  for (int k = n - 1; k >= 0; k--) {
-    if (threadIdx.x == 0) y[k] /= LU[(size_t)k * n + k];
-    __syncthreads();
-    double yk = y[k];
-    for (int i = threadIdx.x; i < k; i += blockDim.x) y[i] -= LU[(size_t)i * n + k] * yk;
-    __syncthreads();
-  }
-  for (int i = threadIdx.x; i < n; i += blockDim.x) x[i] = scale * y[i];
-}
-
 // ---------------------------------------------------------------------------
 
 namespace {
@@ -167,6 +139,86 @@ void csr_transpose(int64_t rows, int64_t cols, const std::vector<int64_t> &rp,
       tci[k] = (int32_t)r;
       tv[k] = v[e];
     }
+}
+
+// Inverse of a dense n x n matrix (row-major) as a sparse matrix: the
+// connected components of its nonzero pattern are found (union-find), each
+// component's dense block is LU-factored with partial pivoting and inverted,
+// and the inverse is emitted in CSR (rows ascending, columns ascending).
+// A1 = B B^T and A2 = BT^T BT couple only the 6 unknowns of one body
+// (genMatrixComponets.m:99-100), so their inverses hold 36 entries per body.
+void block_inverse_csr(const double *A, int64_t n, const char *what, std::vector<int64_t> &rp,
+                       std::vector<int32_t> &ci, std::vector<double> &v) {
+  std::vector<int64_t> root(n);
+  for (int64_t i = 0; i < n; i++) root[i] = i;
+  auto find = [&](int64_t i) {
+    while (root[i] != i) { root[i] = root[root[i]]; i = root[i]; }
+    return i;
+  };
+  for (int64_t i = 0; i < n; i++)
+    for (int64_t j = 0; j < n; j++)
+      if (A[i * n + j] != 0.0) {
+        int64_t a = find(i), b = find(j);
+        if (a != b) root[std::max(a, b)] = std::min(a, b);
+      }
+  std::vector<std::vector<int64_t>> comps;
+  std::vector<int64_t> slot(n, -1);
+  int64_t total = 0;
+  for (int64_t i = 0; i < n; i++) {
+    int64_t r = find(i);
+    if (slot[r] < 0) { slot[r] = (int64_t)comps.size(); comps.emplace_back(); }
+    comps[slot[r]].push_back(i);
+  }
+  for (auto &c : comps) total += (int64_t)c.size() * (int64_t)c.size();
+  if (total > (int64_t)1 << 26)
+    throw Error(NNK_EARG, std::string("mobility: ") + what + " couples too many unknowns for its inverse (" +
+                              std::to_string(total) + " entries > 2^26)");
+  std::vector<int64_t> rowlen(n, 0);
+  std::vector<std::vector<double>> inv(comps.size());
+  for (size_t q = 0; q < comps.size(); q++) {
+    const auto &c = comps[q];
+    const int m = (int)c.size();
+    std::vector<double> M((size_t)m * m);
+    for (int a = 0; a < m; a++)
+      for (int b = 0; b < m; b++) M[(size_t)a * m + b] = A[c[a] * n + c[b]];
+    std::vector<int32_t> piv;
+    host_lu(M, piv, m);
+    for (int k = 0; k < m; k++)
+      if (M[(size_t)k * m + k] == 0.0)
+        throw Error(NNK_EARG, std::string("mobility: ") + what + " is singular");
+    // columns of the inverse: solve L U x = P e_j
+    std::vector<double> X((size_t)m * m, 0.0), y(m);
+    for (int j = 0; j < m; j++) {
+      std::fill(y.begin(), y.end(), 0.0);
+      y[j] = 1.0;
+      for (int k = 0; k < m; k++)
+        if (piv[k] != k) std::swap(y[k], y[piv[k]]);
+      for (int k = 0; k < m; k++)
+        for (int i = k + 1; i < m; i++) y[i] -= M[(size_t)i * m + k] * y[k];
+      for (int k = m - 1; k >= 0; k--) {
+        y[k] /= M[(size_t)k * m + k];
+        for (int i = 0; i < k; i++) y[i] -= M[(size_t)i * m + k] * y[k];
+      }
+      for (int i = 0; i < m; i++) X[(size_t)i * m + j] = y[i];
+    }
+    inv[q] = std::move(X);
+    for (int64_t r : c) rowlen[r] = m;
+  }
+  rp.assign(n + 1, 0);
+  for (int64_t i = 0; i < n; i++) rp[i + 1] = rp[i] + rowlen[i];
+  ci.resize(rp[n]);
+  v.resize(rp[n]);
+  for (size_t q = 0; q < comps.size(); q++) {
+    const auto &c = comps[q];
+    const int m = (int)c.size();
+    for (int a = 0; a < m; a++) {
+      int64_t e = rp[c[a]];
+      for (int b = 0; b < m; b++, e++) {
+        ci[e] = (int32_t)c[b];
+        v[e] = inv[q][(size_t)a * m + b];
+      }
+    }
+  }
 }
 
 }  // namespace
@@ -218,11 +270,8 @@ Mobility::Mobility(Fmm *fmm, Fmm *fmm_sparse, int64_t nb6, const int64_t *Brp, c
   csr_transpose(n3_, nb6, rp2, ci2, v2, trp, tci, tv);
   BTt_.upload(nb6, n3_, trp, tci, tv);
   for (int q = 0; q < 2; q++) {
-    std::vector<double> A(q == 0 ? A1 : A2, (q == 0 ? A1 : A2) + (size_t)nb6 * nb6);
-    std::vector<int32_t> piv;
-    host_lu(A, piv, (int)nb6);
-    (q == 0 ? A1lu_ : A2lu_).upload(A);
-    (q == 0 ? A1piv_ : A2piv_).upload(piv);
+    block_inverse_csr(q == 0 ? A1 : A2, nb6, q == 0 ? "A1" : "A2", rp, ci, v);
+    (q == 0 ? A1inv_ : A2inv_).upload(nb6, nb6, rp, ci, v);
   }
   aos_a_.alloc(sizeof(double) * n3_);
   aos_b_.alloc(sizeof(double) * n3_);
@@ -248,12 +297,6 @@ void Mobility::reduced(const double *v, double *f, hipStream_t s) {
   NNK_HIP(hipGetLastError());
 }
 
-void Mobility::small_solve(const DBuf &lu, const DBuf &piv, const double *b, double *x, double scale,
-                           hipStream_t s) {
-  hipLaunchKernelGGL(k_small_lu_solve, dim3(1), dim3(256), sizeof(double) * nb6_, s, (int)nb6_,
-                     lu.as<double>(), piv.as<int32_t>(), b, x, scale);
-  NNK_HIP(hipGetLastError());
-}
 
 void Mobility::apply(const double *x, double *y, hipStream_t s) {
   kifmm(Fcur_ ? Fcur_ : F_, x, y, false, s);  // S f
@@ -264,11 +307,11 @@ void Mobility::apply(const double *x, double *y, hipStream_t s) {
 void Mobility::precon(const double *x, double *y, hipStream_t s) {
   if (!F4_) throw Error(NNK_ESTATE, "mobility: no coronaRes-4 operator for the preconditioner");
   reduced(x, y, s);                                        // y_f = R^-1 f
-  small_solve(A1lu_, A1piv_, x + n3_, s6a_.as<double>(), 1.0, s);   // A1 \ UO
+  A1inv_.mul(x + n3_, s6a_.as<double>(), false, 1.0, s);          // A1 \ UO
   Bt_.mul(s6a_.as<double>(), t3a_.as<double>(), false, 1.0, s);     // B^T (.)
   kifmm((Fcur_ && F4i_) ? F4i_ : F4_, t3a_.as<double>(), t3b_.as<double>(), false, s);  // S4 (.)
   BTt_.mul(t3b_.as<double>(), s6b_.as<double>(), false, 1.0, s);    // BT^T (.)
-  small_solve(A2lu_, A2piv_, s6b_.as<double>(), y + n3_, -1.0, s);  // -A2 \ (.)
+  A2inv_.mul(s6b_.as<double>(), y + n3_, false, -1.0, s);         // -A2 \ (.)
   BT_.mul(y + n3_, t3a_.as<double>(), false, 1.0, s);               // BT out
   reduced(t3a_.as<double>(), t3b_.as<double>(), s);                 // R^-1 (.)
   hipLaunchKernelGGL(k_axpby, dim3(nblk(n3_)), dim3(256), 0, s, n3_, -1.0, t3b_.as<double>(), 1.0, y);

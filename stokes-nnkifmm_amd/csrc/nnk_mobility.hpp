@@ -60,15 +60,13 @@ class Mobility {
   void op_any(const double *x, double *y, bool use_precon, hipStream_t s);
   void kifmm(Fmm *Fm, const double *f, double *u, bool accumulate, hipStream_t s);
   void reduced(const double *v, double *f, hipStream_t s);
-  void small_solve(const DBuf &lu, const DBuf &piv, const double *b, double *x, double scale,
-                   hipStream_t s);
   double dot(const double *a, const double *b, hipStream_t s);
 
   Fmm *F_, *F4_, *Fi_ = nullptr, *F4i_ = nullptr;
   Fmm *Fcur_ = nullptr;  // operator used by apply(): F_ or Fi_
   int64_t nb6_, N_ = 0, n3_ = 0, n_ = 0;
   DevCsr B_, Bt_, BT_, BTt_;
-  DBuf A1lu_, A1piv_, A2lu_, A2piv_;
+  DevCsr A1inv_, A2inv_;  // inverses of A1, A2 (block diagonal: one 6 x 6 block per body)
   DBuf aos_a_, aos_b_, t3a_, t3b_, s6a_, s6b_, px_, part_;
   bool graph_ = false;
   GraphOp gops_[4];

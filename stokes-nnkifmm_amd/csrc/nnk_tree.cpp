@@ -99,18 +99,29 @@ int build_tree(const double *pot, int64_t npot, const double *tgt, int64_t ntgt,
   return NNK_OK;
 }
 
+int validate_nn_csc(int64_t ncol, int64_t nfine, const double *fine, const int64_t *colptr, const int64_t *row,
+                    const double *val, std::string &err) {
+  if (nfine < 0) { err = "negative fine point count"; return NNK_EARG; }
+  if (!colptr) { err = "null NN colptr"; return NNK_EARG; }
+  if (nfine > 0 && !fine) { err = "null fine points"; return NNK_EARG; }
+  if (colptr[0] != 0) { err = "NN colptr[0] must be 0"; return NNK_EARG; }
+  for (int64_t c = 0; c < ncol; c++)
+    if (colptr[c + 1] < colptr[c]) { err = "NN colptr not monotone"; return NNK_EARG; }
+  const int64_t nnz = colptr[ncol];
+  if (nnz > 0 && (!row || !val)) { err = "null NN row indices or values"; return NNK_EARG; }
+  for (int64_t e = 0; e < nnz; e++)
+    if (row[e] < 0 || row[e] >= nfine) { err = "NN row index out of range"; return NNK_EARG; }
+  return NNK_OK;
+}
+
 int set_nearest(Tree &t, const double *fine, int64_t nfine, const int64_t *colptr,
                 const int64_t *row, const double *val, std::string &err) {
-  if (nfine < 0) { err = "negative fine point count"; return NNK_EARG; }
   if (t.npot != t.ntgt) {
     err = "NEAREST mode needs potential points == target points";
     return NNK_EARG;
   }
-  int64_t nnz = colptr[t.npot];
-  for (int64_t c = 0; c < t.npot; c++)
-    if (colptr[c + 1] < colptr[c]) { err = "NN colptr not monotone"; return NNK_EARG; }
-  for (int64_t e = 0; e < nnz; e++)
-    if (row[e] < 0 || row[e] >= nfine) { err = "NN row index out of range"; return NNK_EARG; }
+  if (int rc = validate_nn_csc(t.npot, nfine, fine, colptr, row, val, err)) return rc;
+  const int64_t nnz = colptr[t.npot];
   t.nearest = true;
   t.nfine = nfine;
   t.fine.assign(fine, fine + 3 * nfine);

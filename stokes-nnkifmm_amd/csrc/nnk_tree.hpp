@@ -57,6 +57,11 @@ int dedup_gpu(const double *pot, int64_t npot, const double *tgt, int64_t ntgt, 
 void build_lists_gpu(Tree &t);
 int set_nearest(Tree &t, const double *fine, int64_t nfine, const int64_t *colptr,
                 const int64_t *row, const double *val, std::string &err);
+// Checks a nearest map (nfine x ncol, CSC with 0-based rows) before any of
+// it is read on the GPU: colptr[0] == 0 and monotone, rows in [0, nfine),
+// no null array that would be read.  Returns NNK_OK or NNK_EARG with err.
+int validate_nn_csc(int64_t ncol, int64_t nfine, const double *fine, const int64_t *colptr, const int64_t *row,
+                    const double *val, std::string &err);
 
 enum { NNK_OK = 0, NNK_EARG = 1, NNK_EMAXDEPTH = 2, NNK_EHIP = 3, NNK_ENOMEM = 4, NNK_ESTATE = 5 };
 

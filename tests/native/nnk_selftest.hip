@@ -1,11 +1,15 @@
 // Test-only entry points (not part of the C ABI): exercise the GEMM kernel
-// against a host product, and read back the FMM's per-node arrays.
+// against a host product, read back the FMM's per-node arrays, probe MFMA
+// lane layouts, and run the host-only planners and input checks of the
+// library (the CPU tests call these without a GPU).
 #include <hip/hip_runtime.h>
 
+#include <cstring>
+#include <string>
 #include <vector>
 
-#include "../nnk_gemm.hpp"
-#include "../nnk_handles.hpp"
+#include "nnk_gemm.hpp"
+#include "nnk_handles.hpp"
 
 using namespace nnk;
 
@@ -122,6 +126,50 @@ extern "C" int nnk_selftest_mfma_bf16_probe(const uint16_t *a, const uint16_t *b
     NNK_HIP(hipMemcpy(d, dd.p, sizeof(float) * 256 * trials, hipMemcpyDeviceToHost));
     return 0;
   } catch (...) {
+    return 1;
+  }
+}
+
+static void copy_err(const std::string &m, char *err, int errlen) {
+  if (err && errlen > 0) {
+    std::strncpy(err, m.c_str(), (size_t)errlen - 1);
+    err[errlen - 1] = 0;
+  }
+}
+
+// Host only: the nearest-map check of nnk_tree_set_nearest.
+extern "C" int nnk_selftest_validate_nn(int64_t ncol, int64_t nfine, const double *fine, const int64_t *colptr,
+                                        const int64_t *row, const double *val, char *err, int errlen) {
+  std::string m;
+  int rc = nnk::validate_nn_csc(ncol, nfine, fine, colptr, row, val, m);
+  copy_err(m, err, errlen);
+  return rc;
+}
+
+// Host only: the mixed-form M2L plan of one level.  sizes[8] = nodes per
+// child class, gv = the classes' gather tables concatenated (189 x sizes[a]
+// each), count = node count.  Returns 0 with merged[0] and pairs[0..1] =
+// (class pairs, merged pairs or -1), or nonzero with the refusal in err.
+extern "C" int nnk_selftest_m2l_plan(const int32_t *sizes, const int32_t *gv, int64_t count, int BN,
+                                     int32_t *merged, int64_t *pairs, char *err, int errlen) {
+  try {
+    std::vector<int32_t> cls[8], g[8];
+    size_t off = 0;
+    for (int a = 0; a < 8; a++) {
+      for (int c = 0; c < sizes[a]; c++) cls[a].push_back(c);
+      const size_t n = (size_t)nnk::class_offset_slots(a).size() * sizes[a];
+      g[a].assign(gv + off, gv + off + n);
+      off += n;
+    }
+    nnk::M2lLevelPlan P;
+    nnk::plan_m2l_level(cls, g, count, BN, P);
+    merged[0] = P.merged ? 1 : 0;
+    pairs[0] = P.class_pairs;
+    pairs[1] = P.merged_pairs;
+    copy_err("", err, errlen);
+    return 0;
+  } catch (const std::exception &e) {
+    copy_err(e.what(), err, errlen);
     return 1;
   }
 }

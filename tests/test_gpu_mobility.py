@@ -136,3 +136,36 @@ def test_gmres_matches_oracle(oracle, nnk, precon):
     assert info["flag"] == 0 and infoo["flag"] == 0
     assert abs(info["iter"] - infoo["iter"]) <= 2, (info["iter"], infoo["iter"])
     assert rel(x, xo) < 1e-7
+
+
+@pytest.mark.parametrize("offset", [0.0, 0.3])
+def test_sphere_gravity_sedimentation_and_offset_torque(nnk, offset):
+    """mobilityProblemGravity (mobilityProblemGravity.m:25-74): F = G and
+    M = -(4/3) pi offset (b3 x G) (Swimmers.m:372-385).  A sphere sediments
+    at U = G / (6 pi mu a) and turns at Omega = M / (8 pi mu a^3) (Stokes'
+    laws, to the discretisation's 2 %); with b1 = e_x, b2 = e_z the body
+    axis b3 = -e_y, so G = -g e_z gives M = -(4/3) pi offset g e_x."""
+    sw = nnk.Swimmers()
+    c, fine = cube_sphere(8), cube_sphere(32)
+    NN = nnk.nearestMatrix(c, fine)
+    sw.addSwimmer(c, fine, NN, [0, 0, 0], [1, 0, 0, 0, 0, 1], np.zeros_like(c), [0, 0, 0], [0, 0, 0])
+    g = 6 * np.pi
+    out = nnk.mobilityProblemGravity(sw, lambda t, x: np.zeros_like(x), 0.0, np.r_[0, 0, 0, 1, 0, 0, 0, 0, 1],
+                                     [0.01, 1.0], ["nodeCapacity", 64, "nearest", 1], 0, 1, 0.1, 1e-8, 300,
+                                     [1e-6, 100], [0.0, 0.0, -g, offset])
+    info = nnk.mobility.last_solve
+    assert info["flag"] == 0
+    np.testing.assert_allclose(sw.F[0], [0, 0, -g])
+    M = -4.0 / 3.0 * np.pi * offset * np.cross([0, -1, 0], [0, 0, -g])
+    np.testing.assert_allclose(sw.Moment[0], M, atol=1e-14)
+    U = out[0:3]
+    assert abs(U[2] + 1) < 0.02 and np.abs(U[:2]).max() < 1e-6, U
+    # out[3:6] = Omega x b1, out[6:9] = Omega x b2 with b1 = e_x, b2 = e_z
+    w1, w2 = out[3:6], out[6:9]
+    Om = np.array([-w2[1], w2[0], w1[1]])
+    np.testing.assert_allclose(np.cross(Om, [1, 0, 0]), w1, atol=1e-12)
+    want = M / (8 * np.pi)
+    if offset == 0.0:
+        assert np.abs(Om).max() < 1e-6, Om
+    else:
+        assert abs(Om[0] / want[0] - 1) < 0.03 and np.abs(Om[1:]).max() < 1e-6 * abs(want[0]) + 1e-9, (Om, want)
