@@ -70,7 +70,26 @@ def build(verbose=False):
     _run([HIPCC, "-shared", "--offload-arch=gfx950", "-o", tmp2, sto, "-L" + LIBDIR, "-lnnkifmm",
           "-Wl,-rpath,$ORIGIN", "-Wl,-rpath,/opt/rocm/lib"])
     os.replace(tmp2, SELFTEST)
+    build_gateway()
     return LIB
+
+
+def build_gateway():
+    """The MATLAB marshalling layer (matlab/nnk_gateway.c, plain C, gcc) as
+    libnnk_gateway.so, and the C program that drives it the way the MEX
+    gateway does (tests/native/gateway_test.c)."""
+    gcc = os.environ.get("CC", "gcc")
+    inc = ["-I" + os.path.join(os.path.dirname(HERE), "include"), "-I" + os.path.join(HERE, "matlab")]
+    gw = os.path.join(LIBDIR, "libnnk_gateway.so")
+    _run([gcc, "-std=c99", "-O2", "-Wall", "-Wextra", "-fPIC", "-shared"] + inc +
+         [os.path.join(HERE, "matlab", "nnk_gateway.c"), "-o", gw + ".tmp", "-L" + LIBDIR, "-lnnkifmm",
+          "-Wl,-rpath,$ORIGIN"])
+    os.replace(gw + ".tmp", gw)
+    exe = os.path.join(LIBDIR, "nnk_gateway_test")
+    _run([gcc, "-std=c99", "-O2", "-Wall", "-Wextra"] + inc +
+         [os.path.join(os.path.dirname(HERE), "tests", "native", "gateway_test.c"), "-o", exe + ".tmp",
+          "-L" + LIBDIR, "-lnnk_gateway", "-lnnkifmm", "-Wl,-rpath,$ORIGIN"])
+    os.replace(exe + ".tmp", exe)
 
 
 if __name__ == "__main__":
