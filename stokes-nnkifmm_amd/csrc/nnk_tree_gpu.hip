@@ -24,7 +24,9 @@
 // stack: siblings have consecutive ids), counted, scanned, filled and
 // sorted with segmented sorts; X is the inverse of W by one radix sort.
 #include <hip/hip_runtime.h>
-#include <hipcub/hipcub.hpp>
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
+#include <rocprim/device/device_segmented_radix_sort.hpp>
 
 #include <algorithm>
 #include <cstring>
@@ -420,12 +422,12 @@ void build_lists_gpu(Tree &t) {
   // ptr[q][n] = sum of sizes of nodes < n (CSR over ids, count+2 entries)
   ptr.alloc(sizeof(int64_t) * 4 * stride);
   size_t tb = 0;
-  NNK_HIP(hipcub::DeviceScan::InclusiveSum(nullptr, tb, cnt.as<int32_t>(), ptr.as<int64_t>(), (int)stride));
+  NNK_HIP(rocprim::inclusive_scan(nullptr, tb, cnt.as<int32_t>(), ptr.as<int64_t>(), (size_t)stride, rocprim::plus<int64_t>()));
   tmp.alloc(tb);
   int64_t tot[4];
   for (int q = 0; q < 3; q++) {
-    NNK_HIP(hipcub::DeviceScan::InclusiveSum(tmp.p, tb, cnt.as<int32_t>() + q * stride,
-                                             ptr.as<int64_t>() + q * stride, (int)stride));
+    NNK_HIP(rocprim::inclusive_scan(tmp.p, tb, cnt.as<int32_t>() + q * stride,
+                                             ptr.as<int64_t>() + q * stride, (size_t)stride, rocprim::plus<int64_t>()));
     NNK_HIP(hipMemcpy(&tot[q], ptr.as<int64_t>() + q * stride + nc + 1, 8, hipMemcpyDeviceToHost));
   }
   DBuf idx[4], lastW, sorted;
@@ -442,12 +444,12 @@ void build_lists_gpu(Tree &t) {
     const int64_t *pq = pU + q * stride;
     sorted.alloc(sizeof(int32_t) * tot[q]);
     size_t sb = 0;
-    NNK_HIP(hipcub::DeviceSegmentedSort::SortKeys(nullptr, sb, idx[q].as<int32_t>(), sorted.as<int32_t>(),
-                                                  (int)tot[q], (int)(nc + 1), pq, pq + 1));
+    NNK_HIP(rocprim::segmented_radix_sort_keys(nullptr, sb, idx[q].as<int32_t>(), sorted.as<int32_t>(),
+                                               (unsigned)tot[q], (unsigned)(nc + 1), pq, pq + 1));
     DBuf st;
     st.alloc(sb);
-    NNK_HIP(hipcub::DeviceSegmentedSort::SortKeys(st.p, sb, idx[q].as<int32_t>(), sorted.as<int32_t>(),
-                                                  (int)tot[q], (int)(nc + 1), pq, pq + 1));
+    NNK_HIP(rocprim::segmented_radix_sort_keys(st.p, sb, idx[q].as<int32_t>(), sorted.as<int32_t>(),
+                                               (unsigned)tot[q], (unsigned)(nc + 1), pq, pq + 1));
     std::swap(idx[q], sorted);
   }
   if (tot[2]) {
@@ -470,14 +472,14 @@ void build_lists_gpu(Tree &t) {
                        v0.as<int32_t>(), cntX.as<int32_t>());
     NNK_HIP(hipGetLastError());
     size_t sb = 0;
-    NNK_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, sb, k0.as<uint64_t>(), k1.as<uint64_t>(), v0.as<int32_t>(),
+    NNK_HIP(rocprim::radix_sort_pairs(nullptr, sb, k0.as<uint64_t>(), k1.as<uint64_t>(), v0.as<int32_t>(),
                                                idx[3].as<int32_t>(), (int)tot[2], 0, 64));
     DBuf st;
     st.alloc(sb);
-    NNK_HIP(hipcub::DeviceRadixSort::SortPairs(st.p, sb, k0.as<uint64_t>(), k1.as<uint64_t>(), v0.as<int32_t>(),
+    NNK_HIP(rocprim::radix_sort_pairs(st.p, sb, k0.as<uint64_t>(), k1.as<uint64_t>(), v0.as<int32_t>(),
                                                idx[3].as<int32_t>(), (int)tot[2], 0, 64));
   }
-  NNK_HIP(hipcub::DeviceScan::InclusiveSum(tmp.p, tb, cntX.as<int32_t>(), ptrX.as<int64_t>(), (int)stride));
+  NNK_HIP(rocprim::inclusive_scan(tmp.p, tb, cntX.as<int32_t>(), ptrX.as<int64_t>(), (size_t)stride, rocprim::plus<int64_t>()));
   for (int q = 0; q < 4; q++) {
     const int64_t *pq = q < 3 ? pU + q * stride : ptrX.as<int64_t>();
     NNK_HIP(hipMemcpy(t.lptr[q].data(), pq, sizeof(int64_t) * stride, hipMemcpyDeviceToHost));
@@ -499,7 +501,7 @@ int dedup_gpu(const double *pot, int64_t npot, const double *tgt, int64_t ntgt, 
   va.alloc(sizeof(int32_t) * n);
   vb.alloc(sizeof(int32_t) * n);
   size_t tb = 0;
-  NNK_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, ka.as<uint64_t>(), kb.as<uint64_t>(), va.as<int32_t>(),
+  NNK_HIP(rocprim::radix_sort_pairs(nullptr, tb, ka.as<uint64_t>(), kb.as<uint64_t>(), va.as<int32_t>(),
                                              vb.as<int32_t>(), (int)n, 0, 64));
   tmp.alloc(tb);
   // lexicographic (x, y, z) order, equal rows in index order: stable sorts
@@ -509,7 +511,7 @@ int dedup_gpu(const double *pot, int64_t npot, const double *tgt, int64_t ntgt, 
     hipLaunchKernelGGL(k_row_key, dim3(blocks(n)), dim3(256), 0, 0, n, R.as<double>(), ord, coord,
                        ka.as<uint64_t>(), va.as<int32_t>());
     NNK_HIP(hipGetLastError());
-    NNK_HIP(hipcub::DeviceRadixSort::SortPairs(tmp.p, tb, ka.as<uint64_t>(), kb.as<uint64_t>(), va.as<int32_t>(),
+    NNK_HIP(rocprim::radix_sort_pairs(tmp.p, tb, ka.as<uint64_t>(), kb.as<uint64_t>(), va.as<int32_t>(),
                                                vb.as<int32_t>(), (int)n, 0, 64));
     std::swap(va, vb);   // va: current order
     ord = va.as<int32_t>();
@@ -519,10 +521,10 @@ int dedup_gpu(const double *pot, int64_t npot, const double *tgt, int64_t ntgt, 
   hipLaunchKernelGGL(k_row_heads, dim3(blocks(n)), dim3(256), 0, 0, n, R.as<double>(), ord, head.as<int32_t>());
   NNK_HIP(hipGetLastError());
   size_t sb = 0;
-  NNK_HIP(hipcub::DeviceScan::InclusiveSum(nullptr, sb, head.as<int32_t>(), gscan.as<int32_t>(), (int)n));
+  NNK_HIP(rocprim::inclusive_scan(nullptr, sb, head.as<int32_t>(), gscan.as<int32_t>(), (size_t)n, rocprim::plus<int32_t>()));
   DBuf stmp;
   stmp.alloc(sb);
-  NNK_HIP(hipcub::DeviceScan::InclusiveSum(stmp.p, sb, head.as<int32_t>(), gscan.as<int32_t>(), (int)n));
+  NNK_HIP(rocprim::inclusive_scan(stmp.p, sb, head.as<int32_t>(), gscan.as<int32_t>(), (size_t)n, rocprim::plus<int32_t>()));
   leader.alloc(sizeof(int32_t) * n);
   rep.alloc(sizeof(int32_t) * n);
   for (int pass = 0; pass < 2; pass++) {
@@ -534,7 +536,7 @@ int dedup_gpu(const double *pot, int64_t npot, const double *tgt, int64_t ntgt, 
   uidx.alloc(sizeof(int32_t) * n);
   hipLaunchKernelGGL(k_row_keep, dim3(blocks(n)), dim3(256), 0, 0, n, rep.as<int32_t>(), keep.as<int32_t>());
   NNK_HIP(hipGetLastError());
-  NNK_HIP(hipcub::DeviceScan::ExclusiveSum(stmp.p, sb, keep.as<int32_t>(), uidx.as<int32_t>(), (int)n));
+  NNK_HIP(rocprim::exclusive_scan(stmp.p, sb, keep.as<int32_t>(), uidx.as<int32_t>(), 0, (size_t)n, rocprim::plus<int32_t>()));
   int32_t last[2];
   NNK_HIP(hipMemcpy(&last[0], keep.as<int32_t>() + n - 1, 4, hipMemcpyDeviceToHost));
   NNK_HIP(hipMemcpy(&last[1], uidx.as<int32_t>() + n - 1, 4, hipMemcpyDeviceToHost));
@@ -574,10 +576,10 @@ int subdivide_gpu(Tree &t, double m, double M, std::string &err) {
   NNK_HIP(hipGetLastError());
   {
     size_t tb = 0;
-    NNK_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, key0.as<uint64_t>(), key1.as<uint64_t>(),
+    NNK_HIP(rocprim::radix_sort_pairs(nullptr, tb, key0.as<uint64_t>(), key1.as<uint64_t>(),
                                                row0.as<int32_t>(), row1.as<int32_t>(), (int)np, 0, 63));
     tmp.alloc(tb);
-    NNK_HIP(hipcub::DeviceRadixSort::SortPairs(tmp.p, tb, key0.as<uint64_t>(), key1.as<uint64_t>(),
+    NNK_HIP(rocprim::radix_sort_pairs(tmp.p, tb, key0.as<uint64_t>(), key1.as<uint64_t>(),
                                                row0.as<int32_t>(), row1.as<int32_t>(), (int)np, 0, 63));
   }
   const uint64_t *key = key1.as<uint64_t>();
@@ -606,10 +608,10 @@ int subdivide_gpu(Tree &t, double m, double M, std::string &err) {
                        P.end.as<int64_t>(), (int64_t)t.capacity, P.split.as<int32_t>());
     NNK_HIP(hipGetLastError());
     size_t tb = 0;
-    NNK_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, P.split.as<int32_t>(), P.scan.as<int32_t>(), (int)P.n));
+    NNK_HIP(rocprim::exclusive_scan(nullptr, tb, P.split.as<int32_t>(), P.scan.as<int32_t>(), 0, (size_t)P.n, rocprim::plus<int32_t>()));
     DBuf stmp;
     stmp.alloc(tb);
-    NNK_HIP(hipcub::DeviceScan::ExclusiveSum(stmp.p, tb, P.split.as<int32_t>(), P.scan.as<int32_t>(), (int)P.n));
+    NNK_HIP(rocprim::exclusive_scan(stmp.p, tb, P.split.as<int32_t>(), P.scan.as<int32_t>(), 0, (size_t)P.n, rocprim::plus<int32_t>()));
     int32_t last[2];
     NNK_HIP(hipMemcpy(&last[0], P.split.as<int32_t>() + P.n - 1, 4, hipMemcpyDeviceToHost));
     NNK_HIP(hipMemcpy(&last[1], P.scan.as<int32_t>() + P.n - 1, 4, hipMemcpyDeviceToHost));
