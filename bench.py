@@ -15,6 +15,11 @@ Forms timed (all reported under "forms"):
                1e-6 relative error)
   mixedh_res8  two-part split-FP16 M2L on power-of-two scaled operands at
                coronaRes 8: the same accuracy with half the MFMAs
+  fp64t_res8   the truncated-SVD operators of mixed3_res8 (cut 1e-8,
+               rank 605) with every operation in FP64: FP64-MFMA R x R M2L
+               and FP64 direct sums (the FP64 comparator below 1e-6)
+  fp64t_res9_tol3e-7
+               the same on the headline form's operators (rank 477)
   mixedh_res9_tol3e-7
                the same at coronaRes 9 with the pseudo-inverses cut at
                3e-7 (rank 477 instead of 605 at 8 / 1e-8, below one
@@ -27,6 +32,9 @@ Prints one JSON line.  Besides the throughput it reports:
                 mixed: 3 BF16 products of 2 R^2 per pair, R the
                 pseudo-inverse rank) over its HIP-event time in the timed
                 region, against the MI355X dense MFMA peak of that type;
+                roofline.matvec: the whole matvec, sum over the kernel
+                classes of (algorithmic flops / that class's peak) over the
+                measured time per matvec, with the per-class terms;
   cpu_baseline  the CPU oracle (the reference's algorithm restated in C with
                 OpenMP) timed on this host on a bounded sample;
   rel_err       ||u - u_direct|| / ||u_direct|| on 4096 sampled targets,
@@ -53,7 +61,10 @@ sys.path.insert(0, ROOT)
 
 FP64_MFMA_PEAK_TFLOPS = 78.6
 BF16_MFMA_PEAK_TFLOPS = 2500.0
+FP64_VALU_PEAK_TFLOPS = 78.6
+FP32_VALU_PEAK_TFLOPS = 157.3
 HBM_PEAK_GBS = 8000.0
+PAIR_FLOPS = 33   # one regularized-Stokeslet source-target interaction
 
 
 def parse():
@@ -64,7 +75,8 @@ def parse():
     ap.add_argument("--n", type=int, default=1 << 20)
     ap.add_argument("--res", type=int, default=6, help="corona resolution of the CPU baseline")
     ap.add_argument("--svd-tol", type=float, default=1e-8, help="pseudo-inverse cut of the mixed forms")
-    ap.add_argument("--forms", default="fp64_res6,fp64_res8,mixed_res6,mixed3_res8,mixedh_res8,mixedh_res9_tol3e-7",
+    ap.add_argument("--forms", default="fp64_res6,fp64_res8,fp64t_res8,fp64t_res9_tol3e-7,mixed_res6,mixed3_res8,"
+                                      "mixedh_res8,mixedh_res9_tol3e-7",
                     help="comma-separated forms <precision>_res<r>[_tol<svd tol>] to time; the last one is the "
                          "headline")
     ap.add_argument("--no-solve", action="store_true",
@@ -73,7 +85,10 @@ def parse():
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default M2L launch shapes (no timing of candidates in the warm-up)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
-    ap.add_argument("--cpu-n", type=int, default=1 << 16, help="CPU baseline sample size")
+    ap.add_argument("--cpu-n", type=int, default=1 << 16, help="CPU baseline sample size (all host threads)")
+    ap.add_argument("--cpu-n1", type=int, default=1 << 12, help="CPU baseline sample size on one thread")
+    ap.add_argument("--solve-only", choices=["sphere", "spheroids", "batched", "adjoint", "ode45"], default=None,
+                    help="run only this mobility solve and print its record (for profiling runs)")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="only run a device-to-device copy of this many MiB (3 times): a kernel of "
                          "known traffic for scaling the FETCH_SIZE / WRITE_SIZE counters")
@@ -82,22 +97,127 @@ def parse():
 
 def cpu_baseline(args, eps, mu):
     """Time the CPU oracle's matvec on a bounded sample of the workload: the
-    same distribution and parameters at N = cpu_n (levels 0-3, ~32 points
-    per leaf as at 2^20)."""
+    same distribution and parameters at N = cpu_n on every host thread, and
+    at N = cpu_n1 on one thread."""
     import oracle
     oracle.build()
-    rng = np.random.Generator(np.random.PCG64(args.seed + 1))
-    p = rng.random((args.cpu_n, 3))
-    q = rng.standard_normal((args.cpu_n, 3))
-    tree = oracle.Tree(p, p, nodeCapacity=64)
+    allc = oracle.num_threads()
+
+    def one(n, threads):
+        oracle.set_threads(threads)
+        rng = np.random.Generator(np.random.PCG64(args.seed + 1))
+        p = rng.random((n, 3))
+        q = rng.standard_normal((n, 3))
+        tree = oracle.Tree(p, p, nodeCapacity=64)
+        t0 = time.perf_counter()
+        tree.matvec(q, eps, mu, args.res, 1)
+        return time.perf_counter() - t0
+    dt = one(args.cpu_n, allc)
+    dt1 = one(args.cpu_n1, 1)
+    oracle.set_threads(allc)
+    what = ("CPU oracle (reference algorithm: per-node surfaces, dense kernel blocks, per-node pivoted LU, "
+            "OpenMP over the nodes of a level), same distribution, eps, nodeCapacity 64, coronaRes "
+            f"{args.res}")
+    return {"value": args.cpu_n / dt / 1e6, "unit": "Mpts/s", "cores": allc, "kind": "port",
+            "sample": f"{what}; N={args.cpu_n} on {allc} threads: {dt:.2f} s for one matvec",
+            "single_core": {"value": args.cpu_n1 / dt1 / 1e6, "unit": "Mpts/s", "cores": 1,
+                            "sample": f"N={args.cpu_n1} on 1 thread: {dt1:.2f} s for one matvec"}}
+
+
+def _oracle_system(sw, eps, cap):
+    """The oracle twin (oracle/mobility.py) of a Swimmers system."""
+    import scipy.sparse as sp
+    import oracle
+    from oracle.mobility import MobilityOracle
+    pts, fpts, NNall = sw.getSwimmerBnd()
+    t = oracle.Tree(pts, pts, nodeCapacity=cap, finePoints=fpts, NN=NNall)
+    _, _, NNs, rot, frot = sw.getSwimmers()
+    sizes, _ = sw.getSwimmerSizes()
+    ptr = np.concatenate([[0], np.cumsum(sizes)])
+    w = np.asarray(sp.csc_matrix(NNs).sum(0)).ravel()
+    pm = sp.csc_matrix(NNs).T @ frot
+    return MobilityOracle(t, eps, 1.0, ptr, rot, w, pm)
+
+
+def cpu_solves(nnk, gpu):
+    """The solves on the CPU oracle (the reference's algorithm, every host
+    thread).  The one-sphere solve runs in full.  The larger ones are timed
+    per GMRES iteration (one saddle-point apply and one preconditioner
+    application) on the stated or a labelled reduced system, and the solve
+    time is estimated as that time x the GPU run's iteration count (x the
+    point ratio for the reduced suspension)."""
+    import importlib as il
+    import oracle
+    oracle.build()
+    geo = il.import_module("stokes-nnkifmm_amd.geometries")
+    out = {"threads": oracle.num_threads(), "kind": "port"}
+    # one sphere (384 / 6144 points, eps 0.01), preconditioned GMRES to 1e-6
+    sw = nnk.Swimmers()
+    c, fine = geo.cube_sphere(8), geo.cube_sphere(32)
+    sw.addSwimmer(c, fine, nnk.nearestMatrix(c, fine), [0, 0, 0], [1, 0, 0, 0, 1, 0], np.zeros_like(c),
+                  [6 * np.pi, 0, 0], [0, 0, 0])
+    m = _oracle_system(sw, 0.01, 64)
+    b = np.zeros(m.n)
+    b[m.n3:m.n3 + 3] = [6 * np.pi, 0, 0]
     t0 = time.perf_counter()
-    tree.matvec(q, eps, mu, args.res, 1)
+    x, info = m.gmres(b, 1e-6, 200, True)
     dt = time.perf_counter() - t0
-    return {"value": args.cpu_n / dt / 1e6, "unit": "Mpts/s", "cores": oracle.num_threads(),
-            "kind": "port",
-            "sample": f"CPU oracle (reference algorithm: per-node surfaces, dense kernel blocks, "
-                      f"per-node pivoted LU, OpenMP over the nodes of a level) on N={args.cpu_n} "
-                      f"uniform points, same eps/nodeCapacity/coronaRes; {dt:.2f} s for one matvec"}
+    g = gpu.get("mobility_solve_one_sphere", {}).get("precon", {})
+    out["one_sphere"] = {"wall_s": round(dt, 3), "iterations": int(info["iter"]), "U_x": float(x[m.n3]),
+                         "gpu_wall_s": g.get("wall_s"), "gpu_iterations": g.get("iterations")}
+
+    def per_iteration(m):
+        x = np.random.default_rng(1).standard_normal(m.n)
+        t0 = time.perf_counter()
+        m.apply(m.precon(x))
+        return time.perf_counter() - t0
+    # spheroid suspension, reduced: 8 spheroids (384 / 6144 points) above a
+    # 12 x 12 wall in an 8^3 box, eps 0.005
+    sw = geo.spheroid_suspension(nbody=8, n=8, wall_n=12, box=8.0)
+    m = _oracle_system(sw, 0.005, 64)
+    it = per_iteration(m)
+    g = gpu.get("mobility_solve_spheroids_wall", {})
+    n_full = g.get("force_points")
+    gi = g.get("fp64_precon", {}).get("iterations")
+    rec = {"system": f"8 spheroids + 12 x 12 wall, {m.N} force points (reduced from the stated 64 + 40 x 40)",
+           "s_per_iteration": round(it, 3)}
+    if n_full and gi:
+        rec["estimated_full_solve_s"] = round(it * n_full / m.N * gi, 1)
+        rec["estimate"] = (f"s per iteration x {n_full}/{m.N} points x {gi} GPU FP64 iterations "
+                           "(the oracle's cost grows about linearly with the points at fixed leaf size)")
+        rec["gpu_fp64_wall_s"] = g.get("fp64_precon", {}).get("wall_s")
+    out["spheroids_wall_reduced"] = rec
+    # squirmer pair (2 x 1536 / 24576 points, eps 0.01, B2 = -5) at its stated size
+    sw = geo.squirmer_pair(n=16, refine=4, gap=0.5, B1=1.0, B2=-5.0)
+    m = _oracle_system(sw, 0.01, 64)
+    it = per_iteration(m)
+    a = gpu.get("adjoint_squirmer_pair", {})
+    gi = max(a.get("adjoint_iterations") or [0])
+    out["squirmer_pair"] = {"force_points": int(m.N), "s_per_iteration": round(it, 3),
+                            "estimated_forward_solve_s": round(it * gi, 1) if gi else None,
+                            "estimated_central_difference_s": round(2 * it * gi, 1) if gi else None,
+                            "estimate": "s per iteration x the GPU solve's iteration count; one central "
+                                        "difference is two forward solves, and the gradient with respect "
+                                        f"to all {3 * m.N} node coordinates by differences takes "
+                                        f"{6 * m.N} of them (the adjoint: one transpose solve)",
+                            "gpu_forward_s": a.get("forward_s"), "gpu_transpose_gmres_s": a.get("transpose_gmres_s")}
+    # batched single bodies: the problem of median size, per iteration
+    bodies, eps = geo.batch_bodies()
+    sizes = np.array([len(b.Geometries[0]) for b in bodies])
+    k = int(np.argsort(sizes)[len(sizes) // 2])
+    m = _oracle_system(bodies[k], float(eps[k]), 64)
+    it = per_iteration(m)
+    bt = gpu.get("batched_solves", {})
+    npts = np.array([len(b.Geometries[0]) for b in bodies], float)
+    mi = bt.get("fp64", {}).get("mean_iterations")
+    rec = {"median_problem_points": int(m.N), "s_per_iteration": round(it, 3)}
+    if mi:
+        rec["estimated_batch_s"] = round(it * float((npts / m.N).sum()) * mi, 1)
+        rec["estimate"] = ("s per iteration of the median problem x sum over the 256 problems of "
+                           "their point ratio x the GPU's mean iteration count")
+        rec["gpu_fp64_wall_s"] = bt.get("fp64", {}).get("wall_s")
+    out["batched_256"] = rec
+    return out
 
 
 def run_form(nnk, lib, tree, fd, args, eps, mu, precision, res, svd_tol):
@@ -135,8 +255,9 @@ def run_form(nnk, lib, tree, fd, args, eps, mu, precision, res, svd_tol):
     K, R = int(st[12]), int(st[13])
     ms_step = dt / args.steps * 1e3
     phases = ["strengths", "S2M", "up", "M2L", "L2L", "P2L", "down", "leaf"]
-    if precision == "fp64":
-        flops = 2.0 * K * K * st[0]
+    if precision in ("fp64", "fp64t"):
+        r = K if precision == "fp64" else R
+        flops = 2.0 * r * r * st[0]
         peak, kern = FP64_MFMA_PEAK_TFLOPS, "k_gemm_gather<160,64,16,1> (M2L, FP64 MFMA)"
     else:
         np_, f16, what = {"mixed": (2, "false", "two-part split-BF16"), "mixed3": (3, "false", "three-part split-BF16"),
@@ -150,7 +271,8 @@ def run_form(nnk, lib, tree, fd, args, eps, mu, precision, res, svd_tol):
     m2l_ms = ms[3] / args.steps
     achieved = flops / (m2l_ms * 1e-3) / 1e12 if m2l_ms > 0 else 0.0
     rec = {"precision": precision, "coronaRes": res, "svd_tol": svd_tol if precision != "fp64" else None,
-           "rank": R if precision != "fp64" else K, "Mpts_s": round(tree.points.shape[0] / (ms_step * 1e-3) / 1e6, 4),
+           "rank": R if precision != "fp64" else K,
+           "matvec_roofline": matvec_roofline(precision, st, K, R, ms_step), "Mpts_s": round(tree.points.shape[0] / (ms_step * 1e-3) / 1e6, 4),
            "ms_per_step": round(ms_step, 3), "setup_s": round(t_setup, 3),
            "phase_ms_per_step": {p: round(float(ms[i] / args.steps), 3) for i, p in enumerate(phases)},
            "m2l_launch_per_level": {str(l): {"split": int(spl[l]), "depth": int(dep[l])}
@@ -160,6 +282,41 @@ def run_form(nnk, lib, tree, fd, args, eps, mu, precision, res, svd_tol):
                    "launches_per_step": float(cnt[3] / args.steps), "achieved_tflops": round(achieved, 3),
                    "peak_tflops": peak}}
     return u, rec
+
+
+def matvec_roofline(precision, st, K, R, ms_step):
+    """Sum over the matvec's kernel classes of (algorithmic flops / that
+    class's peak), over the measured time per matvec.  Work counts from
+    nnk_fmm_stats: 0 M2L pairs, 1 P2P pairs, 2 S2M pairs, 3 L2T pairs,
+    4 M2T pairs, 5 P2L pairs, 6 up-solve nodes, 7 down-solve nodes, 8 M2M
+    parents, 9 L2L nodes.  FP64 form: every translation and solve is a
+    (3Nq)^2 FP64 GEMM per node (two factors per solve).  Truncated forms:
+    R x R translations and the leaves' R x 3Nq up expansions (the down and
+    W expansions are not counted); M2L on 16-bit parts counts every split
+    product (3 or 6); direct sums at 33 flop per pair on the VALU (FP32
+    math in the mixed forms)."""
+    f64m, b16 = FP64_MFMA_PEAK_TFLOPS, BF16_MFMA_PEAK_TFLOPS
+    v64, v32 = FP64_VALU_PEAK_TFLOPS, FP32_VALU_PEAK_TFLOPS
+    direct_peak = v32 if precision in ("mixed", "mixed3", "mixedh") else v64
+    pairs = float(st[1] + st[2] + st[3] + st[4] + st[5])
+    cls = {"direct_sums (P2P, S2M, L2T, M2T, P2L)": (PAIR_FLOPS * pairs, direct_peak)}
+    if precision == "fp64":
+        k2 = 2.0 * K * K
+        cls["M2L"] = (k2 * st[0], f64m)
+        cls["M2M + L2L"] = (k2 * (8 * st[8] + st[9]), f64m)
+        cls["equivalent-density solves"] = (2 * k2 * (st[6] + st[7]), f64m)
+    else:
+        nprod = {"mixed": 3, "mixed3": 6, "mixedh": 3, "fp64t": 1}[precision]
+        cls["M2L"] = (nprod * 2.0 * R * R * st[0], f64m if precision == "fp64t" else b16)
+        cls["M2M + L2L"] = (2.0 * R * R * (8 * st[8] + st[9]), f64m)
+        # up expansion R x 3Nq per leaf with sources (nodes with an up
+        # density minus M2M parents); the down and W expansions (3Nq x R
+        # per target leaf / W node) are left out, which can only lower frac
+        cls["up expansion (leaves)"] = (2.0 * R * K * (st[6] - st[8]), f64m)
+    ideal_ms = sum(f / (pk * 1e12) for f, pk in cls.values()) * 1e3
+    return {"frac": round(ideal_ms / ms_step, 4), "ideal_ms": round(ideal_ms, 3),
+            "classes": {k: {"flops": f, "peak_tflops": pk, "ideal_ms": round(f / (pk * 1e12) * 1e3, 4)}
+                        for k, (f, pk) in cls.items()}}
 
 
 def direct_sample(lib, pts, fd, idx, eps, mu):
@@ -204,8 +361,9 @@ def spheroid_solve(nnk):
     """64 prolate spheroids (aspect 3, semi-major axis 1) above a 40 x 40
     wall in a 20^3 box, 4x refined NEAREST quadrature (~126k force / ~2M
     quadrature points), eps = 0.005, each body pulled by a unit force:
-    GMRES to 1e-6 with the block preconditioner, FP64 and by iterative
-    refinement around the mixed form (FP64 residuals)."""
+    GMRES to 1e-6 with the block preconditioner, FP64, by iterative
+    refinement around the mixed form (FP64 residuals) and by flexible GMRES
+    (FP64 Arnoldi, mixed-form inner solves to 1e-2)."""
     import torch
     import importlib as il
     geo = il.import_module("stokes-nnkifmm_amd.geometries")
@@ -248,6 +406,16 @@ def spheroid_solve(nnk):
         "wall_s": round(dt, 3), "inner_iterations": info["iter"], "outer": info["outer"],
         "relres_fp64": float(np.linalg.norm(r) / np.linalg.norm(b)),
         "body_velocity_diff_vs_fp64": float(np.linalg.norm(xr[-nb:] - x64[-nb:]) / np.linalg.norm(x64[-nb:]))}
+    t0 = time.perf_counter()
+    xf = mob.mobility_refined(sw, B, BT, A1, A2, b, 1e-6, 400, True, innerTol=1e-2, outer="fgmres")
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    info = mob.last_solve
+    r = nnk.mobilityMatrixMulti(xf, sw, B, BT) - b
+    out["mixed_fgmres_precon"] = {
+        "wall_s": round(dt, 3), "inner_iterations": info["iter"], "outer": info["outer"], "inner_tol": 1e-2,
+        "relres_fp64": float(np.linalg.norm(r) / np.linalg.norm(b)),
+        "body_velocity_diff_vs_fp64": float(np.linalg.norm(xf[-nb:] - x64[-nb:]) / np.linalg.norm(x64[-nb:]))}
     return out
 
 
@@ -386,6 +554,11 @@ def main():
         return
     nnk = importlib.import_module("stokes-nnkifmm_amd")
     lib = nnk.lib()
+    if args.solve_only:
+        fn = {"sphere": sphere_solve, "spheroids": spheroid_solve, "batched": batched_solves,
+              "adjoint": adjoint_pair, "ode45": time_stepping_squirmers}[args.solve_only]
+        print(json.dumps({"solve": args.solve_only, "record": fn(nnk)}), flush=True)
+        return
     eps, mu, N = 1e-3, 1.0, args.n
     rng = np.random.Generator(np.random.PCG64(args.seed))
     pts = rng.random((N, 3))
@@ -404,7 +577,7 @@ def main():
         parts = name.split("_")
         prec, res = parts[0], int(parts[1][3:])
         tol = float(parts[2][3:]) if len(parts) > 2 else args.svd_tol
-        if prec not in ("fp64", "mixed", "mixed3", "mixedh") or not parts[1].startswith("res"):
+        if prec not in ("fp64", "fp64t", "mixed", "mixed3", "mixedh") or not parts[1].startswith("res"):
             raise SystemExit(f"unknown form {name}")
         u, rec = run_form(nnk, lib, tree, fd, args, eps, mu, prec, res, 0.0 if prec == "fp64" else tol)
         uo = u.cpu().numpy()[idx]
@@ -424,7 +597,7 @@ def main():
     roof = {"bound": "mfma", "achieved": m["achieved_tflops"], "peak": m["peak_tflops"], "unit": "TFLOP/s",
             "frac": round(m["achieved_tflops"] / m["peak_tflops"], 4), "traffic": traffic,
             "kernel": m["kernel"], "launches_per_step": m["launches_per_step"],
-            "flops_per_step": m["flops_per_step"]}
+            "flops_per_step": m["flops_per_step"], "matvec": head["matvec_roofline"]}
     out = {
         "metric": "FMM matvec Mpts/s + mobility solve time at 1e-6 rel err, FP32/MFMA vs FP64",
         "value": head["Mpts_s"],
@@ -468,6 +641,10 @@ def main():
         print("# time stepping done", file=sys.stderr, flush=True)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args, eps, mu)
+        print("# cpu matvec baseline done", file=sys.stderr, flush=True)
+        if not args.no_solve:
+            out["cpu_baseline"]["solves"] = cpu_solves(nnk, out)
+            print("# cpu solve baselines done", file=sys.stderr, flush=True)
     print(json.dumps(out), flush=True)
 
 

@@ -50,8 +50,12 @@ enum { NNK_LIST_U = 0, NNK_LIST_V = 1, NNK_LIST_W = 2, NNK_LIST_X = 3 };
  * NNK_MIXED3: as NNK_MIXED with three BF16 parts (6 products, ~24 bits).
  * NNK_MIXED_H: as NNK_MIXED with two FP16 parts (3 products, ~22 bits) on
  * operands scaled by powers of two (per operator row; per matvec for the
- * coefficients). */
-enum { NNK_FP64 = 0, NNK_MIXED = 1, NNK_MIXED3 = 2, NNK_MIXED_H = 3 };
+ * coefficients).
+ * NNK_FP64_T: the truncated-SVD pseudo-inverses and R x R translations of
+ * the mixed forms with every operation in FP64 (M2L on the FP64 matrix
+ * cores, FP64 direct sums): the FP64 comparator at the mixed forms'
+ * accuracy. */
+enum { NNK_FP64 = 0, NNK_MIXED = 1, NNK_MIXED3 = 2, NNK_MIXED_H = 3, NNK_FP64_T = 4 };
 
 /* Library version (major*10000 + minor*100 + patch). */
 int32_t nnk_version(void);
@@ -253,6 +257,19 @@ int32_t nnk_mobility_set_graph(nnk_mobility *h, int32_t on);
 int32_t nnk_mobility_solve_refined(nnk_mobility *h, nnk_fmm *inner, nnk_fmm *inner_sparse, const double *rhs,
                                    double tol, int32_t maxit, double inner_tol, int32_t precon, double *x,
                                    int32_t *flag, double *relres, int32_t *iters, int32_t *outer);
+
+/* No counterpart in the reference (its slot is the gmres call of
+ *           Mobility/mobilityProblem.m:53,57): the same solve by flexible
+ *           GMRES (Saad 1993).  The outer Arnoldi process runs in FP64 on the
+ *           FP64 operator of nnk_mobility_create; its k-th basis vector is
+ *           preconditioned by a GMRES solve on `inner` to inner_tol (right-
+ *           preconditioned as in nnk_mobility_solve_refined when precon !=
+ *           0), and x is assembled from those preconditioned vectors.
+ *           Arguments and outputs as nnk_mobility_solve_refined; *outer =
+ *           outer Arnoldi steps (at most 50, no restart). */
+int32_t nnk_mobility_solve_fgmres(nnk_mobility *h, nnk_fmm *inner, nnk_fmm *inner_sparse, const double *rhs,
+                                  double tol, int32_t maxit, double inner_tol, int32_t precon, double *x,
+                                  int32_t *flag, double *relres, int32_t *iters, int32_t *outer);
 
 void nnk_mobility_destroy(nnk_mobility *h);
 

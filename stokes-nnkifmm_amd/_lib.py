@@ -62,6 +62,9 @@ SIGNATURES = {
     "nnk_mobility_solve_refined": (ctypes.c_int32, [_vp, _vp, _vp, _d, ctypes.c_double, ctypes.c_int32,
                                                     ctypes.c_double, ctypes.c_int32, _d, _i32, _d, _i32,
                                                     _i32]),
+    "nnk_mobility_solve_fgmres": (ctypes.c_int32, [_vp, _vp, _vp, _d, ctypes.c_double, ctypes.c_int32,
+                                                    ctypes.c_double, ctypes.c_int32, _d, _i32, _d, _i32,
+                                                    _i32]),
     "nnk_fmm_set_graph": (ctypes.c_int32, [_vp, ctypes.c_int32]),
     "nnk_fmm_set_autotune": (ctypes.c_int32, [_vp, ctypes.c_int32]),
     "nnk_fmm_launch_config": (ctypes.c_int32, [_vp, _i32, _i32, ctypes.c_int32]),

@@ -15,7 +15,7 @@ LIB = os.path.join(LIBDIR, "libnnkifmm.so")
 SELFTEST = os.path.join(LIBDIR, "libnnk_selftest.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["nnk_tree.cpp", "nnk_tree_gpu.hip", "nnk_svd.cpp", "nnk_fmm.hip", "nnk_direct.hip", "nnk_reduced.hip",
+SOURCES = ["nnk_tree.cpp", "nnk_tree_gpu.hip", "nnk_svd.hip", "nnk_fmm.hip", "nnk_direct.hip", "nnk_reduced.hip",
            "nnk_nearest.hip", "nnk_mobility.hip", "nnk_mixed.hip", "nnk_batch.hip", "nnk_capi.cpp"]
 
 # -packed-fp32-ops: no packed FP32 instructions in device code.

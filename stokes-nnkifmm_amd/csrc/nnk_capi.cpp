@@ -158,7 +158,8 @@ int32_t nnk_fmm_create(const nnk_tree *t, double eps, double mu, int32_t res, in
   return guard([&]() -> int32_t {
     if (!t || !out) return fail(NNK_EARG, "nnk_fmm_create: null argument");
     *out = nullptr;
-    if (precision != NNK_FP64 && precision != NNK_MIXED && precision != NNK_MIXED3 && precision != NNK_MIXED_H)
+    if (precision != NNK_FP64 && precision != NNK_MIXED && precision != NNK_MIXED3 && precision != NNK_MIXED_H &&
+        precision != NNK_FP64_T)
       return fail(NNK_EARG, "nnk_fmm_create: unknown precision");
     auto f = std::make_unique<nnk::Fmm>(t->t, eps, mu, res, shells, precision,
                                         svd_tol > 0 ? svd_tol : 1e-8);
@@ -397,6 +398,39 @@ int32_t nnk_mobility_solve_refined(nnk_mobility *h, nnk_fmm *inner, nnk_fmm *inn
     int nout = 0;
     nnk::GmresResult R =
         h->m->refine(db.as<double>(), dx.as<double>(), tol, maxit, inner_tol, precon != 0, 0, nullptr, &nout);
+    h->m->set_inner(nullptr);
+    NNK_HIP(hipMemcpy(x, dx.p, sizeof(double) * n, hipMemcpyDeviceToHost));
+    if (flag) *flag = R.flag;
+    if (relres) *relres = R.relres;
+    if (iters) *iters = R.iter;
+    if (outer) *outer = nout;
+    return NNK_OK;
+  });
+}
+
+int32_t nnk_mobility_solve_fgmres(nnk_mobility *h, nnk_fmm *inner, nnk_fmm *inner_sparse, const double *rhs,
+                                  double tol, int32_t maxit, double inner_tol, int32_t precon, double *x,
+                                  int32_t *flag, double *relres, int32_t *iters, int32_t *outer) {
+  return guard([&]() -> int32_t {
+    if (!h || !inner || !rhs || !x || maxit < 1 || !(tol > 0) || !(inner_tol > 0))
+      return fail(NNK_EARG, "nnk_mobility_solve_fgmres: bad argument");
+    if (inner->npot != h->m->points() || inner->ntgt != h->m->points() ||
+        (inner_sparse && (inner_sparse->npot != h->m->points() || inner_sparse->ntgt != h->m->points())))
+      return fail(NNK_EARG, "nnk_mobility_solve_fgmres: inner operator is not on the mobility tree");
+    const int64_t n = h->m->size();
+    nnk::DBuf db, dx;
+    db.alloc(sizeof(double) * n);
+    dx.alloc(sizeof(double) * n);
+    NNK_HIP(hipMemcpy(db.p, rhs, sizeof(double) * n, hipMemcpyHostToDevice));
+    h->m->set_inner(inner->f.get(), inner_sparse ? inner_sparse->f.get() : nullptr);
+    int nout = 0;
+    nnk::GmresResult R;
+    try {
+      R = h->m->fgmres(db.as<double>(), dx.as<double>(), tol, maxit, inner_tol, precon != 0, 0, nullptr, &nout);
+    } catch (...) {
+      h->m->set_inner(nullptr);
+      throw;
+    }
     h->m->set_inner(nullptr);
     NNK_HIP(hipMemcpy(x, dx.p, sizeof(double) * n, hipMemcpyDeviceToHost));
     if (flag) *flag = R.flag;

@@ -32,7 +32,7 @@
 
 namespace nnk {
 
-int svd_jacobi(const double *A, int n, double *U, double *s, double *V);
+int svd_batch(const double *const *A, int n, int nb, double *const *U, double *const *s, double *const *V);
 
 
 // ---------------------------------------------------------------------------
@@ -526,17 +526,22 @@ void Fmm::split_reduce(const DBuf &slot, int ncols, int M, int ldc, int nsplit, 
 }
 
 void Fmm::gemm_dims(const GemmBatch &b, const double *X, int ldx, double *C, int ldc, int beta,
-                    hipStream_t s, int M, int K, int Mpad, int Kpad, int nsplit, const DBuf *slot) {
+                    hipStream_t s, int M, int K, int Mpad, int Kpad, int nsplit, const DBuf *slot, int tag) {
   if (b.njobs == 0 || b.maxN == 0) return;
   if (Mpad % GBM || Kpad % GBK) throw Error(NNK_ESTATE, "gemm_dims: bad padding");
+  // tag 1: the M2L's own kernel symbol (profiles tell it from the other GEMMs)
+  auto go = [&](int bt, double *P, int ns) {
+    if (tag == 1) launch_gemm<64, 1>(Mpad / GBM, M, K, Mpad, Kpad, X, ldx, C, ldc, bt, b, s, ns, P);
+    else launch_gemm<64, 0>(Mpad / GBM, M, K, Mpad, Kpad, X, ldx, C, ldc, bt, b, s, ns, P);
+  };
   if (nsplit > 1) {
     if (beta != 0 || !slot) throw Error(NNK_ESTATE, "split GEMM needs beta = 0 and a slot list");
     if ((size_t)nsplit * b.njobs * b.maxN * ldc > part_elems_) throw Error(NNK_ESTATE, "split buffer too small");
-    launch_gemm<64, 0>(Mpad / GBM, M, K, Mpad, Kpad, X, ldx, C, ldc, 0, b, s, nsplit, part_.as<double>());
+    go(0, part_.as<double>(), nsplit);
     split_reduce(*slot, b.njobs * b.maxN, M, ldc, nsplit, C, s);
     return;
   }
-  launch_gemm<64, 0>(Mpad / GBM, M, K, Mpad, Kpad, X, ldx, C, ldc, beta, b, s);
+  go(beta, nullptr, 1);
 }
 
 void Fmm::gemm(const GemmBatch &b, const double *X, int ldx, double *C, int ldc, int beta,
@@ -571,7 +576,7 @@ Fmm::Fmm(std::shared_ptr<const Tree> t, double eps, double mu, int res, int shel
     : tp_(std::move(t)), tree_(*tp_), eps_(eps), mu_(mu), res_(res), shells_(shells),
       prec_(precision), tau_(tau) {
   const Tree &T = tree_;
-  if (precision < 0 || precision > 3) throw Error(NNK_EARG, "unknown precision");
+  if (precision < 0 || precision > 4) throw Error(NNK_EARG, "unknown precision");
   if (precision >= 1 && !(tau > 0 && tau < 1)) throw Error(NNK_EARG, "truncation tolerance must be in (0,1)");
   if (res < 2 || res > 16 || shells < 1 || 2 * shells > res)
     throw Error(NNK_EARG, "coronaRes must be in [2,16] and 1 <= coronaShells <= res/2");
@@ -686,16 +691,32 @@ Fmm::Fmm(std::shared_ptr<const Tree> t, double eps, double mu, int res, int shel
   ops_.assign(maxlev_ + 1, nullptr);
   {
     OpCache &C = OpCache::get();
+    std::vector<int> miss;
     for (int l = 2; l <= maxlev_; l++) {
       // FP64 form: operators with T.  Mixed forms: any entry (T unused).
       auto hit = C.find(key(0, l));
       if (!hit && prec_ >= 1) hit = C.find(key(1, l));
-      if (hit) {
-        ops_[l] = std::static_pointer_cast<const LevelOps>(hit);
-      } else {
-        ops_[l] = build_ops(l, prec_ == 0);
-        C.put(key(prec_ == 0 ? 0 : 1, l), ops_[l], ops_[l]->bytes());
-      }
+      if (hit) ops_[l] = std::static_pointer_cast<const LevelOps>(hit);
+      else miss.push_back(l);
+    }
+    // the check-to-equivalent SVDs of all missing levels in one batch
+    const int n = K_, nb = (int)miss.size();
+    std::vector<std::vector<double>> A(nb), U(nb, std::vector<double>((size_t)n * n)),
+        S(nb, std::vector<double>(n)), V(nb, std::vector<double>((size_t)n * n));
+    std::vector<const double *> pa(nb);
+    std::vector<double *> pu(nb), ps(nb), pv(nb);
+    for (int b = 0; b < nb; b++) {
+      A[b] = up_to_down(miss[b]);
+      pa[b] = A[b].data();
+      pu[b] = U[b].data();
+      ps[b] = S[b].data();
+      pv[b] = V[b].data();
+    }
+    if (nb) svd_batch(pa.data(), n, nb, pu.data(), ps.data(), pv.data());
+    for (int b = 0; b < nb; b++) {
+      const int l = miss[b];
+      ops_[l] = build_ops(l, prec_ == 0, std::move(U[b]), std::move(S[b]), std::move(V[b]));
+      C.put(key(prec_ == 0 ? 0 : 1, l), ops_[l], ops_[l]->bytes());
     }
   }
   plan();
@@ -713,7 +734,7 @@ size_t LevelOps::bytes() const {
 
 size_t MixedOps::bytes() const {
   return Pup.bytes + Vr.bytes + Pdn.bytes + Ur.bytes + VrT.bytes + UrT.bytes + Mc.bytes + Lc.bytes + Cf.bytes +
-         Uf.bytes;
+         Uf.bytes + UrF.bytes + UrU.bytes + Cd.bytes;
 }
 
 bool OpKey::operator<(const OpKey &o) const {
@@ -833,30 +854,40 @@ static void form_op(const std::vector<double> &src, const std::vector<double> &t
   NNK_HIP(hipDeviceSynchronize());
 }
 
-std::shared_ptr<const LevelOps> Fmm::build_ops(int l, bool with_T) {
+// K(up surface -> down surface) of the canonical box of level l (3Nq x
+// 3Nq row-major; rows: down-surface points, the system Up.m:31 solves)
+std::vector<double> Fmm::up_to_down(int l) const {
+  const auto codes = surface_codes(res_, shells_);
+  const double w = std::ldexp(w0_, -l);
+  const double e2 = eps_ * eps_, c = 1.0 / (8.0 * kPi * mu_);
+  const double zero[3] = {0, 0, 0};
+  auto up = surface_pts(codes, res_, zero, w, false);
+  auto dn = surface_pts(codes, res_, zero, w, true);
+  const int n = K_;
+  std::vector<double> A((size_t)n * n);
+  for (int p = 0; p < nq_; p++)      // row point (down surface, target)
+    for (int q = 0; q < nq_; q++) {  // column point (up surface, source)
+      double r[3] = {dn[3 * p] - up[3 * q], dn[3 * p + 1] - up[3 * q + 1], dn[3 * p + 2] - up[3 * q + 2]};
+      double r2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+      double invs = (1.0 / std::pow(r2 + e2, 1.5)) * c;
+      for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++)
+          A[(size_t)(3 * p + a) * n + 3 * q + b] = (r[a] * r[b] + (a == b ? r2 + 2 * e2 : 0.0)) * invs;
+    }
+  return A;
+}
+
+std::shared_ptr<const LevelOps> Fmm::build_ops(int l, bool with_T, std::vector<double> U, std::vector<double> S,
+                                               std::vector<double> V) {
   auto op = std::make_shared<LevelOps>();
   LevelOps &O = *op;
   const auto codes = surface_codes(res_, shells_);
   const double w = std::ldexp(w0_, -l);
   const double e2 = eps_ * eps_, c = 1.0 / (8.0 * kPi * mu_);
   const size_t mat = (size_t)Kpad_ * Mpad_;
-  const double zero[3] = {0, 0, 0};
-  // ---- check-to-equivalent factors from K(up -> down) ----
+  // ---- check-to-equivalent factors from the SVD of K(up -> down) ----
   {
-    auto up = surface_pts(codes, res_, zero, w, false);
-    auto dn = surface_pts(codes, res_, zero, w, true);
     const int n = K_;
-    std::vector<double> A((size_t)n * n), U((size_t)n * n), V((size_t)n * n), S(n);
-    for (int p = 0; p < nq_; p++)      // row point (down surface, target)
-      for (int q = 0; q < nq_; q++) {  // column point (up surface, source)
-        double r[3] = {dn[3 * p] - up[3 * q], dn[3 * p + 1] - up[3 * q + 1], dn[3 * p + 2] - up[3 * q + 2]};
-        double r2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
-        double invs = (1.0 / std::pow(r2 + e2, 1.5)) * c;
-        for (int a = 0; a < 3; a++)
-          for (int b = 0; b < 3; b++)
-            A[(size_t)(3 * p + a) * n + 3 * q + b] = (r[a] * r[b] + (a == b ? r2 + 2 * e2 : 0.0)) * invs;
-      }
-    svd_jacobi(A.data(), n, U.data(), S.data(), V.data());
     // up:   x = V (S^-1 U^T b);  down (K(down->up) = A^T): x = U (S^-1 V^T b)
     std::vector<double> Wup(mat, 0.0), Vup(mat, 0.0), Wdn(mat, 0.0), Udn(mat, 0.0);
     for (int k = 0; k < n; k++)
@@ -1204,7 +1235,7 @@ void Fmm::launch_surf(const int32_t *nodes, int n, const int64_t *lptr, const in
   a.e2 = eps_ * eps_;
   a.c = 1.0 / (8.0 * kPi * mu_);
   a.chk = chk;
-  if (prec_ >= 1) hipLaunchKernelGGL(k_surf_eval<true>, dim3(n), dim3(kSurfThreads), 0, s, a);
+  if (fp32_math()) hipLaunchKernelGGL(k_surf_eval<true>, dim3(n), dim3(kSurfThreads), 0, s, a);
   else hipLaunchKernelGGL(k_surf_eval<false>, dim3(n), dim3(kSurfThreads), 0, s, a);
   NNK_HIP(hipGetLastError());
 }
@@ -1232,7 +1263,7 @@ void Fmm::launch_leaf(int far, double *d_u, hipStream_t s) {
   a.out = d_u;
   auto go = [&](auto kern, int threads) { hipLaunchKernelGGL(kern, dim3(n_tleaves_), dim3(threads), 0, s, a); };
   const int T = far ? leaf_threads_ : 256;
-  if (prec_ >= 1) {
+  if (fp32_math()) {
     if (T == 128) go(k_leaf<true, 128>, 128);
     else if (T == 512) go(k_leaf<true, 512>, 512);
     else go(k_leaf<true, 256>, 256);
@@ -1259,7 +1290,7 @@ void Fmm::launch_config(int level, int *split, int *depth) const {
   *depth = 0;
   if (level < 2 || level > maxlev_) return;
   const LevelPlan &P = plans_[level];
-  if (prec_ >= 1) {
+  if (m16()) {
     if (P.mx_m2l_jobs) { *split = P.mx_split; *depth = P.mx_depth; }
   } else if (P.n_down) {
     *split = P.m2l_split;
@@ -1269,9 +1300,10 @@ void Fmm::launch_config(int level, int *split, int *depth) const {
 void Fmm::set_launch_config(int level, int split, int depth) {
   if (level < 2 || level > maxlev_) throw Error(NNK_EARG, "launch config: level without M2L");
   LevelPlan &P = plans_[level];
-  const bool mixed = prec_ >= 1;
+  const bool mixed = m16();
   if (mixed ? !P.mx_m2l_jobs : !P.n_down) throw Error(NNK_EARG, "launch config: level without M2L");
-  const size_t per_split = mixed ? (size_t)P.mx_m2l_jobs * P.mx_m2l_maxN * RP_ : (size_t)P.m2l.njobs * P.m2l.maxN * K_;
+  const size_t per_split = mixed ? (size_t)P.mx_m2l_jobs * P.mx_m2l_maxN * RP_
+                                 : (size_t)P.m2l.njobs * P.m2l.maxN * m2l_ld();
   if (split < 1 || split > 64 || (split > 1 && ((size_t)split * per_split > part_elems_ ||
                                                 !(mixed ? P.mx_slot.p : P.m2l_slot.p))))
     throw Error(NNK_EARG, "launch config: split-K factor out of range");
@@ -1279,7 +1311,8 @@ void Fmm::set_launch_config(int level, int split, int depth) {
     if (depth != 0) throw Error(NNK_EARG, "launch config: the FP64 form takes depth 0");
     P.m2l_split = split;
   } else {
-    const bool ok = depth == 2 || (depth == 3 && !(np_ == 3 && !f16_)) || (depth == 4 && dma_ok(P, split));
+    const bool ok = depth == 2 || (depth == 3 && !(np_ == 3 && !f16_)) ||
+                    (depth >= 4 && depth <= 5 && dma_ok(P, split, depth));
     if (!ok) throw Error(NNK_EARG, "launch config: depth not available for this form");
     P.mx_split = split;
     P.mx_depth = depth;
@@ -1296,24 +1329,25 @@ void Fmm::autotune(hipStream_t s) {
   try {
     for (int l = 2; l <= maxlev_; l++) {
       LevelPlan &P = plans_[l];
-      const bool mixed = prec_ >= 1;
+      const bool mixed = m16();
       if (mixed ? !P.mx_m2l_jobs : !P.n_down) continue;
       const int s0 = mixed ? P.mx_split : P.m2l_split;
       const size_t per_split = mixed ? (size_t)P.mx_m2l_jobs * P.mx_m2l_maxN * RP_
-                                     : (size_t)P.m2l.njobs * P.m2l.maxN * K_;
+                                     : (size_t)P.m2l.njobs * P.m2l.maxN * m2l_ld();
       std::vector<int> splits = {s0};
       if (s0 > 1) splits.push_back(std::max(1, s0 / 2));
       if ((size_t)std::min(64, 2 * s0) * per_split <= part_elems_) splits.push_back(std::min(64, 2 * s0));
       std::vector<int> depths = !mixed ? std::vector<int>{0} : np_ == 3 && !f16_ ? std::vector<int>{2, 4}
-                                                                                  : std::vector<int>{2, 3, 4};
+                                                                                  : std::vector<int>{2, 3, 4, 5};
       float best = 1e30f;
       int bs = s0, bd = mixed ? P.mx_depth : 0;
       for (int sp : splits) {
         if (sp > 1 && !(mixed ? P.mx_slot.p : P.m2l_slot.p)) continue;
         for (int dp : depths) {
-          if (dp == 4 && !dma_ok(P, sp)) continue;
+          if (dp >= 4 && !dma_ok(P, sp, dp)) continue;
           auto run = [&]() {
             if (mixed) launch_m2l_mixed(l, dp, sp, s);
+            else if (prec_ == 4) launch_m2l_t(l, sp, s);
             else gemm(P.m2l, uppot_.as<double>(), K_, chk_.as<double>(), K_, 0, s, 1, sp, &P.m2l_slot);
           };
           run();  // warm

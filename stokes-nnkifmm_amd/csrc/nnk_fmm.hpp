@@ -94,8 +94,11 @@ struct LevelPlan {
   DBuf mx_m2m_slot;
   DBuf mx_m2l;                // GemmJobBf16[8]
   int mx_split = 1, mx_depth = 3;  // split-K factor and register-ring depth of the M2L launch
-                                   // (depth 4: the DMA-staged kernel)
+                                   // (depth 4: the DMA-staged kernel; 5: the same with
+                                   // 64 x 48 wave tiles)
   int mx_maxnb = 0;                // most operator blocks of one M2L job
+  bool mx_tn3 = false;             // the class jobs also exist with 192-column tiles (mx_m2l3)
+  DBuf mx_m2l3;                    // GemmJobBf16[mx_m2l_jobs], tile lists over 192 columns
   DBuf mx_slot;
   std::vector<DBuf> mx_owned;
   int mx_m2l_jobs = 0, mx_m2l_maxN = 0;
@@ -125,6 +128,10 @@ struct MixedOps {
   // and L2L stay FP64: their S^-1 amplifies the rounding of 22-bit inputs
   // beyond the 1e-6 target.)
   DBuf UrF, UrU;
+  // FP64-M2L truncated form: the 316 R x R operators in FP64 At layout
+  // (KpadR x MpadR each, rows R..KpadR-1 zero)
+  DBuf Cd;
+  std::vector<const double *> dop;    // per offset slot (343)
   std::vector<const uint16_t *> cop;  // per offset slot (343)
   std::vector<const float *> uop;
   size_t bytes() const;
@@ -181,7 +188,9 @@ class Fmm {
   // precision 0: FP64 form (reference parity); 1: mixed form (truncated SVD
   // at relative tolerance tau, two-part split-BF16 M2L, FP32-math direct
   // sums); 2: as 1 with three-part split-BF16 M2L; 3: as 1 with two-part
-  // split-FP16 M2L on power-of-two scaled operands
+  // split-FP16 M2L on power-of-two scaled operands; 4: truncated SVD with
+  // every operation in FP64 (R x R M2L on the FP64 matrix cores, FP64
+  // direct sums)
   Fmm(std::shared_ptr<const Tree> t, double eps, double mu, int res, int shells, int precision = 0,
       double tau = 1e-10);
   ~Fmm();
@@ -231,7 +240,11 @@ class Fmm {
 
  private:
   void plan();
-  std::shared_ptr<const LevelOps> build_ops(int level, bool with_T);
+  // K(up -> down) of the canonical box of a level, and the level's
+  // operators from that matrix's SVD (A = U diag(S) V^T, row-major)
+  std::vector<double> up_to_down(int level) const;
+  std::shared_ptr<const LevelOps> build_ops(int level, bool with_T, std::vector<double> U, std::vector<double> S,
+                                            std::vector<double> V);
   // the 316 translation operators of a level into T (tslot: 343 pointers)
   void form_T(int level, DBuf &T, std::vector<double *> &tslot);
   OpKey key(int kind, int level) const;
@@ -246,13 +259,23 @@ class Fmm {
   void split_reduce(const DBuf &slot, int ncols, int M, int ldc, int nsplit, double *C, hipStream_t s);
   void reserve_part(size_t elems);
   void gemm_dims(const GemmBatch &b, const double *X, int ldx, double *C, int ldc, int beta,
-                 hipStream_t s, int M, int K, int Mpad, int Kpad, int nsplit = 1, const DBuf *slot = nullptr);
+                 hipStream_t s, int M, int K, int Mpad, int Kpad, int nsplit = 1, const DBuf *slot = nullptr,
+                 int tag = 0);
+  // forms 1-3: M2L on 16-bit parts and FP32-math direct sums; form 4 keeps
+  // both in FP64
+  bool m16() const { return prec_ >= 1 && prec_ <= 3; }
+  bool fp32_math() const { return prec_ >= 1 && prec_ <= 3; }
+  // leading dimension of the M2L output (FP64: check potentials; truncated
+  // forms: down coefficients)
+  int m2l_ld() const { return prec_ >= 1 ? RP_ : K_; }
+  // form 4: the M2L of level l, FP64 gather GEMM over the R x R operators
+  void launch_m2l_t(int l, int split, hipStream_t s);
   void build_mixed();
   void plan_mixed();
   void apply_mixed(const double *d_f, double *d_u, hipStream_t s);
   void apply_impl(const double *d_f, double *d_u, hipStream_t s);
   void launch_m2l_mixed(int l, int depth, int split, hipStream_t s);
-  bool dma_ok(const LevelPlan &P, int split) const;
+  bool dma_ok(const LevelPlan &P, int split, int depth = 4) const;
   // time the M2L launch candidates of every level on the current densities
   // (register-ring depth 2/3 for the mixed forms, split-K factor s/2, s, 2s)
   // and keep the fastest

@@ -264,35 +264,49 @@ __global__ __launch_bounds__(512, 1) void k_m2l_split(const int M, const int KC,
     }
 }
 
-// The same GEMM with its operand tiles staged by the LDS DMA path (global_load_lds_dwordx4: each lane's 16 bytes
-// go straight from memory to LDS, lane q at byte 16q of the wave's 1 KB
-// fragment) instead of through a register ring and LDS stores.
+// The same GEMM with its operand tiles staged by the LDS DMA path
+// (global_load_lds_dwordx4: each lane's 16 bytes go straight from memory to
+// LDS, lane q at byte 16q of a 1 KB fragment) instead of through a
+// register ring and LDS stores.
 //
-// Three LDS stages, each one chunk of A and X in fragment order (16 NP KB),
-// are separate arrays addressed with static names, so the compiler's wait
-// insertion can tell the stage being read from the ones being filled and
-// does not drain the DMA queue before the LDS reads.  At chunk c each wave
-//   waits until its own DMAs of chunk c have landed (at most the 2 NP of
-//   chunk c+1 still outstanding), meets the others at a barrier (so every wave's
-//   part of chunk c is in LDS and every wave is done with chunk c-1),
+// Three LDS stages, each one chunk of A (8 row tiles) and X (4 TN column
+// tiles) in fragment order, are separate arrays addressed with static
+// names, so the compiler's wait insertion can tell the stage being read from
+// the ones being filled and does not drain the DMA queue before the LDS
+// reads.  At chunk c each wave
+//   waits until its own DMAs of chunk c have landed (at most the LPW of
+//   chunk c+1 still outstanding), meets the others at a barrier (so every
+//   wave's part of chunk c is in LDS and every wave is done with chunk c-1),
 //   issues the DMAs of chunk c+2 into the stage chunk c-1 used, and
 //   multiplies chunk c.
 // The barrier is s_barrier alone: __syncthreads would also wait for all
 // outstanding loads and undo the look-ahead.
 //
+// Tile width: TN = 2 gives the 128 x 128 tile of k_m2l_split (8 waves as
+// 2 x 4 of 64 x 32).  TN = 3 widens every wave to 64 x 48 (a 128 x 192
+// tile): the A fragments each wave reads from LDS then feed 12 instead of 8
+// MFMA tiles, which cuts the LDS traffic per MFMA by a sixth (the kernel is
+// bound by LDS bandwidth, see below), at 48 more registers per wave.  The
+// (8 + 4 TN) NP fragment loads of a chunk are dealt round-robin to the 8
+// waves (load i to wave i mod 8: the A tiles first, then the X tiles), LPW
+// per wave; with TN = 3 a wave's X loads gather three different column
+// tiles, each with its own gather slot.  TN = 3 needs NP = 2 (LPW = 5).
+//
 // FP16 parts (F16, NP = 2): the row unscale factors of each operator block
 // (powers of two, or zero for an all-zero row; 2^-126 at the smallest) are
-// read once per workgroup
-// into an LDS table of their exponent bytes, 128 rows x (a1-a0) blocks, so
-// that no global load sits in the main loop apart from the DMAs and the
-// gather slots: a load whose result is needed at the end of a block would
-// make the compiler wait for every DMA issued before it.  Requires
-// a1 - a0 <= 320 active blocks per split (checked on the host; the 316
-// offsets of a level fit without a split).
+// read once per workgroup into an LDS table of their exponent bytes, 128
+// rows x (a1-a0) blocks, so that no global load sits in the main loop apart
+// from the DMAs and the gather slots: a load whose result is needed at the
+// end of a block would make the compiler wait for every DMA issued before
+// it.  Requires a1 - a0 <= dma_max_blocks(TN) active blocks per split
+// (checked on the host): 320 at TN = 2 (the 316 offsets of a level fit
+// without a split), 192 at TN = 3 (a child class's 189; LDS holds no more).
 // BF16 parts (NP = 2 or 3) have no unscale table; NP = 3 takes 144 KB of
-// stages.  Results are bit-identical to k_m2l_split<NP, D, TAG, F16>: the
-// same MFMAs in the same order, and the unscale exact (a power of two).
+// stages.  Results are bit-identical to k_m2l_split<NP, D, TAG, F16> and
+// across TN: the same MFMAs in the same order per output element, and the
+// unscale exact (a power of two).
 constexpr int kDmaMaxBlocks = 320;
+constexpr int dma_max_blocks(int tn) { return tn == 3 ? 192 : kDmaMaxBlocks; }
 
 template <int N>
 __device__ __forceinline__ void wait_vmcnt() {
@@ -300,7 +314,7 @@ __device__ __forceinline__ void wait_vmcnt() {
   __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | (7 << 4) | (15 << 8));
 }
 
-template <int NP, int TAG, bool F16>
+template <int NP, int TAG, bool F16, int TN = 2>
 __global__ __launch_bounds__(512, 1) void k_m2l_dma(const int M, const int KC, const int RT,
                                                     const uint16_t *__restrict__ X, const int zrow,
                                                     double *__restrict__ C, const int ldc,
@@ -309,19 +323,28 @@ __global__ __launch_bounds__(512, 1) void k_m2l_dma(const int M, const int KC, c
                                                     const int maxN, const double *__restrict__ xun,
                                                     const int beta) {
   static_assert(!F16 || NP == 2, "FP16 parts: two");
-  constexpr int BN = 128, TM = 4, TN = 2, TILE = 512;
+  static_assert(TN == 2 || (TN == 3 && NP == 2), "64 x 48 wave tiles: two parts");
+  constexpr int BN = 64 * TN, TM = 4, TILE = 512;
+  constexpr int XT = 4 * TN;                  // 16-column X tiles of a workgroup tile
+  constexpr int LPW = (8 + XT) * NP / 8;      // fragment loads per wave and chunk
+  constexpr int NXS = TN == 2 ? 1 : 3;        // distinct X column tiles per wave
+  constexpr int MAXB = TN == 3 ? 192 : 512;   // active blocks of a tile
+  constexpr int UB = dma_max_blocks(TN);
   typedef typename std::conditional<F16, f16x8, bf16x8>::type op8;
   typedef const __attribute__((address_space(1))) void *gvp;
   typedef __attribute__((address_space(3))) void *lvp;
   typedef const __attribute__((address_space(1))) f32x4 *gf32x4p;
-  typedef uint16_t Stage[2][NP][8][TILE];  // [A or X][part][fragment tile][lane * 8 + e]
+  struct Stage {
+    uint16_t a[NP][8][TILE];   // [part][row tile][lane * 8 + e]
+    uint16_t x[NP][XT][TILE];  // [part][column tile][lane * 8 + e]
+  };
   __shared__ __attribute__((aligned(1024))) Stage St0;
   __shared__ __attribute__((aligned(1024))) Stage St1;
   __shared__ __attribute__((aligned(1024))) Stage St2;
-  __shared__ __attribute__((aligned(16))) uint32_t ue[F16 ? kDmaMaxBlocks * 32 : 1];
-  __shared__ int act[512];
-  __shared__ const uint16_t *aptr[512];
-  __shared__ const float *uptr[F16 ? 512 : 1];
+  __shared__ __attribute__((aligned(16))) uint32_t ue[F16 ? UB * 32 : 1];
+  __shared__ int act[MAXB];
+  __shared__ const uint16_t *aptr[MAXB];
+  __shared__ const float *uptr[F16 ? MAXB : 1];
 
   const int bid = blockIdx.x;
   const int jb = bid % njobs;
@@ -373,37 +396,70 @@ __global__ __launch_bounds__(512, 1) void k_m2l_dma(const int M, const int KC, c
     __syncthreads();
   }
 
-  // this thread's DMA slot: fragment tile ti = its wave, lane slot tq
-  const int ti = wave, tq = lane;
-  const int my_col = col0 + ti * 16 + (tq & 15);
+  // this wave's fragment loads: load i = wave + 8 q of the chunk's list
+  // [A part 0 tiles 0-7, A part 1, ..., X part 0 tiles 0..XT-1, ...].  The
+  // first XQ0 are A loads (i < 8 NP); with TN = 2 the X loads all read the
+  // column tile `wave` (one gather slot), with TN = 3 the three X loads read
+  // tiles wave, (wave + 8) mod 12 and wave + 4 (gather slots 0, 1, 2).
+  constexpr int XQ0 = TN == 2 ? NP : 2;
+  const int tq = lane;
   const int kof = 8 * (tq >> 4);
-  const size_t aoff = (size_t)(rb * 8 + ti) * TILE + (size_t)tq * 8;
+  int lp[LPW], lt[LPW];  // part and tile of load q
+#pragma unroll
+  for (int q = 0; q < LPW; q++) {
+    const int i = wave + 8 * q;
+    const int j = q >= XQ0 ? i - 8 * NP : i;
+    const int per = q >= XQ0 ? XT : 8;
+    lp[q] = j / per;
+    lt[q] = j - lp[q] * per;
+  }
+  int xtile[NXS];
+#pragma unroll
+  for (int s = 0; s < NXS; s++) xtile[s] = lt[XQ0 + (TN == 2 ? 0 : s)];
+  int my_col[NXS];
+#pragma unroll
+  for (int s = 0; s < NXS; s++) my_col[s] = col0 + xtile[s] * 16 + (tq & 15);
   const size_t slab = (size_t)RT * TILE;
   const gi32p gidx = (gi32p)J.gidx;
-  auto gload = [&](int bi) -> int {
+  auto gload = [&](int bi, int s) -> int {
     const int bc = bi < a1 ? bi : a1 - 1;
-    const int cc = my_col < J.N ? my_col : J.N - 1;
+    const int cc = my_col[s] < J.N ? my_col[s] : J.N - 1;
     return gidx[(size_t)act[bc] * J.N + cc];
   };
-  auto gslot = [&](int raw, int bi) -> int { return (bi < a1 && my_col < J.N && raw >= 0) ? raw : zrow; };
-  int fblk = -1, gcur = zrow, gnext = total > 0 ? gload(a0) : -1;
+  auto gslot = [&](int raw, int bi, int s) -> int {
+    return (bi < a1 && my_col[s] < J.N && raw >= 0) ? raw : zrow;
+  };
+  int fblk = -1, gcur[NXS], gnext[NXS];
+#pragma unroll
+  for (int s = 0; s < NXS; s++) {
+    gcur[s] = zrow;
+    gnext[s] = total > 0 ? gload(a0, s) : -1;
+  }
 
   // DMAs of chunk c (clamped to the last chunk, so that every step issues
-  // exactly 2 NP of them and the wait counts stay static)
+  // exactly LPW of them and the wait counts stay static)
   auto fetch = [&](int c, Stage &Sd) {
     const int cq = c < total ? c : total - 1;
     const int q = cq / KC, kc = cq - q * KC, bi = a0 + q;
     if (bi != fblk) {
       fblk = bi;
-      gcur = gslot(gnext, bi);
-      gnext = gload(bi + 1);
-    }
-    const uint16_t *Ab = aptr[bi] + aoff + (size_t)kc * NP * slab;
-    const uint16_t *Xb = X + ((size_t)gcur * KC + kc) * NP * 32 + kof;
 #pragma unroll
-    for (int p = 0; p < NP; p++) {
-      __builtin_amdgcn_global_load_lds((gvp)(Ab + p * slab), (lvp)&Sd[0][p][ti][0], 16, 0, 0);
-      __builtin_amdgcn_global_load_lds((gvp)(Xb + p * 32), (lvp)&Sd[1][p][ti][0], 16, 0, 0);
+      for (int s = 0; s < NXS; s++) {
+        gcur[s] = gslot(gnext[s], bi, s);
+        gnext[s] = gload(bi + 1, s);
+      }
+    }
+#pragma unroll
+    for (int l = 0; l < LPW; l++) {
+      if (l < XQ0) {
+        const uint16_t *Ab = aptr[bi] + (size_t)(rb * 8 + lt[l]) * TILE + (size_t)tq * 8 +
+                             ((size_t)kc * NP + lp[l]) * slab;
+        __builtin_amdgcn_global_load_lds((gvp)Ab, (lvp)&Sd.a[lp[l]][lt[l]][0], 16, 0, 0);
+      } else {
+        const int g = gcur[TN == 2 ? 0 : l - XQ0];
+        const uint16_t *Xb = X + (((size_t)g * KC + kc) * NP + lp[l]) * 32 + kof;
+        __builtin_amdgcn_global_load_lds((gvp)Xb, (lvp)&Sd.x[lp[l]][lt[l]][0], 16, 0, 0);
+      }
     }
   };
 
@@ -426,9 +482,9 @@ __global__ __launch_bounds__(512, 1) void k_m2l_dma(const int M, const int KC, c
 #pragma unroll
     for (int p = 0; p < NP; p++) {
 #pragma unroll
-      for (int i = 0; i < TM; i++) a[p][i] = *reinterpret_cast<const op8 *>(&Sc[0][p][wm * 4 + i][lane * 8]);
+      for (int i = 0; i < TM; i++) a[p][i] = *reinterpret_cast<const op8 *>(&Sc.a[p][wm * 4 + i][lane * 8]);
 #pragma unroll
-      for (int j = 0; j < TN; j++) x[p][j] = *reinterpret_cast<const op8 *>(&Sc[1][p][wn * 2 + j][lane * 8]);
+      for (int j = 0; j < TN; j++) x[p][j] = *reinterpret_cast<const op8 *>(&Sc.x[p][wn * TN + j][lane * 8]);
     }
 #pragma unroll
     for (int i = 0; i < TM; i++)
@@ -466,7 +522,7 @@ __global__ __launch_bounds__(512, 1) void k_m2l_dma(const int M, const int KC, c
     }
   };
   auto step = [&](int cc, const Stage &Sc, Stage &Sd) {
-    wait_vmcnt<2 * NP>();
+    wait_vmcnt<LPW>();
     __builtin_amdgcn_s_barrier();
     fetch(cc + 2, Sd);
     if (cc < total) {
@@ -491,7 +547,7 @@ __global__ __launch_bounds__(512, 1) void k_m2l_dma(const int M, const int KC, c
   for (int i = 0; i < TM; i++)
 #pragma unroll
     for (int j = 0; j < TN; j++) {
-      const int col = col0 + wn * 32 + j * 16 + lr;
+      const int col = col0 + wn * 16 * TN + j * 16 + lr;
       if (col >= J.N) continue;
       double *out = nsplit > 1 ? P + ((size_t)(sp * njobs + jb) * maxN + col) * ldc
                                : C + (size_t)J.cidx[col] * ldc;

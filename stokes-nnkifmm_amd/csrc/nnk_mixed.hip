@@ -318,27 +318,39 @@ void Fmm::build_mixed() {
       std::vector<const double *> P(no, X.Pdn.as<double>());
       GemmBatch b2 = batch(no, P, R, &ids, &ids);
       gemm_dims(b2, Y.as<double>(), K, CT.as<double>(), MpadR_, 0, 0, R, K, MpadR_, Kpad_);
-      const size_t per = (size_t)KCm_ * np_ * RT_ * 512;
-      X.Cf.alloc(sizeof(uint16_t) * per * no);
-      const int64_t tot = (int64_t)per * no;
-      if (f16_) {
-        X.Uf.alloc(sizeof(float) * (size_t)RPm_ * no);
-        const int64_t nr = (int64_t)RPm_ * no;
-        hipLaunchKernelGGL(k_op_rowscale, dim3((unsigned)((nr + 255) / 256)), dim3(256), 0, 0, CT.as<double>(),
-                           MpadR_, R, no, RPm_, X.Uf.as<float>());
-        hipLaunchKernelGGL(k_split_op_h, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, 0, CT.as<double>(),
-                           MpadR_, R, no, KCm_, RT_, X.Uf.as<float>(), X.Cf.as<uint16_t>());
+      if (prec_ == 4) {  // FP64 operators, At layout padded to KpadR rows
+        const size_t opd = (size_t)KpadR_ * MpadR_;
+        X.Cd.alloc(sizeof(double) * opd * no);
+        X.Cd.zero();
+        for (int j = 0; j < no; j++)
+          NNK_HIP(hipMemcpyAsync(X.Cd.as<double>() + opd * j, CT.as<double>() + (size_t)j * R * MpadR_,
+                                 sizeof(double) * (size_t)R * MpadR_, hipMemcpyDeviceToDevice, 0));
+        NNK_HIP(hipDeviceSynchronize());
+        X.dop.assign(343, nullptr);
+        for (int j = 0; j < no; j++) X.dop[slots[j]] = X.Cd.as<double>() + opd * j;
       } else {
-        hipLaunchKernelGGL(k_split_op, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, 0, CT.as<double>(),
-                           MpadR_, R, no, KCm_, np_, RT_, X.Cf.as<uint16_t>());
-      }
-      NNK_HIP(hipGetLastError());
-      NNK_HIP(hipDeviceSynchronize());
-      X.cop.assign(343, nullptr);
-      X.uop.assign(343, nullptr);
-      for (int j = 0; j < no; j++) {
-        X.cop[slots[j]] = X.Cf.as<uint16_t>() + per * j;
-        if (f16_) X.uop[slots[j]] = X.Uf.as<float>() + (size_t)RPm_ * j;
+        const size_t per = (size_t)KCm_ * np_ * RT_ * 512;
+        X.Cf.alloc(sizeof(uint16_t) * per * no);
+        const int64_t tot = (int64_t)per * no;
+        if (f16_) {
+          X.Uf.alloc(sizeof(float) * (size_t)RPm_ * no);
+          const int64_t nr = (int64_t)RPm_ * no;
+          hipLaunchKernelGGL(k_op_rowscale, dim3((unsigned)((nr + 255) / 256)), dim3(256), 0, 0, CT.as<double>(),
+                             MpadR_, R, no, RPm_, X.Uf.as<float>());
+          hipLaunchKernelGGL(k_split_op_h, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, 0, CT.as<double>(),
+                             MpadR_, R, no, KCm_, RT_, X.Uf.as<float>(), X.Cf.as<uint16_t>());
+        } else {
+          hipLaunchKernelGGL(k_split_op, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, 0, CT.as<double>(),
+                             MpadR_, R, no, KCm_, np_, RT_, X.Cf.as<uint16_t>());
+        }
+        NNK_HIP(hipGetLastError());
+        NNK_HIP(hipDeviceSynchronize());
+        X.cop.assign(343, nullptr);
+        X.uop.assign(343, nullptr);
+        for (int j = 0; j < no; j++) {
+          X.cop[slots[j]] = X.Cf.as<uint16_t>() + per * j;
+          if (f16_) X.uop[slots[j]] = X.Uf.as<float>() + (size_t)RPm_ * j;
+        }
       }
     }
     // ---- M2M (this level is the child level; parent level l-1 >= 2) ----
@@ -390,10 +402,12 @@ void Fmm::build_mixed() {
   chat_.alloc(sizeof(double) * (size_t)nc * RP_);
   dhat_.alloc(sizeof(double) * (size_t)nc * RP_);
   // split coefficients, one all-zero row (slot nc) for absent sources
-  chat_x_.alloc(sizeof(uint16_t) * (size_t)(nc + 1) * np_ * RP_);
+  if (m16()) {
+    chat_x_.alloc(sizeof(uint16_t) * (size_t)(nc + 1) * np_ * RP_);
+    chat_x_.zero();
+  }
   chat_.zero();
   dhat_.zero();
-  chat_x_.zero();
   amax_.alloc(sizeof(unsigned long long));
   xun_.alloc(sizeof(double));
   if (f16_) {
@@ -486,7 +500,7 @@ void Fmm::plan_mixed() {
     // class job; one job over all 316 offsets then walks fewer (tile,
     // operator block) pairs (plan_m2l_level decides and checks the tables).
     M2lLevelPlan mp;
-    plan_m2l_level(H.cls, H.gv, tree_.count, MBN, mp);
+    if (m16()) plan_m2l_level(H.cls, H.gv, tree_.count, MBN, mp);
     const bool merged = mp.merged;
     const std::vector<int> &msl = mp.msl;
     const std::vector<int32_t> &mcols = mp.mcols;
@@ -525,7 +539,7 @@ void Fmm::plan_mixed() {
     }
     for (int a = 0; a < 8; a++) {
       if (H.cls[a].empty()) continue;
-      if (!merged) {
+      if (m16() && !merged) {
         std::vector<const uint16_t *> ops;
         std::vector<const float *> uops;
         for (int sl : class_offsets(a)) {
@@ -595,11 +609,74 @@ void Fmm::plan_mixed() {
       for (int j = 0; j < nj; j++) P.mx_maxnb = std::max(P.mx_maxnb, mj[j].nb);
       P.mx_split = split_for((int64_t)nj * (RPm_ / MBM) * ((mN + MBN - 1) / MBN), minnb);
       P.mx_depth = dma_ok(P, P.mx_split) ? 4 : (np_ == 3 && !f16_) ? 2 : 3;
+      // class jobs of two-part forms: also as 192-column tiles for the
+      // 64 x 48 wave-tile DMA kernel (its tile lists checked like the others)
+      P.mx_tn3 = !merged && np_ == 2 && P.mx_maxnb <= dma_max_blocks(3);
+      if (P.mx_tn3) {
+        std::vector<GemmJobBf16> mj3 = mj;
+        for (int j = 0, a = 0; a < 8; a++) {
+          if (H.cls[a].empty()) continue;
+          std::vector<int32_t> tp, tl;
+          tile_lists(H.gv[a], mj3[j].nb, mj3[j].N, 3 * 64, tp, tl);
+          check_gather_job(H.gv[a], mj3[j].nb, mj3[j].N, 3 * 64, tp, tl, tree_.count, "M2L job (192-column tiles)");
+          P.mx_owned.emplace_back();
+          P.mx_owned.back().upload(tp);
+          mj3[j].tptr = P.mx_owned.back().as<int32_t>();
+          P.mx_owned.emplace_back();
+          P.mx_owned.back().upload(tl);
+          mj3[j].tlist = P.mx_owned.back().as<int32_t>();
+          j++;
+        }
+        P.mx_m2l3.upload(mj3);
+      }
       P.mx_slot.upload(slot);  // also for the autotuner's split candidates
       reserve_part((size_t)std::min(64, 2 * P.mx_split) * nj * mN * RP_);
     }
     P.mx_l2l.njobs = (int)lj.size();
     if (!lj.empty()) P.mx_l2l.jobs.upload(lj);
+    if (prec_ == 4 && P.n_down) {  // FP64 R x R M2L jobs, one per child class, as in the FP64 form
+      P.m2l = GemmBatch();
+      std::vector<GemmJob> jobs;
+      int minnb = 512;
+      for (int a = 0; a < 8; a++) {
+        if (H.cls[a].empty()) continue;
+        std::vector<const double *> ops;
+        for (int sl : class_offset_slots(a)) {
+          if (!X.dop[sl]) throw Error(NNK_ESTATE, "FP64 truncated M2L: missing translation operator");
+          ops.push_back(X.dop[sl]);
+        }
+        const int nb = (int)ops.size(), N = (int)H.cls[a].size();
+        GemmJob J{};
+        P.m2l.owned.emplace_back();
+        P.m2l.owned.back().upload(ops);
+        J.A = P.m2l.owned.back().as<const double *>();
+        P.m2l.owned.emplace_back();
+        P.m2l.owned.back().upload(H.gv[a]);
+        J.gidx = P.m2l.owned.back().as<int32_t>();
+        P.m2l.owned.emplace_back();
+        P.m2l.owned.back().upload(H.cls[a]);
+        J.cidx = P.m2l.owned.back().as<int32_t>();
+        J.nb = nb;
+        J.N = N;
+        std::vector<int32_t> tp, tl;
+        tile_lists(H.gv[a], nb, N, 64, tp, tl);
+        check_gather_job(H.gv[a], nb, N, 64, tp, tl, tree_.count, "FP64 truncated M2L job");
+        P.m2l.owned.emplace_back();
+        P.m2l.owned.back().upload(tp);
+        J.tptr = P.m2l.owned.back().as<int32_t>();
+        P.m2l.owned.emplace_back();
+        P.m2l.owned.back().upload(tl);
+        J.tlist = P.m2l.owned.back().as<int32_t>();
+        jobs.push_back(J);
+        P.m2l.maxN = std::max(P.m2l.maxN, N);
+        minnb = std::min(minnb, nb);
+      }
+      P.m2l.njobs = (int)jobs.size();
+      P.m2l.jobs.upload(jobs);
+      const int nj = P.m2l.njobs, mN = P.m2l.maxN;
+      P.m2l_split = split_for((int64_t)nj * (MpadR_ / GBM) * ((mN + 63) / 64), minnb);
+      reserve_part((size_t)std::min(64, 2 * P.m2l_split) * nj * mN * RP_);
+    }
     if (f16_) {  // split-FP16 down expansion
       auto hjob = [&](std::vector<GemmJobBf16> &jobs, int &maxN, const std::vector<const uint16_t *> &ops,
                       const std::vector<const float *> &uops, const std::vector<int32_t> &g,
@@ -644,46 +721,60 @@ void Fmm::plan_mixed() {
   upall_.upload(upall);
 }
 
-// whether the DMA-staged FP16 M2L kernel can run level plan P with split-K
-// factor `split`: its unscale table holds kDmaMaxBlocks blocks per split
-bool Fmm::dma_ok(const LevelPlan &P, int split) const {
+// whether the DMA-staged M2L kernel can run level plan P with split-K
+// factor `split`: depth 4 (128-column tiles) holds kDmaMaxBlocks blocks per
+// split in its FP16 unscale table; depth 5 (192-column tiles, two-part
+// forms, class jobs) holds dma_max_blocks(3) blocks of a tile in LDS
+bool Fmm::dma_ok(const LevelPlan &P, int split, int depth) const {
   if (split < 1) return false;
+  if (depth == 5) return P.mx_tn3 && P.mx_maxnb <= dma_max_blocks(3);
   return !f16_ || (P.mx_maxnb + split - 1) / split <= kDmaMaxBlocks;
 }
 
 // M2L of level l in the mixed form with register-ring depth `depth` (2 or
-// 3; 4 selects the DMA-staged kernel) and split-K factor `split` (the part
-// buffer must hold it)
+// 3; 4 selects the DMA-staged kernel, 5 the DMA-staged kernel with 64 x 48
+// wave tiles) and split-K factor `split` (the part buffer must hold it)
 void Fmm::launch_m2l_mixed(int l, int depth, int split, hipStream_t s) {
   LevelPlan &P = plans_[l];
   if (!P.mx_m2l_jobs) return;
-  const int nrb = RPm_ / MBM, nct = (P.mx_m2l_maxN + MBN - 1) / MBN;
+  const int bn = depth == 5 ? 3 * 64 : MBN;
+  const int nrb = RPm_ / MBM, nct = (P.mx_m2l_maxN + bn - 1) / bn;
   const int nj = P.mx_m2l_jobs, mN = P.mx_m2l_maxN;
   if (split > 1 && (size_t)split * nj * mN * RP_ > part_elems_) throw Error(NNK_ESTATE, "mixed M2L: split buffer too small");
   const dim3 grid((unsigned)(nj * split * nrb * nct));
   const int zrow = tree_.count;
   double *d = dhat_.as<double>();
   double *pp = split > 1 ? part_.as<double>() : nullptr;
+  const GemmJobBf16 *jobs = (depth == 5 ? P.mx_m2l3 : P.mx_m2l).as<GemmJobBf16>();
   auto launch = [&](auto kern) {
-    hipLaunchKernelGGL(kern, grid, dim3(512), 0, s, R_, KCm_, RT_, chat_x_.as<uint16_t>(), zrow, d, RP_,
-                       P.mx_m2l.as<GemmJobBf16>(), nj, nrb, split, pp, mN, xun_.as<double>(), 0);
+    hipLaunchKernelGGL(kern, grid, dim3(512), 0, s, R_, KCm_, RT_, chat_x_.as<uint16_t>(), zrow, d, RP_, jobs, nj,
+                       nrb, split, pp, mN, xun_.as<double>(), 0);
   };
-  if (depth == 4 && !dma_ok(P, split))
+  if (depth >= 4 && !dma_ok(P, split, depth))
     throw Error(NNK_ESTATE, "mixed M2L: too many operator blocks for the DMA-staged kernel");
+  if (depth == 5 && np_ != 2) throw Error(NNK_ESTATE, "mixed M2L: 64 x 48 wave tiles need two parts");
   if (f16_) {
-    if (depth == 4) launch(k_m2l_dma<2, 1, true>);
+    if (depth == 5) launch(k_m2l_dma<2, 1, true, 3>);
+    else if (depth == 4) launch(k_m2l_dma<2, 1, true>);
     else if (depth == 2) launch(k_m2l_split<2, 2, 1, true>);
     else launch(k_m2l_split<2, 3, 1, true>);
   } else if (np_ == 3) {
     if (depth == 4) launch(k_m2l_dma<3, 1, false>);
     else launch(k_m2l_split<3, 2, 1, false>);  // a third register stage would spill
   } else {
-    if (depth == 4) launch(k_m2l_dma<2, 1, false>);
+    if (depth == 5) launch(k_m2l_dma<2, 1, false, 3>);
+    else if (depth == 4) launch(k_m2l_dma<2, 1, false>);
     else if (depth == 2) launch(k_m2l_split<2, 2, 1, false>);
     else launch(k_m2l_split<2, 3, 1, false>);
   }
   NNK_HIP(hipGetLastError());
   if (split > 1) split_reduce(P.mx_slot, nj * mN, R_, RP_, split, d, s);
+}
+
+void Fmm::launch_m2l_t(int l, int split, hipStream_t s) {
+  LevelPlan &P = plans_[l];
+  gemm_dims(P.m2l, chat_.as<double>(), RP_, dhat_.as<double>(), RP_, 0, s, R_, R_, MpadR_, KpadR_, split,
+            &P.m2l_slot, 1);
 }
 
 // one launch of the split-FP16 GEMM (no split-K, register-ring depth 2):
@@ -730,7 +821,7 @@ void Fmm::apply_mixed(const double *d_f, double *d_u, hipStream_t s) {
     gemm_dims(P.mx_m2m, c, RP_, c, RP_, 0, s, R, R, MpadR_, KpadR_, P.mx_m2m_split, &P.mx_m2m_slot);
     gemm_dims(P.mx_upP, chk, K, c, RP_, 0, s, R, K, MpadR_, Kpad_);
   }
-  if (n_upall_) {
+  if (n_upall_ && m16()) {
     const int64_t tot = (int64_t)n_upall_ * KCm_ * 32;
     if (f16_) {
       hipLaunchKernelGGL(k_amax_reset, dim3(1), dim3(64), 0, s, amax_.as<unsigned long long>());
@@ -753,6 +844,10 @@ void Fmm::apply_mixed(const double *d_f, double *d_u, hipStream_t s) {
     if (P.mx_m2l_jobs) {
       mark(3, true, s);
       launch_m2l_mixed(l, P.mx_depth, P.mx_split, s);
+      mark(3, false, s);
+    } else if (prec_ == 4 && P.m2l.njobs) {
+      mark(3, true, s);
+      launch_m2l_t(l, P.m2l_split, s);
       mark(3, false, s);
     }
     mark(4, true, s);

@@ -86,6 +86,30 @@ __global__ __launch_bounds__(256) void k_multidot(int64_t n, int k, const double
   }
 }
 
+// out[j] = sum_{q < nb} partial[j * nb + q], q ascending (the order the
+// host used when it summed the partial dot products itself)
+__global__ void k_sum_partials(int k, int nb, const double *__restrict__ partial, double *__restrict__ out) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= k) return;
+  double t = 0.0;
+  for (int q = 0; q < nb; q++) t += partial[(size_t)j * nb + q];
+  out[j] = t;
+}
+
+// *nrm = sqrt(sum of the nb partials of w . w); y = w / *nrm when it is > 0
+__global__ void k_norm_from_partials(int nb, const double *__restrict__ partial, double *__restrict__ nrm) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  double t = 0.0;
+  for (int q = 0; q < nb; q++) t += partial[q];
+  *nrm = sqrt(t);
+}
+__global__ void k_scale_by_norm(int64_t n, const double *__restrict__ nrm, const double *__restrict__ w,
+                                double *__restrict__ y) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const double h = *nrm;
+  if (i < n && h > 0) y[i] = (1.0 / h) * w[i];
+}
+
 // w -= sum_j h[j] V_j
 __global__ void k_subtract_combo(int64_t n, int k, const double *__restrict__ V,
                                  const double *__restrict__ h, double *__restrict__ w) {
@@ -350,17 +374,49 @@ double Mobility::dot(const double *a, const double *b, hipStream_t s) {
   return t;
 }
 
+// w -= V[0..kk) c with c = V^T w, twice (classical Gram-Schmidt with one
+// reorthogonalisation), then y = w / ||w||; nothing waits on the host.  The
+// coefficients of the two passes land in hc[0..kk) and hc[m1..m1+kk), the
+// norm in hc[2 m1].  The partial dot products are summed on the device in a
+// fixed order (k_sum_partials).
+void Mobility::cgs2(const double *V, int kk, double *w, double *y, double *hc, int m1, double *part,
+                    hipStream_t s) {
+  const int64_t n = n_;
+  const int nb = 256;
+  for (int pass = 0; pass < 2; pass++) {
+    double *c = hc + (size_t)pass * m1;
+    hipLaunchKernelGGL(k_multidot, dim3(nb), dim3(256), 0, s, n, kk, V, w, part);
+    hipLaunchKernelGGL(k_sum_partials, dim3((kk + 63) / 64), dim3(64), 0, s, kk, nb, part, c);
+    hipLaunchKernelGGL(k_subtract_combo, dim3(nblk(n)), dim3(256), 0, s, n, kk, V, c, w);
+  }
+  hipLaunchKernelGGL(k_multidot, dim3(nb), dim3(256), 0, s, n, 1, w, w, part);
+  hipLaunchKernelGGL(k_norm_from_partials, dim3(1), dim3(64), 0, s, nb, part, hc + 2 * (size_t)m1);
+  hipLaunchKernelGGL(k_scale_by_norm, dim3(nblk(n)), dim3(256), 0, s, n, hc + 2 * (size_t)m1, w, y);
+  NNK_HIP(hipGetLastError());
+}
+
+// the Arnoldi column of step k from the two passes' coefficients and the
+// norm (one host wait per GMRES iteration): h[j] = c1[j] + c2[j], h[k+1]
+void Mobility::read_column(const double *hc, int m1, int kk, std::vector<double> &h, std::vector<double> &hh,
+                           hipStream_t s) {
+  NNK_HIP(hipMemcpyAsync(hh.data(), hc, sizeof(double) * (2 * (size_t)m1 + 1), hipMemcpyDeviceToHost, s));
+  NNK_HIP(hipStreamSynchronize(s));
+  for (int j = 0; j < kk; j++) h[j] = hh[j] + hh[(size_t)m1 + j];
+  h[kk] = hh[2 * (size_t)m1];
+}
+
 GmresResult Mobility::gmres(const double *b, double *x, double tol, int maxit, bool use_precon,
                             hipStream_t s, std::vector<double> *resvec) {
   GmresResult R{};
   const int64_t n = n_;
   const int m = std::max(1, (int)std::min<int64_t>(maxit, n));
   const int nb = 256;
-  DBuf V, w, hdev, part;
+  DBuf V, w, hdev, part, hc;
   V.alloc(sizeof(double) * (size_t)(m + 1) * n);
   w.alloc(sizeof(double) * n);
   hdev.alloc(sizeof(double) * (m + 1));
   part.alloc(sizeof(double) * (size_t)(m + 1) * nb);
+  hc.alloc(sizeof(double) * (2 * (size_t)(m + 1) + 1));
   NNK_HIP(hipMemsetAsync(x, 0, sizeof(double) * n, s));
   const double bnorm = std::sqrt(dot(b, b, s));
   if (resvec) resvec->assign(1, bnorm);
@@ -371,36 +427,16 @@ GmresResult Mobility::gmres(const double *b, double *x, double tol, int maxit, b
   }
   double *Vp = V.as<double>();
   hipLaunchKernelGGL(k_axpby, dim3(nblk(n)), dim3(256), 0, s, n, 1.0 / bnorm, b, 0.0, Vp);
-  std::vector<double> H((size_t)(m + 1) * m, 0.0), cs(m), sn(m), g(m + 1, 0.0), h(m + 1), h2(m + 1),
-      hp((size_t)(m + 1) * nb);
+  std::vector<double> H((size_t)(m + 1) * m, 0.0), cs(m), sn(m), g(m + 1, 0.0), h(m + 1),
+      hh(2 * (size_t)(m + 1) + 1);
   g[0] = bnorm;
   int k = 0;
   bool done = false;
-  auto project = [&](int kk, std::vector<double> &coef) {
-    hipLaunchKernelGGL(k_multidot, dim3(nb), dim3(256), 0, s, n, kk, Vp, w.as<double>(), part.as<double>());
-    NNK_HIP(hipGetLastError());
-    NNK_HIP(hipMemcpyAsync(hp.data(), part.p, sizeof(double) * (size_t)kk * nb, hipMemcpyDeviceToHost, s));
-    NNK_HIP(hipStreamSynchronize(s));
-    for (int j = 0; j < kk; j++) {
-      double t = 0;
-      for (int q = 0; q < nb; q++) t += hp[(size_t)j * nb + q];
-      coef[j] = t;
-    }
-    NNK_HIP(hipMemcpyAsync(hdev.p, coef.data(), sizeof(double) * kk, hipMemcpyHostToDevice, s));
-    hipLaunchKernelGGL(k_subtract_combo, dim3(nblk(n)), dim3(256), 0, s, n, kk, Vp, hdev.as<double>(),
-                       w.as<double>());
-    NNK_HIP(hipGetLastError());
-  };
   for (k = 0; k < m && !done; k++) {
     op_any(Vp + (size_t)k * n, w.as<double>(), use_precon, s);
-    project(k + 1, h);
-    project(k + 1, h2);
-    for (int j = 0; j <= k; j++) h[j] += h2[j];
-    double hn = std::sqrt(dot(w.as<double>(), w.as<double>(), s));
-    h[k + 1] = hn;
-    if (hn > 0)
-      hipLaunchKernelGGL(k_axpby, dim3(nblk(n)), dim3(256), 0, s, n, 1.0 / hn, w.as<double>(), 0.0,
-                         Vp + (size_t)(k + 1) * n);
+    cgs2(Vp, k + 1, w.as<double>(), Vp + (size_t)(k + 1) * n, hc.as<double>(), m + 1, part.as<double>(), s);
+    read_column(hc.as<double>(), m + 1, k + 1, h, hh, s);
+    const double hn = h[k + 1];
     for (int j = 0; j < k; j++) {  // previous rotations
       double t = cs[j] * h[j] + sn[j] * h[j + 1];
       h[j + 1] = -sn[j] * h[j] + cs[j] * h[j + 1];
@@ -475,6 +511,93 @@ GmresResult Mobility::refine(const double *b, double *x, double tol, int maxit, 
   R.iter = used;
   R.flag = R.relres <= tol ? 0 : 1;
   if (outer) *outer = it;
+  NNK_HIP(hipStreamSynchronize(s));
+  return R;
+}
+
+GmresResult Mobility::fgmres(const double *b, double *x, double tol, int maxit, double inner_tol,
+                             bool use_precon, hipStream_t s, std::vector<double> *resvec, int *outer) {
+  if (!Fi_) throw Error(NNK_ESTATE, "mobility: no inner operator for flexible GMRES");
+  const int64_t n = n_;
+  const int m = 50;  // outer Arnoldi steps (no restart)
+  const int nb = 256;
+  DBuf V, Z, w, hdev, part, hc;
+  V.alloc(sizeof(double) * (size_t)(m + 1) * n);
+  Z.alloc(sizeof(double) * (size_t)m * n);
+  w.alloc(sizeof(double) * n);
+  hdev.alloc(sizeof(double) * (m + 1));
+  part.alloc(sizeof(double) * (size_t)(m + 1) * nb);
+  hc.alloc(sizeof(double) * (2 * (size_t)(m + 1) + 1));
+  GmresResult R{};
+  NNK_HIP(hipMemsetAsync(x, 0, sizeof(double) * n, s));
+  const double bnorm = std::sqrt(dot(b, b, s));
+  if (resvec) resvec->assign(1, bnorm);
+  if (outer) *outer = 0;
+  if (bnorm == 0.0) {
+    R.flag = 0;
+    NNK_HIP(hipStreamSynchronize(s));
+    return R;
+  }
+  double *Vp = V.as<double>(), *Zp = Z.as<double>();
+  hipLaunchKernelGGL(k_axpby, dim3(nblk(n)), dim3(256), 0, s, n, 1.0 / bnorm, b, 0.0, Vp);
+  std::vector<double> H((size_t)(m + 1) * m, 0.0), cs(m), sn(m), g(m + 1, 0.0), h(m + 1),
+      hh(2 * (size_t)(m + 1) + 1);
+  g[0] = bnorm;
+  int k = 0, used = 0;
+  bool done = false;
+  for (k = 0; k < m && !done; k++) {
+    // z_k: the inner (mixed-form) solve of A z = v_k, right-preconditioned
+    Fcur_ = Fi_;
+    GmresResult gi;
+    try {
+      gi = gmres(Vp + (size_t)k * n, Zp + (size_t)k * n, inner_tol, std::max(1, maxit - used), use_precon, s,
+                 nullptr);
+    } catch (...) {
+      Fcur_ = nullptr;
+      throw;
+    }
+    Fcur_ = nullptr;
+    used += std::max(gi.iter, 1);
+    apply(Zp + (size_t)k * n, w.as<double>(), s);  // FP64 operator
+    cgs2(Vp, k + 1, w.as<double>(), Vp + (size_t)(k + 1) * n, hc.as<double>(), m + 1, part.as<double>(), s);
+    read_column(hc.as<double>(), m + 1, k + 1, h, hh, s);
+    const double hn = h[k + 1];
+    for (int j = 0; j < k; j++) {
+      const double t = cs[j] * h[j] + sn[j] * h[j + 1];
+      h[j + 1] = -sn[j] * h[j] + cs[j] * h[j + 1];
+      h[j] = t;
+    }
+    const double r = std::hypot(h[k], h[k + 1]);
+    cs[k] = r > 0 ? h[k] / r : 1.0;
+    sn[k] = r > 0 ? h[k + 1] / r : 0.0;
+    h[k] = r;
+    h[k + 1] = 0.0;
+    g[k + 1] = -sn[k] * g[k];
+    g[k] = cs[k] * g[k];
+    for (int j = 0; j <= k; j++) H[(size_t)j * m + k] = h[j];
+    if (resvec) resvec->push_back(std::fabs(g[k + 1]));
+    if (std::fabs(g[k + 1]) <= tol * bnorm || hn == 0.0 || used >= maxit) done = true;
+  }
+  const int kk = k;
+  std::vector<double> y(kk, 0.0);
+  for (int i = kk - 1; i >= 0; i--) {
+    double t = g[i];
+    for (int j = i + 1; j < kk; j++) t -= H[(size_t)i * m + j] * y[j];
+    y[i] = H[(size_t)i * m + i] != 0 ? t / H[(size_t)i * m + i] : 0.0;
+  }
+  // x = Z y, then the true FP64 residual
+  NNK_HIP(hipMemcpyAsync(hdev.p, y.data(), sizeof(double) * kk, hipMemcpyHostToDevice, s));
+  NNK_HIP(hipMemsetAsync(w.p, 0, sizeof(double) * n, s));
+  hipLaunchKernelGGL(k_subtract_combo, dim3(nblk(n)), dim3(256), 0, s, n, kk, Zp, hdev.as<double>(),
+                     w.as<double>());
+  hipLaunchKernelGGL(k_axpby, dim3(nblk(n)), dim3(256), 0, s, n, -1.0, w.as<double>(), 0.0, x);
+  apply(x, w.as<double>(), s);
+  hipLaunchKernelGGL(k_axpby, dim3(nblk(n)), dim3(256), 0, s, n, 1.0, b, -1.0, w.as<double>());
+  NNK_HIP(hipGetLastError());
+  R.relres = std::sqrt(dot(w.as<double>(), w.as<double>(), s)) / bnorm;
+  R.iter = used;
+  R.flag = R.relres <= tol ? 0 : 1;
+  if (outer) *outer = kk;
   NNK_HIP(hipStreamSynchronize(s));
   return R;
 }

@@ -44,6 +44,15 @@ class Mobility {
   // iteration count in iter and the outer count in outer.
   GmresResult refine(const double *b, double *x, double tol, int maxit, double inner_tol, bool use_precon,
                      hipStream_t s, std::vector<double> *resvec, int *outer);
+  // Flexible GMRES (Saad 1993): an FP64 Arnoldi process on the FP64
+  // operator whose k-th basis vector is preconditioned by an inner GMRES
+  // solve (to inner_tol, on the inner operator, itself right-preconditioned
+  // when use_precon) that may differ from step to step; x = Z y from the
+  // stored preconditioned vectors Z.  At most 50 outer steps (no restart)
+  // and maxit inner iterations in total.  Returns the inner iteration count
+  // in iter and the outer step count in outer.
+  GmresResult fgmres(const double *b, double *x, double tol, int maxit, double inner_tol, bool use_precon,
+                     hipStream_t s, std::vector<double> *resvec, int *outer);
   // inner operators of the refinement: the matvec and (optional) the
   // coronaRes-4 operator of the preconditioner used inside the inner solves
   void set_inner(Fmm *inner, Fmm *inner_sparse = nullptr) {
@@ -61,6 +70,9 @@ class Mobility {
   void kifmm(Fmm *Fm, const double *f, double *u, bool accumulate, hipStream_t s);
   void reduced(const double *v, double *f, hipStream_t s);
   double dot(const double *a, const double *b, hipStream_t s);
+  void cgs2(const double *V, int kk, double *w, double *y, double *hc, int m1, double *part, hipStream_t s);
+  void read_column(const double *hc, int m1, int kk, std::vector<double> &h, std::vector<double> &hh,
+                   hipStream_t s);
 
   Fmm *F_, *F4_, *Fi_ = nullptr, *F4i_ = nullptr;
   Fmm *Fcur_ = nullptr;  // operator used by apply(): F_ or Fi_

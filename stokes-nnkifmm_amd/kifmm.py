@@ -16,7 +16,7 @@ from .octree import parse_options
 
 _OPTS = {"GPU": 0, "parThreads": 0, "blockSize": 0.1, "GMRES": 0, "coronaRes": 6, "coronaShells": 1,
          "format": 1, "precision": "fp64", "svdTol": 1e-8, "hipGraph": 0, "autotune": 0}
-_PREC = {"fp64": 0, "mixed": 1, "mixed3": 2, "mixedh": 3}
+_PREC = {"fp64": 0, "mixed": 1, "mixed3": 2, "mixedh": 3, "fp64t": 4}
 
 
 def _is_torch(x):
@@ -51,7 +51,9 @@ class KIFMM:
     accepted for compatibility; the matvec always runs on the GPU and never
     materialises a kernel block.  Build-only options: 'precision' ('fp64',
     the reference's numbers, 'mixed' or 'mixed3': two- or three-part
-    split-BF16 M2L, 'mixedh': two-part split-FP16 M2L on scaled operands), 'svdTol' (the mixed forms' pseudo-inverse cut) and
+    split-BF16 M2L, 'mixedh': two-part split-FP16 M2L on scaled operands,
+    'fp64t': the mixed forms' truncated operators with every operation in
+    FP64), 'svdTol' (the truncated forms' pseudo-inverse cut) and
     'hipGraph' (1: replay each matvec, and the mobility solve's GMRES
     operator, from a captured HIP graph) and 'autotune' (1: pick the M2L
     launch shapes by timing candidates on the first matvec)."""

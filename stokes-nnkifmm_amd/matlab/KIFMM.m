@@ -7,8 +7,8 @@ classdef KIFMM
     %               'GMRES', 0|1, 'format', 1|2, 'coronaRes', r,
     %               'coronaShells', s)
     %   Options beyond the reference: 'precision' ('fp64' (default),
-    %   'mixed', 'mixed3', 'mixedh') and 'svdTol' (pseudo-inverse cut of
-    %   the mixed forms).  GPU, parThreads and blockSize are accepted and
+    %   'mixed', 'mixed3', 'mixedh', 'fp64t') and 'svdTol' (pseudo-inverse
+    %   cut of the truncated forms).  GPU, parThreads and blockSize are accepted and
     %   unused: the library always runs on the GPU.
     properties
         tree;
@@ -49,7 +49,7 @@ classdef KIFMM
             pot = this.run('rsolve', this.arguments.coronaRes, vel);
         end
         function id = handle(this, res)
-            p = find(strcmp(this.arguments.precision, {'fp64', 'mixed', 'mixed3', 'mixedh'})) - 1;
+            p = find(strcmp(this.arguments.precision, {'fp64', 'mixed', 'mixed3', 'mixedh', 'fp64t'})) - 1;
             if isempty(p)
                 error('KIFMM: unknown precision ''%s''', this.arguments.precision);
             end

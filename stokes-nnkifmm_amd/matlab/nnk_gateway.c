@@ -359,6 +359,17 @@ int32_t nnkgw_mobility_solve_refined(nnkgw_id mob, nnkgw_id inner, nnkgw_id inne
                                         outer));
 }
 
+int32_t nnkgw_mobility_solve_fgmres(nnkgw_id mob, nnkgw_id inner, nnkgw_id inner4, const double *rhs,
+                                    double tol, int32_t maxit, double inner_tol, int32_t precon, double *x,
+                                    int32_t *flag, double *relres, int32_t *iters, int32_t *outer) {
+  nnk_mobility *m = (nnk_mobility *)get(mob, NNKGW_MOBILITY);
+  nnk_fmm *fi = (nnk_fmm *)get(inner, NNKGW_FMM);
+  nnk_fmm *f4 = inner4 ? (nnk_fmm *)get(inner4, NNKGW_FMM) : NULL;
+  if (!m || !fi || (inner4 && !f4)) return fail(NNK_EARG, "nnkifmm: flexible GMRES: not a live handle");
+  return lib(nnk_mobility_solve_fgmres(m, fi, f4, rhs, tol, maxit, inner_tol, precon, x, flag, relres, iters,
+                                       outer));
+}
+
 int32_t nnkgw_mobility_size(nnkgw_id mob, int64_t *n) {
   if (!get(mob, NNKGW_MOBILITY) || !n) return fail(NNK_EARG, "nnkifmm: size: not a live mobility handle");
   *n = g_tab[mob - 1].size;

@@ -88,6 +88,9 @@ int32_t nnkgw_mobility_solve(nnkgw_id mob, const double *rhs, double tol, int32_
 int32_t nnkgw_mobility_solve_refined(nnkgw_id mob, nnkgw_id inner, nnkgw_id inner4, const double *rhs,
                                      double tol, int32_t maxit, double inner_tol, int32_t precon, double *x,
                                      int32_t *flag, double *relres, int32_t *iters, int32_t *outer);
+int32_t nnkgw_mobility_solve_fgmres(nnkgw_id mob, nnkgw_id inner, nnkgw_id inner4, const double *rhs,
+                                    double tol, int32_t maxit, double inner_tol, int32_t precon, double *x,
+                                    int32_t *flag, double *relres, int32_t *iters, int32_t *outer);
 int32_t nnkgw_mobility_size(nnkgw_id mob, int64_t *n);
 
 /* handle-class delete(): release one handle (its object is destroyed once

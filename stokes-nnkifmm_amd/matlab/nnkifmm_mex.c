@@ -23,8 +23,8 @@
  *   id = nnkifmm_mex('mobility', fmm, fmm4 or [], B.', BT.', A1.', A2.')
  *   y  = nnkifmm_mex('mobility_apply' | 'precon', mob, x)
  *   [x, flag, relres, iter, resvec] = nnkifmm_mex('solve', mob, rhs, tol, maxit, precon)
- *   [x, flag, relres, iter, outer] = nnkifmm_mex('solve_refined', mob, inner, inner4 or [],
- *                                                rhs, tol, maxit, innerTol, precon)
+ *   [x, flag, relres, iter, outer] = nnkifmm_mex('solve_refined' | 'solve_fgmres', mob, inner,
+ *                                                inner4 or [], rhs, tol, maxit, innerTol, precon)
  *        nnkifmm_mex('opcache', bytes)
  *        nnkifmm_mex('free', id)
  */
@@ -199,7 +199,7 @@ void mexFunction(int nlhs, mxArray *plhs[], int nrhs, const mxArray *prhs[]) {
       mxDestroyArray(rv);
     }
 
-  } else if (!strcmp(cmd, "solve_refined")) {
+  } else if (!strcmp(cmd, "solve_refined") || !strcmp(cmd, "solve_fgmres")) {
     need(nrhs, 9, cmd);
     nnkgw_id id = id_of(prhs[1]);
     int64_t n;
@@ -208,10 +208,9 @@ void mexFunction(int nlhs, mxArray *plhs[], int nrhs, const mxArray *prhs[]) {
     int32_t flag = 1, it = 0, outer = 0;
     double relres = 0;
     plhs[0] = mxCreateDoubleMatrix(n, 1, mxREAL);
-    ok(nnkgw_mobility_solve_refined(id, id_of(prhs[2]), id_of(prhs[3]), mxGetPr(prhs[4]), mxGetScalar(prhs[5]),
-                                    (int32_t)mxGetScalar(prhs[6]), mxGetScalar(prhs[7]),
-                                    (int32_t)mxGetScalar(prhs[8]), mxGetPr(plhs[0]), &flag, &relres, &it,
-                                    &outer));
+    ok((!strcmp(cmd, "solve_fgmres") ? nnkgw_mobility_solve_fgmres : nnkgw_mobility_solve_refined)(
+        id, id_of(prhs[2]), id_of(prhs[3]), mxGetPr(prhs[4]), mxGetScalar(prhs[5]), (int32_t)mxGetScalar(prhs[6]),
+        mxGetScalar(prhs[7]), (int32_t)mxGetScalar(prhs[8]), mxGetPr(plhs[0]), &flag, &relres, &it, &outer));
     if (nlhs > 1) plhs[1] = mxCreateDoubleScalar(flag);
     if (nlhs > 2) plhs[2] = mxCreateDoubleScalar(relres);
     if (nlhs > 3) plhs[3] = mxCreateDoubleScalar(it);

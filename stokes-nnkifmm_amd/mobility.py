@@ -148,13 +148,18 @@ def mobility_gmres(swimmers, B, BT, A1, A2, rhs, tol, maxIter, precon):
 
 
 def mobility_refined(swimmers, B, BT, A1, A2, rhs, tol, maxIter, precon, innerTol=1e-4, svdTol=1e-8,
-                     innerPrecision="mixed"):
-    """The solve of mobility_gmres by iterative refinement (no counterpart in
-    the reference): residuals with the FP64 operator, corrections by GMRES on
-    the mixed-precision KIFMM of the same tree to innerTol (preconditioned
-    with mixed-precision coronaRes-4 operators too).  The answer meets tol on
+                     innerPrecision="mixed", outer="refine"):
+    """The solve of mobility_gmres around a mixed-precision inner solver (no
+    counterpart in the reference): GMRES on the mixed-precision KIFMM of the
+    same tree to innerTol (preconditioned with mixed-precision coronaRes-4
+    operators too) inside an FP64 outer iteration, either iterative
+    refinement (outer="refine": FP64 residual, inner correction, add) or
+    flexible GMRES (outer="fgmres": FP64 Arnoldi on the FP64 operator, each
+    basis vector preconditioned by an inner solve).  The answer meets tol on
     the unpreconditioned FP64 system.  Records flag, relres, iter (inner
     iterations) and outer in mobility.last_solve."""
+    if outer not in ("refine", "fgmres"):
+        raise ValueError(f"mobility_refined: unknown outer iteration {outer!r}")
     from .kifmm import _Handle
     m = _mobility(swimmers, B, BT, A1, A2, precon=bool(precon))
     fmm = swimmers.FMM
@@ -172,14 +177,14 @@ def mobility_refined(swimmers, B, BT, A1, A2, rhs, tol, maxIter, precon, innerTo
     x = np.empty_like(b)
     flag = np.zeros(1, np.int32)
     it = np.zeros(1, np.int32)
-    outer = np.zeros(1, np.int32)
     rel = np.zeros(1)
-    check(lib().nnk_mobility_solve_refined(m.h, inner[0].h, inner[1].h if inner[1] else None, ptr(b), float(tol),
-                                           int(maxIter), float(innerTol),
-                                           int(bool(precon)), ptr(x), ptr(flag, _lib._i32), ptr(rel),
-                                           ptr(it, _lib._i32), ptr(outer, _lib._i32)))
+    nout = np.zeros(1, np.int32)
+    solve = lib().nnk_mobility_solve_fgmres if outer == "fgmres" else lib().nnk_mobility_solve_refined
+    check(solve(m.h, inner[0].h, inner[1].h if inner[1] else None, ptr(b), float(tol), int(maxIter),
+                float(innerTol), int(bool(precon)), ptr(x), ptr(flag, _lib._i32), ptr(rel), ptr(it, _lib._i32),
+                ptr(nout, _lib._i32)))
     last_solve.clear()
-    last_solve.update(flag=int(flag[0]), relres=float(rel[0]), iter=int(it[0]), outer=int(outer[0]))
+    last_solve.update(flag=int(flag[0]), relres=float(rel[0]), iter=int(it[0]), outer=int(nout[0]))
     return x
 
 

@@ -173,3 +173,18 @@ extern "C" int nnk_selftest_m2l_plan(const int32_t *sizes, const int32_t *gv, in
     return 1;
   }
 }
+
+namespace nnk {
+int svd_jacobi(const double *A, int n, double *U, double *s, double *V);
+}
+
+// The check-to-equivalent SVD (GPU one-sided Jacobi) of an n x n row-major
+// matrix; returns the sweeps used, or -1 with the message in err.
+extern "C" int nnk_selftest_svd(const double *A, int n, double *U, double *s, double *V, char *err, int errlen) {
+  try {
+    return nnk::svd_jacobi(A, n, U, s, V);
+  } catch (const std::exception &e) {
+    copy_err(e.what(), err, errlen);
+    return -1;
+  }
+}

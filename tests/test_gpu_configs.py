@@ -95,6 +95,25 @@ def test_suspension_refined_solve(nnk, suspension):
     assert rel(xr[-nb:], x64[-nb:]) < 1e-4
 
 
+def test_suspension_fgmres_solve(nnk, suspension):
+    """Flexible GMRES (FP64 Arnoldi on the FP64 operator, every basis vector
+    preconditioned by a mixed-form inner GMRES to 1e-2): meets 1e-6 on the
+    oracle's FP64 system in a few outer steps, and its body velocities agree
+    with the FP64 solve's."""
+    sw, B, BT, A1, A2, m = suspension
+    mob = importlib.import_module("stokes-nnkifmm_amd.mobility")
+    b = mob._rhs(sw, lambda t, x: np.zeros_like(x), 0.0)
+    x64 = mob.mobility_gmres(sw, B, BT, A1, A2, b, 1e-6, 400, True)
+    xf = mob.mobility_refined(sw, B, BT, A1, A2, b, 1e-6, 400, True, innerTol=1e-2, outer="fgmres")
+    info = dict(mob.last_solve)
+    print(f"[fgmres] {info}", flush=True)
+    assert info["flag"] == 0 and 1 <= info["outer"] <= 10, info
+    r = m.apply(xf) - b
+    assert np.linalg.norm(r) <= 1.05e-6 * np.linalg.norm(b)
+    nb = 6 * sw.swimmerNo()
+    assert rel(xf[-nb:], x64[-nb:]) < 1e-4
+
+
 def _oracle_apply(oracle, sw, eps, x):
     pout, fout, NN, prot, frot = sw.getSwimmers()
     n = len(pout)

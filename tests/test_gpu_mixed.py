@@ -119,11 +119,14 @@ def test_refined_solve_reaches_fp64_accuracy(oracle, nnk, precon):
 @pytest.mark.parametrize("precision,split", [("mixedh", 1), ("mixedh", 2), ("mixed", 1), ("mixed3", 2)])
 def test_mixedh_m2l_kernels_agree_bitwise(nnk, precision, split):
     """The split-precision M2L of every level run by the register-ring
-    kernel (depth 2 and, but for the three-part form, 3) and by the
-    DMA-staged kernel (depth 4): the same products in the same order, so
-    the matvecs are bit-identical for one split-K factor.  For the FP16
-    form, operator rows that are all zero (unscale 0) and rows scaled by
-    widely different powers of two come with the clustered points."""
+    kernel (depth 2 and, but for the three-part form, 3), by the DMA-staged
+    kernel (depth 4) and, for the two-part forms, by the DMA-staged kernel
+    with 64 x 48 wave tiles (depth 5, on the levels run as class jobs;
+    depth 4 elsewhere): the same products in the same order, so the matvecs
+    are bit-identical for one split-K factor (with split-K, depth 5 groups
+    the partial sums differently: equal to rounding).  For the FP16 form, operator
+    rows that are all zero (unscale 0) and rows scaled by widely different
+    powers of two come with the clustered points."""
     rng = np.random.default_rng(44)
     p = np.concatenate([rng.random((30000, 3)), 1e-3 * rng.random((3000, 3)) + 0.5])
     q = rng.standard_normal(p.shape)
@@ -146,8 +149,26 @@ def test_mixedh_m2l_kernels_agree_bitwise(nnk, precision, split):
         lib.nnk_fmm_launch_config(h, chk_s.ctypes.data_as(i32), chk_d.ctypes.data_as(i32), 16)
         assert (chk_s[levels] == split).all() and (chk_d[levels] == depth).all()
         out[depth] = f.computeVel(q)
+    if precision != "mixed3":
+        wide = []
+        for l in levels:
+            s = np.zeros(16, np.int32)
+            d = np.zeros(16, np.int32)
+            s[l], d[l] = split, 5
+            if lib.nnk_fmm_set_launch_config(h, s.ctypes.data_as(i32), d.ctypes.data_as(i32), 16) == 0:
+                wide.append(int(l))
+        print(f"[m2l] {precision} split {split}: 64 x 48 wave tiles at levels {wide}", flush=True)
+        assert wide   # the levels with class jobs (the merged-offset levels keep depth 4)
+        out[5] = f.computeVel(q)
     for depth in depths[1:]:
         np.testing.assert_array_equal(out[depth], out[2])
+    if 5 in out:
+        # 192-column tiles split their (larger) active block lists at other
+        # points, so with split-K only the grouping of the partial sums differs
+        if split == 1:
+            np.testing.assert_array_equal(out[5], out[2])
+        else:
+            assert rel(out[5], out[2]) < 1e-13
 
 
 def test_launch_config_rejects_what_a_form_cannot_run(nnk):
@@ -176,7 +197,8 @@ def test_launch_config_rejects_what_a_form_cannot_run(nnk):
     l64, lb, lh = first_m2l_level(f64), first_m2l_level(fb), first_m2l_level(fh)
     assert setcfg(f64, l64, 1, 3) != 0          # the FP64 form has no ring depth
     assert setcfg(fb, lb, 1, 3) != 0            # three parts: no third register stage
-    assert setcfg(fh, lh, 1, 5) != 0
+    assert setcfg(fb, lb, 1, 5) != 0            # three parts: no 64 x 48 wave tiles
+    assert setcfg(fh, lh, 1, 6) != 0
     assert setcfg(fh, lh, 0, 4) == 0            # split 0: level left as it is
     assert setcfg(fh, lh, 65, 3) != 0
     assert setcfg(fh, 0, 1, 3) != 0             # level without M2L
@@ -184,3 +206,32 @@ def test_launch_config_rejects_what_a_form_cannot_run(nnk):
     np.testing.assert_array_equal(f64.computeVel(p), u64)
     np.testing.assert_array_equal(fb.computeVel(p), ub)
     np.testing.assert_array_equal(fh.computeVel(p), uh)
+
+
+@pytest.mark.parametrize("case", ["uniform", "blobs", "distinct"])
+def test_fp64t_against_direct_and_mixed3(oracle, nnk, case):
+    """The FP64 truncated form (the mixed forms' pseudo-inverses and R x R
+    translations with FP64 M2L and FP64 direct sums) at 8 points per edge:
+    below 1e-6 against the FP64 direct sum on a uniform tree (1e-5 on the
+    adaptive ones, whose deep levels the truncation serves less well), and
+    the three-part BF16 form (same operators, 24-bit M2L, FP32-math direct
+    sums) within 5e-7 of it."""
+    rng = np.random.default_rng(41)
+    if case == "uniform":
+        p = t = rng.random((30000, 3)); cap, eps = 64, 1e-3
+    elif case == "blobs":
+        p = t = blobs(rng); cap, eps = 16, 1e-3
+    else:
+        p = rng.random((8000, 3)); t = np.concatenate([rng.random((3000, 3)), p[:1000]]); cap, eps = 40, 5e-3
+    tree = nnk.OcTree(p, t, "nodeCapacity", cap)
+    q = rng.standard_normal((int(tree.potentials.sum()), 3))
+    print(f"[fp64t] case={case} points={len(tree.points)} nodes={tree.nodeCount}", flush=True)
+    ut = nnk.KIFMM(tree, [eps, 1.0], "coronaRes", 8, "precision", "fp64t").computeVel(q)
+    u3 = nnk.KIFMM(tree, [eps, 1.0], "coronaRes", 8, "precision", "mixed3").computeVel(q)
+    tg = tree.getTargetPoints()
+    idx = np.sort(rng.choice(len(tg), min(2000, len(tg)), replace=False))
+    ud = oracle.direct(tree.getPotentialPoints(), q, tg[idx], eps)
+    et = rel(ut[idx], ud)
+    print(f"[fp64t] rel err vs direct {et:.3e}, vs mixed3 {rel(u3, ut):.3e}", flush=True)
+    assert et < (1e-6 if case == "uniform" else 1e-5)
+    assert rel(u3, ut) < 5e-7

@@ -138,6 +138,30 @@ def test_gmres_matches_oracle(oracle, nnk, precon):
     assert rel(x, xo) < 1e-7
 
 
+def test_fgmres_outer_iteration(oracle, nnk):
+    """Flexible GMRES on the two-sphere + wall system.  With an FP64 inner
+    operator solved to 1e-12, the first preconditioned direction already
+    solves the system: one outer step (the flexible Arnoldi relation
+    A Z = V H).  With a mixed inner operator at 1e-2 it takes a few steps
+    and meets 1e-8 under the oracle's operator, matching the oracle's
+    GMRES solution."""
+    sw, B, BT, A1, A2, m = _system(nnk, oracle, True)
+    from importlib import import_module
+    mob = import_module("stokes-nnkifmm_amd.mobility")
+    b = mob._rhs(sw, lambda t, x: np.zeros_like(x), 0.0)
+    x1 = mob.mobility_refined(sw, B, BT, A1, A2, b, 1e-8, 300, True, innerTol=1e-12, innerPrecision="fp64",
+                              outer="fgmres")
+    i1 = dict(mob.last_solve)
+    assert i1["flag"] == 0 and i1["outer"] == 1, i1
+    x2 = mob.mobility_refined(sw, B, BT, A1, A2, b, 1e-8, 300, True, innerTol=1e-2, outer="fgmres")
+    i2 = dict(mob.last_solve)
+    assert i2["flag"] == 0 and 2 <= i2["outer"] <= 12, i2
+    for x in (x1, x2):
+        assert np.linalg.norm(m.apply(x) - b) <= 1.05e-8 * np.linalg.norm(b)
+    xo, _ = m.gmres(b, 1e-10, 300, True)
+    assert rel(x2, xo) < 1e-6
+
+
 @pytest.mark.parametrize("offset", [0.0, 0.3])
 def test_sphere_gravity_sedimentation_and_offset_torque(nnk, offset):
     """mobilityProblemGravity (mobilityProblemGravity.m:25-74): F = G and
