@@ -283,6 +283,22 @@ int32_t nnk_direct_apply_device(const double *d_src, int64_t ns, const double *d
                                 const double *d_tgt, int64_t nt, double eps, double mu,
                                 double *d_u, void *stream);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): nnk_direct_apply_device with the same kernels and the
+ * same summation order (bit-identical results), but its per-source-part
+ * partial sums go to a static device array of 16 MiB (2^21 doubles) that
+ * the library holds on every GPU it runs on, instead of a stream-ordered
+ * allocation.  It needs 24 * S * nt bytes, S = source parts (S = 1, none,
+ * for nt > 130,816 targets; S * nt < 655,360 otherwise), so every size
+ * fits; a size that did not would be refused with NNK_EARG.  No hipMalloc,
+ * hipFree or allocating library call is made.  Every call on one GPU shares
+ * that GPU's array: calls that overlap on one GPU (other streams, other
+ * threads) race. */
+int32_t nnk_direct_apply_device_noalloc(const double *d_src, int64_t ns, const double *d_q,
+                                        const double *d_tgt, int64_t nt, double eps, double mu,
+                                        double *d_u, void *stream);
+
 /* No counterpart in the reference (which times with tic/toc): per-phase
  * HIP-event timing of nnk_fmm_apply* on the apply stream.  Phases:
  * 0 strengths, 1 S2M, 2 upward GEMMs (M2M + solves), 3 M2L, 4 L2L, 5 P2L,

@@ -44,6 +44,8 @@ SIGNATURES = {
                                           ctypes.c_double, _d]),
     "nnk_direct_apply_device": (ctypes.c_int32, [_vp, ctypes.c_int64, _vp, _vp, ctypes.c_int64,
                                                  ctypes.c_double, ctypes.c_double, _vp, _vp]),
+    "nnk_direct_apply_device_noalloc": (ctypes.c_int32, [_vp, ctypes.c_int64, _vp, _vp, ctypes.c_int64,
+                                                         ctypes.c_double, ctypes.c_double, _vp, _vp]),
     "nnk_fmm_set_profiling": (ctypes.c_int32, [_vp, ctypes.c_int32]),
     "nnk_fmm_phase_times": (ctypes.c_int32, [_vp, _d, _i64, ctypes.c_int32]),
     "nnk_fmm_stats": (ctypes.c_int32, [_vp, _i64, ctypes.c_int32]),

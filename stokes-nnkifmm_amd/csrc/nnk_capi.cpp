@@ -26,6 +26,8 @@ int64_t nearest_matrix(const double *coarse, int64_t n, const double *fine, int6
                        int64_t *colptr, int64_t *row, double *val, int64_t cap);
 void direct_device(const double *src, int64_t ns, const double *q, const double *tgt, int64_t nt,
                    double eps, double mu, double *out, hipStream_t s);
+void direct_device_noalloc(const double *src, int64_t ns, const double *q, const double *tgt, int64_t nt,
+                           double eps, double mu, double *out, hipStream_t s);
 }
 
 namespace {
@@ -472,6 +474,19 @@ int32_t nnk_direct_apply_device(const double *d_src, int64_t ns, const double *d
       return fail(NNK_EARG, "nnk_direct_apply_device: bad argument");
     if (!(eps > 0) || !(mu > 0)) return fail(NNK_EARG, "nnk_direct_apply_device: eps and mu must be > 0");
     nnk::direct_device(d_src, ns, d_q, d_tgt, nt, eps, mu, d_u, (hipStream_t)stream);
+    return NNK_OK;
+  });
+}
+
+int32_t nnk_direct_apply_device_noalloc(const double *d_src, int64_t ns, const double *d_q,
+                                        const double *d_tgt, int64_t nt, double eps, double mu,
+                                        double *d_u, void *stream) {
+  return guard([&]() -> int32_t {
+    if (ns < 0 || nt < 0 || (ns && (!d_src || !d_q)) || (nt && (!d_tgt || !d_u)))
+      return fail(NNK_EARG, "nnk_direct_apply_device_noalloc: bad argument");
+    if (!(eps > 0) || !(mu > 0))
+      return fail(NNK_EARG, "nnk_direct_apply_device_noalloc: eps and mu must be > 0");
+    nnk::direct_device_noalloc(d_src, ns, d_q, d_tgt, nt, eps, mu, d_u, (hipStream_t)stream);
     return NNK_OK;
   });
 }
