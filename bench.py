@@ -265,9 +265,11 @@ def run_form(nnk, lib, tree, fd, args, eps, mu, precision, res, svd_tol):
         flops = (6 if np_ == 3 else 3) * 2.0 * R * R * st[0]   # 16-bit products per split product
         peak = BF16_MFMA_PEAK_TFLOPS
         used = {int(dep[l]) for l in range(16) if spl[l] > 0}
-        names = ([f"k_m2l_split<{np_},D,1,{f16}>"] if used - {4} else []) + \
-                ([f"k_m2l_dma<{np_},1,{f16}>"] if 4 in used else [])
-        kern = " / ".join(names) + f" (M2L, {what} MFMA{', LDS-DMA staged' if used == {4} else ''})"
+        names = ([f"k_m2l_split<{np_},D,1,{f16}>"] if used & {2, 3} else []) + \
+                ([f"k_m2l_dma<{np_},1,{f16}>"] if 4 in used else []) + \
+                ([f"k_m2l_dma<{np_},1,{f16},3> (64 x 48 wave tiles)"] if 5 in used else []) + \
+                ([f"k_m2l_dma<{np_},1,{f16},2,4> (four LDS stages)"] if 6 in used else [])
+        kern = " / ".join(names) + f" (M2L, {what} MFMA{', LDS-DMA staged' if used <= {4, 5, 6} else ''})"
     m2l_ms = ms[3] / args.steps
     achieved = flops / (m2l_ms * 1e-3) / 1e12 if m2l_ms > 0 else 0.0
     rec = {"precision": precision, "coronaRes": res, "svd_tol": svd_tol if precision != "fp64" else None,
@@ -317,6 +319,15 @@ def matvec_roofline(precision, st, K, R, ms_step):
     return {"frac": round(ideal_ms / ms_step, 4), "ideal_ms": round(ideal_ms, 3),
             "classes": {k: {"flops": f, "peak_tflops": pk, "ideal_ms": round(f / (pk * 1e12) * 1e3, 4)}
                         for k, (f, pk) in cls.items()}}
+
+
+def handle_ideal_ms(lib, h, precision):
+    """Ideal time (ms) of one matvec of a KIFMM handle: the roofline sum of
+    matvec_roofline over its work counts."""
+    P = ctypes.POINTER
+    st = np.zeros(14, np.int64)
+    lib.nnk_fmm_stats(h.h, st.ctypes.data_as(P(ctypes.c_int64)), 14)
+    return matvec_roofline(precision, st, int(st[12]), int(st[13]), 1.0)["ideal_ms"]
 
 
 def direct_sample(lib, pts, fd, idx, eps, mu):
@@ -394,7 +405,19 @@ def spheroid_solve(nnk):
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     info = mob.last_solve
-    out["fp64_precon"] = {"wall_s": round(dt, 3), "iterations": info["iter"], "relres": info["relres"]}
+    # roofline of the solve: per iteration one coronaRes-6 and one
+    # coronaRes-4 matvec at their ideal (roofline-sum) times; the leaf-block
+    # solves, SpMVs and Arnoldi vectors are not counted (a lower bound)
+    lib = nnk.lib()
+    i6 = handle_ideal_ms(lib, sw.FMM._handle(), "fp64")
+    i4 = handle_ideal_ms(lib, sw.FMM._handle(res=4), "fp64")
+    ideal_s = (info["iter"] + 2) * (i6 + i4) * 1e-3
+    out["fp64_precon"] = {"wall_s": round(dt, 3), "iterations": info["iter"], "relres": info["relres"],
+                          "roofline": {"frac": round(ideal_s / dt, 4), "ideal_s": round(ideal_s, 4),
+                                       "matvec_ideal_ms": {"coronaRes6": i6, "coronaRes4": i4},
+                                       "counted": "iterations + 2 applications of the res-6 and res-4 "
+                                                  "FP64 matvecs (roofline sum over kernel classes); "
+                                                  "leaf-block solves, SpMVs and Arnoldi not counted"}}
     t0 = time.perf_counter()
     xr = mob.mobility_refined(sw, B, BT, A1, A2, b, 1e-6, 400, True, innerTol=1e-3)
     torch.cuda.synchronize()

@@ -1312,7 +1312,7 @@ void Fmm::set_launch_config(int level, int split, int depth) {
     P.m2l_split = split;
   } else {
     const bool ok = depth == 2 || (depth == 3 && !(np_ == 3 && !f16_)) ||
-                    (depth >= 4 && depth <= 5 && dma_ok(P, split, depth));
+                    (depth >= 4 && depth <= 6 && dma_ok(P, split, depth));
     if (!ok) throw Error(NNK_EARG, "launch config: depth not available for this form");
     P.mx_split = split;
     P.mx_depth = depth;
@@ -1338,7 +1338,7 @@ void Fmm::autotune(hipStream_t s) {
       if (s0 > 1) splits.push_back(std::max(1, s0 / 2));
       if ((size_t)std::min(64, 2 * s0) * per_split <= part_elems_) splits.push_back(std::min(64, 2 * s0));
       std::vector<int> depths = !mixed ? std::vector<int>{0} : np_ == 3 && !f16_ ? std::vector<int>{2, 4}
-                                                                                  : std::vector<int>{2, 3, 4, 5};
+                                                                                  : std::vector<int>{2, 3, 4, 5, 6};
       float best = 1e30f;
       int bs = s0, bd = mixed ? P.mx_depth : 0;
       for (int sp : splits) {

@@ -166,30 +166,32 @@ __global__ __launch_bounds__(512, 1) void k_m2l_split(const int M, const int KC,
     if constexpr (F16) return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
     else return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
   };
+  // the products term by term over all tiles (an accumulator's MFMAs are
+  // TM TN instructions apart; per accumulator the order is unchanged)
   auto compute = [&](int s) {
     op8 a[NP][TM], x[NP][TN];
 #pragma unroll
-    for (int p = 0; p < NP; p++) {
+    for (int p = NP - 1; p >= 0; p--) {
 #pragma unroll
       for (int i = 0; i < TM; i++) a[p][i] = *reinterpret_cast<const op8 *>(&As[s][p][wm * 4 + i][lane * 8]);
 #pragma unroll
-      for (int j = 0; j < TN; j++) x[p][j] = *reinterpret_cast<const op8 *>(&Xs[s][p][wn * 2 + j][lane * 8]);
+      for (int j = 0; j < TN; j++)
+        x[NP - 1 - p][j] = *reinterpret_cast<const op8 *>(&Xs[s][NP - 1 - p][wn * 2 + j][lane * 8]);
     }
+    auto term = [&](int pa, int px) {
 #pragma unroll
-    for (int i = 0; i < TM; i++)
+      for (int i = 0; i < TM; i++)
 #pragma unroll
-      for (int j = 0; j < TN; j++) {
-        f32x4 t = acc[i][j];
-        if constexpr (NP == 3) {
-          t = mfma(a[2][i], x[0][j], t);
-          t = mfma(a[1][i], x[1][j], t);
-          t = mfma(a[0][i], x[2][j], t);
-        }
-        t = mfma(a[1][i], x[0][j], t);
-        t = mfma(a[0][i], x[1][j], t);
-        t = mfma(a[0][i], x[0][j], t);
-        acc[i][j] = t;
-      }
+        for (int j = 0; j < TN; j++) acc[i][j] = mfma(a[pa][i], x[px][j], acc[i][j]);
+    };
+    if constexpr (NP == 3) {
+      term(2, 0);
+      term(1, 1);
+      term(0, 2);
+    }
+    term(1, 0);
+    term(0, 1);
+    term(0, 0);
   };
   const int mrow0 = rb * 128 + wm * 64 + 4 * lg;  // this lane's first output row
   // row unscale factors of block bi (FP16 form), loaded before the block's
@@ -269,16 +271,18 @@ __global__ __launch_bounds__(512, 1) void k_m2l_split(const int M, const int KC,
 // LDS, lane q at byte 16q of a 1 KB fragment) instead of through a
 // register ring and LDS stores.
 //
-// Three LDS stages, each one chunk of A (8 row tiles) and X (4 TN column
-// tiles) in fragment order, are separate arrays addressed with static
-// names, so the compiler's wait insertion can tell the stage being read from
-// the ones being filled and does not drain the DMA queue before the LDS
-// reads.  At chunk c each wave
+// NS LDS stages (3, or 4 for 128-column tiles of two parts), each one chunk
+// of A (8 row tiles) and X (4 TN column tiles) in fragment order, are
+// separate arrays addressed with static names, so the compiler's wait
+// insertion can tell the stage being read from the ones being filled and
+// does not drain the DMA queue before the LDS reads.  At chunk c each wave
 //   waits until its own DMAs of chunk c have landed (at most the LPW of
-//   chunk c+1 still outstanding), meets the others at a barrier (so every
-//   wave's part of chunk c is in LDS and every wave is done with chunk c-1),
-//   issues the DMAs of chunk c+2 into the stage chunk c-1 used, and
-//   multiplies chunk c.
+//   each of chunks c+1 .. c+NS-2 still outstanding), meets the others at a
+//   barrier (so every wave's part of chunk c is in LDS and every wave is
+//   done with chunk c-1), issues the DMAs of chunk c+NS-1 into the stage
+//   chunk c-1 used, and multiplies chunk c.  Four stages keep three chunks
+//   of loads in flight instead of two; their 128 KB leave room for an
+//   exponent table of 192 blocks (class jobs).
 // The barrier is s_barrier alone: __syncthreads would also wait for all
 // outstanding loads and undo the look-ahead.
 //
@@ -306,7 +310,7 @@ __global__ __launch_bounds__(512, 1) void k_m2l_split(const int M, const int KC,
 // across TN: the same MFMAs in the same order per output element, and the
 // unscale exact (a power of two).
 constexpr int kDmaMaxBlocks = 320;
-constexpr int dma_max_blocks(int tn) { return tn == 3 ? 192 : kDmaMaxBlocks; }
+constexpr int dma_max_blocks(int tn, int ns = 3) { return tn == 3 || ns == 4 ? 192 : kDmaMaxBlocks; }
 
 template <int N>
 __device__ __forceinline__ void wait_vmcnt() {
@@ -314,7 +318,7 @@ __device__ __forceinline__ void wait_vmcnt() {
   __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | (7 << 4) | (15 << 8));
 }
 
-template <int NP, int TAG, bool F16, int TN = 2>
+template <int NP, int TAG, bool F16, int TN = 2, int NS = 3>
 __global__ __launch_bounds__(512, 1) void k_m2l_dma(const int M, const int KC, const int RT,
                                                     const uint16_t *__restrict__ X, const int zrow,
                                                     double *__restrict__ C, const int ldc,
@@ -324,12 +328,13 @@ __global__ __launch_bounds__(512, 1) void k_m2l_dma(const int M, const int KC, c
                                                     const int beta) {
   static_assert(!F16 || NP == 2, "FP16 parts: two");
   static_assert(TN == 2 || (TN == 3 && NP == 2), "64 x 48 wave tiles: two parts");
+  static_assert(NS == 3 || (NS == 4 && TN == 2 && NP == 2), "four stages: 128-column tiles, two parts");
   constexpr int BN = 64 * TN, TM = 4, TILE = 512;
   constexpr int XT = 4 * TN;                  // 16-column X tiles of a workgroup tile
   constexpr int LPW = (8 + XT) * NP / 8;      // fragment loads per wave and chunk
   constexpr int NXS = TN == 2 ? 1 : 3;        // distinct X column tiles per wave
-  constexpr int MAXB = TN == 3 ? 192 : 512;   // active blocks of a tile
-  constexpr int UB = dma_max_blocks(TN);
+  constexpr int UB = dma_max_blocks(TN, NS);
+  constexpr int MAXB = UB < 512 ? UB : 512;   // active blocks of a tile
   typedef typename std::conditional<F16, f16x8, bf16x8>::type op8;
   typedef const __attribute__((address_space(1))) void *gvp;
   typedef __attribute__((address_space(3))) void *lvp;
@@ -338,9 +343,11 @@ __global__ __launch_bounds__(512, 1) void k_m2l_dma(const int M, const int KC, c
     uint16_t a[NP][8][TILE];   // [part][row tile][lane * 8 + e]
     uint16_t x[NP][XT][TILE];  // [part][column tile][lane * 8 + e]
   };
+  typedef typename std::conditional<NS == 4, Stage, uint32_t[4]>::type Stage3;
   __shared__ __attribute__((aligned(1024))) Stage St0;
   __shared__ __attribute__((aligned(1024))) Stage St1;
   __shared__ __attribute__((aligned(1024))) Stage St2;
+  __shared__ __attribute__((aligned(1024))) Stage3 St3;
   __shared__ __attribute__((aligned(16))) uint32_t ue[F16 ? UB * 32 : 1];
   __shared__ int act[MAXB];
   __shared__ const uint16_t *aptr[MAXB];
@@ -477,30 +484,34 @@ __global__ __launch_bounds__(512, 1) void k_m2l_dma(const int M, const int KC, c
     if constexpr (F16) return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
     else return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
   };
+  // All fragments of the chunk are read first; the products then run term
+  // by term over all TM x TN tiles, so an accumulator's next MFMA is
+  // TM TN instructions after its previous one instead of depending on the
+  // one just issued.  Each accumulator still takes the terms in the same
+  // order (bit-identical results).
   auto compute = [&](const Stage &Sc) {
     op8 a[NP][TM], x[NP][TN];
 #pragma unroll
-    for (int p = 0; p < NP; p++) {
+    for (int p = NP - 1; p >= 0; p--) {
 #pragma unroll
       for (int i = 0; i < TM; i++) a[p][i] = *reinterpret_cast<const op8 *>(&Sc.a[p][wm * 4 + i][lane * 8]);
 #pragma unroll
-      for (int j = 0; j < TN; j++) x[p][j] = *reinterpret_cast<const op8 *>(&Sc.x[p][wn * TN + j][lane * 8]);
+      for (int j = 0; j < TN; j++) x[NP - 1 - p][j] = *reinterpret_cast<const op8 *>(&Sc.x[NP - 1 - p][wn * TN + j][lane * 8]);
     }
+    auto term = [&](int pa, int px) {
 #pragma unroll
-    for (int i = 0; i < TM; i++)
+      for (int i = 0; i < TM; i++)
 #pragma unroll
-      for (int j = 0; j < TN; j++) {
-        f32x4 t = acc[i][j];
-        if constexpr (NP == 3) {
-          t = mfma(a[2][i], x[0][j], t);
-          t = mfma(a[1][i], x[1][j], t);
-          t = mfma(a[0][i], x[2][j], t);
-        }
-        t = mfma(a[1][i], x[0][j], t);
-        t = mfma(a[0][i], x[1][j], t);
-        t = mfma(a[0][i], x[0][j], t);
-        acc[i][j] = t;
-      }
+        for (int j = 0; j < TN; j++) acc[i][j] = mfma(a[pa][i], x[px][j], acc[i][j]);
+    };
+    if constexpr (NP == 3) {
+      term(2, 0);
+      term(1, 1);
+      term(0, 2);
+    }
+    term(1, 0);
+    term(0, 1);
+    term(0, 0);
   };
   // promotion of block q's FP32 sums (FP16 parts: each row times its
   // unscale factor)
@@ -521,10 +532,12 @@ __global__ __launch_bounds__(512, 1) void k_m2l_dma(const int M, const int KC, c
       }
     }
   };
+  // step cc: chunks cc+1 .. cc+NS-2 may still be in flight; chunk cc+NS-1
+  // goes into the stage chunk cc-1 used
   auto step = [&](int cc, const Stage &Sc, Stage &Sd) {
-    wait_vmcnt<LPW>();
+    wait_vmcnt<(NS - 2) * LPW>();
     __builtin_amdgcn_s_barrier();
-    fetch(cc + 2, Sd);
+    fetch(cc + NS - 1, Sd);
     if (cc < total) {
       compute(Sc);
       if ((cc + 1) % KC == 0) promote(cc / KC);
@@ -533,10 +546,20 @@ __global__ __launch_bounds__(512, 1) void k_m2l_dma(const int M, const int KC, c
   if (total > 0) {
     fetch(0, St0);
     fetch(1, St1);
-    for (int c = 0; c < total; c += 3) {
-      step(c, St0, St2);
-      step(c + 1, St1, St0);
-      step(c + 2, St2, St1);
+    if constexpr (NS == 4) {
+      fetch(2, St2);
+      for (int c = 0; c < total; c += 4) {
+        step(c, St0, St3);
+        step(c + 1, St1, St0);
+        step(c + 2, St2, St1);
+        step(c + 3, St3, St2);
+      }
+    } else {
+      for (int c = 0; c < total; c += 3) {
+        step(c, St0, St2);
+        step(c + 1, St1, St0);
+        step(c + 2, St2, St1);
+      }
     }
     wait_vmcnt<0>();  // no DMA may land after the workgroup ends
   }

@@ -723,17 +723,20 @@ void Fmm::plan_mixed() {
 
 // whether the DMA-staged M2L kernel can run level plan P with split-K
 // factor `split`: depth 4 (128-column tiles) holds kDmaMaxBlocks blocks per
-// split in its FP16 unscale table; depth 5 (192-column tiles, two-part
-// forms, class jobs) holds dma_max_blocks(3) blocks of a tile in LDS
+// split in its FP16 unscale table; depths 5 (192-column tiles) and 6 (four
+// LDS stages) run the class jobs of two-part forms and hold 192 blocks of
+// a tile in LDS
 bool Fmm::dma_ok(const LevelPlan &P, int split, int depth) const {
   if (split < 1) return false;
   if (depth == 5) return P.mx_tn3 && P.mx_maxnb <= dma_max_blocks(3);
+  if (depth == 6) return P.mx_tn3 && P.mx_maxnb <= dma_max_blocks(2, 4);
   return !f16_ || (P.mx_maxnb + split - 1) / split <= kDmaMaxBlocks;
 }
 
 // M2L of level l in the mixed form with register-ring depth `depth` (2 or
 // 3; 4 selects the DMA-staged kernel, 5 the DMA-staged kernel with 64 x 48
-// wave tiles) and split-K factor `split` (the part buffer must hold it)
+// wave tiles, 6 the DMA-staged kernel with four LDS stages) and split-K
+// factor `split` (the part buffer must hold it)
 void Fmm::launch_m2l_mixed(int l, int depth, int split, hipStream_t s) {
   LevelPlan &P = plans_[l];
   if (!P.mx_m2l_jobs) return;
@@ -752,9 +755,10 @@ void Fmm::launch_m2l_mixed(int l, int depth, int split, hipStream_t s) {
   };
   if (depth >= 4 && !dma_ok(P, split, depth))
     throw Error(NNK_ESTATE, "mixed M2L: too many operator blocks for the DMA-staged kernel");
-  if (depth == 5 && np_ != 2) throw Error(NNK_ESTATE, "mixed M2L: 64 x 48 wave tiles need two parts");
+  if (depth >= 5 && np_ != 2) throw Error(NNK_ESTATE, "mixed M2L: this DMA-staged kernel needs two parts");
   if (f16_) {
-    if (depth == 5) launch(k_m2l_dma<2, 1, true, 3>);
+    if (depth == 6) launch(k_m2l_dma<2, 1, true, 2, 4>);
+    else if (depth == 5) launch(k_m2l_dma<2, 1, true, 3>);
     else if (depth == 4) launch(k_m2l_dma<2, 1, true>);
     else if (depth == 2) launch(k_m2l_split<2, 2, 1, true>);
     else launch(k_m2l_split<2, 3, 1, true>);
@@ -762,7 +766,8 @@ void Fmm::launch_m2l_mixed(int l, int depth, int split, hipStream_t s) {
     if (depth == 4) launch(k_m2l_dma<3, 1, false>);
     else launch(k_m2l_split<3, 2, 1, false>);  // a third register stage would spill
   } else {
-    if (depth == 5) launch(k_m2l_dma<2, 1, false, 3>);
+    if (depth == 6) launch(k_m2l_dma<2, 1, false, 2, 4>);
+    else if (depth == 5) launch(k_m2l_dma<2, 1, false, 3>);
     else if (depth == 4) launch(k_m2l_dma<2, 1, false>);
     else if (depth == 2) launch(k_m2l_split<2, 2, 1, false>);
     else launch(k_m2l_split<2, 3, 1, false>);

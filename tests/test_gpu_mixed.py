@@ -121,10 +121,11 @@ def test_mixedh_m2l_kernels_agree_bitwise(nnk, precision, split):
     """The split-precision M2L of every level run by the register-ring
     kernel (depth 2 and, but for the three-part form, 3), by the DMA-staged
     kernel (depth 4) and, for the two-part forms, by the DMA-staged kernel
-    with 64 x 48 wave tiles (depth 5, on the levels run as class jobs;
-    depth 4 elsewhere): the same products in the same order, so the matvecs
-    are bit-identical for one split-K factor (with split-K, depth 5 groups
-    the partial sums differently: equal to rounding).  For the FP16 form, operator
+    with four LDS stages (depth 6) and with 64 x 48 wave tiles (depth 5),
+    both on the levels run as class jobs (depth 4 elsewhere): the same
+    products in the same order, so the matvecs are bit-identical for one
+    split-K factor (with split-K, depth 5's 192-column tiles group the
+    partial sums differently: equal to rounding).  For the FP16 form, operator
     rows that are all zero (unscale 0) and rows scaled by widely different
     powers of two come with the clustered points."""
     rng = np.random.default_rng(44)
@@ -150,17 +151,18 @@ def test_mixedh_m2l_kernels_agree_bitwise(nnk, precision, split):
         assert (chk_s[levels] == split).all() and (chk_d[levels] == depth).all()
         out[depth] = f.computeVel(q)
     if precision != "mixed3":
-        wide = []
-        for l in levels:
-            s = np.zeros(16, np.int32)
-            d = np.zeros(16, np.int32)
-            s[l], d[l] = split, 5
-            if lib.nnk_fmm_set_launch_config(h, s.ctypes.data_as(i32), d.ctypes.data_as(i32), 16) == 0:
-                wide.append(int(l))
-        print(f"[m2l] {precision} split {split}: 64 x 48 wave tiles at levels {wide}", flush=True)
-        assert wide   # the levels with class jobs (the merged-offset levels keep depth 4)
-        out[5] = f.computeVel(q)
-    for depth in depths[1:]:
+        for dk in (6, 5):   # four LDS stages; 64 x 48 wave tiles (class-job levels; others keep depth 4)
+            wide = []
+            for l in levels:
+                s = np.zeros(16, np.int32)
+                d = np.zeros(16, np.int32)
+                s[l], d[l] = split, dk
+                if lib.nnk_fmm_set_launch_config(h, s.ctypes.data_as(i32), d.ctypes.data_as(i32), 16) == 0:
+                    wide.append(int(l))
+            print(f"[m2l] {precision} split {split}: depth {dk} at levels {wide}", flush=True)
+            assert wide
+            out[dk] = f.computeVel(q)
+    for depth in depths[1:] + ((6,) if 6 in out else ()):
         np.testing.assert_array_equal(out[depth], out[2])
     if 5 in out:
         # 192-column tiles split their (larger) active block lists at other
@@ -198,7 +200,7 @@ def test_launch_config_rejects_what_a_form_cannot_run(nnk):
     assert setcfg(f64, l64, 1, 3) != 0          # the FP64 form has no ring depth
     assert setcfg(fb, lb, 1, 3) != 0            # three parts: no third register stage
     assert setcfg(fb, lb, 1, 5) != 0            # three parts: no 64 x 48 wave tiles
-    assert setcfg(fh, lh, 1, 6) != 0
+    assert setcfg(fh, lh, 1, 7) != 0
     assert setcfg(fh, lh, 0, 4) == 0            # split 0: level left as it is
     assert setcfg(fh, lh, 65, 3) != 0
     assert setcfg(fh, 0, 1, 3) != 0             # level without M2L
