@@ -147,7 +147,7 @@ __global__ __launch_bounds__(64 * NW) void k_bp2p(int nitems, const int2 *__rest
     }
     __syncthreads();
     if (lane < nt)
-      for (int j = wv; j < cnt; j += NW) st.acc(j, tx, ty, tz, e2, ux, uy, uz);
+      st.run(wv, NW, cnt, tx, ty, tz, e2, ux, uy, uz);
     __syncthreads();
   }
   red[wv][3 * lane] = ux;
@@ -247,7 +247,7 @@ __global__ __launch_bounds__(256) void k_bp2pT(int nitems, const int2 *__restric
     }
     __syncthreads();
     if (lane < nt)
-      for (int j = wv; j < cnt; j += 4) st.acc(j, tx, ty, tz, e2, ux, uy, uz);
+      st.run(wv, 4, cnt, tx, ty, tz, e2, ux, uy, uz);
     __syncthreads();
   }
   red[wv][3 * lane] = ux;

@@ -155,7 +155,7 @@ __global__ __launch_bounds__(kSurfThreads) void k_surf_eval(SurfArgs a) {
         }
         __syncthreads();
         if (act)
-          for (int j = sub; j < cnt; j += nsub) st.acc(j, tx, ty, tz, a.e2, ux, uy, uz);
+          st.run(sub, nsub, cnt, tx, ty, tz, a.e2, ux, uy, uz);
         __syncthreads();
       }
     }
@@ -245,7 +245,7 @@ __global__ __launch_bounds__(T) void k_leaf(LeafArgs a) {
     double ux = 0, uy = 0, uz = 0;
     auto consume = [&](int cnt) {
       if (act)
-        for (int j = sub; j < cnt; j += nsub) st.acc(j, tx, ty, tz, a.e2, ux, uy, uz);
+        st.run(sub, nsub, cnt, tx, ty, tz, a.e2, ux, uy, uz);
     };
     auto surface = [&](int node, int down, const double *dens) {
       const double *box = a.corners + 6 * (size_t)(node - 1);

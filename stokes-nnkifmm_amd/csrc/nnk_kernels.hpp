@@ -97,10 +97,21 @@ template <int T> struct SrcStage<false, T> {
     stokeslet_acc(tx, ty, tz, p[3 * j], p[3 * j + 1], p[3 * j + 2], q[3 * j], q[3 * j + 1], q[3 * j + 2], e2,
                   ux, uy, uz);
   }
+  // sources j = j0, j0 + step, ... < cnt, four per iteration (the same
+  // sequence of FP64 accumulations as one at a time)
+  __device__ __forceinline__ void run(int j0, int step, int cnt, double tx, double ty, double tz, double e2,
+                                      double &ux, double &uy, double &uz) const {
+    int j = j0;
+    for (; j + 3 * step < cnt; j += 4 * step) {
+#pragma unroll
+      for (int k = 0; k < 4; k++) acc(j + k * step, tx, ty, tz, e2, ux, uy, uz);
+    }
+    for (; j < cnt; j += step) acc(j, tx, ty, tz, e2, ux, uy, uz);
+  }
 };
 template <int T> struct SrcStage<true, T> {
   double p[3 * T];
-  float q[4 * T];
+  __attribute__((aligned(16))) float q[4 * T];
   __device__ __forceinline__ void put(int i, double x, double y, double z, double qx, double qy, double qz) {
     p[3 * i] = x; p[3 * i + 1] = y; p[3 * i + 2] = z;
     q[4 * i] = (float)qx; q[4 * i + 1] = (float)qy; q[4 * i + 2] = (float)qz;
@@ -109,6 +120,36 @@ template <int T> struct SrcStage<true, T> {
                                       double &uy, double &uz) const {
     stokeslet_acc_f32q(tx, ty, tz, p[3 * j], p[3 * j + 1], p[3 * j + 2], q[4 * j], q[4 * j + 1], q[4 * j + 2],
                        e2, ux, uy, uz);
+  }
+  // sources j = j0, j0 + step, ... < cnt, four per iteration: the four
+  // FP32 contributions are added in FP32 (in order) and then once into the
+  // FP64 sums, which saves three conversions and FP64 adds per pair
+  __device__ __forceinline__ void run(int j0, int step, int cnt, double tx, double ty, double tz, double e2,
+                                      double &ux, double &uy, double &uz) const {
+    const float e2f = (float)e2;
+    int j = j0;
+    for (; j + 3 * step < cnt; j += 4 * step) {
+      float gx = 0.f, gy = 0.f, gz = 0.f;
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        const int i = j + k * step;
+        const float4 f = *reinterpret_cast<const float4 *>(&q[4 * i]);
+        const float rx = (float)(tx - p[3 * i]), ry = (float)(ty - p[3 * i + 1]), rz = (float)(tz - p[3 * i + 2]);
+        const float r2 = rx * rx + ry * ry + rz * rz;
+        const float re = r2 + e2f;
+        const float inv = __builtin_amdgcn_rsqf(re);
+        const float inv3 = inv * inv * inv;
+        const float iso = (re + e2f) * inv3;
+        const float rq = (rx * f.x + ry * f.y + rz * f.z) * inv3;
+        gx += iso * f.x + rq * rx;
+        gy += iso * f.y + rq * ry;
+        gz += iso * f.z + rq * rz;
+      }
+      ux += (double)gx;
+      uy += (double)gy;
+      uz += (double)gz;
+    }
+    for (; j < cnt; j += step) acc(j, tx, ty, tz, e2, ux, uy, uz);
   }
 };
 
