@@ -195,9 +195,10 @@ int32_t nnk_fmm_set_graph(nnk_fmm *h, int32_t on);
 
 /* No counterpart in the reference: with on = 1 the first later apply
  *           times the M2L launch candidates of every level on its own data
- *           (split-K factor s/2, s, 2s of the default s; for the mixed forms
- *           also the register-ring depth 2 or 3 and the DMA-staged
- *           kernel, depth 4) and keeps the fastest.  The
+ *           (split-K factor s/2, s, 2s of the default s; for the FP64
+ *           forms also the per-offset pair jobs, depth 1, where planned;
+ *           for the mixed forms also the register-ring depth 2 or 3 and
+ *           the DMA-staged kernels, depths 4-6) and keeps the fastest.  The
  *           choice changes summation order only (results stay within
  *           rounding of each other and deterministic afterwards).
  *           nnk_fmm_launch_config reports the per-level choice (split,
@@ -208,11 +209,17 @@ int32_t nnk_fmm_launch_config(const nnk_fmm *h, int32_t *split, int32_t *depth, 
  *           0..nlev-1 (entries with split 0 are left as they are), e.g. to
  *           replay a configuration that nnk_fmm_launch_config reported.
  *           split: split-K factor 1..64 within the reserved part buffer;
- *           depth: 0 for the FP64 form; 2 or 3 (register-ring depth) for
+ *           depth: 0 for the FP64 forms ("fp64", "fp64t") with their
+ *           class tiles, or 1 for their per-offset pair jobs (split 1;
+ *           levels where the planner built them: sparse adaptive levels,
+ *           whose class tiles would mostly multiply empty columns); 2 or 3
+ *           (register-ring depth) for
  *           the mixed forms, 2 only for "mixed3"; 4 selects the DMA-staged
  *           kernel (any mixed form; for "mixedh" at most 320 operator blocks
- *           per split-K part).  Results are bit-identical across the depths
- *           of one split.  NNK_EARG for a level without M2L or a value the
+ *           per split-K part; 5 and 6 its 64 x 48 wave-tile and four-stage
+ *           variants, class jobs of at most 192 blocks).  Mixed-form results
+ *           are bit-identical across the depths of one split; the FP64
+ *           forms' two layouts agree to rounding.  NNK_EARG for a level without M2L or a value the
  *           form cannot run; a captured graph is recaptured on next use. */
 int32_t nnk_fmm_set_launch_config(nnk_fmm *h, const int32_t *split, const int32_t *depth, int32_t nlev);
 /* No counterpart in the reference: the workgroup size of the leaf kernel

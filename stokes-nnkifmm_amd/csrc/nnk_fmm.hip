@@ -25,6 +25,7 @@
 #include <map>
 #include <mutex>
 #include <tuple>
+#include <type_traits>
 
 #include "nnk_fmm.hpp"
 #include "nnk_gemm.hpp"
@@ -113,9 +114,15 @@ template <bool F32>
 __global__ __launch_bounds__(kSurfThreads) void k_surf_eval(SurfArgs a) {
   constexpr int T = kSurfThreads;
   const int n = a.nodes[blockIdx.x];
-  __shared__ SrcStage<F32, T> st;
+  // FP32 math: positions as FP32 offsets from this node's centre (the
+  // smaller box of an X-list pair; S2M pairs share the box)
+  typedef typename std::conditional<F32, SrcStageRel<T>, SrcStage<false, T>>::type Stage;
+  __shared__ Stage st;
   __shared__ double red[T * 3];
   const double *box = a.corners + 6 * (size_t)(n - 1);
+  double cx, cy, cz;
+  box_centre(box, cx, cy, cz);
+  const float e2f = (float)a.e2;
   const int tid = threadIdx.x;
   int64_t lb = 0, le = 1;
   if (a.lptr) { lb = a.lptr[n]; le = a.lptr[n + 1]; }
@@ -150,12 +157,20 @@ __global__ __launch_bounds__(kSurfThreads) void k_surf_eval(SurfArgs a) {
         const int cnt = min(T, se - s0);
         if (tid < cnt) {
           const size_t s = (size_t)(s0 + tid);
-          st.put(tid, a.src_pos[3 * s], a.src_pos[3 * s + 1], a.src_pos[3 * s + 2], a.src_q[3 * s],
-                 a.src_q[3 * s + 1], a.src_q[3 * s + 2]);
+          if constexpr (F32)
+            st.put(tid, a.src_pos[3 * s] - cx, a.src_pos[3 * s + 1] - cy, a.src_pos[3 * s + 2] - cz,
+                   a.src_q[3 * s], a.src_q[3 * s + 1], a.src_q[3 * s + 2]);
+          else
+            st.put(tid, a.src_pos[3 * s], a.src_pos[3 * s + 1], a.src_pos[3 * s + 2], a.src_q[3 * s],
+                   a.src_q[3 * s + 1], a.src_q[3 * s + 2]);
         }
         __syncthreads();
-        if (act)
-          st.run(sub, nsub, cnt, tx, ty, tz, a.e2, ux, uy, uz);
+        if (act) {
+          if constexpr (F32)
+            st.run(sub, nsub, cnt, (float)(tx - cx), (float)(ty - cy), (float)(tz - cz), e2f, ux, uy, uz);
+          else
+            st.run(sub, nsub, cnt, tx, ty, tz, a.e2, ux, uy, uz);
+        }
         __syncthreads();
       }
     }
@@ -212,7 +227,14 @@ constexpr int kLeafMaxU = 512;
 template <bool F32, int T>
 __global__ __launch_bounds__(T) void k_leaf(LeafArgs a) {
   const int n = a.leaves[blockIdx.x];
-  __shared__ SrcStage<F32, T> st;
+  // FP32 math: the surface sources (L2T, M2T) go through the offset stage
+  // with the origin at the centre of the surface's node (this leaf for
+  // L2T, the smaller W node for M2T); the U-list sources keep the FP64
+  // separation
+  __shared__ union {
+    SrcStage<F32, T> pts;
+    SrcStageRel<T> srf;
+  } st;
   __shared__ double red[T * 3];
   __shared__ int upre[kLeafMaxU + 1];  // source-count prefix over the U list
   const int tid = threadIdx.x;
@@ -245,20 +267,28 @@ __global__ __launch_bounds__(T) void k_leaf(LeafArgs a) {
     double ux = 0, uy = 0, uz = 0;
     auto consume = [&](int cnt) {
       if (act)
-        st.run(sub, nsub, cnt, tx, ty, tz, a.e2, ux, uy, uz);
+        st.pts.run(sub, nsub, cnt, tx, ty, tz, a.e2, ux, uy, uz);
     };
     auto surface = [&](int node, int down, const double *dens) {
       const double *box = a.corners + 6 * (size_t)(node - 1);
+      double cx = 0, cy = 0, cz = 0;
+      if constexpr (F32) box_centre(box, cx, cy, cz);
+      const float txf = (float)(tx - cx), tyf = (float)(ty - cy), tzf = (float)(tz - cz);
       for (int p0 = 0; p0 < a.nq; p0 += T) {
         const int cnt = min(T, a.nq - p0);
         if (tid < cnt) {
           double x, y, z;
           surf_point(box, down, a.surf_idx[p0 + tid], a.n1, x, y, z);
           const double *q = dens + (size_t)(node - 1) * a.K + 3 * (p0 + tid);
-          st.put(tid, x, y, z, q[0], q[1], q[2]);
+          if constexpr (F32) st.srf.put(tid, x - cx, y - cy, z - cz, q[0], q[1], q[2]);
+          else st.pts.put(tid, x, y, z, q[0], q[1], q[2]);
         }
         __syncthreads();
-        consume(cnt);
+        if constexpr (F32) {
+          if (act) st.srf.run(sub, nsub, cnt, txf, tyf, tzf, (float)a.e2, ux, uy, uz);
+        } else {
+          consume(cnt);
+        }
         __syncthreads();
       }
     };
@@ -268,8 +298,8 @@ __global__ __launch_bounds__(T) void k_leaf(LeafArgs a) {
         const int cnt = min(T, se - s0);
         if (tid < cnt) {
           const size_t s = (size_t)(s0 + tid);
-          st.put(tid, a.src_pos[3 * s], a.src_pos[3 * s + 1], a.src_pos[3 * s + 2], a.src_q[3 * s],
-                 a.src_q[3 * s + 1], a.src_q[3 * s + 2]);
+          st.pts.put(tid, a.src_pos[3 * s], a.src_pos[3 * s + 1], a.src_pos[3 * s + 2], a.src_q[3 * s],
+                     a.src_q[3 * s + 1], a.src_q[3 * s + 2]);
         }
         __syncthreads();
         consume(cnt);
@@ -291,8 +321,8 @@ __global__ __launch_bounds__(T) void k_leaf(LeafArgs a) {
             else hi = mid;
           }
           const size_t s = (size_t)(a.src_ptr[a.uidx[u0 + lo]] + (g - upre[lo]));
-          st.put(tid, a.src_pos[3 * s], a.src_pos[3 * s + 1], a.src_pos[3 * s + 2], a.src_q[3 * s],
-                 a.src_q[3 * s + 1], a.src_q[3 * s + 2]);
+          st.pts.put(tid, a.src_pos[3 * s], a.src_pos[3 * s + 1], a.src_pos[3 * s + 2], a.src_q[3 * s],
+                     a.src_q[3 * s + 1], a.src_q[3 * s + 2]);
         }
         __syncthreads();
         consume(cnt);
@@ -416,6 +446,20 @@ __global__ void k_split_reduce(const int32_t *__restrict__ slot, int M, int ldc,
   }
 }
 
+// per-offset M2L: C[:, node[j]] = the sum of the product columns of target
+// j's pairs (Y column p = rows 0..M-1 at p * M), in their listed order
+__global__ void k_pair_sum(const int32_t *__restrict__ node, const int32_t *__restrict__ ptr,
+                           const int32_t *__restrict__ idx, const double *__restrict__ Y, int M,
+                           double *__restrict__ C, int ldc) {
+  const int j = blockIdx.x;
+  const int t = node[j], e0 = ptr[j], e1 = ptr[j + 1];
+  for (int r = threadIdx.x; r < M; r += blockDim.x) {
+    double v = 0.0;
+    for (int e = e0; e < e1; e++) v += Y[(size_t)idx[e] * M + r];
+    C[(size_t)t * ldc + r] = v;
+  }
+}
+
 void tile_lists(const std::vector<int32_t> &gidx, int nb, int N, int BN, std::vector<int32_t> &ptr,
                 std::vector<int32_t> &list) {
   const int nct = (N + BN - 1) / BN;
@@ -493,6 +537,101 @@ void plan_m2l_level(const std::vector<int32_t> *cls, const std::vector<int32_t> 
   if (P.merged) check_gather_job(P.mg, (int)P.msl.size(), Nall, BN, P.mtp, P.mtl, count, "merged M2L job");
 }
 
+void plan_m2l_pairs_host(const std::vector<int32_t> *cls, const std::vector<int32_t> *gv, int64_t count, int BN,
+                         int64_t M, int64_t cap_elems, M2lPairPlan &P) {
+  P = M2lPairPlan{};
+  if (BN <= 0) throw Error(NNK_ESTATE, "per-offset M2L plan: bad tile width");
+  // class tiles: one (column tile, operator block) product per active block
+  // of each BN-column tile
+  std::vector<int> nslot(343, 0);
+  for (int a = 0; a < 8; a++) {
+    if (cls[a].empty()) continue;
+    const std::vector<int> sls = class_offset_slots(a);
+    const int nb = (int)sls.size(), Na = (int)cls[a].size();
+    std::vector<int32_t> tp, tl;
+    tile_lists(gv[a], nb, Na, BN, tp, tl);
+    check_gather_job(gv[a], nb, Na, BN, tp, tl, count, "M2L class job");
+    P.class_work += (int64_t)tl.size() * BN;
+    for (int b = 0; b < nb; b++)
+      for (int c = 0; c < Na; c++)
+        if (gv[a][(size_t)b * Na + c] >= 0) nslot[sls[b]]++;
+  }
+  int64_t tiles = 0;
+  for (int sl = 0; sl < 343; sl++) {
+    P.npairs += nslot[sl];
+    tiles += (nslot[sl] + BN - 1) / BN;
+  }
+  // every pair's product column (M doubles) is written once and read once
+  // by the sums: 16 M bytes against the column's 2 M^2 FP64 flops, at
+  // ~45 TF/s for the products and ~4 TB/s for the round trip, costs
+  // 90 / M of a column's product
+  P.pair_work = tiles * BN + (P.npairs * 90 + M - 1) / std::max<int64_t>(M, 1);
+  P.use = P.npairs > 0 && P.pair_work < P.class_work && P.npairs * M <= cap_elems;
+  if (!P.use) return;
+  // pairs grouped by offset slot (increasing), within a slot by class and
+  // column; each target's list then comes out ordered by slot
+  std::vector<int64_t> base(344, 0);
+  for (int sl = 0; sl < 343; sl++) base[sl + 1] = base[sl] + nslot[sl];
+  std::vector<int64_t> fill(base.begin(), base.end() - 1);
+  P.src.assign(P.npairs, -1);
+  std::vector<int32_t> pair_tgt(P.npairs, -1);
+  for (int sl = 0; sl < 343; sl++) {
+    if (!nslot[sl]) continue;
+    for (int a = 0; a < 8; a++) {
+      if (cls[a].empty()) continue;
+      const std::vector<int> sls = class_offset_slots(a);
+      const int Na = (int)cls[a].size();
+      for (size_t b = 0; b < sls.size(); b++) {
+        if (sls[b] != sl) continue;
+        for (int c = 0; c < Na; c++) {
+          const int32_t g = gv[a][b * Na + c];
+          if (g < 0) continue;
+          const int64_t p = fill[sl]++;
+          P.src[p] = g;
+          pair_tgt[p] = cls[a][c];
+        }
+      }
+    }
+  }
+  // jobs: the BN-pair tiles of each slot in slot order, dealt to the 8
+  // XCDs in contiguous runs (tile k of run x becomes job 8k + x), so that
+  // the tiles of one operator run side by side on one XCD and share its
+  // rows through that L2 (workgroup b of the launch goes to XCD b mod 8,
+  // and the job count is a multiple of 8)
+  std::vector<std::array<int64_t, 3>> tl;  // slot, first pair, pairs
+  for (int sl = 0; sl < 343; sl++)
+    for (int64_t p0 = base[sl]; p0 < base[sl + 1]; p0 += BN)
+      tl.push_back({sl, p0, std::min<int64_t>(BN, base[sl + 1] - p0)});
+  const int64_t T = (int64_t)tl.size();
+  int64_t runmax = 0;
+  for (int x = 0; x < 8; x++) runmax = std::max(runmax, (x + 1) * T / 8 - x * T / 8);
+  const int64_t nj = 8 * runmax;
+  if (nj > INT32_MAX || P.npairs > INT32_MAX) throw Error(NNK_ESTATE, "per-offset M2L plan: too many pairs");
+  P.job_slot.assign(nj, -1);
+  P.job_p0.assign(nj, 0);
+  P.job_n.assign(nj, 0);
+  for (int x = 0; x < 8; x++)
+    for (int64_t k = 0, t = x * T / 8; t < (x + 1) * T / 8; k++, t++) {
+      P.job_slot[8 * k + x] = (int32_t)tl[t][0];
+      P.job_p0[8 * k + x] = (int32_t)tl[t][1];
+      P.job_n[8 * k + x] = (int32_t)tl[t][2];
+    }
+  // per target (the level's class columns, each once): its pairs
+  std::vector<int32_t> pos(count, -1);
+  for (int a = 0; a < 8; a++)
+    for (int32_t t : cls[a]) {
+      if (t < 0 || t >= count || pos[t] >= 0) throw Error(NNK_ESTATE, "per-offset M2L plan: bad target list");
+      pos[t] = (int32_t)P.node.size();
+      P.node.push_back(t);
+    }
+  P.ptr.assign(P.node.size() + 1, 0);
+  for (int64_t p = 0; p < P.npairs; p++) P.ptr[pos[pair_tgt[p]] + 1]++;
+  for (size_t j = 0; j < P.node.size(); j++) P.ptr[j + 1] += P.ptr[j];
+  std::vector<int32_t> at(P.ptr.begin(), P.ptr.end() - 1);
+  P.idx.assign(P.npairs, -1);
+  for (int64_t p = 0; p < P.npairs; p++) P.idx[at[pos[pair_tgt[p]]]++] = (int32_t)p;
+}
+
 int Fmm::split_for(int64_t tiles, int nb) {
   // aim at ~4 workgroups per CU, with at least 4 operator blocks per split
   const int64_t want = 1024;
@@ -517,6 +656,78 @@ void Fmm::reserve_part(size_t elems) {
   if (elems <= part_elems_) return;
   part_.alloc(sizeof(double) * elems);
   part_elems_ = elems;
+}
+
+// pair buffer limit of the per-offset M2L (doubles): 2 GiB
+constexpr int64_t kPairBufMaxElems = (int64_t)1 << 28;
+
+void Fmm::plan_m2l_pairs(LevelPlan &P, const std::vector<int32_t> *cls, const std::vector<int32_t> *gv,
+                         const std::vector<const double *> *ops, int M) {
+  P.m2l_npairs = 0;
+  P.m2l_pair_on = false;
+  P.m2lp = GemmBatch();
+  M2lPairPlan H;
+  plan_m2l_pairs_host(cls, gv, tree_.count, 64, M, kPairBufMaxElems, H);
+  if (!H.use) return;
+  std::vector<const double *> slot_op(343, nullptr);
+  for (int a = 0; a < 8; a++) {
+    if (cls[a].empty()) continue;
+    const std::vector<int> sls = class_offset_slots(a);
+    if (ops[a].size() != sls.size()) throw Error(NNK_ESTATE, "per-offset M2L plan: operator count");
+    for (size_t b = 0; b < sls.size(); b++) slot_op[sls[b]] = ops[a][b];
+  }
+  const int nj = (int)H.job_slot.size();
+  std::vector<const double *> jop(nj, nullptr);
+  for (int j = 0; j < nj; j++)
+    if (H.job_slot[j] >= 0 && !(jop[j] = slot_op[H.job_slot[j]]))
+      throw Error(NNK_ESTATE, "per-offset M2L plan: missing translation operator");
+  std::vector<int32_t> col(H.npairs);
+  for (int64_t p = 0; p < H.npairs; p++) col[p] = (int32_t)p;
+  const std::vector<int32_t> tp = {0, 1}, tl = {0};  // one tile, its one block
+  GemmBatch &B = P.m2lp;
+  B.owned.emplace_back(make_dbuf(jop));
+  const double *const *dop = B.owned.back().as<const double *>();
+  B.owned.emplace_back(make_dbuf(H.src));
+  const int32_t *dsrc = B.owned.back().as<int32_t>();
+  B.owned.emplace_back(make_dbuf(col));
+  const int32_t *dcol = B.owned.back().as<int32_t>();
+  B.owned.emplace_back(make_dbuf(tp));
+  const int32_t *dtp = B.owned.back().as<int32_t>();
+  B.owned.emplace_back(make_dbuf(tl));
+  const int32_t *dtl = B.owned.back().as<int32_t>();
+  std::vector<GemmJob> jobs(nj);
+  for (int j = 0; j < nj; j++) {
+    GemmJob &J = jobs[j];
+    J.A = dop + j;
+    J.gidx = dsrc + H.job_p0[j];  // empty jobs (N = 0) return before any read
+    J.cidx = dcol + H.job_p0[j];
+    J.nb = 1;
+    J.N = H.job_n[j];
+    J.tptr = dtp;
+    J.tlist = dtl;
+    B.maxN = std::max(B.maxN, J.N);
+  }
+  B.njobs = nj;
+  B.jobs.upload(jobs);
+  P.m2lp_nodes = (int)H.node.size();
+  P.m2lp_node.upload(H.node);
+  P.m2lp_ptr.upload(H.ptr);
+  P.m2lp_idx.upload(H.idx);
+  if ((size_t)H.npairs * M > pairbuf_elems_) {
+    pairbuf_.alloc(sizeof(double) * (size_t)H.npairs * M);
+    pairbuf_elems_ = (size_t)H.npairs * M;
+  }
+  P.m2l_npairs = (int)H.npairs;
+  P.m2l_pair_on = true;
+}
+
+void Fmm::launch_m2l_pairs(const LevelPlan &P, const double *X, int ldx, double *C, int ldc, int M, int Mpad,
+                           int Kpad, hipStream_t s) {
+  if (!P.m2l_npairs) throw Error(NNK_ESTATE, "per-offset M2L: no plan for this level");
+  gemm_dims(P.m2lp, X, ldx, pairbuf_.as<double>(), M, 0, s, M, M, Mpad, Kpad, 1, nullptr, 1);
+  hipLaunchKernelGGL(k_pair_sum, dim3((unsigned)P.m2lp_nodes), dim3(256), 0, s, P.m2lp_node.as<int32_t>(),
+                     P.m2lp_ptr.as<int32_t>(), P.m2lp_idx.as<int32_t>(), pairbuf_.as<double>(), M, C, ldc);
+  NNK_HIP(hipGetLastError());
 }
 
 void Fmm::split_reduce(const DBuf &slot, int ncols, int M, int ldc, int nsplit, double *C, hipStream_t s) {
@@ -1072,12 +1283,13 @@ void Fmm::plan() {
     maxn = std::max(maxn, dn.size());
     if (P.n_down) {
       std::vector<GemmJob> jobs, ljobs;
+      std::vector<const double *> cops[8];  // per class: its operators in block order
       const size_t mat = (size_t)Kpad_ * Mpad_;
       for (int a = 0; a < 8; a++) {
         if (cls[a].empty()) continue;
         auto offs = class_offsets(a);
         std::map<int, int> pos;
-        std::vector<const double *> A;
+        std::vector<const double *> &A = cops[a];
         for (size_t b = 0; b < offs.size(); b++) {
           pos[offset_slot(offs[b][0], offs[b][1], offs[b][2])] = (int)b;
           A.push_back(O.tslot[offset_slot(offs[b][0], offs[b][1], offs[b][2])]);
@@ -1134,6 +1346,7 @@ void Fmm::plan() {
         P.m2l_slot.upload(slot);  // also for the autotuner's split candidates
         reserve_part((size_t)std::min(64, 2 * P.m2l_split) * nj * mN * K_);
       }
+      if (prec_ == 0) plan_m2l_pairs(P, hl_[l].cls, hl_[l].gv, cops, K_);
       std::vector<GemmJob> j1, j2;
       add_job(P.dn_solve1, j1, {O.Wdn.as<double>()}, &dn, nullptr, P.n_down);
       finish(P.dn_solve1, j1);
@@ -1293,7 +1506,8 @@ void Fmm::launch_config(int level, int *split, int *depth) const {
   if (m16()) {
     if (P.mx_m2l_jobs) { *split = P.mx_split; *depth = P.mx_depth; }
   } else if (P.n_down) {
-    *split = P.m2l_split;
+    *split = P.m2l_pair_on ? 1 : P.m2l_split;
+    *depth = P.m2l_pair_on ? 1 : 0;
   }
 }
 
@@ -1308,8 +1522,16 @@ void Fmm::set_launch_config(int level, int split, int depth) {
                                                 !(mixed ? P.mx_slot.p : P.m2l_slot.p))))
     throw Error(NNK_EARG, "launch config: split-K factor out of range");
   if (!mixed) {
-    if (depth != 0) throw Error(NNK_EARG, "launch config: the FP64 form takes depth 0");
-    P.m2l_split = split;
+    if (depth == 1) {
+      if (!P.m2l_npairs) throw Error(NNK_EARG, "launch config: no per-offset M2L plan at this level");
+      if (split != 1) throw Error(NNK_EARG, "launch config: the per-offset M2L takes split 1");
+      P.m2l_pair_on = true;
+    } else if (depth == 0) {
+      P.m2l_split = split;
+      P.m2l_pair_on = false;
+    } else {
+      throw Error(NNK_EARG, "launch config: the FP64 forms take depth 0 or 1");
+    }
   } else {
     const bool ok = depth == 2 || (depth == 3 && !(np_ == 3 && !f16_)) ||
                     (depth >= 4 && depth <= 6 && dma_ok(P, split, depth));
@@ -1337,16 +1559,22 @@ void Fmm::autotune(hipStream_t s) {
       std::vector<int> splits = {s0};
       if (s0 > 1) splits.push_back(std::max(1, s0 / 2));
       if ((size_t)std::min(64, 2 * s0) * per_split <= part_elems_) splits.push_back(std::min(64, 2 * s0));
-      std::vector<int> depths = !mixed ? std::vector<int>{0} : np_ == 3 && !f16_ ? std::vector<int>{2, 4}
-                                                                                  : std::vector<int>{2, 3, 4, 5, 6};
+      // FP64 forms: depth 0 = class tiles, 1 = per-offset pair jobs (split 1)
+      std::vector<int> depths = !mixed ? (P.m2l_npairs ? std::vector<int>{0, 1} : std::vector<int>{0})
+                                : np_ == 3 && !f16_ ? std::vector<int>{2, 4}
+                                                    : std::vector<int>{2, 3, 4, 5, 6};
       float best = 1e30f;
-      int bs = s0, bd = mixed ? P.mx_depth : 0;
+      int bs = s0, bd = mixed ? P.mx_depth : P.m2l_pair_on ? 1 : 0;
       for (int sp : splits) {
         if (sp > 1 && !(mixed ? P.mx_slot.p : P.m2l_slot.p)) continue;
         for (int dp : depths) {
           if (dp >= 4 && !dma_ok(P, sp, dp)) continue;
+          if (!mixed && dp == 1 && sp != s0) continue;  // one pair-job candidate
           auto run = [&]() {
             if (mixed) launch_m2l_mixed(l, dp, sp, s);
+            else if (dp == 1 && prec_ == 4)
+              launch_m2l_pairs(P, chat_.as<double>(), RP_, dhat_.as<double>(), RP_, R_, MpadR_, KpadR_, s);
+            else if (dp == 1) launch_m2l_pairs(P, uppot_.as<double>(), K_, chk_.as<double>(), K_, K_, Mpad_, Kpad_, s);
             else if (prec_ == 4) launch_m2l_t(l, sp, s);
             else gemm(P.m2l, uppot_.as<double>(), K_, chk_.as<double>(), K_, 0, s, 1, sp, &P.m2l_slot);
           };
@@ -1362,7 +1590,8 @@ void Fmm::autotune(hipStream_t s) {
         }
       }
       if (mixed) { P.mx_split = bs; P.mx_depth = bd; }
-      else P.m2l_split = bs;
+      else if (bd == 1) P.m2l_pair_on = true;
+      else { P.m2l_split = bs; P.m2l_pair_on = false; }
     }
   } catch (...) {
     prof_ = prof;
@@ -1437,7 +1666,8 @@ void Fmm::apply_impl(const double *d_f, double *d_u, hipStream_t s) {
     LevelPlan &P = plans_[l];
     if (!P.n_down) continue;
     mark(3, true, s);
-    gemm(P.m2l, up, K_, chk, K_, 0, s, 1, P.m2l_split, &P.m2l_slot);
+    if (P.m2l_pair_on) launch_m2l_pairs(P, up, K_, chk, K_, K_, Mpad_, Kpad_, s);
+    else gemm(P.m2l, up, K_, chk, K_, 0, s, 1, P.m2l_split, &P.m2l_slot);
     mark(3, false, s);
     mark(4, true, s);
     gemm(P.l2l, dn, K_, chk, K_, 1, s);

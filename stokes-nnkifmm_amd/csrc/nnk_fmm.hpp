@@ -63,6 +63,27 @@ struct M2lLevelPlan {
 void plan_m2l_level(const std::vector<int32_t> *cls, const std::vector<int32_t> *gv, int64_t count, int BN,
                     M2lLevelPlan &out);
 
+// The per-offset M2L plan of one level for the FP64 forms (sparse adaptive
+// levels): the V-list (target, source) pairs grouped by offset slot, as
+// jobs of at most BN pairs of one slot, each product into its own column
+// of a pair buffer, then a fixed-order sum per target.  Chosen (use) where
+// its padded work, with the pair buffer's round trip, is below that of the
+// class tiles (BN-column tiles times their active blocks) and the buffer
+// (npairs x M doubles) fits cap_elems.  Jobs are laid out for the 8 XCDs:
+// the slot-ordered tiles are split into 8 contiguous runs and tile k of
+// run x is job 8k + x (empty jobs: n = 0, slot = -1).  node/ptr/idx: per
+// target its pairs, ordered by slot.  Tables are checked as for the class
+// jobs (count = node count).
+struct M2lPairPlan {
+  bool use = false;
+  int64_t class_work = 0, pair_work = 0, npairs = 0;
+  std::vector<int32_t> job_slot, job_p0, job_n;
+  std::vector<int32_t> src;             // per pair: source slot
+  std::vector<int32_t> node, ptr, idx;
+};
+void plan_m2l_pairs_host(const std::vector<int32_t> *cls, const std::vector<int32_t> *gv, int64_t count, int BN,
+                         int64_t M, int64_t cap_elems, M2lPairPlan &out);
+
 struct GemmBatch {
   DBuf jobs;                 // GemmJob[n]
   std::vector<DBuf> owned;   // index/pointer arrays referenced by the jobs
@@ -85,6 +106,16 @@ struct LevelPlan {
   int m2l_split = 1;          // split-K factor of the M2L launch (both forms)
   DBuf m2l_slot;              // job * maxN + column -> output slot (-1 none), for the split reduction
   GemmBatch l2l;              // 8 classes, parent down density (accumulates)
+  // Either FP64 form, where the class tiles would mostly multiply empty
+  // columns (sparse adaptive levels): the M2L as jobs of at most 64 V-list
+  // pairs of one offset, each product into its own column of the pair
+  // buffer, then a fixed-order sum per target (by offset slot).  Built
+  // where the planner's work estimate favours it; m2l_pair_on selects it.
+  int m2l_npairs = 0;         // pairs of the per-offset plan (0: none built)
+  bool m2l_pair_on = false;
+  GemmBatch m2lp;             // one operator, <= 64 pairs per job
+  int m2lp_nodes = 0;         // targets of the per-target sums (the level's down nodes)
+  DBuf m2lp_node, m2lp_ptr, m2lp_idx;  // target slot; its pairs [ptr[j], ptr[j+1]) of idx
   GemmBatch dn_solve1, dn_solve2;
   int n_x = 0;
   DBuf x_nodes;               // nodes with a non-empty X list
@@ -270,6 +301,15 @@ class Fmm {
   int m2l_ld() const { return prec_ >= 1 ? RP_ : K_; }
   // form 4: the M2L of level l, FP64 gather GEMM over the R x R operators
   void launch_m2l_t(int l, int split, hipStream_t s);
+  // Either FP64 form: the per-offset pair plan of level P (cls[a]: output
+  // slots of child class a, gv[a]: its nb x N gather table, ops[a]: its nb
+  // operators in class block order, M: product rows), built and selected
+  // where its padded work plus the pair buffer's round trip is below the
+  // class tiles'; the launch of the plan
+  void plan_m2l_pairs(LevelPlan &P, const std::vector<int32_t> *cls, const std::vector<int32_t> *gv,
+                      const std::vector<const double *> *ops, int M);
+  void launch_m2l_pairs(const LevelPlan &P, const double *X, int ldx, double *C, int ldc, int M, int Mpad,
+                        int Kpad, hipStream_t s);
   void build_mixed();
   void plan_mixed();
   void apply_mixed(const double *d_f, double *d_u, hipStream_t s);
@@ -325,6 +365,8 @@ class Fmm {
                double *xun, hipStream_t s);
   DBuf part_;                       // split-K partial tiles
   size_t part_elems_ = 0;
+  DBuf pairbuf_;                    // per-offset M2L: one product column per V-list pair
+  size_t pairbuf_elems_ = 0;
   int n_upall_ = 0;
   void reduced_factor();
   bool red_ready_ = false;

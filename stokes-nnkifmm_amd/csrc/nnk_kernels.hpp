@@ -153,6 +153,68 @@ template <int T> struct SrcStage<true, T> {
   }
 };
 
+// FP32-math stage for the evaluations that involve a surface (S2M, P2L,
+// L2T, M2T): positions are FP32 offsets from an origin o, formed in FP64
+// and rounded once (at staging for the sources, per target for the
+// targets).  The separation t - s is then one FP32 subtraction.  Its
+// rounding error is at most 2^-24 (|t - o| + |s - o|) per component.
+// The callers put o at the centre of the smaller of the two boxes (width
+// w).  The surface geometry keeps every such pair at least w apart and the
+// point nearer o within 1.5 sqrt(3) w of it, so the error stays within a
+// few FP32 ulps of |r|, the size of the kernel's own FP32 rounding.  (P2P
+// pairs can be arbitrarily close and keep the FP64 separation of
+// SrcStage.)
+template <int T> struct SrcStageRel {
+  float4 p[T];  // x, y, z offset from the origin; w unused
+  float4 q[T];  // strength; w unused
+  __device__ __forceinline__ void put(int i, double dx, double dy, double dz, double qx, double qy,
+                                      double qz) {
+    p[i] = float4{(float)dx, (float)dy, (float)dz, 0.f};
+    q[i] = float4{(float)qx, (float)qy, (float)qz, 0.f};
+  }
+  __device__ __forceinline__ static void pair(const float4 &s, const float4 &f, float tx, float ty, float tz,
+                                              float e2f, float &gx, float &gy, float &gz) {
+    const float rx = tx - s.x, ry = ty - s.y, rz = tz - s.z;
+    const float r2 = rx * rx + ry * ry + rz * rz;
+    const float re = r2 + e2f;
+    const float inv = __builtin_amdgcn_rsqf(re);
+    const float inv3 = inv * inv * inv;
+    const float iso = (re + e2f) * inv3;
+    const float rq = (rx * f.x + ry * f.y + rz * f.z) * inv3;
+    gx = __builtin_fmaf(rq, rx, __builtin_fmaf(iso, f.x, gx));
+    gy = __builtin_fmaf(rq, ry, __builtin_fmaf(iso, f.y, gy));
+    gz = __builtin_fmaf(rq, rz, __builtin_fmaf(iso, f.z, gz));
+  }
+  // sources j = j0, j0 + step, ... < cnt onto the target offset (tx, ty,
+  // tz): each group of four is summed in FP32, then added to the FP64 sums
+  __device__ __forceinline__ void run(int j0, int step, int cnt, float tx, float ty, float tz, float e2f,
+                                      double &ux, double &uy, double &uz) const {
+    int j = j0;
+    for (; j + 3 * step < cnt; j += 4 * step) {
+      float gx = 0.f, gy = 0.f, gz = 0.f;
+#pragma unroll
+      for (int k = 0; k < 4; k++) pair(p[j + k * step], q[j + k * step], tx, ty, tz, e2f, gx, gy, gz);
+      ux += (double)gx;
+      uy += (double)gy;
+      uz += (double)gz;
+    }
+    for (; j < cnt; j += step) {
+      float gx = 0.f, gy = 0.f, gz = 0.f;
+      pair(p[j], q[j], tx, ty, tz, e2f, gx, gy, gz);
+      ux += (double)gx;
+      uy += (double)gy;
+      uz += (double)gz;
+    }
+  }
+};
+
+// centre of box [lo0 lo1 lo2 hi0 hi1 hi2]
+__device__ __forceinline__ void box_centre(const double *__restrict__ box, double &cx, double &cy, double &cz) {
+  cx = 0.5 * (box[0] + box[3]);
+  cy = 0.5 * (box[1] + box[4]);
+  cz = 0.5 * (box[2] + box[5]);
+}
+
 template <bool F32>
 __device__ __forceinline__ void stk(double tx, double ty, double tz, double sx, double sy, double sz,
                                     double qx, double qy, double qz, double e2, double &ux, double &uy,

@@ -637,10 +637,11 @@ void Fmm::plan_mixed() {
     if (prec_ == 4 && P.n_down) {  // FP64 R x R M2L jobs, one per child class, as in the FP64 form
       P.m2l = GemmBatch();
       std::vector<GemmJob> jobs;
+      std::vector<const double *> cops[8];  // per class: its operators in block order
       int minnb = 512;
       for (int a = 0; a < 8; a++) {
         if (H.cls[a].empty()) continue;
-        std::vector<const double *> ops;
+        std::vector<const double *> &ops = cops[a];
         for (int sl : class_offset_slots(a)) {
           if (!X.dop[sl]) throw Error(NNK_ESTATE, "FP64 truncated M2L: missing translation operator");
           ops.push_back(X.dop[sl]);
@@ -676,6 +677,7 @@ void Fmm::plan_mixed() {
       const int nj = P.m2l.njobs, mN = P.m2l.maxN;
       P.m2l_split = split_for((int64_t)nj * (MpadR_ / GBM) * ((mN + 63) / 64), minnb);
       reserve_part((size_t)std::min(64, 2 * P.m2l_split) * nj * mN * RP_);
+      plan_m2l_pairs(P, H.cls, H.gv, cops, R_);
     }
     if (f16_) {  // split-FP16 down expansion
       auto hjob = [&](std::vector<GemmJobBf16> &jobs, int &maxN, const std::vector<const uint16_t *> &ops,
@@ -852,7 +854,8 @@ void Fmm::apply_mixed(const double *d_f, double *d_u, hipStream_t s) {
       mark(3, false, s);
     } else if (prec_ == 4 && P.m2l.njobs) {
       mark(3, true, s);
-      launch_m2l_t(l, P.m2l_split, s);
+      if (P.m2l_pair_on) launch_m2l_pairs(P, chat_.as<double>(), RP_, dhat_.as<double>(), RP_, R_, MpadR_, KpadR_, s);
+      else launch_m2l_t(l, P.m2l_split, s);
       mark(3, false, s);
     }
     mark(4, true, s);

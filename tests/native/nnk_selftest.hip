@@ -174,6 +174,55 @@ extern "C" int nnk_selftest_m2l_plan(const int32_t *sizes, const int32_t *gv, in
   }
 }
 
+// Host only: the per-offset M2L plan of one level (FP64 forms).  sizes[8]
+// and gv as for nnk_selftest_m2l_plan, but class a's column c is the node
+// slot base_a + c (base_a = sizes[0] + ... + sizes[a-1], count >= their
+// sum); M and cap: product rows and pair-buffer limit in doubles.
+// info[0..4] = use, class work, pair work, pairs, jobs.  With use and
+// room (maxjobs, maxpairs, maxnodes) the tables are copied out: job_slot,
+// job_p0, job_n (jobs); src, idx (pairs); node, ptr (nodes, nodes + 1).
+// Returns 0, 2 if the tables did not fit, or 1 with the refusal in err.
+extern "C" int nnk_selftest_m2l_pairs(const int32_t *sizes, const int32_t *gv, int64_t count, int BN, int64_t M,
+                                      int64_t cap, int64_t *info, int32_t *job_slot, int32_t *job_p0,
+                                      int32_t *job_n, int64_t maxjobs, int32_t *src, int32_t *idx,
+                                      int64_t maxpairs, int32_t *node, int32_t *ptr, int64_t maxnodes, char *err,
+                                      int errlen) {
+  try {
+    std::vector<int32_t> cls[8], g[8];
+    size_t off = 0;
+    int32_t base = 0;
+    for (int a = 0; a < 8; a++) {
+      for (int c = 0; c < sizes[a]; c++) cls[a].push_back(base + c);
+      base += sizes[a];
+      const size_t n = (size_t)nnk::class_offset_slots(a).size() * sizes[a];
+      g[a].assign(gv + off, gv + off + n);
+      off += n;
+    }
+    nnk::M2lPairPlan P;
+    nnk::plan_m2l_pairs_host(cls, g, count, BN, M, cap, P);
+    info[0] = P.use ? 1 : 0;
+    info[1] = P.class_work;
+    info[2] = P.pair_work;
+    info[3] = P.npairs;
+    info[4] = (int64_t)P.job_slot.size();
+    copy_err("", err, errlen);
+    if (!P.use) return 0;
+    if ((int64_t)P.job_slot.size() > maxjobs || P.npairs > maxpairs || (int64_t)P.node.size() > maxnodes)
+      return 2;
+    std::copy(P.job_slot.begin(), P.job_slot.end(), job_slot);
+    std::copy(P.job_p0.begin(), P.job_p0.end(), job_p0);
+    std::copy(P.job_n.begin(), P.job_n.end(), job_n);
+    std::copy(P.src.begin(), P.src.end(), src);
+    std::copy(P.idx.begin(), P.idx.end(), idx);
+    std::copy(P.node.begin(), P.node.end(), node);
+    std::copy(P.ptr.begin(), P.ptr.end(), ptr);
+    return 0;
+  } catch (const std::exception &e) {
+    copy_err(e.what(), err, errlen);
+    return 1;
+  }
+}
+
 namespace nnk {
 int svd_jacobi(const double *A, int n, double *U, double *s, double *V);
 }

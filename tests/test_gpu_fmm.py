@@ -65,6 +65,47 @@ def test_matvec_matches_oracle(oracle, nnk, name, res):
     assert rel(u, uo) < TOL, rel(u, uo)
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp64t"])
+@pytest.mark.parametrize("name", ["blobs", "shell"])
+def test_pair_layout_matches_class_tiles(oracle, nnk, name, precision):
+    """The FP64 forms' per-offset M2L (the V-list pairs of one offset per
+    job, each product into its own column, then a fixed-order sum per
+    target), which the planner picks on sparse adaptive levels, against the
+    class tiles on the same levels: equal to rounding, deterministic, and
+    (FP64 form) equal to the oracle within the FP64 tolerance."""
+    pot, tgt, cap, eps, mu = _case(name)
+    t = nnk.OcTree(pot, tgt, "nodeCapacity", cap)
+    q = np.random.default_rng(2).standard_normal((int(t.potentials.sum()), 3))
+    f = nnk.KIFMM(t, [eps, mu], "coronaRes", 6, "precision", precision)
+    lib, h = nnk.lib(), f._handle().h
+    i32 = ctypes.POINTER(ctypes.c_int32)
+    s0 = np.zeros(16, np.int32)
+    d0 = np.zeros(16, np.int32)
+    assert lib.nnk_fmm_launch_config(h, s0.ctypes.data_as(i32), d0.ctypes.data_as(i32), 16) == 0
+    pl = np.flatnonzero(d0 == 1)
+    print(f"[pairs] {name} {precision}: per-offset levels {pl.tolist()} of {np.flatnonzero(s0).tolist()}",
+          flush=True)
+    assert len(pl) >= 1 and (s0[pl] == 1).all()
+    u1 = f.computeVel(q)
+    np.testing.assert_array_equal(f.computeVel(q), u1)
+
+    def setcfg(split, depth):
+        s = np.zeros(16, np.int32)
+        d = np.zeros(16, np.int32)
+        s[pl], d[pl] = split, depth
+        return lib.nnk_fmm_set_launch_config(h, s.ctypes.data_as(i32), d.ctypes.data_as(i32), 16)
+
+    assert setcfg(2, 1) != 0          # the per-offset jobs take no split-K
+    assert setcfg(1, 0) == 0          # class tiles on the same levels
+    u0 = f.computeVel(q)
+    assert rel(u1, u0) < 1e-12, rel(u1, u0)   # summation order only, as for the autotuner's choices
+    assert setcfg(1, 1) == 0
+    np.testing.assert_array_equal(f.computeVel(q), u1)
+    if precision == "fp64":
+        uo = oracle.Tree(pot, tgt, nodeCapacity=cap).matvec(q, eps, mu, 6, 1)
+        assert rel(u1, uo) < TOL, rel(u1, uo)
+
+
 def test_golden_fixtures(nnk):
     for name in ("adaptive_res4.npz", "distinct_res6.npz"):
         g = dict(np.load(os.path.join(GOLD, name)))

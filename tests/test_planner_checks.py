@@ -7,7 +7,12 @@
     chosen only where it walks fewer (tile, block) pairs, its table is
     built only for levels of at most 8 x 128 nodes, and a plan whose gather
     table holds an index outside [-1, nodeCount) is refused before any
-    launch.
+    launch;
+  * the FP64 forms' per-offset M2L plan (sparse adaptive levels): chosen
+    only where its padded work is below the class tiles', every V-list
+    pair exactly once, jobs of at most 64 pairs of one offset laid out as
+    contiguous runs per XCD (job 8k + x), per-target lists ordered by
+    offset slot, out-of-range gathers refused.
 """
 import ctypes
 import os
@@ -26,6 +31,9 @@ def st(nnk):
                                              ctypes.c_char_p, ctypes.c_int]
     lib.nnk_selftest_m2l_plan.argtypes = [P(ctypes.c_int32), P(ctypes.c_int32), ctypes.c_int64, ctypes.c_int,
                                           P(ctypes.c_int32), P(ctypes.c_int64), ctypes.c_char_p, ctypes.c_int]
+    i32, i64 = P(ctypes.c_int32), ctypes.c_int64
+    lib.nnk_selftest_m2l_pairs.argtypes = [i32, i32, i64, ctypes.c_int, i64, i64, P(ctypes.c_int64), i32, i32, i32,
+                                           i64, i32, i32, i64, i32, i32, i64, ctypes.c_char_p, ctypes.c_int]
     return lib
 
 
@@ -97,4 +105,104 @@ def test_m2l_plan_refuses_out_of_range_gathers(st, bad):
     rc, _, _, msg = _plan(st, 8, 73, rng, corrupt=bad)
     assert rc == 1 and "gather index out of range" in msg, msg
     rc, _, _, msg = _plan(st, 200, 5000, rng, corrupt=5000 if bad == 73 else bad)
+    assert rc == 1 and "gather index out of range" in msg, msg
+
+
+def _class_slots(a):
+    """The V-list offsets of a child of class a (bits x, y, z) as slots of
+    the 7^3 offset table, in block order: every offset within [-2 - g,
+    3 - g] per axis (g the class bit) that is not adjacent (max |o| >= 2)."""
+    g = ((a >> 2) & 1, (a >> 1) & 1, a & 1)
+    return [(ox + 3) * 49 + (oy + 3) * 7 + (oz + 3)
+            for ox in range(-2 - g[0], 4 - g[0]) for oy in range(-2 - g[1], 4 - g[1])
+            for oz in range(-2 - g[2], 4 - g[2]) if max(abs(ox), abs(oy), abs(oz)) >= 2]
+
+
+def _pairs(st, sizes, gv, count, M=456, cap=1 << 28):
+    sizes = np.asarray(sizes, np.int32)
+    info = np.zeros(5, np.int64)
+    nj, npair, nn = 8 * 4096, max(1, int((gv >= 0).sum())), int(sizes.sum())
+    js, jp, jn = (np.zeros(nj, np.int32) for _ in range(3))
+    src, idx = np.zeros(npair, np.int32), np.zeros(npair, np.int32)
+    node, ptr = np.zeros(nn, np.int32), np.zeros(nn + 1, np.int32)
+    err = ctypes.create_string_buffer(256)
+    i32 = ctypes.c_int32
+    rc = st.nnk_selftest_m2l_pairs(_p(sizes, i32), _p(gv, i32), count, 64, M, cap, _p(info, ctypes.c_int64),
+                                   _p(js, i32), _p(jp, i32), _p(jn, i32), nj, _p(src, i32), _p(idx, i32), npair,
+                                   _p(node, i32), _p(ptr, i32), nn, err, 256)
+    return rc, info, (js, jp, jn, src, idx, node, ptr), err.value.decode()
+
+
+def _sparse_level(rng, sizes, fill, count):
+    return np.concatenate([np.where(rng.random(189 * n) < fill, rng.integers(0, count, 189 * n), -1)
+                           for n in sizes]).astype(np.int32)
+
+
+def test_m2l_pair_plan_tables(st):
+    rng = np.random.default_rng(3)
+    sizes = [40, 7, 90, 0, 33, 64, 5, 120]
+    count = 2000
+    gv = _sparse_level(rng, sizes, 0.08, count)
+    rc, info, (js, jp, jn, src, idx, node, ptr), msg = _pairs(st, sizes, gv, count)
+    assert rc == 0, msg
+    use, cw, pw, npairs, nj = info.tolist()
+    assert use == 1 and pw < cw and npairs == int((gv >= 0).sum()) and nj % 8 == 0
+    # expected pairs: (slot, target, source) from the gather tables
+    want, off, base = [], 0, 0
+    for a, n in enumerate(sizes):
+        sl = _class_slots(a)
+        g = gv[off:off + 189 * n].reshape(189, n)
+        for b in range(189):
+            for c in range(n):
+                if g[b, c] >= 0:
+                    want.append((sl[b], base + c, int(g[b, c])))
+        off += 189 * n
+        base += n
+    # jobs: at most 64 pairs of one slot; tiles in slot order, contiguous
+    # runs per XCD (job 8k + x), and together they cover pairs 0..npairs-1
+    js, jp, jn = js[:nj], jp[:nj], jn[:nj]
+    assert (jn <= 64).all() and ((jn == 0) == (js == -1)).all()
+    seq = [(int(js[8 * k + x]), int(jp[8 * k + x]), int(jn[8 * k + x]))
+           for x in range(8) for k in range(nj // 8) if jn[8 * k + x] > 0]
+    assert [t[1] for t in seq] == sorted(t[1] for t in seq)
+    covered = np.concatenate([np.arange(p0, p0 + n) for _, p0, n in seq])
+    assert np.array_equal(covered, np.arange(npairs))
+    assert [t[0] for t in seq] == sorted(t[0] for t in seq)
+    runs = [sum(1 for k in range(nj // 8) if jn[8 * k + x] > 0) for x in range(8)]
+    assert max(runs) - min(runs) <= 1
+    pair_slot = np.empty(npairs, np.int64)
+    for sl, p0, n in seq:
+        pair_slot[p0:p0 + n] = sl
+    # per target: its pairs, ordered by slot; together every pair once
+    assert sorted(node.tolist()) == list(range(sum(sizes)))
+    got = []
+    for j, t in enumerate(node.tolist()):
+        ps = idx[ptr[j]:ptr[j + 1]]
+        assert list(pair_slot[ps]) == sorted(pair_slot[ps])
+        got += [(int(pair_slot[p]), t, int(src[p])) for p in ps]
+    assert sorted(got) == sorted(want)
+    assert np.array_equal(np.sort(idx), np.arange(npairs))
+
+
+def test_m2l_pair_plan_choice(st):
+    rng = np.random.default_rng(4)
+    # a full uniform level: the class tiles walk no empty column
+    sizes = [64] * 8
+    gv = rng.integers(0, 600, 189 * 64 * 8).astype(np.int32)
+    rc, info, _, msg = _pairs(st, sizes, gv, 600)
+    assert rc == 0 and info[0] == 0 and info[1] == 8 * 189 * 64, msg
+    # sparse, but the pair buffer would pass its limit: class tiles
+    gv = _sparse_level(rng, sizes, 0.05, 600)
+    rc, info, _, msg = _pairs(st, sizes, gv, 600, M=456, cap=456 * 100)
+    assert rc == 0 and info[0] == 0 and info[3] > 100, msg
+    rc, info, _, msg = _pairs(st, sizes, gv, 600)
+    assert rc == 0 and info[0] == 1, msg
+
+
+@pytest.mark.parametrize("bad", [-2, 600])
+def test_m2l_pair_plan_refuses_out_of_range_gathers(st, bad):
+    rng = np.random.default_rng(5)
+    gv = _sparse_level(rng, [30] * 8, 0.1, 600)
+    gv[len(gv) // 3] = bad
+    rc, _, _, msg = _pairs(st, [30] * 8, gv, 600)
     assert rc == 1 and "gather index out of range" in msg, msg
