@@ -258,17 +258,20 @@ def run_form(nnk, lib, tree, fd, args, eps, mu, precision, res, svd_tol):
     if precision in ("fp64", "fp64t"):
         r = K if precision == "fp64" else R
         flops = 2.0 * r * r * st[0]
-        peak, kern = FP64_MFMA_PEAK_TFLOPS, "k_gemm_gather<160,64,16,1> (M2L, FP64 MFMA)"
+        pairs = any(int(dep[l]) == 1 for l in range(16) if spl[l] > 0)
+        peak, kern = FP64_MFMA_PEAK_TFLOPS, "k_gemm_gather<160,64,16,1> (M2L, FP64 MFMA" + \
+            ("; per-offset pair jobs + k_pair_sum at some levels)" if pairs else ")")
     else:
         np_, f16, what = {"mixed": (2, "false", "two-part split-BF16"), "mixed3": (3, "false", "three-part split-BF16"),
                           "mixedh": (2, "true", "two-part split-FP16")}[precision]
         flops = (6 if np_ == 3 else 3) * 2.0 * R * R * st[0]   # 16-bit products per split product
         peak = BF16_MFMA_PEAK_TFLOPS
         used = {int(dep[l]) for l in range(16) if spl[l] > 0}
-        names = ([f"k_m2l_split<{np_},D,1,{f16}>"] if used & {2, 3} else []) + \
-                ([f"k_m2l_dma<{np_},1,{f16}>"] if 4 in used else []) + \
+        names = ([f"k_m2l_split<{np_},D,1,{f16}>"] if used & {2, 3, 12, 13} else []) + \
+                ([f"k_m2l_dma<{np_},1,{f16}>"] if used & {4, 14} else []) + \
                 ([f"k_m2l_dma<{np_},1,{f16},3> (64 x 48 wave tiles)"] if 5 in used else []) + \
-                ([f"k_m2l_dma<{np_},1,{f16},2,4> (four LDS stages)"] if 6 in used else [])
+                ([f"k_m2l_dma<{np_},1,{f16},2,4> (four LDS stages)"] if 6 in used else []) + \
+                (["per-offset pair jobs + k_pair_sum at some levels"] if used & {12, 13, 14} else [])
         kern = " / ".join(names) + f" (M2L, {what} MFMA{', LDS-DMA staged' if used <= {4, 5, 6} else ''})"
     m2l_ms = ms[3] / args.steps
     achieved = flops / (m2l_ms * 1e-3) / 1e12 if m2l_ms > 0 else 0.0

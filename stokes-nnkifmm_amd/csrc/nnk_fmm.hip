@@ -538,7 +538,7 @@ void plan_m2l_level(const std::vector<int32_t> *cls, const std::vector<int32_t> 
 }
 
 void plan_m2l_pairs_host(const std::vector<int32_t> *cls, const std::vector<int32_t> *gv, int64_t count, int BN,
-                         int64_t M, int64_t cap_elems, M2lPairPlan &P) {
+                         int64_t M, int64_t cap_elems, int64_t rt, int64_t base_work, M2lPairPlan &P) {
   P = M2lPairPlan{};
   if (BN <= 0) throw Error(NNK_ESTATE, "per-offset M2L plan: bad tile width");
   // class tiles: one (column tile, operator block) product per active block
@@ -562,11 +562,10 @@ void plan_m2l_pairs_host(const std::vector<int32_t> *cls, const std::vector<int3
     tiles += (nslot[sl] + BN - 1) / BN;
   }
   // every pair's product column (M doubles) is written once and read once
-  // by the sums: 16 M bytes against the column's 2 M^2 FP64 flops, at
-  // ~45 TF/s for the products and ~4 TB/s for the round trip, costs
-  // 90 / M of a column's product
-  P.pair_work = tiles * BN + (P.npairs * 90 + M - 1) / std::max<int64_t>(M, 1);
-  P.use = P.npairs > 0 && P.pair_work < P.class_work && P.npairs * M <= cap_elems;
+  // by the sums: 16 M bytes, counted as rt / M of a column's product
+  P.pair_work = tiles * BN + (P.npairs * rt + M - 1) / std::max<int64_t>(M, 1);
+  const int64_t other = base_work >= 0 ? base_work : P.class_work;
+  P.use = P.npairs > 0 && P.pair_work < other && P.npairs * M <= cap_elems;
   if (!P.use) return;
   // pairs grouped by offset slot (increasing), within a slot by class and
   // column; each target's list then comes out ordered by slot
@@ -658,8 +657,6 @@ void Fmm::reserve_part(size_t elems) {
   part_elems_ = elems;
 }
 
-// pair buffer limit of the per-offset M2L (doubles): 2 GiB
-constexpr int64_t kPairBufMaxElems = (int64_t)1 << 28;
 
 void Fmm::plan_m2l_pairs(LevelPlan &P, const std::vector<int32_t> *cls, const std::vector<int32_t> *gv,
                          const std::vector<const double *> *ops, int M) {
@@ -667,7 +664,7 @@ void Fmm::plan_m2l_pairs(LevelPlan &P, const std::vector<int32_t> *cls, const st
   P.m2l_pair_on = false;
   P.m2lp = GemmBatch();
   M2lPairPlan H;
-  plan_m2l_pairs_host(cls, gv, tree_.count, 64, M, kPairBufMaxElems, H);
+  plan_m2l_pairs_host(cls, gv, tree_.count, 64, M, kPairBufMaxElems, kPairRtFP64, -1, H);
   if (!H.use) return;
   std::vector<const double *> slot_op(343, nullptr);
   for (int a = 0; a < 8; a++) {
@@ -725,6 +722,11 @@ void Fmm::launch_m2l_pairs(const LevelPlan &P, const double *X, int ldx, double 
                            int Kpad, hipStream_t s) {
   if (!P.m2l_npairs) throw Error(NNK_ESTATE, "per-offset M2L: no plan for this level");
   gemm_dims(P.m2lp, X, ldx, pairbuf_.as<double>(), M, 0, s, M, M, Mpad, Kpad, 1, nullptr, 1);
+  pair_sum(P, M, C, ldc, s);
+}
+
+void Fmm::pair_sum(const LevelPlan &P, int M, double *C, int ldc, hipStream_t s) {
+  if ((size_t)P.m2l_npairs * M > pairbuf_elems_) throw Error(NNK_ESTATE, "per-offset M2L: pair buffer too small");
   hipLaunchKernelGGL(k_pair_sum, dim3((unsigned)P.m2lp_nodes), dim3(256), 0, s, P.m2lp_node.as<int32_t>(),
                      P.m2lp_ptr.as<int32_t>(), P.m2lp_idx.as<int32_t>(), pairbuf_.as<double>(), M, C, ldc);
   NNK_HIP(hipGetLastError());
@@ -1504,7 +1506,10 @@ void Fmm::launch_config(int level, int *split, int *depth) const {
   if (level < 2 || level > maxlev_) return;
   const LevelPlan &P = plans_[level];
   if (m16()) {
-    if (P.mx_m2l_jobs) { *split = P.mx_split; *depth = P.mx_depth; }
+    if (P.mx_m2l_jobs) {
+      *split = P.m2l_pair_on ? 1 : P.mx_split;
+      *depth = P.m2l_pair_on ? 10 + P.mx_pair_depth : P.mx_depth;
+    }
   } else if (P.n_down) {
     *split = P.m2l_pair_on ? 1 : P.m2l_split;
     *depth = P.m2l_pair_on ? 1 : 0;
@@ -1532,12 +1537,19 @@ void Fmm::set_launch_config(int level, int split, int depth) {
     } else {
       throw Error(NNK_EARG, "launch config: the FP64 forms take depth 0 or 1");
     }
+  } else if (depth >= 12 && depth <= 14) {  // per-offset pair jobs, kernel depth 2-4
+    if (!P.m2l_npairs) throw Error(NNK_EARG, "launch config: no per-offset M2L plan at this level");
+    if (split != 1) throw Error(NNK_EARG, "launch config: the per-offset M2L takes split 1");
+    if (depth == 13 && np_ == 3 && !f16_) throw Error(NNK_EARG, "launch config: depth not available for this form");
+    P.mx_pair_depth = depth - 10;
+    P.m2l_pair_on = true;
   } else {
     const bool ok = depth == 2 || (depth == 3 && !(np_ == 3 && !f16_)) ||
                     (depth >= 4 && depth <= 6 && dma_ok(P, split, depth));
     if (!ok) throw Error(NNK_EARG, "launch config: depth not available for this form");
     P.mx_split = split;
     P.mx_depth = depth;
+    P.m2l_pair_on = false;
   }
   gop_.reset();
 }
@@ -1559,19 +1571,25 @@ void Fmm::autotune(hipStream_t s) {
       std::vector<int> splits = {s0};
       if (s0 > 1) splits.push_back(std::max(1, s0 / 2));
       if ((size_t)std::min(64, 2 * s0) * per_split <= part_elems_) splits.push_back(std::min(64, 2 * s0));
-      // FP64 forms: depth 0 = class tiles, 1 = per-offset pair jobs (split 1)
+      // FP64 forms: depth 0 = class tiles, 1 = per-offset pair jobs (split
+      // 1); mixed forms: 12-14 = per-offset pair jobs through the depth 2-4
+      // kernel (split 1)
       std::vector<int> depths = !mixed ? (P.m2l_npairs ? std::vector<int>{0, 1} : std::vector<int>{0})
                                 : np_ == 3 && !f16_ ? std::vector<int>{2, 4}
                                                     : std::vector<int>{2, 3, 4, 5, 6};
+      if (mixed && P.m2l_npairs) {
+        if (np_ == 3 && !f16_) depths.insert(depths.end(), {12, 14});
+        else depths.insert(depths.end(), {12, 13, 14});
+      }
       float best = 1e30f;
-      int bs = s0, bd = mixed ? P.mx_depth : P.m2l_pair_on ? 1 : 0;
+      int bs = s0, bd = mixed ? (P.m2l_pair_on ? 10 + P.mx_pair_depth : P.mx_depth) : P.m2l_pair_on ? 1 : 0;
       for (int sp : splits) {
         if (sp > 1 && !(mixed ? P.mx_slot.p : P.m2l_slot.p)) continue;
         for (int dp : depths) {
-          if (dp >= 4 && !dma_ok(P, sp, dp)) continue;
-          if (!mixed && dp == 1 && sp != s0) continue;  // one pair-job candidate
+          if (dp >= 4 && dp <= 6 && !dma_ok(P, sp, dp)) continue;
+          if ((dp == 1 || dp > 10) && sp != s0) continue;  // one candidate per pair-job launch
           auto run = [&]() {
-            if (mixed) launch_m2l_mixed(l, dp, sp, s);
+            if (mixed) launch_m2l_mixed(l, dp, dp > 10 ? 1 : sp, s);
             else if (dp == 1 && prec_ == 4)
               launch_m2l_pairs(P, chat_.as<double>(), RP_, dhat_.as<double>(), RP_, R_, MpadR_, KpadR_, s);
             else if (dp == 1) launch_m2l_pairs(P, uppot_.as<double>(), K_, chk_.as<double>(), K_, K_, Mpad_, Kpad_, s);
@@ -1589,7 +1607,8 @@ void Fmm::autotune(hipStream_t s) {
           if (ms < best) { best = ms; bs = sp; bd = dp; }
         }
       }
-      if (mixed) { P.mx_split = bs; P.mx_depth = bd; }
+      if (mixed && bd > 10) { P.m2l_pair_on = true; P.mx_pair_depth = bd - 10; }
+      else if (mixed) { P.mx_split = bs; P.mx_depth = bd; P.m2l_pair_on = false; }
       else if (bd == 1) P.m2l_pair_on = true;
       else { P.m2l_split = bs; P.m2l_pair_on = false; }
     }

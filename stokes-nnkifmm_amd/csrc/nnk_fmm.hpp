@@ -63,12 +63,13 @@ struct M2lLevelPlan {
 void plan_m2l_level(const std::vector<int32_t> *cls, const std::vector<int32_t> *gv, int64_t count, int BN,
                     M2lLevelPlan &out);
 
-// The per-offset M2L plan of one level for the FP64 forms (sparse adaptive
-// levels): the V-list (target, source) pairs grouped by offset slot, as
-// jobs of at most BN pairs of one slot, each product into its own column
-// of a pair buffer, then a fixed-order sum per target.  Chosen (use) where
-// its padded work, with the pair buffer's round trip, is below that of the
-// class tiles (BN-column tiles times their active blocks) and the buffer
+// The per-offset M2L plan of one level (sparse adaptive levels): the
+// V-list (target, source) pairs grouped by offset slot, as jobs of at most
+// BN pairs of one slot, each product into its own column of a pair buffer,
+// then a fixed-order sum per target.  Chosen (use) where its padded work
+// (BN per job tile, plus rt / M per pair for the pair buffer's round trip)
+// is below base_work (or, with base_work < 0, below the class tiles'
+// work: BN-column tiles times their active blocks) and the buffer
 // (npairs x M doubles) fits cap_elems.  Jobs are laid out for the 8 XCDs:
 // the slot-ordered tiles are split into 8 contiguous runs and tile k of
 // run x is job 8k + x (empty jobs: n = 0, slot = -1).  node/ptr/idx: per
@@ -82,7 +83,14 @@ struct M2lPairPlan {
   std::vector<int32_t> node, ptr, idx;
 };
 void plan_m2l_pairs_host(const std::vector<int32_t> *cls, const std::vector<int32_t> *gv, int64_t count, int BN,
-                         int64_t M, int64_t cap_elems, M2lPairPlan &out);
+                         int64_t M, int64_t cap_elems, int64_t rt, int64_t base_work, M2lPairPlan &out);
+// pair buffer limit of the per-offset M2L (doubles): 2 GiB
+constexpr int64_t kPairBufMaxElems = (int64_t)1 << 28;
+// the pair buffer's round trip (16 M bytes per pair) relative to a pair
+// column's product, times M: ~4 TB/s for the round trip against ~45 TF/s
+// for 2 M^2 FP64 flops, or ~700 TF/s for the 6 M^2 flops of two-part
+// 16-bit products
+constexpr int64_t kPairRtFP64 = 90, kPairRt16 = 470;
 
 struct GemmBatch {
   DBuf jobs;                 // GemmJob[n]
@@ -116,6 +124,9 @@ struct LevelPlan {
   GemmBatch m2lp;             // one operator, <= 64 pairs per job
   int m2lp_nodes = 0;         // targets of the per-target sums (the level's down nodes)
   DBuf m2lp_node, m2lp_ptr, m2lp_idx;  // target slot; its pairs [ptr[j], ptr[j+1]) of idx
+  DBuf mx_m2lp;               // mixed forms: the same jobs as GemmJobBf16[mx_m2lp_jobs]
+  int mx_m2lp_jobs = 0;
+  int mx_pair_depth = 0;      // register-ring depth (2, 3) or 4 (DMA-staged) of their launch
   GemmBatch dn_solve1, dn_solve2;
   int n_x = 0;
   DBuf x_nodes;               // nodes with a non-empty X list
@@ -310,6 +321,13 @@ class Fmm {
                       const std::vector<const double *> *ops, int M);
   void launch_m2l_pairs(const LevelPlan &P, const double *X, int ldx, double *C, int ldc, int M, int Mpad,
                         int Kpad, hipStream_t s);
+  // the mixed forms' per-offset pair plan of a level (base_work: the padded
+  // work of the class or merged jobs it replaces) and its launch by the
+  // 16-bit M2L kernel of register-ring depth 2/3 or the DMA-staged one (4)
+  void plan_m2l_pairs16(LevelPlan &P, const HostLevel &H, const MixedOps &X, int64_t base_work);
+  void launch_m2l_pairs16(const LevelPlan &P, int depth, hipStream_t s);
+  // the per-target sums of a per-offset plan (Y: M rows per pair)
+  void pair_sum(const LevelPlan &P, int M, double *C, int ldc, hipStream_t s);
   void build_mixed();
   void plan_mixed();
   void apply_mixed(const double *d_f, double *d_u, hipStream_t s);

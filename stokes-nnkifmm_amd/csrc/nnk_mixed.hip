@@ -631,6 +631,9 @@ void Fmm::plan_mixed() {
       }
       P.mx_slot.upload(slot);  // also for the autotuner's split candidates
       reserve_part((size_t)std::min(64, 2 * P.mx_split) * nj * mN * RP_);
+      // sparse levels: the per-offset pair jobs where they do less padded
+      // work than the class or merged jobs
+      plan_m2l_pairs16(P, H, X, (merged ? mp.merged_pairs : mp.class_pairs) * MBN);
     }
     P.mx_l2l.njobs = (int)lj.size();
     if (!lj.empty()) P.mx_l2l.jobs.upload(lj);
@@ -728,6 +731,93 @@ void Fmm::plan_mixed() {
 // split in its FP16 unscale table; depths 5 (192-column tiles) and 6 (four
 // LDS stages) run the class jobs of two-part forms and hold 192 blocks of
 // a tile in LDS
+void Fmm::plan_m2l_pairs16(LevelPlan &P, const HostLevel &H, const MixedOps &X, int64_t base_work) {
+  P.m2l_npairs = 0;
+  P.m2l_pair_on = false;
+  P.mx_m2lp_jobs = 0;
+  M2lPairPlan HP;
+  plan_m2l_pairs_host(H.cls, H.gv, tree_.count, MBN, R_, kPairBufMaxElems, kPairRt16, base_work, HP);
+  if (!HP.use) return;
+  const int nj = (int)HP.job_slot.size();
+  std::vector<const uint16_t *> jop(nj, nullptr);
+  std::vector<const float *> juop(nj, nullptr);
+  for (int j = 0; j < nj; j++) {
+    const int sl = HP.job_slot[j];
+    if (sl < 0) continue;
+    if (!(jop[j] = X.cop[sl]) || (f16_ && !(juop[j] = X.uop[sl])))
+      throw Error(NNK_ESTATE, "per-offset mixed M2L: missing translation operator");
+  }
+  std::vector<int32_t> col(HP.npairs);
+  for (int64_t p = 0; p < HP.npairs; p++) col[p] = (int32_t)p;
+  const std::vector<int32_t> tp = {0, 1}, tl = {0};  // one tile, its one block
+  auto up = [&](const auto &v) {
+    P.mx_owned.emplace_back();
+    P.mx_owned.back().upload(v);
+    return P.mx_owned.back().p;
+  };
+  const uint16_t *const *dA = (const uint16_t *const *)up(jop);
+  const float *const *dU = (const float *const *)up(juop);
+  const int32_t *dsrc = (const int32_t *)up(HP.src);
+  const int32_t *dcol = (const int32_t *)up(col);
+  const int32_t *dtp = (const int32_t *)up(tp);
+  const int32_t *dtl = (const int32_t *)up(tl);
+  std::vector<GemmJobBf16> jobs(nj);
+  for (int j = 0; j < nj; j++) {
+    GemmJobBf16 &J = jobs[j];
+    J.A = dA + j;
+    J.U = dU + j;
+    J.gidx = dsrc + HP.job_p0[j];  // empty jobs (N = 0) return before any read
+    J.cidx = dcol + HP.job_p0[j];
+    J.nb = 1;
+    J.N = HP.job_n[j];
+    J.tptr = dtp;
+    J.tlist = dtl;
+  }
+  P.mx_m2lp.upload(jobs);
+  P.mx_m2lp_jobs = nj;
+  P.m2lp_nodes = (int)HP.node.size();
+  P.m2lp_node.upload(HP.node);
+  P.m2lp_ptr.upload(HP.ptr);
+  P.m2lp_idx.upload(HP.idx);
+  if ((size_t)HP.npairs * R_ > pairbuf_elems_) {
+    pairbuf_.alloc(sizeof(double) * (size_t)HP.npairs * R_);
+    pairbuf_elems_ = (size_t)HP.npairs * R_;
+  }
+  P.m2l_npairs = (int)HP.npairs;
+  P.m2l_pair_on = true;
+  P.mx_pair_depth = 4;
+}
+
+// the per-offset jobs of level P through the 16-bit M2L kernel of depth 2,
+// 3 or 4 (no split-K: one block per job), then the per-target sums
+void Fmm::launch_m2l_pairs16(const LevelPlan &P, int depth, hipStream_t s) {
+  if (!P.m2l_npairs || !P.mx_m2lp_jobs) throw Error(NNK_ESTATE, "per-offset mixed M2L: no plan for this level");
+  if (depth < 2 || depth > 4 || (depth == 3 && np_ == 3 && !f16_))
+    throw Error(NNK_ESTATE, "per-offset mixed M2L: depth not available for this form");
+  const int nrb = RPm_ / MBM, nj = P.mx_m2lp_jobs;
+  const dim3 grid((unsigned)(nj * nrb));  // one 128-column tile per job
+  const GemmJobBf16 *jobs = P.mx_m2lp.as<GemmJobBf16>();
+  double *Y = pairbuf_.as<double>();
+  auto launch = [&](auto kern) {
+    hipLaunchKernelGGL(kern, grid, dim3(512), 0, s, R_, KCm_, RT_, chat_x_.as<uint16_t>(), (int)tree_.count, Y,
+                       R_, jobs, nj, nrb, 1, nullptr, MBN, xun_.as<double>(), 0);
+  };
+  if (f16_) {
+    if (depth == 4) launch(k_m2l_dma<2, 1, true>);
+    else if (depth == 2) launch(k_m2l_split<2, 2, 1, true>);
+    else launch(k_m2l_split<2, 3, 1, true>);
+  } else if (np_ == 3) {
+    if (depth == 4) launch(k_m2l_dma<3, 1, false>);
+    else launch(k_m2l_split<3, 2, 1, false>);
+  } else {
+    if (depth == 4) launch(k_m2l_dma<2, 1, false>);
+    else if (depth == 2) launch(k_m2l_split<2, 2, 1, false>);
+    else launch(k_m2l_split<2, 3, 1, false>);
+  }
+  NNK_HIP(hipGetLastError());
+  pair_sum(P, R_, dhat_.as<double>(), RP_, s);
+}
+
 bool Fmm::dma_ok(const LevelPlan &P, int split, int depth) const {
   if (split < 1) return false;
   if (depth == 5) return P.mx_tn3 && P.mx_maxnb <= dma_max_blocks(3);
@@ -742,6 +832,11 @@ bool Fmm::dma_ok(const LevelPlan &P, int split, int depth) const {
 void Fmm::launch_m2l_mixed(int l, int depth, int split, hipStream_t s) {
   LevelPlan &P = plans_[l];
   if (!P.mx_m2l_jobs) return;
+  if (depth > 10) {  // 12-14: the per-offset pair jobs, kernel depth 2-4
+    if (split != 1) throw Error(NNK_ESTATE, "per-offset mixed M2L: no split-K");
+    launch_m2l_pairs16(P, depth - 10, s);
+    return;
+  }
   const int bn = depth == 5 ? 3 * 64 : MBN;
   const int nrb = RPm_ / MBM, nct = (P.mx_m2l_maxN + bn - 1) / bn;
   const int nj = P.mx_m2l_jobs, mN = P.mx_m2l_maxN;
@@ -850,7 +945,8 @@ void Fmm::apply_mixed(const double *d_f, double *d_u, hipStream_t s) {
     LevelPlan &P = plans_[l];
     if (P.mx_m2l_jobs) {
       mark(3, true, s);
-      launch_m2l_mixed(l, P.mx_depth, P.mx_split, s);
+      if (P.m2l_pair_on) launch_m2l_pairs16(P, P.mx_pair_depth, s);
+      else launch_m2l_mixed(l, P.mx_depth, P.mx_split, s);
       mark(3, false, s);
     } else if (prec_ == 4 && P.m2l.njobs) {
       mark(3, true, s);

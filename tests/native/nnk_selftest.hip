@@ -199,7 +199,7 @@ extern "C" int nnk_selftest_m2l_pairs(const int32_t *sizes, const int32_t *gv, i
       off += n;
     }
     nnk::M2lPairPlan P;
-    nnk::plan_m2l_pairs_host(cls, g, count, BN, M, cap, P);
+    nnk::plan_m2l_pairs_host(cls, g, count, BN, M, cap, 90, -1, P);
     info[0] = P.use ? 1 : 0;
     info[1] = P.class_work;
     info[2] = P.pair_work;

@@ -173,6 +173,45 @@ def test_mixedh_m2l_kernels_agree_bitwise(nnk, precision, split):
             assert rel(out[5], out[2]) < 1e-13
 
 
+@pytest.mark.parametrize("precision", ["mixed", "mixed3", "mixedh"])
+def test_pair_jobs_match_class_jobs(nnk, precision):
+    """The mixed forms' per-offset pair jobs (sparse levels of an adaptive
+    tree: one offset's V-list pairs per job, each block product promoted
+    to FP64 on its own, then summed per target in offset-slot order) form
+    the same per-block products as the class and merged jobs and add them
+    in the same order: bit-identical at split 1, through every kernel that
+    runs them (register-ring depth 2/3, DMA-staged 4)."""
+    rng = np.random.default_rng(46)
+    p = blobs(rng)
+    q = rng.standard_normal(p.shape)
+    f = nnk.KIFMM(nnk.OcTree(p, p, "nodeCapacity", 16), [1e-3, 1.0], "precision", precision)
+    lib, h = nnk.lib(), f._handle().h
+    i32 = ctypes.POINTER(ctypes.c_int32)
+    s0 = np.zeros(16, np.int32)
+    d0 = np.zeros(16, np.int32)
+    assert lib.nnk_fmm_launch_config(h, s0.ctypes.data_as(i32), d0.ctypes.data_as(i32), 16) == 0
+    pl = np.flatnonzero(d0 > 10)
+    print(f"[pairs] {precision}: per-offset levels {pl.tolist()} of {np.flatnonzero(s0).tolist()}", flush=True)
+    assert len(pl) >= 1 and (s0[pl] == 1).all()
+    u = f.computeVel(q)
+
+    def setcfg(split, depth):
+        s = np.zeros(16, np.int32)
+        d = np.zeros(16, np.int32)
+        s[pl], d[pl] = split, depth
+        return lib.nnk_fmm_set_launch_config(h, s.ctypes.data_as(i32), d.ctypes.data_as(i32), 16)
+
+    assert setcfg(2, 14) != 0      # no split-K for the pair jobs
+    kds = (2, 4) if precision == "mixed3" else (2, 3, 4)
+    if precision == "mixed3":
+        assert setcfg(1, 13) != 0  # three parts: no third register stage
+    for kd in kds:
+        assert setcfg(1, 10 + kd) == 0
+        np.testing.assert_array_equal(f.computeVel(q), u)
+        assert setcfg(1, kd) == 0  # class or merged jobs on the same levels
+        np.testing.assert_array_equal(f.computeVel(q), u)
+
+
 def test_launch_config_rejects_what_a_form_cannot_run(nnk):
     rng = np.random.default_rng(45)
     p = rng.random((20000, 3))
