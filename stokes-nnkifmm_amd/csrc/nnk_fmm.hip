@@ -1537,15 +1537,16 @@ void Fmm::set_launch_config(int level, int split, int depth) {
     } else {
       throw Error(NNK_EARG, "launch config: the FP64 forms take depth 0 or 1");
     }
-  } else if (depth >= 12 && depth <= 14) {  // per-offset pair jobs, kernel depth 2-4
+  } else if ((depth >= 12 && depth <= 14) || depth == 17) {  // per-offset pair jobs, kernel depth 2-4, 7
     if (!P.m2l_npairs) throw Error(NNK_EARG, "launch config: no per-offset M2L plan at this level");
     if (split != 1) throw Error(NNK_EARG, "launch config: the per-offset M2L takes split 1");
-    if (depth == 13 && np_ == 3 && !f16_) throw Error(NNK_EARG, "launch config: depth not available for this form");
+    if ((depth == 13 || depth == 17) && np_ == 3 && !f16_)
+      throw Error(NNK_EARG, "launch config: depth not available for this form");
     P.mx_pair_depth = depth - 10;
     P.m2l_pair_on = true;
   } else {
     const bool ok = depth == 2 || (depth == 3 && !(np_ == 3 && !f16_)) ||
-                    (depth >= 4 && depth <= 6 && dma_ok(P, split, depth));
+                    (depth >= 4 && depth <= 7 && dma_ok(P, split, depth) && !(depth == 7 && np_ == 3 && !f16_));
     if (!ok) throw Error(NNK_EARG, "launch config: depth not available for this form");
     P.mx_split = split;
     P.mx_depth = depth;
@@ -1572,21 +1573,21 @@ void Fmm::autotune(hipStream_t s) {
       if (s0 > 1) splits.push_back(std::max(1, s0 / 2));
       if ((size_t)std::min(64, 2 * s0) * per_split <= part_elems_) splits.push_back(std::min(64, 2 * s0));
       // FP64 forms: depth 0 = class tiles, 1 = per-offset pair jobs (split
-      // 1); mixed forms: 12-14 = per-offset pair jobs through the depth 2-4
+      // 1); mixed forms: 12-14, 17 = per-offset pair jobs through the depth 2-4, 7
       // kernel (split 1)
       std::vector<int> depths = !mixed ? (P.m2l_npairs ? std::vector<int>{0, 1} : std::vector<int>{0})
                                 : np_ == 3 && !f16_ ? std::vector<int>{2, 4}
-                                                    : std::vector<int>{2, 3, 4, 5, 6};
+                                                    : std::vector<int>{2, 3, 4, 5, 6, 7};
       if (mixed && P.m2l_npairs) {
         if (np_ == 3 && !f16_) depths.insert(depths.end(), {12, 14});
-        else depths.insert(depths.end(), {12, 13, 14});
+        else depths.insert(depths.end(), {12, 13, 14, 17});
       }
       float best = 1e30f;
       int bs = s0, bd = mixed ? (P.m2l_pair_on ? 10 + P.mx_pair_depth : P.mx_depth) : P.m2l_pair_on ? 1 : 0;
       for (int sp : splits) {
         if (sp > 1 && !(mixed ? P.mx_slot.p : P.m2l_slot.p)) continue;
         for (int dp : depths) {
-          if (dp >= 4 && dp <= 6 && !dma_ok(P, sp, dp)) continue;
+          if (dp >= 4 && dp <= 7 && !dma_ok(P, sp, dp)) continue;
           if ((dp == 1 || dp > 10) && sp != s0) continue;  // one candidate per pair-job launch
           auto run = [&]() {
             if (mixed) launch_m2l_mixed(l, dp, dp > 10 ? 1 : sp, s);

@@ -318,7 +318,7 @@ __device__ __forceinline__ void wait_vmcnt() {
   __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | (7 << 4) | (15 << 8));
 }
 
-template <int NP, int TAG, bool F16, int TN = 2, int NS = 3>
+template <int NP, int TAG, bool F16, int TN = 2, int NS = 3, bool PF = false>
 __global__ __launch_bounds__(512, 1) void k_m2l_dma(const int M, const int KC, const int RT,
                                                     const uint16_t *__restrict__ X, const int zrow,
                                                     double *__restrict__ C, const int ldc,
@@ -329,6 +329,7 @@ __global__ __launch_bounds__(512, 1) void k_m2l_dma(const int M, const int KC, c
   static_assert(!F16 || NP == 2, "FP16 parts: two");
   static_assert(TN == 2 || (TN == 3 && NP == 2), "64 x 48 wave tiles: two parts");
   static_assert(NS == 3 || (NS == 4 && TN == 2 && NP == 2), "four stages: 128-column tiles, two parts");
+  static_assert(!PF || (TN == 2 && NS == 3), "fragment prefetch: 128-column tiles, three stages");
   constexpr int BN = 64 * TN, TM = 4, TILE = 512;
   constexpr int XT = 4 * TN;                  // 16-column X tiles of a workgroup tile
   constexpr int LPW = (8 + XT) * NP / 8;      // fragment loads per wave and chunk
@@ -489,20 +490,25 @@ __global__ __launch_bounds__(512, 1) void k_m2l_dma(const int M, const int KC, c
   // TM TN instructions after its previous one instead of depending on the
   // one just issued.  Each accumulator still takes the terms in the same
   // order (bit-identical results).
-  auto compute = [&](const Stage &Sc) {
+  struct Frag {
     op8 a[NP][TM], x[NP][TN];
+  };
+  auto readfrag = [&](const Stage &Sc, Frag &F) {
 #pragma unroll
     for (int p = NP - 1; p >= 0; p--) {
 #pragma unroll
-      for (int i = 0; i < TM; i++) a[p][i] = *reinterpret_cast<const op8 *>(&Sc.a[p][wm * 4 + i][lane * 8]);
+      for (int i = 0; i < TM; i++) F.a[p][i] = *reinterpret_cast<const op8 *>(&Sc.a[p][wm * 4 + i][lane * 8]);
 #pragma unroll
-      for (int j = 0; j < TN; j++) x[NP - 1 - p][j] = *reinterpret_cast<const op8 *>(&Sc.x[NP - 1 - p][wn * TN + j][lane * 8]);
+      for (int j = 0; j < TN; j++)
+        F.x[NP - 1 - p][j] = *reinterpret_cast<const op8 *>(&Sc.x[NP - 1 - p][wn * TN + j][lane * 8]);
     }
+  };
+  auto mul = [&](const Frag &F) {
     auto term = [&](int pa, int px) {
 #pragma unroll
       for (int i = 0; i < TM; i++)
 #pragma unroll
-        for (int j = 0; j < TN; j++) acc[i][j] = mfma(a[pa][i], x[px][j], acc[i][j]);
+        for (int j = 0; j < TN; j++) acc[i][j] = mfma(F.a[pa][i], F.x[px][j], acc[i][j]);
     };
     if constexpr (NP == 3) {
       term(2, 0);
@@ -512,6 +518,11 @@ __global__ __launch_bounds__(512, 1) void k_m2l_dma(const int M, const int KC, c
     term(1, 0);
     term(0, 1);
     term(0, 0);
+  };
+  auto compute = [&](const Stage &Sc) {
+    Frag F;
+    readfrag(Sc, F);
+    mul(F);
   };
   // promotion of block q's FP32 sums (FP16 parts: each row times its
   // unscale factor)
@@ -543,7 +554,43 @@ __global__ __launch_bounds__(512, 1) void k_m2l_dma(const int M, const int KC, c
       if ((cc + 1) % KC == 0) promote(cc / KC);
     }
   };
-  if (total > 0) {
+  // fragment prefetch (PF): step cc reads the fragments of chunk cc+1 from
+  // LDS into one register set while the MFMAs of chunk cc run on the other,
+  // so that the LDS reads of one chunk overlap the products of the previous
+  // one instead of following the barrier.  The DMAs of chunk cc+3 go into
+  // the stage chunk cc was read from (every wave has waited for its reads of
+  // chunk cc before the barrier), so three stages still hold two chunks of
+  // loads in flight.
+  auto pstep = [&](int cc, const Stage &Sn, Stage &Sd, const Frag &Fc, Frag &Fn) {
+    // vmcnt: this wave's loads of chunk cc+1 have landed (those of cc+2 may
+    // be outstanding); lgkmcnt(0): its LDS reads of chunk cc are done
+    __builtin_amdgcn_s_waitcnt((LPW & 15) | ((LPW >> 4) << 14) | (7 << 4));
+    __builtin_amdgcn_s_barrier();
+    fetch(cc + 3, Sd);
+    readfrag(Sn, Fn);
+    if (cc < total) {
+      mul(Fc);
+      if ((cc + 1) % KC == 0) promote(cc / KC);
+    }
+  };
+  if (total > 0 && PF) {
+    Frag F0, F1;
+    fetch(0, St0);
+    fetch(1, St1);
+    wait_vmcnt<LPW>();
+    __builtin_amdgcn_s_barrier();
+    readfrag(St0, F0);
+    fetch(2, St2);
+    for (int c = 0; c < total; c += 6) {
+      pstep(c, St1, St0, F0, F1);
+      pstep(c + 1, St2, St1, F1, F0);
+      pstep(c + 2, St0, St2, F0, F1);
+      pstep(c + 3, St1, St0, F1, F0);
+      pstep(c + 4, St2, St1, F0, F1);
+      pstep(c + 5, St0, St2, F1, F0);
+    }
+    wait_vmcnt<0>();  // no DMA may land after the workgroup ends
+  } else if (total > 0) {
     fetch(0, St0);
     fetch(1, St1);
     if constexpr (NS == 4) {

@@ -792,7 +792,7 @@ void Fmm::plan_m2l_pairs16(LevelPlan &P, const HostLevel &H, const MixedOps &X, 
 // 3 or 4 (no split-K: one block per job), then the per-target sums
 void Fmm::launch_m2l_pairs16(const LevelPlan &P, int depth, hipStream_t s) {
   if (!P.m2l_npairs || !P.mx_m2lp_jobs) throw Error(NNK_ESTATE, "per-offset mixed M2L: no plan for this level");
-  if (depth < 2 || depth > 4 || (depth == 3 && np_ == 3 && !f16_))
+  if (depth < 2 || depth > 7 || (depth >= 5 && depth <= 6) || ((depth == 3 || depth == 7) && np_ == 3 && !f16_))
     throw Error(NNK_ESTATE, "per-offset mixed M2L: depth not available for this form");
   const int nrb = RPm_ / MBM, nj = P.mx_m2lp_jobs;
   const dim3 grid((unsigned)(nj * nrb));  // one 128-column tile per job
@@ -803,14 +803,16 @@ void Fmm::launch_m2l_pairs16(const LevelPlan &P, int depth, hipStream_t s) {
                        R_, jobs, nj, nrb, 1, nullptr, MBN, xun_.as<double>(), 0);
   };
   if (f16_) {
-    if (depth == 4) launch(k_m2l_dma<2, 1, true>);
+    if (depth == 7) launch(k_m2l_dma<2, 1, true, 2, 3, true>);
+    else if (depth == 4) launch(k_m2l_dma<2, 1, true>);
     else if (depth == 2) launch(k_m2l_split<2, 2, 1, true>);
     else launch(k_m2l_split<2, 3, 1, true>);
   } else if (np_ == 3) {
     if (depth == 4) launch(k_m2l_dma<3, 1, false>);
     else launch(k_m2l_split<3, 2, 1, false>);
   } else {
-    if (depth == 4) launch(k_m2l_dma<2, 1, false>);
+    if (depth == 7) launch(k_m2l_dma<2, 1, false, 2, 3, true>);
+    else if (depth == 4) launch(k_m2l_dma<2, 1, false>);
     else if (depth == 2) launch(k_m2l_split<2, 2, 1, false>);
     else launch(k_m2l_split<2, 3, 1, false>);
   }
@@ -827,7 +829,8 @@ bool Fmm::dma_ok(const LevelPlan &P, int split, int depth) const {
 
 // M2L of level l in the mixed form with register-ring depth `depth` (2 or
 // 3; 4 selects the DMA-staged kernel, 5 the DMA-staged kernel with 64 x 48
-// wave tiles, 6 the DMA-staged kernel with four LDS stages) and split-K
+// wave tiles, 6 the DMA-staged kernel with four LDS stages, 7 the
+// DMA-staged kernel with fragment prefetch) and split-K
 // factor `split` (the part buffer must hold it)
 void Fmm::launch_m2l_mixed(int l, int depth, int split, hipStream_t s) {
   LevelPlan &P = plans_[l];
@@ -854,7 +857,8 @@ void Fmm::launch_m2l_mixed(int l, int depth, int split, hipStream_t s) {
     throw Error(NNK_ESTATE, "mixed M2L: too many operator blocks for the DMA-staged kernel");
   if (depth >= 5 && np_ != 2) throw Error(NNK_ESTATE, "mixed M2L: this DMA-staged kernel needs two parts");
   if (f16_) {
-    if (depth == 6) launch(k_m2l_dma<2, 1, true, 2, 4>);
+    if (depth == 7) launch(k_m2l_dma<2, 1, true, 2, 3, true>);
+    else if (depth == 6) launch(k_m2l_dma<2, 1, true, 2, 4>);
     else if (depth == 5) launch(k_m2l_dma<2, 1, true, 3>);
     else if (depth == 4) launch(k_m2l_dma<2, 1, true>);
     else if (depth == 2) launch(k_m2l_split<2, 2, 1, true>);
@@ -863,7 +867,8 @@ void Fmm::launch_m2l_mixed(int l, int depth, int split, hipStream_t s) {
     if (depth == 4) launch(k_m2l_dma<3, 1, false>);
     else launch(k_m2l_split<3, 2, 1, false>);  // a third register stage would spill
   } else {
-    if (depth == 6) launch(k_m2l_dma<2, 1, false, 2, 4>);
+    if (depth == 7) launch(k_m2l_dma<2, 1, false, 2, 3, true>);
+    else if (depth == 6) launch(k_m2l_dma<2, 1, false, 2, 4>);
     else if (depth == 5) launch(k_m2l_dma<2, 1, false, 3>);
     else if (depth == 4) launch(k_m2l_dma<2, 1, false>);
     else if (depth == 2) launch(k_m2l_split<2, 2, 1, false>);

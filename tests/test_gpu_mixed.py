@@ -121,6 +121,8 @@ def test_mixedh_m2l_kernels_agree_bitwise(nnk, precision, split):
     """The split-precision M2L of every level run by the register-ring
     kernel (depth 2 and, but for the three-part form, 3), by the DMA-staged
     kernel (depth 4) and, for the two-part forms, by the DMA-staged kernel
+    with fragment prefetch (depth 7: the LDS reads of a chunk overlap the
+    products of the previous one), by the DMA-staged kernel
     with four LDS stages (depth 6) and with 64 x 48 wave tiles (depth 5),
     both on the levels run as class jobs (depth 4 elsewhere): the same
     products in the same order, so the matvecs are bit-identical for one
@@ -132,7 +134,7 @@ def test_mixedh_m2l_kernels_agree_bitwise(nnk, precision, split):
     p = np.concatenate([rng.random((30000, 3)), 1e-3 * rng.random((3000, 3)) + 0.5])
     q = rng.standard_normal(p.shape)
     f = nnk.KIFMM(nnk.OcTree(p, p, "nodeCapacity", 48), [1e-3, 1.0], "precision", precision)
-    depths = (2, 4) if precision == "mixed3" else (2, 3, 4)
+    depths = (2, 4) if precision == "mixed3" else (2, 3, 4, 7)
     lib, h = nnk.lib(), f._handle().h
     i32 = ctypes.POINTER(ctypes.c_int32)
     s0 = np.zeros(16, np.int32)
@@ -180,7 +182,8 @@ def test_pair_jobs_match_class_jobs(nnk, precision):
     to FP64 on its own, then summed per target in offset-slot order) form
     the same per-block products as the class and merged jobs and add them
     in the same order: bit-identical at split 1, through every kernel that
-    runs them (register-ring depth 2/3, DMA-staged 4)."""
+    runs them (register-ring depth 2/3, DMA-staged 4, DMA-staged with
+    fragment prefetch 7)."""
     rng = np.random.default_rng(46)
     p = blobs(rng)
     q = rng.standard_normal(p.shape)
@@ -202,9 +205,10 @@ def test_pair_jobs_match_class_jobs(nnk, precision):
         return lib.nnk_fmm_set_launch_config(h, s.ctypes.data_as(i32), d.ctypes.data_as(i32), 16)
 
     assert setcfg(2, 14) != 0      # no split-K for the pair jobs
-    kds = (2, 4) if precision == "mixed3" else (2, 3, 4)
+    kds = (2, 4) if precision == "mixed3" else (2, 3, 4, 7)
     if precision == "mixed3":
         assert setcfg(1, 13) != 0  # three parts: no third register stage
+        assert setcfg(1, 17) != 0  # nor fragment prefetch (its registers would spill)
     for kd in kds:
         assert setcfg(1, 10 + kd) == 0
         np.testing.assert_array_equal(f.computeVel(q), u)
@@ -239,7 +243,8 @@ def test_launch_config_rejects_what_a_form_cannot_run(nnk):
     assert setcfg(f64, l64, 1, 3) != 0          # the FP64 form has no ring depth
     assert setcfg(fb, lb, 1, 3) != 0            # three parts: no third register stage
     assert setcfg(fb, lb, 1, 5) != 0            # three parts: no 64 x 48 wave tiles
-    assert setcfg(fh, lh, 1, 7) != 0
+    assert setcfg(fb, lb, 1, 7) != 0            # three parts: no fragment prefetch
+    assert setcfg(fh, lh, 1, 8) != 0
     assert setcfg(fh, lh, 0, 4) == 0            # split 0: level left as it is
     assert setcfg(fh, lh, 65, 3) != 0
     assert setcfg(fh, 0, 1, 3) != 0             # level without M2L
