@@ -198,7 +198,7 @@ int32_t nnk_fmm_set_graph(nnk_fmm *h, int32_t on);
  *           (split-K factor s/2, s, 2s of the default s; for the FP64
  *           forms also the per-offset pair jobs, depth 1, where planned;
  *           for the mixed forms also the register-ring depth 2 or 3 and
- *           the DMA-staged kernels, depths 4-7, and where planned the
+ *           the DMA-staged kernels, depths 4-8, and where planned the
  *           per-offset pair jobs, 12-14, 17) and keeps the fastest.  The
  *           choice changes summation order only (results stay within
  *           rounding of each other and deterministic afterwards).
@@ -220,7 +220,8 @@ int32_t nnk_fmm_launch_config(const nnk_fmm *h, int32_t *split, int32_t *depth, 
  *           per split-K part; 5 and 6 its 64 x 48 wave-tile and four-stage
  *           variants, class jobs of at most 192 blocks; 7 depth 4 with the
  *           LDS fragment reads of each chunk overlapping the products of the
- *           previous one, two-part forms, depth 4's limits); 12-14 and 17 run the per-offset
+ *           previous one, two-part forms, depth 4's limits; 8 the same with
+ *           four LDS stages, depth 6's limits); 12-14 and 17 run the per-offset
  *           pair jobs (split 1, levels where the planner built them) through
  *           the kernel of depth 2-4 or 7.  Mixed-form results
  *           are bit-identical across the depths of one split; the FP64

@@ -1546,7 +1546,7 @@ void Fmm::set_launch_config(int level, int split, int depth) {
     P.m2l_pair_on = true;
   } else {
     const bool ok = depth == 2 || (depth == 3 && !(np_ == 3 && !f16_)) ||
-                    (depth >= 4 && depth <= 7 && dma_ok(P, split, depth) && !(depth == 7 && np_ == 3 && !f16_));
+                    (depth >= 4 && depth <= 8 && dma_ok(P, split, depth) && !(depth >= 7 && np_ == 3 && !f16_));
     if (!ok) throw Error(NNK_EARG, "launch config: depth not available for this form");
     P.mx_split = split;
     P.mx_depth = depth;
@@ -1577,7 +1577,7 @@ void Fmm::autotune(hipStream_t s) {
       // kernel (split 1)
       std::vector<int> depths = !mixed ? (P.m2l_npairs ? std::vector<int>{0, 1} : std::vector<int>{0})
                                 : np_ == 3 && !f16_ ? std::vector<int>{2, 4}
-                                                    : std::vector<int>{2, 3, 4, 5, 6, 7};
+                                                    : std::vector<int>{2, 3, 4, 5, 6, 7, 8};
       if (mixed && P.m2l_npairs) {
         if (np_ == 3 && !f16_) depths.insert(depths.end(), {12, 14});
         else depths.insert(depths.end(), {12, 13, 14, 17});
@@ -1587,7 +1587,7 @@ void Fmm::autotune(hipStream_t s) {
       for (int sp : splits) {
         if (sp > 1 && !(mixed ? P.mx_slot.p : P.m2l_slot.p)) continue;
         for (int dp : depths) {
-          if (dp >= 4 && dp <= 7 && !dma_ok(P, sp, dp)) continue;
+          if (dp >= 4 && dp <= 8 && !dma_ok(P, sp, dp)) continue;
           if ((dp == 1 || dp > 10) && sp != s0) continue;  // one candidate per pair-job launch
           auto run = [&]() {
             if (mixed) launch_m2l_mixed(l, dp, dp > 10 ? 1 : sp, s);

@@ -329,7 +329,7 @@ __global__ __launch_bounds__(512, 1) void k_m2l_dma(const int M, const int KC, c
   static_assert(!F16 || NP == 2, "FP16 parts: two");
   static_assert(TN == 2 || (TN == 3 && NP == 2), "64 x 48 wave tiles: two parts");
   static_assert(NS == 3 || (NS == 4 && TN == 2 && NP == 2), "four stages: 128-column tiles, two parts");
-  static_assert(!PF || (TN == 2 && NS == 3), "fragment prefetch: 128-column tiles, three stages");
+  static_assert(!PF || TN == 2, "fragment prefetch: 128-column tiles");
   constexpr int BN = 64 * TN, TM = 4, TILE = 512;
   constexpr int XT = 4 * TN;                  // 16-column X tiles of a workgroup tile
   constexpr int LPW = (8 + XT) * NP / 8;      // fragment loads per wave and chunk
@@ -557,16 +557,18 @@ __global__ __launch_bounds__(512, 1) void k_m2l_dma(const int M, const int KC, c
   // fragment prefetch (PF): step cc reads the fragments of chunk cc+1 from
   // LDS into one register set while the MFMAs of chunk cc run on the other,
   // so that the LDS reads of one chunk overlap the products of the previous
-  // one instead of following the barrier.  The DMAs of chunk cc+3 go into
+  // one instead of following the barrier.  The DMAs of chunk cc+NS go into
   // the stage chunk cc was read from (every wave has waited for its reads of
-  // chunk cc before the barrier), so three stages still hold two chunks of
+  // chunk cc before the barrier), so NS stages still hold NS-1 chunks of
   // loads in flight.
+  constexpr int PW = (NS - 2) * LPW;
   auto pstep = [&](int cc, const Stage &Sn, Stage &Sd, const Frag &Fc, Frag &Fn) {
-    // vmcnt: this wave's loads of chunk cc+1 have landed (those of cc+2 may
-    // be outstanding); lgkmcnt(0): its LDS reads of chunk cc are done
-    __builtin_amdgcn_s_waitcnt((LPW & 15) | ((LPW >> 4) << 14) | (7 << 4));
+    // vmcnt: this wave's loads of chunk cc+1 have landed (those of cc+2 ..
+    // cc+NS-1 may be outstanding); lgkmcnt(0): its LDS reads of chunk cc
+    // are done
+    __builtin_amdgcn_s_waitcnt((PW & 15) | ((PW >> 4) << 14) | (7 << 4));
     __builtin_amdgcn_s_barrier();
-    fetch(cc + 3, Sd);
+    fetch(cc + NS, Sd);
     readfrag(Sn, Fn);
     if (cc < total) {
       mul(Fc);
@@ -577,17 +579,28 @@ __global__ __launch_bounds__(512, 1) void k_m2l_dma(const int M, const int KC, c
     Frag F0, F1;
     fetch(0, St0);
     fetch(1, St1);
-    wait_vmcnt<LPW>();
+    if constexpr (NS == 4) fetch(2, St2);
+    wait_vmcnt<PW>();
     __builtin_amdgcn_s_barrier();
     readfrag(St0, F0);
-    fetch(2, St2);
-    for (int c = 0; c < total; c += 6) {
-      pstep(c, St1, St0, F0, F1);
-      pstep(c + 1, St2, St1, F1, F0);
-      pstep(c + 2, St0, St2, F0, F1);
-      pstep(c + 3, St1, St0, F1, F0);
-      pstep(c + 4, St2, St1, F0, F1);
-      pstep(c + 5, St0, St2, F1, F0);
+    if constexpr (NS == 4) {
+      fetch(3, St3);
+      for (int c = 0; c < total; c += 4) {
+        pstep(c, St1, St0, F0, F1);
+        pstep(c + 1, St2, St1, F1, F0);
+        pstep(c + 2, St3, St2, F0, F1);
+        pstep(c + 3, St0, St3, F1, F0);
+      }
+    } else {
+      fetch(2, St2);
+      for (int c = 0; c < total; c += 6) {
+        pstep(c, St1, St0, F0, F1);
+        pstep(c + 1, St2, St1, F1, F0);
+        pstep(c + 2, St0, St2, F0, F1);
+        pstep(c + 3, St1, St0, F1, F0);
+        pstep(c + 4, St2, St1, F0, F1);
+        pstep(c + 5, St0, St2, F1, F0);
+      }
     }
     wait_vmcnt<0>();  // no DMA may land after the workgroup ends
   } else if (total > 0) {

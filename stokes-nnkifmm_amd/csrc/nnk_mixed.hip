@@ -823,14 +823,15 @@ void Fmm::launch_m2l_pairs16(const LevelPlan &P, int depth, hipStream_t s) {
 bool Fmm::dma_ok(const LevelPlan &P, int split, int depth) const {
   if (split < 1) return false;
   if (depth == 5) return P.mx_tn3 && P.mx_maxnb <= dma_max_blocks(3);
-  if (depth == 6) return P.mx_tn3 && P.mx_maxnb <= dma_max_blocks(2, 4);
+  if (depth == 6 || depth == 8) return P.mx_tn3 && P.mx_maxnb <= dma_max_blocks(2, 4);
   return !f16_ || (P.mx_maxnb + split - 1) / split <= kDmaMaxBlocks;
 }
 
 // M2L of level l in the mixed form with register-ring depth `depth` (2 or
 // 3; 4 selects the DMA-staged kernel, 5 the DMA-staged kernel with 64 x 48
-// wave tiles, 6 the DMA-staged kernel with four LDS stages, 7 the
-// DMA-staged kernel with fragment prefetch) and split-K
+// wave tiles, 6 the DMA-staged kernel with four LDS stages, 7 and 8 the
+// DMA-staged kernel with fragment prefetch, three and four LDS stages) and
+// split-K
 // factor `split` (the part buffer must hold it)
 void Fmm::launch_m2l_mixed(int l, int depth, int split, hipStream_t s) {
   LevelPlan &P = plans_[l];
@@ -857,7 +858,8 @@ void Fmm::launch_m2l_mixed(int l, int depth, int split, hipStream_t s) {
     throw Error(NNK_ESTATE, "mixed M2L: too many operator blocks for the DMA-staged kernel");
   if (depth >= 5 && np_ != 2) throw Error(NNK_ESTATE, "mixed M2L: this DMA-staged kernel needs two parts");
   if (f16_) {
-    if (depth == 7) launch(k_m2l_dma<2, 1, true, 2, 3, true>);
+    if (depth == 8) launch(k_m2l_dma<2, 1, true, 2, 4, true>);
+    else if (depth == 7) launch(k_m2l_dma<2, 1, true, 2, 3, true>);
     else if (depth == 6) launch(k_m2l_dma<2, 1, true, 2, 4>);
     else if (depth == 5) launch(k_m2l_dma<2, 1, true, 3>);
     else if (depth == 4) launch(k_m2l_dma<2, 1, true>);
@@ -867,7 +869,8 @@ void Fmm::launch_m2l_mixed(int l, int depth, int split, hipStream_t s) {
     if (depth == 4) launch(k_m2l_dma<3, 1, false>);
     else launch(k_m2l_split<3, 2, 1, false>);  // a third register stage would spill
   } else {
-    if (depth == 7) launch(k_m2l_dma<2, 1, false, 2, 3, true>);
+    if (depth == 8) launch(k_m2l_dma<2, 1, false, 2, 4, true>);
+    else if (depth == 7) launch(k_m2l_dma<2, 1, false, 2, 3, true>);
     else if (depth == 6) launch(k_m2l_dma<2, 1, false, 2, 4>);
     else if (depth == 5) launch(k_m2l_dma<2, 1, false, 3>);
     else if (depth == 4) launch(k_m2l_dma<2, 1, false>);

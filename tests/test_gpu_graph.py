@@ -74,8 +74,8 @@ def test_autotuned_launch_shapes(nnk, precision):
                                            16) == 0
     assert (split[2:] >= 0).all() and split[2] >= 1
     # FP64: 0 class tiles, 1 per-offset pair jobs; mixed: ring depth 2/3,
-    # DMA-staged 4-7, per-offset pair jobs 12-14, 17
-    assert set(depth[split > 0]) <= ({0, 1} if precision == "fp64" else {2, 3, 4, 5, 6, 7, 12, 13, 14, 17})
+    # DMA-staged 4-8, per-offset pair jobs 12-14, 17
+    assert set(depth[split > 0]) <= ({0, 1} if precision == "fp64" else {2, 3, 4, 5, 6, 7, 8, 12, 13, 14, 17})
     threads = ctypes.c_int32()
     assert nnk.lib().nnk_fmm_leaf_threads(f._handle().h, ctypes.byref(threads)) == 0
     assert threads.value in (128, 256, 512)

@@ -123,7 +123,8 @@ def test_mixedh_m2l_kernels_agree_bitwise(nnk, precision, split):
     kernel (depth 4) and, for the two-part forms, by the DMA-staged kernel
     with fragment prefetch (depth 7: the LDS reads of a chunk overlap the
     products of the previous one), by the DMA-staged kernel
-    with four LDS stages (depth 6) and with 64 x 48 wave tiles (depth 5),
+    with four LDS stages (depth 6; with fragment prefetch, 8) and with
+    64 x 48 wave tiles (depth 5),
     both on the levels run as class jobs (depth 4 elsewhere): the same
     products in the same order, so the matvecs are bit-identical for one
     split-K factor (with split-K, depth 5's 192-column tiles group the
@@ -153,7 +154,9 @@ def test_mixedh_m2l_kernels_agree_bitwise(nnk, precision, split):
         assert (chk_s[levels] == split).all() and (chk_d[levels] == depth).all()
         out[depth] = f.computeVel(q)
     if precision != "mixed3":
-        for dk in (6, 5):   # four LDS stages; 64 x 48 wave tiles (class-job levels; others keep depth 4)
+        # four LDS stages, without and with fragment prefetch; 64 x 48 wave
+        # tiles (class-job levels; others keep depth 4)
+        for dk in (6, 8, 5):
             wide = []
             for l in levels:
                 s = np.zeros(16, np.int32)
@@ -164,7 +167,7 @@ def test_mixedh_m2l_kernels_agree_bitwise(nnk, precision, split):
             print(f"[m2l] {precision} split {split}: depth {dk} at levels {wide}", flush=True)
             assert wide
             out[dk] = f.computeVel(q)
-    for depth in depths[1:] + ((6,) if 6 in out else ()):
+    for depth in depths[1:] + ((6, 8) if 6 in out else ()):
         np.testing.assert_array_equal(out[depth], out[2])
     if 5 in out:
         # 192-column tiles split their (larger) active block lists at other
@@ -244,7 +247,7 @@ def test_launch_config_rejects_what_a_form_cannot_run(nnk):
     assert setcfg(fb, lb, 1, 3) != 0            # three parts: no third register stage
     assert setcfg(fb, lb, 1, 5) != 0            # three parts: no 64 x 48 wave tiles
     assert setcfg(fb, lb, 1, 7) != 0            # three parts: no fragment prefetch
-    assert setcfg(fh, lh, 1, 8) != 0
+    assert setcfg(fh, lh, 1, 9) != 0
     assert setcfg(fh, lh, 0, 4) == 0            # split 0: level left as it is
     assert setcfg(fh, lh, 65, 3) != 0
     assert setcfg(fh, 0, 1, 3) != 0             # level without M2L
