@@ -380,8 +380,10 @@ def spheroid_solve(nnk):
     (FP64 Arnoldi, mixed-form inner solves to 1e-2)."""
     import torch
     import importlib as il
+    nvtx = torch.cuda.nvtx   # ROCTx ranges: a kernel trace splits by them
     geo = il.import_module("stokes-nnkifmm_amd.geometries")
     mob = il.import_module("stokes-nnkifmm_amd.mobility")
+    nvtx.range_push("spheroids: set-up")
     t0 = time.perf_counter()
     sw = geo.spheroid_suspension()
     t_geo = time.perf_counter() - t0
@@ -403,10 +405,14 @@ def spheroid_solve(nnk):
     out["nearest_matrix_whole_system"] = {
         "coarse": int(len(pts)), "fine": int(len(fine_all)), "wall_s": round(t_nn, 3), "nnz": int(NNg.nnz),
         "pairs_brute_force": float(len(pts)) * float(len(fine_all))}
+    torch.cuda.synchronize()
+    nvtx.range_pop()
+    nvtx.range_push("spheroids: FP64 solve")
     t0 = time.perf_counter()
     x64 = mob.mobility_gmres(sw, B, BT, A1, A2, b, 1e-6, 400, True)
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
+    nvtx.range_pop()
     info = mob.last_solve
     # roofline of the solve: per iteration one coronaRes-6 and one
     # coronaRes-4 matvec at their ideal (roofline-sum) times; the leaf-block
@@ -421,10 +427,12 @@ def spheroid_solve(nnk):
                                        "counted": "iterations + 2 applications of the res-6 and res-4 "
                                                   "FP64 matvecs (roofline sum over kernel classes); "
                                                   "leaf-block solves, SpMVs and Arnoldi not counted"}}
+    nvtx.range_push("spheroids: refinement solve")
     t0 = time.perf_counter()
     xr = mob.mobility_refined(sw, B, BT, A1, A2, b, 1e-6, 400, True, innerTol=1e-3)
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
+    nvtx.range_pop()
     info = mob.last_solve
     r = nnk.mobilityMatrixMulti(xr, sw, B, BT) - b
     nb = 6 * sw.swimmerNo()
@@ -432,14 +440,27 @@ def spheroid_solve(nnk):
         "wall_s": round(dt, 3), "inner_iterations": info["iter"], "outer": info["outer"],
         "relres_fp64": float(np.linalg.norm(r) / np.linalg.norm(b)),
         "body_velocity_diff_vs_fp64": float(np.linalg.norm(xr[-nb:] - x64[-nb:]) / np.linalg.norm(x64[-nb:]))}
+    nvtx.range_push("spheroids: flexible GMRES solve")
     t0 = time.perf_counter()
     xf = mob.mobility_refined(sw, B, BT, A1, A2, b, 1e-6, 400, True, innerTol=1e-2, outer="fgmres")
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
+    nvtx.range_pop()
     info = mob.last_solve
     r = nnk.mobilityMatrixMulti(xf, sw, B, BT) - b
+    # roofline of the solve: each inner iteration one mixed coronaRes-6 and
+    # one mixed coronaRes-4 matvec, each outer iteration the FP64 pair, at
+    # their ideal (roofline-sum) times (a lower bound, as for the FP64 solve)
+    inner6, inner4 = sw._mobility.inner[("mixed", 1e-8, True)]
+    m6, m4 = handle_ideal_ms(lib, inner6, "mixed"), handle_ideal_ms(lib, inner4, "mixed")
+    ideal_f = (info["iter"] * (m6 + m4) + (info["outer"] + 2) * (i6 + i4)) * 1e-3
     out["mixed_fgmres_precon"] = {
         "wall_s": round(dt, 3), "inner_iterations": info["iter"], "outer": info["outer"], "inner_tol": 1e-2,
+        "roofline": {"frac": round(ideal_f / dt, 4), "ideal_s": round(ideal_f, 4),
+                     "matvec_ideal_ms": {"mixed_coronaRes6": m6, "mixed_coronaRes4": m4},
+                     "counted": "inner iterations x the mixed res-6 and res-4 matvecs + (outer + 2) x the "
+                                "FP64 pair (roofline sum over kernel classes); leaf-block solves, SpMVs and "
+                                "Arnoldi not counted"},
         "relres_fp64": float(np.linalg.norm(r) / np.linalg.norm(b)),
         "body_velocity_diff_vs_fp64": float(np.linalg.norm(xf[-nb:] - x64[-nb:]) / np.linalg.norm(x64[-nb:]))}
     return out

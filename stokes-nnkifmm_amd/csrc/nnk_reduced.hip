@@ -2,10 +2,11 @@
 // for every target leaf, solve K(leaf sources -> leaf targets) x = v, or in
 // NEAREST mode (K(fine -> targets) kron(NN_leaf, I3)) x = v (ReducedSolve.m
 // :170-187).  The 3n x 3n blocks are formed and LU-factored with partial
-// pivoting once per tree on the GPU (one workgroup per leaf); each solve is
-// a pair of triangular substitutions.  MATLAB's mldivide would use Cholesky
-// on the symmetric non-NEAREST blocks; LU gives the same solution up to
-// rounding.
+// pivoting once per tree on the GPU (one workgroup per leaf), and each
+// block's inverse is formed from its factors; each solve is then one
+// block-diagonal matrix-vector product.  MATLAB's mldivide would use
+// Cholesky on the symmetric non-NEAREST blocks; both give the solution up
+// to rounding (relative error of order cond(A) times the FP64 epsilon).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -217,81 +218,53 @@ __global__ void k_red_perm(RedArgs a, int nleaves) {
   }
 }
 
-constexpr int kRedSolveThreads = 512;
+// The inverse of each leaf block from its LU factors (set-up only): thread
+// j of workgroup (leaf, column group) computes column j of
+// A^-1 = U^-1 L^-1 P e_j by forward and back substitution, in the
+// row-major working block W (W[i n3 + j]), where the lanes of a wavefront
+// own consecutive columns, so every access of W is coalesced and every
+// access of the factors is the same address across the lanes.
+constexpr int kRedInvCols = 64;
+__global__ __launch_bounds__(kRedInvCols) void k_red_inv(RedArgs a, double *__restrict__ W) {
+  const int li = blockIdx.x;
+  const int n = a.leaves[li];
+  const int n3 = 3 * (a.tgt_ptr[n + 1] - a.tgt_ptr[n]);
+  const int j = blockIdx.y * kRedInvCols + threadIdx.x;
+  if (j >= n3) return;  // no barriers below
+  const double *A = a.blk + a.off[li];  // column-major factors: A[k n3 + i] = LU[i][k]
+  const int32_t *perm = a.perm + a.poff[li];
+  double *X = W + a.off[li];
+  for (int i = 0; i < n3; i++) {  // unit lower triangle
+    double t = perm[i] == j ? 1.0 : 0.0;
+    for (int k = 0; k < i; k++) t -= A[(size_t)k * n3 + i] * X[(size_t)k * n3 + j];
+    X[(size_t)i * n3 + j] = t;
+  }
+  for (int i = n3 - 1; i >= 0; i--) {  // upper triangle
+    double t = X[(size_t)i * n3 + j];
+    for (int k = i + 1; k < n3; k++) t -= A[(size_t)k * n3 + i] * X[(size_t)k * n3 + j];
+    X[(size_t)i * n3 + j] = t / A[(size_t)i * n3 + i];
+  }
+}
 
-// x = A^-1 v for each leaf (v, x indexed by target ordinal, 3 per point)
-// with the column-major LU factors (Ac[k * n3 + i] = LU[i][k]).  Blocked
-// substitution: each 64-row diagonal block is solved by wavefront 0 alone,
-// passing the solved values between lanes with shuffles (no barrier inside
-// a block); then all threads of the workgroup update the remaining rows
-// (two barriers per block instead of two per row).
-__global__ __launch_bounds__(kRedSolveThreads) void k_red_solve(RedArgs a, const double *__restrict__ v,
-                                                   double *__restrict__ x) {
+// x = A^-1 v for each leaf with the column-major inverse
+// (Inv[j n3 + i] = (A^-1)_ij): one thread per row, the leaf's right-hand
+// side in LDS, the columns read in order (coalesced across the rows), the
+// sum in column order.
+constexpr int kRedApplyThreads = 256;
+__global__ __launch_bounds__(kRedApplyThreads) void k_red_apply(RedArgs a, const double *__restrict__ v,
+                                                                double *__restrict__ x) {
   extern __shared__ double b[];
   const int li = blockIdx.x;
   const int n = a.leaves[li];
   const int tb = a.tgt_ptr[n], nt = a.tgt_ptr[n + 1] - tb;
   const int n3 = 3 * nt;
-  const double *A = a.blk + a.off[li];
-  const int32_t *perm = a.perm + a.poff[li];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  for (int i = tid; i < n3; i += kRedSolveThreads) {
-    const int r = perm[i];
-    b[i] = v[3 * (size_t)a.tgt_out[tb + r / 3] + r % 3];
-  }
+  const double *Inv = a.blk + a.off[li];
+  for (int i = threadIdx.x; i < n3; i += kRedApplyThreads) b[i] = v[3 * (size_t)a.tgt_out[tb + i / 3] + i % 3];
   __syncthreads();
-  // unit lower triangle, top down
-  for (int kb = 0; kb < n3; kb += 64) {
-    const int ke = min(kb + 64, n3);
-    if (wave == 0) {  // lane l holds row kb + l of the diagonal block
-      const int r = kb + lane;
-      double L[64];
-#pragma unroll
-      for (int kk = 0; kk < 64; kk++) L[kk] = (kb + kk < ke && r < ke) ? A[(size_t)(kb + kk) * n3 + r] : 0.0;
-      double br = r < ke ? b[r] : 0.0;
-#pragma unroll
-      for (int kk = 0; kk < 64; kk++) {
-        const double bk = __shfl(br, kk);
-        if (lane > kk) br -= L[kk] * bk;
-      }
-      if (r < ke) b[r] = br;
-    }
-    __syncthreads();
-    for (int i = ke + tid; i < n3; i += kRedSolveThreads) {
-      double t = b[i];
-      for (int k = kb; k < ke; k++) t -= A[(size_t)k * n3 + i] * b[k];
-      b[i] = t;
-    }
-    __syncthreads();
-  }
-  // upper triangle, bottom up
-  for (int ke = n3; ke > 0; ke -= 64) {
-    const int kb = max(ke - 64, 0);
-    if (wave == 0) {
-      const int r = kb + lane;
-      double U[64];
-#pragma unroll
-      for (int kk = 0; kk < 64; kk++) U[kk] = (kb + kk < ke && r < ke) ? A[(size_t)(kb + kk) * n3 + r] : 0.0;
-      double br = r < ke ? b[r] : 0.0;
-#pragma unroll
-      for (int kk = 63; kk >= 0; kk--) {
-        if (kb + kk >= ke) continue;                    // the same for the whole wavefront
-        const double xk = __shfl(br, kk) / __shfl(U[kk], kk);   // lane kk holds the diagonal
-        if (lane == kk) br = xk;
-        else if (lane < kk) br -= U[kk] * xk;
-      }
-      if (r < ke) {
-        b[r] = br;
-        x[3 * (size_t)a.tgt_out[tb + r / 3] + r % 3] = br;
-      }
-    }
-    __syncthreads();
-    for (int i = tid; i < kb; i += kRedSolveThreads) {
-      double t = b[i];
-      for (int k = kb; k < ke; k++) t -= A[(size_t)k * n3 + i] * b[k];
-      b[i] = t;
-    }
-    __syncthreads();
+  for (int i = threadIdx.x; i < n3; i += kRedApplyThreads) {
+    double acc = 0.0;
+    for (int jj = 0; jj < n3; jj++) acc += Inv[(size_t)jj * n3 + i] * b[jj];
+    x[3 * (size_t)a.tgt_out[tb + i / 3] + i % 3] = acc;
   }
 }
 
@@ -336,7 +309,22 @@ void Fmm::reduced_factor() {
     hipLaunchKernelGGL(k_red_transpose, dim3(n_tleaves_), dim3(256), 0, 0, a);
     hipLaunchKernelGGL(k_red_perm, dim3((n_tleaves_ + 63) / 64), dim3(64), 0, 0, a, n_tleaves_);
     NNK_HIP(hipGetLastError());
+    // the blocks' inverses replace their factors: each solve is then one
+    // pass over them at HBM bandwidth instead of a chain of substitutions
+    DBuf W;
+    W.alloc(sizeof(double) * std::max<int64_t>(off.back(), 1));
+    hipLaunchKernelGGL(k_red_inv, dim3(n_tleaves_, (red_maxn_ + kRedInvCols - 1) / kRedInvCols), dim3(kRedInvCols),
+                       0, 0, a, W.as<double>());
+    NNK_HIP(hipGetLastError());
+    RedArgs w = a;
+    w.blk = W.as<double>();
+    hipLaunchKernelGGL(k_red_transpose, dim3(n_tleaves_), dim3(256), 0, 0, w);
+    NNK_HIP(hipGetLastError());
+    NNK_HIP(hipDeviceSynchronize());
+    red_blk_ = std::move(W);
   }
+  red_piv_ = DBuf();
+  red_perm_ = DBuf();
   NNK_HIP(hipDeviceSynchronize());
   red_ready_ = true;
 }
@@ -368,7 +356,7 @@ void Fmm::reduced_solve(const double *d_v, double *d_f, hipStream_t s) {
   if (!red_ready_) reduced_factor();
   if (!n_tleaves_) return;
   RedArgs a = red_args();
-  hipLaunchKernelGGL(k_red_solve, dim3(n_tleaves_), dim3(kRedSolveThreads), sizeof(double) * red_maxn_, s, a, d_v,
+  hipLaunchKernelGGL(k_red_apply, dim3(n_tleaves_), dim3(kRedApplyThreads), sizeof(double) * red_maxn_, s, a, d_v,
                      d_f);
   NNK_HIP(hipGetLastError());
 }
