@@ -451,7 +451,7 @@ def spheroid_solve(nnk):
     # roofline of the solve: each inner iteration one mixed coronaRes-6 and
     # one mixed coronaRes-4 matvec, each outer iteration the FP64 pair, at
     # their ideal (roofline-sum) times (a lower bound, as for the FP64 solve)
-    inner6, inner4 = sw._mobility.inner[("mixed", 1e-8, True)]
+    inner6, inner4 = sw._mobility.inner[("mixed", 1e-8, True, 6)]
     m6, m4 = handle_ideal_ms(lib, inner6, "mixed"), handle_ideal_ms(lib, inner4, "mixed")
     ideal_f = (info["iter"] * (m6 + m4) + (info["outer"] + 2) * (i6 + i4)) * 1e-3
     out["mixed_fgmres_precon"] = {

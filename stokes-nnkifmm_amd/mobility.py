@@ -148,31 +148,37 @@ def mobility_gmres(swimmers, B, BT, A1, A2, rhs, tol, maxIter, precon):
 
 
 def mobility_refined(swimmers, B, BT, A1, A2, rhs, tol, maxIter, precon, innerTol=1e-4, svdTol=1e-8,
-                     innerPrecision="mixed", outer="refine"):
+                     innerPrecision="mixed", outer="refine", innerRes=None):
     """The solve of mobility_gmres around a mixed-precision inner solver (no
     counterpart in the reference): GMRES on the mixed-precision KIFMM of the
     same tree to innerTol (preconditioned with mixed-precision coronaRes-4
     operators too) inside an FP64 outer iteration, either iterative
     refinement (outer="refine": FP64 residual, inner correction, add) or
     flexible GMRES (outer="fgmres": FP64 Arnoldi on the FP64 operator, each
-    basis vector preconditioned by an inner solve).  The answer meets tol on
-    the unpreconditioned FP64 system.  Records flag, relres, iter (inner
-    iterations) and outer in mobility.last_solve."""
+    basis vector preconditioned by an inner solve).  innerRes sets the
+    corona resolution of the inner operator (default: the FP64 operator's);
+    a coarser one makes every inner iteration cheaper, and the outer
+    iteration, on the FP64 operator, still decides convergence.  The answer
+    meets tol on the unpreconditioned FP64 system.  Records flag, relres,
+    iter (inner iterations) and outer in mobility.last_solve."""
     if outer not in ("refine", "fgmres"):
         raise ValueError(f"mobility_refined: unknown outer iteration {outer!r}")
     from .kifmm import _Handle
     m = _mobility(swimmers, B, BT, A1, A2, precon=bool(precon))
     fmm = swimmers.FMM
     a = fmm.arguments
-    key = (innerPrecision, float(svdTol), bool(precon))
-    inner = getattr(m, "inner", {}).get(key)
+    res = int(innerRes) if innerRes is not None else int(a["coronaRes"])
+    key = (innerPrecision, float(svdTol), bool(precon), res)
+    if not hasattr(m, "inner"):
+        m.inner = {}
+    inner = m.inner.get(key)
     if inner is None:
         tune = bool(a.get("autotune"))
-        inner = (_Handle(fmm.tree, a["kernelPar"][0], a["kernelPar"][1], a["coronaRes"], a["coronaShells"],
+        inner = (_Handle(fmm.tree, a["kernelPar"][0], a["kernelPar"][1], res, a["coronaShells"],
                          innerPrecision, svdTol, autotune=tune),
                  _Handle(fmm.tree, a["kernelPar"][0], a["kernelPar"][1], 4, a["coronaShells"],
                          innerPrecision, svdTol, autotune=tune) if precon else None)
-        m.inner = {key: inner}
+        m.inner[key] = inner
     b = c64(rhs).ravel()
     x = np.empty_like(b)
     flag = np.zeros(1, np.int32)
