@@ -131,7 +131,7 @@ struct LevelPlan {
   int n_x = 0;
   DBuf x_nodes;               // nodes with a non-empty X list
   // mixed form
-  GemmBatch mx_m2m, mx_upP, mx_l2l, mx_xP, mx_downU, mx_wV;
+  GemmBatch mx_m2m, mx_l2l, mx_xP;
   int mx_m2m_split = 1;       // as m2m_split, for the mixed form's M2M
   DBuf mx_m2m_slot;
   DBuf mx_m2l;                // GemmJobBf16[8]
@@ -386,6 +386,12 @@ class Fmm {
   DBuf pairbuf_;                    // per-offset M2L: one product column per V-list pair
   size_t pairbuf_elems_ = 0;
   int n_upall_ = 0;
+  // mixed forms: the per-node expansions of every level, one job per level
+  // in one launch each (their nodes do not depend on other levels' results
+  // at the point where they run): up expansion of the leaves with sources,
+  // equivalent densities of the W-list nodes, down expansion of the target
+  // leaves (FP64 down expansion only)
+  GemmBatch mx_upP_all_, mx_wV_all_, mx_downU_all_;
   void reduced_factor();
   bool red_ready_ = false;
   DBuf red_blk_, red_off_, red_piv_, red_perm_, red_poff_, red_ord2loc_;

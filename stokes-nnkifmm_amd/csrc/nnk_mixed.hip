@@ -427,6 +427,26 @@ void Fmm::build_mixed() {
 void Fmm::plan_mixed() {
   const size_t matRR = (size_t)KpadR_ * MpadR_;
   std::vector<int32_t> upall;
+  // one job per level of a batch over all levels
+  std::vector<GemmJob> jobs_upP, jobs_wV, jobs_downU;
+  mx_upP_all_ = GemmBatch();
+  mx_wV_all_ = GemmBatch();
+  mx_downU_all_ = GemmBatch();
+  auto jobl = [&](GemmBatch &B, std::vector<GemmJob> &jobs, const double *A, const std::vector<int32_t> &g) {
+    if (g.empty()) return;
+    GemmJob J{};
+    B.owned.emplace_back();
+    B.owned.back().upload(std::vector<const double *>{A});
+    J.A = B.owned.back().as<const double *>();
+    B.owned.emplace_back();
+    B.owned.back().upload(g);
+    J.gidx = B.owned.back().as<int32_t>();
+    J.cidx = J.gidx;  // the same node list in and out
+    J.nb = 1;
+    J.N = (int)g.size();
+    jobs.push_back(J);
+    B.maxN = std::max(B.maxN, J.N);
+  };
   auto job1 = [&](GemmBatch &B, const double *A, const std::vector<int32_t> &g,
                   const std::vector<int32_t> &c) {
     if (g.empty()) return;
@@ -478,9 +498,9 @@ void Fmm::plan_mixed() {
         reserve_part((size_t)P.mx_m2m_split * N * RP_);
       }
     }
-    job1(P.mx_upP, X.Pup.as<double>(), H.up_leaves, H.up_leaves);
-    job1(P.mx_wV, X.Vr.as<double>(), H.wnodes, H.wnodes);
-    job1(P.mx_downU, X.Ur.as<double>(), H.tleaves, H.tleaves);
+    jobl(mx_upP_all_, jobs_upP, X.Pup.as<double>(), H.up_leaves);
+    jobl(mx_wV_all_, jobs_wV, X.Vr.as<double>(), H.wnodes);
+    if (!f16_) jobl(mx_downU_all_, jobs_downU, X.Ur.as<double>(), H.tleaves);
     {
       std::vector<int32_t> xs;
       for (int32_t n : H.xnodes) xs.push_back(n - 1);
@@ -724,6 +744,13 @@ void Fmm::plan_mixed() {
   }
   n_upall_ = (int)upall.size();
   upall_.upload(upall);
+  auto fin = [](GemmBatch &B, const std::vector<GemmJob> &jobs) {
+    B.njobs = (int)jobs.size();
+    if (B.njobs) B.jobs.upload(jobs);
+  };
+  fin(mx_upP_all_, jobs_upP);
+  fin(mx_wV_all_, jobs_wV);
+  fin(mx_downU_all_, jobs_downU);
 }
 
 // whether the DMA-staged M2L kernel can run level plan P with split-K
@@ -926,10 +953,13 @@ void Fmm::apply_mixed(const double *d_f, double *d_u, hipStream_t s) {
     mark(1, false, s);
   }
   mark(2, true, s);
+  // the leaves' up expansions (every level at once), then M2M level by
+  // level: a parent's children at the level below are leaves or parents
+  // done in the previous step
+  gemm_dims(mx_upP_all_, chk, K, c, RP_, 0, s, R, K, MpadR_, Kpad_);
   for (int l = maxlev_; l >= 2; l--) {
     LevelPlan &P = plans_[l];
     gemm_dims(P.mx_m2m, c, RP_, c, RP_, 0, s, R, R, MpadR_, KpadR_, P.mx_m2m_split, &P.mx_m2m_slot);
-    gemm_dims(P.mx_upP, chk, K, c, RP_, 0, s, R, K, MpadR_, Kpad_);
   }
   if (n_upall_ && m16()) {
     const int64_t tot = (int64_t)n_upall_ * KCm_ * 32;
@@ -946,8 +976,7 @@ void Fmm::apply_mixed(const double *d_f, double *d_u, hipStream_t s) {
     }
     NNK_HIP(hipGetLastError());
   }
-  for (int l = maxlev_; l >= 2; l--)
-    gemm_dims(plans_[l].mx_wV, c, RP_, uppot_.as<double>(), K, 0, s, K, R, Mpad_, KpadR_);
+  gemm_dims(mx_wV_all_, c, RP_, uppot_.as<double>(), K, 0, s, K, R, Mpad_, KpadR_);
   mark(2, false, s);
   for (int l = 2; l <= maxlev_; l++) {
     LevelPlan &P = plans_[l];
@@ -971,14 +1000,17 @@ void Fmm::apply_mixed(const double *d_f, double *d_u, hipStream_t s) {
       gemm_dims(P.mx_xP, chk, K, d, RP_, 1, s, R, K, MpadR_, Kpad_);
       mark(5, false, s);
     }
-    mark(6, true, s);
     if (f16_) {  // U_R is orthonormal: the down expansion keeps its accuracy in split FP16
+      mark(6, true, s);
       split_h(P.dn_rows, P.n_dn_rows, d, RP_, R, KCm_, dhat_x_.as<uint16_t>(), dun_lv_.as<double>() + l, s);
       launch_h16(P.hx_dn, P.hx_dn_n, P.hx_dn_maxN, K, KCm_, dhat_x_.as<uint16_t>(), downpot_.as<double>(), K,
                  dun_lv_.as<double>() + l, 0, s);
-    } else {
-      gemm_dims(P.mx_downU, d, RP_, downpot_.as<double>(), K, 0, s, K, R, Mpad_, KpadR_);
+      mark(6, false, s);
     }
+  }
+  if (!f16_) {  // FP64 down expansion of the target leaves of every level at once
+    mark(6, true, s);
+    gemm_dims(mx_downU_all_, d, RP_, downpot_.as<double>(), K, 0, s, K, R, Mpad_, KpadR_);
     mark(6, false, s);
   }
   mark(7, true, s);
