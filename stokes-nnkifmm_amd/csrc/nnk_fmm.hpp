@@ -392,6 +392,9 @@ class Fmm {
   // equivalent densities of the W-list nodes, down expansion of the target
   // leaves (FP64 down expansion only)
   GemmBatch mx_upP_all_, mx_wV_all_, mx_downU_all_;
+  // mixed forms: the X-list owners of every level, for one P2L launch
+  DBuf x_all_;
+  int n_x_all_ = 0;
   void reduced_factor();
   bool red_ready_ = false;
   DBuf red_blk_, red_off_, red_piv_, red_perm_, red_poff_, red_ord2loc_;

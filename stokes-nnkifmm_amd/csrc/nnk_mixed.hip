@@ -429,6 +429,7 @@ void Fmm::plan_mixed() {
   std::vector<int32_t> upall;
   // one job per level of a batch over all levels
   std::vector<GemmJob> jobs_upP, jobs_wV, jobs_downU;
+  std::vector<int32_t> xall;  // node ids (from 1), as the per-level X lists
   mx_upP_all_ = GemmBatch();
   mx_wV_all_ = GemmBatch();
   mx_downU_all_ = GemmBatch();
@@ -505,6 +506,7 @@ void Fmm::plan_mixed() {
       std::vector<int32_t> xs;
       for (int32_t n : H.xnodes) xs.push_back(n - 1);
       job1(P.mx_xP, X.Pdn.as<double>(), xs, xs);
+      xall.insert(xall.end(), H.xnodes.begin(), H.xnodes.end());
     }
     // split-BF16 M2L jobs (one per child class) and FP64 L2L jobs
     std::vector<GemmJobBf16> mj;
@@ -748,6 +750,8 @@ void Fmm::plan_mixed() {
     B.njobs = (int)jobs.size();
     if (B.njobs) B.jobs.upload(jobs);
   };
+  n_x_all_ = (int)xall.size();
+  x_all_.upload(xall);
   fin(mx_upP_all_, jobs_upP);
   fin(mx_wV_all_, jobs_wV);
   fin(mx_downU_all_, jobs_downU);
@@ -978,6 +982,11 @@ void Fmm::apply_mixed(const double *d_f, double *d_u, hipStream_t s) {
   }
   gemm_dims(mx_wV_all_, c, RP_, uppot_.as<double>(), K, 0, s, K, R, Mpad_, KpadR_);
   mark(2, false, s);
+  if (n_x_all_) {  // P2L of every level (the up pass is done with the check potentials)
+    mark(5, true, s);
+    launch_surf(x_all_.as<int32_t>(), n_x_all_, xptr_.as<int64_t>(), xidx_.as<int32_t>(), 0, 0, chk, s);
+    mark(5, false, s);
+  }
   for (int l = 2; l <= maxlev_; l++) {
     LevelPlan &P = plans_[l];
     if (P.mx_m2l_jobs) {
@@ -996,7 +1005,6 @@ void Fmm::apply_mixed(const double *d_f, double *d_u, hipStream_t s) {
     mark(4, false, s);
     if (P.n_x) {
       mark(5, true, s);
-      launch_surf(P.x_nodes.as<int32_t>(), P.n_x, xptr_.as<int64_t>(), xidx_.as<int32_t>(), 0, 0, chk, s);
       gemm_dims(P.mx_xP, chk, K, d, RP_, 1, s, R, K, MpadR_, Kpad_);
       mark(5, false, s);
     }
