@@ -271,8 +271,10 @@ def run_form(nnk, lib, tree, fd, args, eps, mu, precision, res, svd_tol):
                 ([f"k_m2l_dma<{np_},1,{f16}>"] if used & {4, 14} else []) + \
                 ([f"k_m2l_dma<{np_},1,{f16},3> (64 x 48 wave tiles)"] if 5 in used else []) + \
                 ([f"k_m2l_dma<{np_},1,{f16},2,4> (four LDS stages)"] if 6 in used else []) + \
-                (["per-offset pair jobs + k_pair_sum at some levels"] if used & {12, 13, 14} else [])
-        kern = " / ".join(names) + f" (M2L, {what} MFMA{', LDS-DMA staged' if used <= {4, 5, 6} else ''})"
+                ([f"k_m2l_dma<{np_},1,{f16},2,3,true> (fragment prefetch)"] if used & {7, 17} else []) + \
+                ([f"k_m2l_dma<{np_},1,{f16},2,4,true> (fragment prefetch, four LDS stages)"] if 8 in used else []) + \
+                (["per-offset pair jobs + k_pair_sum at some levels"] if used & {12, 13, 14, 17} else [])
+        kern = " / ".join(names) + f" (M2L, {what} MFMA{', LDS-DMA staged' if used <= {4, 5, 6, 7, 8} else ''})"
     m2l_ms = ms[3] / args.steps
     achieved = flops / (m2l_ms * 1e-3) / 1e12 if m2l_ms > 0 else 0.0
     rec = {"precision": precision, "coronaRes": res, "svd_tol": svd_tol if precision != "fp64" else None,
