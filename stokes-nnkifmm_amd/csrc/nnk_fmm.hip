@@ -1364,6 +1364,14 @@ void Fmm::plan() {
     hl_[l].dn = dn;
     hl_[l].xnodes = xs;
   }
+  {
+    std::vector<int32_t> xall;  // node ids (from 1), level by level
+    for (int l = 2; l <= maxlev_; l++) xall.insert(xall.end(), hl_[l].xnodes.begin(), hl_[l].xnodes.end());
+    n_x_all_ = (int)xall.size();
+    x_all_.upload(xall);
+    if (n_x_all_ && prec_ == 0) xchk_.alloc(sizeof(double) * (size_t)nc * K_);
+    else xchk_ = DBuf();
+  }
   tmp_.alloc(sizeof(double) * maxn * K_);
   // remaining work counts
   std::vector<int64_t> srcn(nc + 2, 0), tgtn(nc + 2, 0);
@@ -1430,6 +1438,13 @@ void Fmm::strengths(const double *d_f, hipStream_t s) {
                      tree_.nearest ? nullptr : src_from_.as<int32_t>(), ctr_ptr_.as<int64_t>(),
                      ctr_col_.as<int32_t>(), ctr_w_.as<double>(), d_f, src_q_.as<double>());
   NNK_HIP(hipGetLastError());
+}
+
+// C[node] += X[node] for the listed node ids (from 1), rows of K
+__global__ __launch_bounds__(256) void k_add_rows(const int32_t *__restrict__ nodes, const double *__restrict__ X,
+                                                  double *__restrict__ C, int K) {
+  const size_t o = (size_t)(nodes[blockIdx.x] - 1) * K;
+  for (int i = threadIdx.x; i < K; i += 256) C[o + i] += X[o + i];
 }
 
 void Fmm::launch_surf(const int32_t *nodes, int n, const int64_t *lptr, const int32_t *lidx, int down,
@@ -1682,6 +1697,12 @@ void Fmm::apply_impl(const double *d_f, double *d_u, hipStream_t s) {
     gemm(P.up_solve2, tmp, K_, up, K_, 0, s);
   }
   mark(2, false, s);
+  if (n_x_all_) {  // P2L of every level into its own buffer
+    mark(5, true, s);
+    launch_surf(x_all_.as<int32_t>(), n_x_all_, xptr_.as<int64_t>(), xidx_.as<int32_t>(), 0, 0,
+                xchk_.as<double>(), s);
+    mark(5, false, s);
+  }
   for (int l = 2; l <= maxlev_; l++) {
     LevelPlan &P = plans_[l];
     if (!P.n_down) continue;
@@ -1692,9 +1713,11 @@ void Fmm::apply_impl(const double *d_f, double *d_u, hipStream_t s) {
     mark(4, true, s);
     gemm(P.l2l, dn, K_, chk, K_, 1, s);
     mark(4, false, s);
-    if (P.n_x) {
+    if (P.n_x) {  // chk += the P2L check potentials (as one addition each)
       mark(5, true, s);
-      launch_surf(P.x_nodes.as<int32_t>(), P.n_x, xptr_.as<int64_t>(), xidx_.as<int32_t>(), 0, 1, chk, s);
+      hipLaunchKernelGGL(k_add_rows, dim3(P.n_x), dim3(256), 0, s, P.x_nodes.as<int32_t>(), xchk_.as<double>(), chk,
+                         K_);
+      NNK_HIP(hipGetLastError());
       mark(5, false, s);
     }
     mark(6, true, s);

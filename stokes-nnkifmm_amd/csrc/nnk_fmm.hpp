@@ -392,8 +392,10 @@ class Fmm {
   // equivalent densities of the W-list nodes, down expansion of the target
   // leaves (FP64 down expansion only)
   GemmBatch mx_upP_all_, mx_wV_all_, mx_downU_all_;
-  // mixed forms: the X-list owners of every level, for one P2L launch
-  DBuf x_all_;
+  // the X-list owners of every level, for one P2L launch (both forms); the
+  // FP64 form puts the P2L check potentials in xchk_ and adds them level by
+  // level after M2L and L2L
+  DBuf x_all_, xchk_;
   int n_x_all_ = 0;
   void reduced_factor();
   bool red_ready_ = false;

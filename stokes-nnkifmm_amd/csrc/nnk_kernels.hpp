@@ -68,7 +68,7 @@ __device__ __forceinline__ void stokeslet_acc_f32q(double tx, double ty, double 
   const float re = r2 + e2f;
   const float inv = __builtin_amdgcn_rsqf(re);
   const float inv3 = inv * inv * inv;
-  const float iso = (re + e2f) * inv3;
+  const float iso = __builtin_fmaf(e2f, inv3, inv);  // (re + e2) / re^1.5
   const float rq = (rx * fx + ry * fy + rz * fz) * inv3;
   ux += (double)(iso * fx + rq * rx);
   uy += (double)(iso * fy + rq * ry);
@@ -139,7 +139,7 @@ template <int T> struct SrcStage<true, T> {
         const float re = r2 + e2f;
         const float inv = __builtin_amdgcn_rsqf(re);
         const float inv3 = inv * inv * inv;
-        const float iso = (re + e2f) * inv3;
+        const float iso = __builtin_fmaf(e2f, inv3, inv);  // (re + e2) / re^1.5
         const float rq = (rx * f.x + ry * f.y + rz * f.z) * inv3;
         gx += iso * f.x + rq * rx;
         gy += iso * f.y + rq * ry;
@@ -179,7 +179,7 @@ template <int T> struct SrcStageRel {
     const float re = r2 + e2f;
     const float inv = __builtin_amdgcn_rsqf(re);
     const float inv3 = inv * inv * inv;
-    const float iso = (re + e2f) * inv3;
+    const float iso = __builtin_fmaf(e2f, inv3, inv);  // (re + e2) / re^1.5
     const float rq = (rx * f.x + ry * f.y + rz * f.z) * inv3;
     gx = __builtin_fmaf(rq, rx, __builtin_fmaf(iso, f.x, gx));
     gy = __builtin_fmaf(rq, ry, __builtin_fmaf(iso, f.y, gy));

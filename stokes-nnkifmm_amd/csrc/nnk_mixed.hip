@@ -429,7 +429,6 @@ void Fmm::plan_mixed() {
   std::vector<int32_t> upall;
   // one job per level of a batch over all levels
   std::vector<GemmJob> jobs_upP, jobs_wV, jobs_downU;
-  std::vector<int32_t> xall;  // node ids (from 1), as the per-level X lists
   mx_upP_all_ = GemmBatch();
   mx_wV_all_ = GemmBatch();
   mx_downU_all_ = GemmBatch();
@@ -506,7 +505,6 @@ void Fmm::plan_mixed() {
       std::vector<int32_t> xs;
       for (int32_t n : H.xnodes) xs.push_back(n - 1);
       job1(P.mx_xP, X.Pdn.as<double>(), xs, xs);
-      xall.insert(xall.end(), H.xnodes.begin(), H.xnodes.end());
     }
     // split-BF16 M2L jobs (one per child class) and FP64 L2L jobs
     std::vector<GemmJobBf16> mj;
@@ -750,8 +748,6 @@ void Fmm::plan_mixed() {
     B.njobs = (int)jobs.size();
     if (B.njobs) B.jobs.upload(jobs);
   };
-  n_x_all_ = (int)xall.size();
-  x_all_.upload(xall);
   fin(mx_upP_all_, jobs_upP);
   fin(mx_wV_all_, jobs_wV);
   fin(mx_downU_all_, jobs_downU);
