@@ -404,13 +404,44 @@ std::vector<double> surface_pts(const std::vector<int32_t> &codes, int res, cons
 inline int offset_slot(int ox, int oy, int oz) { return (ox + 3) * 49 + (oy + 3) * 7 + (oz + 3); }
 
 // the 189 V-list offsets available to a child of class a (a = 4gx+2gy+gz)
+// the M2L visiting order of the 316 offsets (m2l_slot_order)
+const std::vector<int> &visit_order() {
+  static const std::vector<int> order = [] {
+    std::vector<int> o;
+    for (int p = 0; p < 8; p++) {
+      auto vals = [](int par) {
+        std::vector<int> v;
+        for (int x = -3; x <= 3; x++)
+          if ((x & 1) == par) v.push_back(x);
+        return v;
+      };
+      const std::vector<int> X = vals((p >> 2) & 1), Y = vals((p >> 1) & 1), Z = vals(p & 1);
+      int row = 0;
+      for (size_t i = 0; i < X.size(); i++)
+        for (size_t jj = 0; jj < Y.size(); jj++, row++) {
+          const size_t j = (i & 1) ? Y.size() - 1 - jj : jj;
+          for (size_t kk = 0; kk < Z.size(); kk++) {
+            const size_t k = (row & 1) ? Z.size() - 1 - kk : kk;
+            if (std::max({std::abs(X[i]), std::abs(Y[j]), std::abs(Z[k])}) < 2) continue;
+            o.push_back(offset_slot(X[i], Y[j], Z[k]));
+          }
+        }
+    }
+    if (o.size() != 316) throw Error(NNK_ESTATE, "M2L offset order: not 316 offsets");
+    return o;
+  }();
+  return order;
+}
+
 std::vector<std::array<int, 3>> class_offsets(int a) {
-  int g[3] = {(a >> 2) & 1, (a >> 1) & 1, a & 1};
+  const int g[3] = {(a >> 2) & 1, (a >> 1) & 1, a & 1};
   std::vector<std::array<int, 3>> out;
-  for (int ox = -2 - g[0]; ox <= 3 - g[0]; ox++)
-    for (int oy = -2 - g[1]; oy <= 3 - g[1]; oy++)
-      for (int oz = -2 - g[2]; oz <= 3 - g[2]; oz++)
-        if (std::max({std::abs(ox), std::abs(oy), std::abs(oz)}) >= 2) out.push_back({ox, oy, oz});
+  for (int sl : visit_order()) {
+    const int o[3] = {sl / 49 - 3, (sl / 7) % 7 - 3, sl % 7 - 3};
+    bool in = true;
+    for (int d = 0; d < 3; d++) in = in && o[d] >= -2 - g[d] && o[d] <= 3 - g[d];
+    if (in) out.push_back({o[0], o[1], o[2]});
+  }
   return out;
 }
 
@@ -490,15 +521,11 @@ void check_gather_job(const std::vector<int32_t> &gidx, int nb, int N, int BN, c
     if (b < 0 || b >= nb) fail("tile-list block out of range");
 }
 
+const std::vector<int> &m2l_slot_order() { return visit_order(); }
+
 std::vector<int> class_offset_slots(int a) {
-  const int g[3] = {(a >> 2) & 1, (a >> 1) & 1, a & 1};
   std::vector<int> sls;
-  for (int ox = -2 - g[0]; ox <= 3 - g[0]; ox++)
-    for (int oy = -2 - g[1]; oy <= 3 - g[1]; oy++)
-      for (int oz = -2 - g[2]; oz <= 3 - g[2]; oz++) {
-        if (std::max({std::abs(ox), std::abs(oy), std::abs(oz)}) < 2) continue;
-        sls.push_back(offset_slot(ox, oy, oz));
-      }
+  for (const auto &o : class_offsets(a)) sls.push_back(offset_slot(o[0], o[1], o[2]));
   return sls;
 }
 
@@ -513,9 +540,7 @@ void plan_m2l_level(const std::vector<int32_t> *cls, const std::vector<int32_t> 
     P.class_pairs += (int64_t)P.ctl[a].size();
   }
   std::vector<int> moff(343, -1);
-  for (int sl = 0; sl < 343; sl++) {
-    const int ox = sl / 49 - 3, oy = (sl / 7) % 7 - 3, oz = sl % 7 - 3;
-    if (std::max({std::abs(ox), std::abs(oy), std::abs(oz)}) < 2) continue;
+  for (int sl : m2l_slot_order()) {
     moff[sl] = (int)P.msl.size();
     P.msl.push_back(sl);
   }
@@ -567,14 +592,21 @@ void plan_m2l_pairs_host(const std::vector<int32_t> *cls, const std::vector<int3
   const int64_t other = base_work >= 0 ? base_work : P.class_work;
   P.use = P.npairs > 0 && P.pair_work < other && P.npairs * M <= cap_elems;
   if (!P.use) return;
-  // pairs grouped by offset slot (increasing), within a slot by class and
-  // column; each target's list then comes out ordered by slot
-  std::vector<int64_t> base(344, 0);
-  for (int sl = 0; sl < 343; sl++) base[sl + 1] = base[sl] + nslot[sl];
-  std::vector<int64_t> fill(base.begin(), base.end() - 1);
+  // pairs grouped by offset slot (in the M2L visiting order,
+  // m2l_slot_order), within a slot by class and column; each target's list
+  // then comes out in that order, as its class and merged jobs add them
+  std::vector<int64_t> base(343, 0), fill(343, 0);
+  {
+    int64_t at = 0;
+    for (int sl : m2l_slot_order()) {
+      base[sl] = at;
+      at += nslot[sl];
+    }
+  }
+  fill = base;
   P.src.assign(P.npairs, -1);
   std::vector<int32_t> pair_tgt(P.npairs, -1);
-  for (int sl = 0; sl < 343; sl++) {
+  for (int sl : m2l_slot_order()) {
     if (!nslot[sl]) continue;
     for (int a = 0; a < 8; a++) {
       if (cls[a].empty()) continue;
@@ -598,9 +630,9 @@ void plan_m2l_pairs_host(const std::vector<int32_t> *cls, const std::vector<int3
   // rows through that L2 (workgroup b of the launch goes to XCD b mod 8,
   // and the job count is a multiple of 8)
   std::vector<std::array<int64_t, 3>> tl;  // slot, first pair, pairs
-  for (int sl = 0; sl < 343; sl++)
-    for (int64_t p0 = base[sl]; p0 < base[sl + 1]; p0 += BN)
-      tl.push_back({sl, p0, std::min<int64_t>(BN, base[sl + 1] - p0)});
+  for (int sl : m2l_slot_order())
+    for (int64_t p0 = base[sl]; p0 < base[sl] + nslot[sl]; p0 += BN)
+      tl.push_back({sl, p0, std::min<int64_t>(BN, base[sl] + nslot[sl] - p0)});
   const int64_t T = (int64_t)tl.size();
   int64_t runmax = 0;
   for (int x = 0; x < 8; x++) runmax = std::max(runmax, (x + 1) * T / 8 - x * T / 8);

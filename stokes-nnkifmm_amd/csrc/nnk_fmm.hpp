@@ -40,9 +40,19 @@ void tile_lists(const std::vector<int32_t> &gidx, int nb, int N, int BN, std::ve
 void check_gather_job(const std::vector<int32_t> &gidx, int nb, int N, int BN, const std::vector<int32_t> &tptr,
                       const std::vector<int32_t> &tlist, int64_t count, const char *what);
 
-// The V-list offsets a child of class a (bits x, y, z) can see, as slots
-// (ox+3)*49 + (oy+3)*7 + (oz+3) of the 7^3 offset table, in the order of
-// that class's gather-table blocks (189 of them).
+// The 316 V-list offsets as slots (ox+3)*49 + (oy+3)*7 + (oz+3) of the 7^3
+// offset table, in the order the M2L visits them (every layout: class,
+// merged and per-offset jobs): grouped by the parity of (ox, oy, oz), and
+// within a group along a boustrophedon path (z fastest) on which
+// consecutive offsets differ by 2 in one coordinate.  A column tile's
+// targets are the children of one class of consecutive parents (a compact
+// box, longest in z), so the sources it gathers for consecutive offsets
+// are mostly the same nodes, which the L2 still holds.
+const std::vector<int> &m2l_slot_order();
+
+// The V-list offsets a child of class a (bits x, y, z) can see, as slots,
+// in the order of that class's gather-table blocks (189 of them):
+// m2l_slot_order restricted to the class.
 std::vector<int> class_offset_slots(int a);
 
 // Host plan of one level's split-precision M2L: per child class a job over

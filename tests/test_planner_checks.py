@@ -108,14 +108,59 @@ def test_m2l_plan_refuses_out_of_range_gathers(st, bad):
     assert rc == 1 and "gather index out of range" in msg, msg
 
 
+def _slot(ox, oy, oz):
+    return (ox + 3) * 49 + (oy + 3) * 7 + (oz + 3)
+
+
+def _visit_paths():
+    """Per parity class p = 4 (ox mod 2) + 2 (oy mod 2) + (oz mod 2), the
+    boustrophedon path over its offsets in [-3, 3]^3: x ascending, y
+    ascending on even x steps and descending on odd ones, z ascending on
+    even rows and descending on odd ones (rows counted through the class)."""
+    paths = []
+    for p in range(8):
+        X, Y, Z = ([v for v in range(-3, 4) if v & 1 == (p >> s) & 1] for s in (2, 1, 0))
+        path, row = [], 0
+        for i, x in enumerate(X):
+            for y in (Y if i % 2 == 0 else Y[::-1]):
+                path += [(x, y, z) for z in (Z if row % 2 == 0 else Z[::-1])]
+                row += 1
+        paths.append(path)
+    return paths
+
+
+def _visit_order():
+    """The M2L visiting order of the 316 V-list offsets (as slots): the
+    parity classes in turn, each along its path, near offsets (max |o| < 2)
+    left out."""
+    return [_slot(*o) for path in _visit_paths() for o in path if max(map(abs, o)) >= 2]
+
+
 def _class_slots(a):
     """The V-list offsets of a child of class a (bits x, y, z) as slots of
-    the 7^3 offset table, in block order: every offset within [-2 - g,
-    3 - g] per axis (g the class bit) that is not adjacent (max |o| >= 2)."""
+    the 7^3 offset table, in block order: the visiting order restricted to
+    the offsets within [-2 - g, 3 - g] per axis (g the class bit)."""
     g = ((a >> 2) & 1, (a >> 1) & 1, a & 1)
-    return [(ox + 3) * 49 + (oy + 3) * 7 + (oz + 3)
-            for ox in range(-2 - g[0], 4 - g[0]) for oy in range(-2 - g[1], 4 - g[1])
-            for oz in range(-2 - g[2], 4 - g[2]) if max(abs(ox), abs(oy), abs(oz)) >= 2]
+    out = []
+    for sl in _visit_order():
+        o = (sl // 49 - 3, (sl // 7) % 7 - 3, sl % 7 - 3)
+        if all(-2 - g[d] <= o[d] <= 3 - g[d] for d in range(3)):
+            out.append(sl)
+    return out
+
+
+def test_visit_order_is_a_path_per_parity_class():
+    order = _visit_order()
+    # every non-adjacent offset of the 7^3 table once
+    assert sorted(order) == sorted(_slot(x, y, z) for x in range(-3, 4) for y in range(-3, 4)
+                                   for z in range(-3, 4) if max(abs(x), abs(y), abs(z)) >= 2)
+    # within a class, consecutive offsets of the path differ by 2 in one coordinate
+    for path in _visit_paths():
+        for u, v in zip(path, path[1:]):
+            d = sorted(abs(u[k] - v[k]) for k in range(3))
+            assert d == [0, 0, 2], (u, v)
+    # each child class sees 189 offsets
+    assert all(len(_class_slots(a)) == 189 for a in range(8))
 
 
 def _pairs(st, sizes, gv, count, M=456, cap=1 << 28):
@@ -158,7 +203,7 @@ def test_m2l_pair_plan_tables(st):
                     want.append((sl[b], base + c, int(g[b, c])))
         off += 189 * n
         base += n
-    # jobs: at most 64 pairs of one slot; tiles in slot order, contiguous
+    # jobs: at most 64 pairs of one slot; tiles in visiting order, contiguous
     # runs per XCD (job 8k + x), and together they cover pairs 0..npairs-1
     js, jp, jn = js[:nj], jp[:nj], jn[:nj]
     assert (jn <= 64).all() and ((jn == 0) == (js == -1)).all()
@@ -167,18 +212,19 @@ def test_m2l_pair_plan_tables(st):
     assert [t[1] for t in seq] == sorted(t[1] for t in seq)
     covered = np.concatenate([np.arange(p0, p0 + n) for _, p0, n in seq])
     assert np.array_equal(covered, np.arange(npairs))
-    assert [t[0] for t in seq] == sorted(t[0] for t in seq)
+    rank = {sl: i for i, sl in enumerate(_visit_order())}
+    assert [rank[t[0]] for t in seq] == sorted(rank[t[0]] for t in seq)
     runs = [sum(1 for k in range(nj // 8) if jn[8 * k + x] > 0) for x in range(8)]
     assert max(runs) - min(runs) <= 1
     pair_slot = np.empty(npairs, np.int64)
     for sl, p0, n in seq:
         pair_slot[p0:p0 + n] = sl
-    # per target: its pairs, ordered by slot; together every pair once
+    # per target: its pairs, in the visiting order; together every pair once
     assert sorted(node.tolist()) == list(range(sum(sizes)))
     got = []
     for j, t in enumerate(node.tolist()):
         ps = idx[ptr[j]:ptr[j + 1]]
-        assert list(pair_slot[ps]) == sorted(pair_slot[ps])
+        assert [rank[int(pair_slot[p])] for p in ps] == sorted(rank[int(pair_slot[p])] for p in ps)
         got += [(int(pair_slot[p]), t, int(src[p])) for p in ps]
     assert sorted(got) == sorted(want)
     assert np.array_equal(np.sort(idx), np.arange(npairs))
