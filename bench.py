@@ -13,6 +13,9 @@ Forms timed (all reported under "forms"):
                matrix cores, FP32-math direct sums
   mixed3_res8  the same with three-part split-BF16 M2L at coronaRes 8 (below
                1e-6 relative error)
+  mixed3_res9_tol3e-7
+               three-part split-BF16 M2L on the headline form's operators
+               (coronaRes 9, cut 3e-7, rank 477)
   mixedh_res8  two-part split-FP16 M2L on power-of-two scaled operands at
                coronaRes 8: the same accuracy with half the MFMAs
   fp64t_res8   the truncated-SVD operators of mixed3_res8 (cut 1e-8,
@@ -76,7 +79,7 @@ def parse():
     ap.add_argument("--res", type=int, default=6, help="corona resolution of the CPU baseline")
     ap.add_argument("--svd-tol", type=float, default=1e-8, help="pseudo-inverse cut of the mixed forms")
     ap.add_argument("--forms", default="fp64_res6,fp64_res8,fp64t_res8,fp64t_res9_tol3e-7,mixed_res6,mixed3_res8,"
-                                      "mixedh_res8,mixedh_res9_tol3e-7",
+                                      "mixed3_res9_tol3e-7,mixedh_res8,mixedh_res9_tol3e-7",
                     help="comma-separated forms <precision>_res<r>[_tol<svd tol>] to time; the last one is the "
                          "headline")
     ap.add_argument("--no-solve", action="store_true",
