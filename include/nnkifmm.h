@@ -135,7 +135,9 @@ int32_t nnk_fmm_reduced_apply_device(nnk_fmm *h, const double *d_f, double *d_u,
 /* Replaces: KIFMM.reducedSolve  Classes/KIFMM.m:200-242 (Mobility/ReducedSolve.m):
  *           per target leaf, K(leaf sources -> leaf targets) \ v, with the NN
  *           map folded in NEAREST mode.  Needs potentials == targets.  v, f:
- *           N x 3.  The leaf blocks are LU-factored once, on first use. */
+ *           N x 3.  The leaf blocks are LU-factored and inverted once,
+ *           on first use; each solve is one block-diagonal product (same
+ *           solution to rounding). */
 int32_t nnk_fmm_reduced_solve(nnk_fmm *h, const double *v, double *f);
 int32_t nnk_fmm_reduced_solve_device(nnk_fmm *h, const double *d_v, double *d_f, void *stream);
 
