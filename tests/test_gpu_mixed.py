@@ -116,8 +116,10 @@ def test_refined_solve_reaches_fp64_accuracy(oracle, nnk, precon):
     assert rel(xr, x64) < 1e-7
 
 
-@pytest.mark.parametrize("precision,split", [("mixedh", 1), ("mixedh", 2), ("mixed", 1), ("mixed3", 2)])
-def test_mixedh_m2l_kernels_agree_bitwise(nnk, precision, split):
+@pytest.mark.parametrize("precision,split,res,tol", [("mixedh", 1, 6, 1e-8), ("mixedh", 2, 6, 1e-8),
+                                                   ("mixed", 1, 6, 1e-8), ("mixed3", 2, 6, 1e-8),
+                                                   ("mixedh", 1, 9, 3e-7)])
+def test_mixedh_m2l_kernels_agree_bitwise(nnk, precision, split, res, tol):
     """The split-precision M2L of every level run by the register-ring
     kernel (depth 2 and, but for the three-part form, 3), by the DMA-staged
     kernel (depth 4) and, for the two-part forms, by the DMA-staged kernel
@@ -130,11 +132,20 @@ def test_mixedh_m2l_kernels_agree_bitwise(nnk, precision, split):
     split-K factor (with split-K, depth 5's 192-column tiles group the
     partial sums differently: equal to rounding).  For the FP16 form, operator
     rows that are all zero (unscale 0) and rows scaled by widely different
-    powers of two come with the clustered points."""
+    powers of two come with the clustered points.  Res 6 (R = 428) and the
+    headline form's res 9 / 3e-7 (R = 477) leave different parts of the
+    last 128-row block as zero padding."""
     rng = np.random.default_rng(44)
     p = np.concatenate([rng.random((30000, 3)), 1e-3 * rng.random((3000, 3)) + 0.5])
     q = rng.standard_normal(p.shape)
-    f = nnk.KIFMM(nnk.OcTree(p, p, "nodeCapacity", 48), [1e-3, 1.0], "precision", precision)
+    f = nnk.KIFMM(nnk.OcTree(p, p, "nodeCapacity", 48), [1e-3, 1.0], "precision", precision,
+                  "coronaRes", res, "svdTol", tol)
+    st = np.zeros(14, np.int64)
+    assert nnk.lib().nnk_fmm_stats(f._handle().h, st.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), 14) == 0
+    rank = int(st[13])
+    print(f"[m2l] {precision} res {res} tol {tol}: rank {rank}", flush=True)
+    if res == 9:  # the upper waves of the last row block hold 1-32 rows
+        assert 64 < rank % 128 <= 96
     depths = (2, 4) if precision == "mixed3" else (2, 3, 4, 7)
     lib, h = nnk.lib(), f._handle().h
     i32 = ctypes.POINTER(ctypes.c_int32)
