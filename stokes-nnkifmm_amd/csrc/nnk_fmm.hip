@@ -696,7 +696,7 @@ void Fmm::plan_m2l_pairs(LevelPlan &P, const std::vector<int32_t> *cls, const st
   P.m2l_pair_on = false;
   P.m2lp = GemmBatch();
   M2lPairPlan H;
-  plan_m2l_pairs_host(cls, gv, tree_.count, 64, M, kPairBufMaxElems, kPairRtFP64, -1, H);
+  plan_m2l_pairs_host(cls, gv, tree_.count, GBN1, M, kPairBufMaxElems, kPairRtFP64, -1, H);
   if (!H.use) return;
   std::vector<const double *> slot_op(343, nullptr);
   for (int a = 0; a < 8; a++) {
@@ -776,8 +776,8 @@ void Fmm::gemm_dims(const GemmBatch &b, const double *X, int ldx, double *C, int
   if (Mpad % GBM || Kpad % GBK) throw Error(NNK_ESTATE, "gemm_dims: bad padding");
   // tag 1: the M2L's own kernel symbol (profiles tell it from the other GEMMs)
   auto go = [&](int bt, double *P, int ns) {
-    if (tag == 1) launch_gemm<64, 1>(Mpad / GBM, M, K, Mpad, Kpad, X, ldx, C, ldc, bt, b, s, ns, P);
-    else launch_gemm<64, 0>(Mpad / GBM, M, K, Mpad, Kpad, X, ldx, C, ldc, bt, b, s, ns, P);
+    if (tag == 1) launch_gemm<GBN1, 1>(Mpad / GBM, M, K, Mpad, Kpad, X, ldx, C, ldc, bt, b, s, ns, P);
+    else launch_gemm<GBN0, 0>(Mpad / GBM, M, K, Mpad, Kpad, X, ldx, C, ldc, bt, b, s, ns, P);
   };
   if (nsplit > 1) {
     if (beta != 0 || !slot) throw Error(NNK_ESTATE, "split GEMM needs beta = 0 and a slot list");
@@ -797,15 +797,15 @@ void Fmm::gemm(const GemmBatch &b, const double *X, int ldx, double *C, int ldc,
     // partial tiles, then the fixed-order reduction (stores: beta must be 0)
     if (beta != 0 || !slot) throw Error(NNK_ESTATE, "split GEMM needs beta = 0 and a slot list");
     if ((size_t)nsplit * b.njobs * b.maxN * ldc > part_elems_) throw Error(NNK_ESTATE, "split buffer too small");
-    if (tag == 1) launch_gemm<64, 1>(nrb, K_, K_, Mpad_, Kpad_, X, ldx, C, ldc, 0, b, s, nsplit, part_.as<double>());
-    else launch_gemm<64, 0>(nrb, K_, K_, Mpad_, Kpad_, X, ldx, C, ldc, 0, b, s, nsplit, part_.as<double>());
+    if (tag == 1) launch_gemm<GBN1, 1>(nrb, K_, K_, Mpad_, Kpad_, X, ldx, C, ldc, 0, b, s, nsplit, part_.as<double>());
+    else launch_gemm<GBN0, 0>(nrb, K_, K_, Mpad_, Kpad_, X, ldx, C, ldc, 0, b, s, nsplit, part_.as<double>());
     split_reduce(*slot, b.njobs * b.maxN, K_, ldc, nsplit, C, s);
     return;
   }
   if (tag == 1) {
-    launch_gemm<64, 1>(nrb, K_, K_, Mpad_, Kpad_, X, ldx, C, ldc, beta, b, s);
+    launch_gemm<GBN1, 1>(nrb, K_, K_, Mpad_, Kpad_, X, ldx, C, ldc, beta, b, s);
   } else {
-    launch_gemm<64, 0>(nrb, K_, K_, Mpad_, Kpad_, X, ldx, C, ldc, beta, b, s);
+    launch_gemm<GBN0, 0>(nrb, K_, K_, Mpad_, Kpad_, X, ldx, C, ldc, beta, b, s);
   }
 }
 
@@ -1280,7 +1280,7 @@ void Fmm::plan() {
       add_job(P.m2m, jobs, A, &g, &nl, (int)nl.size());
       finish(P.m2m, jobs);
       hl_[l].m2m_g = g;
-      const int64_t tiles = (int64_t)(Mpad_ / GBM) * (((int64_t)nl.size() + 63) / 64);
+      const int64_t tiles = (int64_t)(Mpad_ / GBM) * (((int64_t)nl.size() + GBN0 - 1) / GBN0);
       P.m2m_split = split_m2m(tiles, nl.size() * (size_t)K_);
       if (P.m2m_split > 1) {
         P.m2m_slot.upload(nl);
@@ -1349,8 +1349,8 @@ void Fmm::plan() {
         add_job(P.m2l, jobs, A, &g, &cls[a], N);
         {  // active blocks per 64-column tile of the M2L launch
           std::vector<int32_t> tp, tl;
-          tile_lists(g, nb, N, 64, tp, tl);
-          check_gather_job(g, nb, N, 64, tp, tl, T.count, "M2L job");
+          tile_lists(g, nb, N, GBN1, tp, tl);
+          check_gather_job(g, nb, N, GBN1, tp, tl, T.count, "M2L job");
           P.m2l.owned.emplace_back(make_dbuf(tp));
           jobs.back().tptr = P.m2l.owned.back().as<int32_t>();
           P.m2l.owned.emplace_back(make_dbuf(tl));
@@ -1376,7 +1376,7 @@ void Fmm::plan() {
           minnb = std::min(minnb, (int)class_offsets(a).size());
           j++;
         }
-        P.m2l_split = split_for((int64_t)nj * (Mpad_ / GBM) * ((mN + 63) / 64), minnb);
+        P.m2l_split = split_for((int64_t)nj * (Mpad_ / GBM) * ((mN + GBN1 - 1) / GBN1), minnb);
         P.m2l_slot.upload(slot);  // also for the autotuner's split candidates
         reserve_part((size_t)std::min(64, 2 * P.m2l_split) * nj * mN * K_);
       }

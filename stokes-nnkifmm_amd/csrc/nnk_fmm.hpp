@@ -16,8 +16,10 @@ namespace nnk {
 
 struct RedArgs;
 
-// rows per workgroup tile and k depth per chunk of the FP64 gather GEMM
-constexpr int GBM = 160, GBK = 16;
+// rows per workgroup tile and k depth per chunk of the FP64 gather GEMM;
+// columns per tile of its M2L launches (GBN1) and of the others (GBN0:
+// expansions, M2M, L2L, check-to-equivalent solves)
+constexpr int GBM = 160, GBK = 16, GBN1 = 64, GBN0 = 32;
 
 struct GemmJob {
   const double *const *A;  // nb operator pointers, each At[Kpad][Mpad] (k-major)

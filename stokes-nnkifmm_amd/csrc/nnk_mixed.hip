@@ -491,7 +491,7 @@ void Fmm::plan_mixed() {
       P.mx_m2m.njobs = 1;
       P.mx_m2m.maxN = N;
       P.mx_m2m.jobs.upload(jobs);
-      const int64_t tiles = (int64_t)(MpadR_ / GBM) * (((int64_t)N + 63) / 64);
+      const int64_t tiles = (int64_t)(MpadR_ / GBM) * (((int64_t)N + GBN0 - 1) / GBN0);
       P.mx_m2m_split = split_m2m(tiles, (size_t)N * RP_);
       if (P.mx_m2m_split > 1) {
         P.mx_m2m_slot.upload(H.up_nonleaves);
@@ -683,8 +683,8 @@ void Fmm::plan_mixed() {
         J.nb = nb;
         J.N = N;
         std::vector<int32_t> tp, tl;
-        tile_lists(H.gv[a], nb, N, 64, tp, tl);
-        check_gather_job(H.gv[a], nb, N, 64, tp, tl, tree_.count, "FP64 truncated M2L job");
+        tile_lists(H.gv[a], nb, N, GBN1, tp, tl);
+        check_gather_job(H.gv[a], nb, N, GBN1, tp, tl, tree_.count, "FP64 truncated M2L job");
         P.m2l.owned.emplace_back();
         P.m2l.owned.back().upload(tp);
         J.tptr = P.m2l.owned.back().as<int32_t>();
@@ -698,7 +698,7 @@ void Fmm::plan_mixed() {
       P.m2l.njobs = (int)jobs.size();
       P.m2l.jobs.upload(jobs);
       const int nj = P.m2l.njobs, mN = P.m2l.maxN;
-      P.m2l_split = split_for((int64_t)nj * (MpadR_ / GBM) * ((mN + 63) / 64), minnb);
+      P.m2l_split = split_for((int64_t)nj * (MpadR_ / GBM) * ((mN + GBN1 - 1) / GBN1), minnb);
       reserve_part((size_t)std::min(64, 2 * P.m2l_split) * nj * mN * RP_);
       plan_m2l_pairs(P, H.cls, H.gv, cops, R_);
     }
