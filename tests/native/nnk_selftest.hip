@@ -13,9 +13,11 @@
 
 using namespace nnk;
 
+// bn: the tile width the library launches (GBN1 = 64 for M2L, GBN0 = 32
+// for the other GEMMs)
 extern "C" int nnk_selftest_gemm(int M, int K, int Mpad, int Kpad, const double *At, int nb,
                                  const double *X, int nx, int ldx, const int *gidx,
-                                 const int *cidx, int N, double *C, int nc, int ldc, int beta) {
+                                 const int *cidx, int N, double *C, int nc, int ldc, int beta, int bn) {
   try {
     DBuf dA, dX, dC, dG, dCi, dP, dJ;
     size_t mat = (size_t)Kpad * Mpad;
@@ -35,12 +37,17 @@ extern "C" int nnk_selftest_gemm(int M, int K, int Mpad, int Kpad, const double 
     j.nb = nb;
     j.N = N;
     dJ.upload(std::vector<GemmJob>{j});
-    const int nrb = Mpad / 160;
-    if (Kpad % 16) return 2;
-    dim3 grid(nrb * ((N + 63) / 64));
-    hipLaunchKernelGGL((k_gemm_gather<160, 64, 16, 0>), grid, dim3(256), 0, 0, M, K, Mpad, Kpad,
-                       dX.as<double>(), ldx, dC.as<double>(), ldc, beta, dJ.as<GemmJob>(), 1, nrb, 1,
-                       nullptr, N);
+    const int nrb = Mpad / GBM;
+    if (Kpad % GBK || (bn != GBN0 && bn != GBN1)) return 2;
+    dim3 grid(nrb * ((N + bn - 1) / bn));
+    if (bn == GBN1)
+      hipLaunchKernelGGL((k_gemm_gather<GBM, GBN1, GBK, 0>), grid, dim3(256), 0, 0, M, K, Mpad, Kpad,
+                         dX.as<double>(), ldx, dC.as<double>(), ldc, beta, dJ.as<GemmJob>(), 1, nrb, 1,
+                         nullptr, N);
+    else
+      hipLaunchKernelGGL((k_gemm_gather<GBM, GBN0, GBK, 0>), grid, dim3(256), 0, 0, M, K, Mpad, Kpad,
+                         dX.as<double>(), ldx, dC.as<double>(), ldc, beta, dJ.as<GemmJob>(), 1, nrb, 1,
+                         nullptr, N);
     NNK_HIP(hipGetLastError());
     NNK_HIP(hipDeviceSynchronize());
     NNK_HIP(hipMemcpy(C, dC.p, sizeof(double) * (size_t)nc * ldc, hipMemcpyDeviceToHost));

@@ -194,7 +194,11 @@ def test_linearity_at_scale(nnk, torch):
     assert (torch.linalg.norm(ax[idx] - ud) / torch.linalg.norm(ud)).item() < 1e-4
 
 
-def test_gemm_selftest(selftest):
+@pytest.mark.parametrize("bn", [64, 32])
+def test_gemm_selftest(selftest, bn):
+    """The FP64 gather GEMM at both tile widths the library launches (64
+    columns for M2L, 32 for the expansions, M2M, L2L and solves) against the
+    host product."""
     d = ctypes.POINTER(ctypes.c_double)
     ip = ctypes.POINTER(ctypes.c_int)
     rng = np.random.default_rng(7)
@@ -211,5 +215,5 @@ def test_gemm_selftest(selftest):
     C = C0.copy()
     assert selftest.nnk_selftest_gemm(M, K, Mpad, Kpad, At.ctypes.data_as(d), nb, X.ctypes.data_as(d), nx, K,
                                       g.ctypes.data_as(ip), ci.ctypes.data_as(ip), N, C.ctypes.data_as(d),
-                                      nc, K, 1) == 0
+                                      nc, K, 1, bn) == 0
     np.testing.assert_allclose(C, ref, rtol=0, atol=1e-12 * np.abs(ref).max())
