@@ -201,7 +201,10 @@ int32_t nnk_fmm_set_graph(nnk_fmm *h, int32_t on);
  *           forms also the per-offset pair jobs, depth 1, where planned;
  *           for the mixed forms also the register-ring depth 2 or 3 and
  *           the DMA-staged kernels, depths 4-8, and where planned the
- *           per-offset pair jobs, 12-14, 17) and keeps the fastest.  The
+ *           per-offset pair jobs, 12-14, 17) and keeps the fastest
+ *           (candidates within 5 % of the fastest are timed three more
+ *           times, interleaved, and keep their best time; within 0.5 %
+ *           the earlier candidate in that order wins).  The
  *           choice changes summation order only (results stay within
  *           rounding of each other and deterministic afterwards).
  *           nnk_fmm_launch_config reports the per-level choice (split,

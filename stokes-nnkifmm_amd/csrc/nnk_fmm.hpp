@@ -347,8 +347,8 @@ class Fmm {
   void launch_m2l_mixed(int l, int depth, int split, hipStream_t s);
   bool dma_ok(const LevelPlan &P, int split, int depth = 4) const;
   // time the M2L launch candidates of every level on the current densities
-  // (register-ring depth 2/3 for the mixed forms, split-K factor s/2, s, 2s)
-  // and keep the fastest
+  // (split-K factor s/2, s, 2s; the kernel variants of the form) and keep
+  // the fastest, re-timing the near-ties
   void autotune(hipStream_t s);
   void autotune_leaf(hipStream_t s);
   int leaf_threads_ = 256;          // threads per target leaf in the leaf kernel
