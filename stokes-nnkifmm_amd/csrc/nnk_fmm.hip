@@ -445,6 +445,11 @@ std::vector<std::array<int, 3>> class_offsets(int a) {
   return out;
 }
 
+// v in decreasing order of w[v[i]] (ties in increasing order of v[i])
+void order_by_work(std::vector<int32_t> &v, const std::vector<int64_t> &w) {
+  std::sort(v.begin(), v.end(), [&](int32_t a, int32_t b) { return w[a] != w[b] ? w[a] > w[b] : a < b; });
+}
+
 template <class T> DBuf make_dbuf(const std::vector<T> &v) {
   DBuf d;
   d.upload(v);
@@ -888,6 +893,7 @@ Fmm::Fmm(std::shared_ptr<const Tree> t, double eps, double mu, int res, int shel
   }
   nsrc_ = src_pos.size() / 3;
   src_ptr_.upload(src_ptr);
+  src_ptr_h_ = src_ptr;
   src_pos_.upload(src_pos);
   src_q_.alloc(sizeof(double) * 3 * std::max<int64_t>(nsrc_, 1));
   if (!T.nearest) src_from_.upload(src_from);
@@ -922,6 +928,23 @@ Fmm::Fmm(std::shared_ptr<const Tree> t, double eps, double mu, int res, int shel
   for (int n = 1; n <= nc; n++) {
     if (T.pot_leaf[n]) pl.push_back(n);
     if (T.tgt_leaf[n]) tl.push_back(n);
+  }
+  // One workgroup runs each leaf of these lists.  They are launched in
+  // decreasing order of their pair work (S2M: sources; leaf kernel:
+  // targets x (U-list sources + surface points of the leaf and its W list)),
+  // so that the heaviest start first instead of finishing last on a few
+  // CUs.  The results do not depend on the order.
+  {
+    auto nsrc = [&](int n) -> int64_t { return src_ptr[n + 1] - src_ptr[n]; };
+    std::vector<int64_t> wp(nc + 1, 0), wt(nc + 1, 0);
+    for (int n : pl) wp[n] = nsrc(n);
+    for (int n : tl) {
+      int64_t u = 0;
+      for (int64_t e = T.lptr[0][n]; e < T.lptr[0][n + 1]; e++) u += nsrc(T.lidx[0][e]);
+      wt[n] = (int64_t)(tgt_ptr[n + 1] - tgt_ptr[n]) * (u + (int64_t)nq_ * (1 + T.lptr[2][n + 1] - T.lptr[2][n]));
+    }
+    order_by_work(pl, wp);
+    order_by_work(tl, wt);
   }
   n_pleaves_ = (int)pl.size();
   n_tleaves_ = (int)tl.size();
@@ -1397,8 +1420,15 @@ void Fmm::plan() {
     hl_[l].xnodes = xs;
   }
   {
-    std::vector<int32_t> xall;  // node ids (from 1), level by level
+    // node ids (from 1) of every level's X-list owners, heaviest P2L first
+    // (one workgroup each; see the leaf lists)
+    std::vector<int32_t> xall;
     for (int l = 2; l <= maxlev_; l++) xall.insert(xall.end(), hl_[l].xnodes.begin(), hl_[l].xnodes.end());
+    std::vector<int64_t> wx(T.count + 1, 0);
+    for (int n : xall)
+      for (int64_t e = T.lptr[3][n]; e < T.lptr[3][n + 1]; e++)
+        wx[n] += src_ptr_h_[T.lidx[3][e] + 1] - src_ptr_h_[T.lidx[3][e]];
+    order_by_work(xall, wx);
     n_x_all_ = (int)xall.size();
     x_all_.upload(xall);
     if (n_x_all_ && prec_ == 0) xchk_.alloc(sizeof(double) * (size_t)nc * K_);

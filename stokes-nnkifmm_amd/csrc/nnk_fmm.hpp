@@ -371,6 +371,7 @@ class Fmm {
   DBuf tgt_ptr_, tgt_pos_, tgt_out_;
   DBuf uptr_, uidx_, wptr_, widx_, xptr_, xidx_;
   DBuf pleaves_, tleaves_;
+  std::vector<int32_t> src_ptr_h_;  // host copy of src_ptr_ (launch order by work)
   int n_pleaves_ = 0, n_tleaves_ = 0;
   // work buffers
   DBuf chk_, uppot_, downpot_, tmp_;
