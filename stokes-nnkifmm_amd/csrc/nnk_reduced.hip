@@ -218,38 +218,98 @@ __global__ void k_red_perm(RedArgs a, int nleaves) {
   }
 }
 
-// The inverse of each leaf block from its LU factors (set-up only): thread
-// j of workgroup (leaf, column group) computes column j of
-// A^-1 = U^-1 L^-1 P e_j by forward and back substitution, in the
-// row-major working block W (W[i n3 + j]), where the lanes of a wavefront
-// own consecutive columns, so every access of W is coalesced and every
-// access of the factors is the same address across the lanes.
-constexpr int kRedInvCols = 64;
+// The inverse of each leaf block from its LU factors (set-up only), in the
+// row-major working block W (W[i n3 + j]).  A workgroup of one wavefront
+// owns 64 consecutive columns j of A^-1 = U^-1 L^-1 P (lane = column) and
+// walks the rows in blocks of kRedInvB: each lane keeps its column's entries
+// of the current row block in registers, subtracts the rows solved before
+// (each kRedInvB x kRedInvB factor tile staged in LDS and read as
+// broadcasts; the solved entries of the lane's own column read from W,
+// coalesced across the lanes), then solves the diagonal tile in registers.
+// Forward with the unit lower triangle (row blocks ascending), then back
+// with the upper one (descending).  Each solved entry is read from W once
+// per row block instead of once per row.
+constexpr int kRedInvCols = 64, kRedInvB = 32;
 __global__ __launch_bounds__(kRedInvCols) void k_red_inv(RedArgs a, double *__restrict__ W) {
+  constexpr int B = kRedInvB;
+  __shared__ double F[B][B + 1];           // F[kk][r]: factor entry (row i0 + r, column k0 + kk)
+  __shared__ double Xs[B][kRedInvCols];    // the row block of the 64 columns (diagonal solve)
   const int li = blockIdx.x;
   const int n = a.leaves[li];
   const int n3 = 3 * (a.tgt_ptr[n + 1] - a.tgt_ptr[n]);
-  const int j = blockIdx.y * kRedInvCols + threadIdx.x;
-  if (j >= n3) return;  // no barriers below
+  if ((int)blockIdx.y * kRedInvCols >= n3) return;  // the whole workgroup
+  const int tid = threadIdx.x;
+  const int j = blockIdx.y * kRedInvCols + tid;
+  const bool col = j < n3;
   const double *A = a.blk + a.off[li];  // column-major factors: A[k n3 + i] = LU[i][k]
   const int32_t *perm = a.perm + a.poff[li];
   double *X = W + a.off[li];
-  for (int i = 0; i < n3; i++) {  // unit lower triangle
-    double t = perm[i] == j ? 1.0 : 0.0;
-    for (int k = 0; k < i; k++) t -= A[(size_t)k * n3 + i] * X[(size_t)k * n3 + j];
-    X[(size_t)i * n3 + j] = t;
+  const int nb = (n3 + B - 1) / B;
+  auto stage = [&](int i0, int k0) {
+    __syncthreads();  // the previous tile has been read
+    for (int idx = tid; idx < B * B; idx += kRedInvCols) {
+      const int kk = idx / B, r = idx % B, k = k0 + kk, i = i0 + r;
+      F[kk][r] = (k < n3 && i < n3) ? A[(size_t)k * n3 + i] : 0.0;
+    }
+    __syncthreads();
+  };
+  // subtract the already solved rows k0 .. k0+B-1 of this column times the
+  // tile, then hand the row block to the diagonal solve
+  double acc[B];
+  auto update = [&](int k0) {
+    for (int kk = 0; kk < B; kk++) {
+      const double xk = (col && k0 + kk < n3) ? X[(size_t)(k0 + kk) * n3 + j] : 0.0;
+#pragma unroll
+      for (int r = 0; r < B; r++) acc[r] -= F[kk][r] * xk;
+    }
+  };
+  auto store = [&](int i0) {
+    if (col)
+      for (int r = 0; r < B && i0 + r < n3; r++) X[(size_t)(i0 + r) * n3 + j] = Xs[r][tid];
+  };
+  for (int I = 0; I < nb; I++) {  // L y = P e_j
+    const int i0 = I * B;
+#pragma unroll
+    for (int r = 0; r < B; r++) acc[r] = (col && i0 + r < n3 && perm[i0 + r] == j) ? 1.0 : 0.0;
+    for (int K = 0; K < I; K++) {
+      stage(i0, K * B);
+      update(K * B);
+    }
+    stage(i0, i0);
+#pragma unroll
+    for (int r = 0; r < B; r++) Xs[r][tid] = acc[r];
+    for (int r = 1; r < B; r++) {  // unit lower diagonal tile (each lane its own column)
+      double t = Xs[r][tid];
+      for (int q = 0; q < r; q++) t -= F[q][r] * Xs[q][tid];
+      Xs[r][tid] = t;
+    }
+    store(i0);
   }
-  for (int i = n3 - 1; i >= 0; i--) {  // upper triangle
-    double t = X[(size_t)i * n3 + j];
-    for (int k = i + 1; k < n3; k++) t -= A[(size_t)k * n3 + i] * X[(size_t)k * n3 + j];
-    X[(size_t)i * n3 + j] = t / A[(size_t)i * n3 + i];
+  for (int I = nb - 1; I >= 0; I--) {  // U x = y
+    const int i0 = I * B;
+#pragma unroll
+    for (int r = 0; r < B; r++) acc[r] = (col && i0 + r < n3) ? X[(size_t)(i0 + r) * n3 + j] : 0.0;
+    for (int K = I + 1; K < nb; K++) {
+      stage(i0, K * B);
+      update(K * B);
+    }
+    stage(i0, i0);
+#pragma unroll
+    for (int r = 0; r < B; r++) Xs[r][tid] = acc[r];
+    for (int r = B - 1; r >= 0; r--) {  // upper diagonal tile
+      double t = Xs[r][tid];
+      for (int q = r + 1; q < B; q++) t -= F[q][r] * Xs[q][tid];
+      Xs[r][tid] = i0 + r < n3 ? t / F[r][r] : 0.0;
+    }
+    store(i0);
   }
 }
 
 // x = A^-1 v for each leaf with the column-major inverse
-// (Inv[j n3 + i] = (A^-1)_ij): one thread per row, the leaf's right-hand
-// side in LDS, the columns read in order (coalesced across the rows), the
-// sum in column order.
+// (Inv[j n3 + i] = (A^-1)_ij): workgroup (leaf, row group) holds the leaf's
+// right-hand side in LDS and gives each of its rows one thread; the columns
+// are read in order (coalesced across the rows), the sum in column order.
+// Large blocks spread over several workgroups instead of one CU.
 constexpr int kRedApplyThreads = 256;
 __global__ __launch_bounds__(kRedApplyThreads) void k_red_apply(RedArgs a, const double *__restrict__ v,
                                                                 double *__restrict__ x) {
@@ -258,10 +318,13 @@ __global__ __launch_bounds__(kRedApplyThreads) void k_red_apply(RedArgs a, const
   const int n = a.leaves[li];
   const int tb = a.tgt_ptr[n], nt = a.tgt_ptr[n + 1] - tb;
   const int n3 = 3 * nt;
+  const int i0 = blockIdx.y * kRedApplyThreads;
+  if (i0 >= n3) return;  // the whole workgroup
   const double *Inv = a.blk + a.off[li];
   for (int i = threadIdx.x; i < n3; i += kRedApplyThreads) b[i] = v[3 * (size_t)a.tgt_out[tb + i / 3] + i % 3];
   __syncthreads();
-  for (int i = threadIdx.x; i < n3; i += kRedApplyThreads) {
+  const int i = i0 + threadIdx.x;
+  if (i < n3) {
     double acc = 0.0;
     for (int jj = 0; jj < n3; jj++) acc += Inv[(size_t)jj * n3 + i] * b[jj];
     x[3 * (size_t)a.tgt_out[tb + i / 3] + i % 3] = acc;
@@ -356,8 +419,8 @@ void Fmm::reduced_solve(const double *d_v, double *d_f, hipStream_t s) {
   if (!red_ready_) reduced_factor();
   if (!n_tleaves_) return;
   RedArgs a = red_args();
-  hipLaunchKernelGGL(k_red_apply, dim3(n_tleaves_), dim3(kRedApplyThreads), sizeof(double) * red_maxn_, s, a, d_v,
-                     d_f);
+  hipLaunchKernelGGL(k_red_apply, dim3(n_tleaves_, (red_maxn_ + kRedApplyThreads - 1) / kRedApplyThreads),
+                     dim3(kRedApplyThreads), sizeof(double) * red_maxn_, s, a, d_v, d_f);
   NNK_HIP(hipGetLastError());
 }
 

@@ -63,6 +63,21 @@ def test_reduced_solve_matches_oracle(oracle, nnk):
     assert rel(f, fo) < 1e-10
 
 
+def test_reduced_solve_large_leaf_blocks(oracle, nnk):
+    """Eight leaves of about 375 points (blocks of about 1,100 unknowns:
+    many 32-row blocks of the inverse with a partial last one, several
+    64-column workgroups per leaf, and several 256-row workgroups per leaf
+    in the apply) against the oracle's per-leaf LU solve."""
+    rng = np.random.default_rng(11)
+    p = rng.random((3000, 3))
+    v = rng.standard_normal(p.shape)
+    t = nnk.OcTree(p, p, "nodeCapacity", 500)
+    f = nnk.KIFMM(t, [0.05, 1.0]).reducedSolve(v)
+    fo = oracle.Tree(p, p, nodeCapacity=500).reduced_solve(v, 0.05, 1.0)
+    print(f"[reduced] large blocks: rel diff {rel(f, fo):.3e}", flush=True)
+    assert rel(f, fo) < 1e-9
+
+
 def _system(nnk, oracle, two_bodies):
     """One sphere (NEAREST, 384 / 6144 points) or two spheres over a plane
     wall (multi-level tree)."""
