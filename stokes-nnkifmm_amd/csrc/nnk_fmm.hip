@@ -210,6 +210,7 @@ struct LeafArgs {
   const double *uppot, *downpot;
   double e2, c;
   int far;          // 0: sources of the leaf itself only (reducedPass.m)
+  int tc;           // targets per workgroup and pass (<= T); blockIdx.y picks the chunk
   double *out;      // ntgt x 3
 };
 
@@ -254,8 +255,8 @@ __global__ __launch_bounds__(T) void k_leaf(LeafArgs a) {
     }
     __syncthreads();
   }
-  for (int t0 = 0; t0 < nt; t0 += T) {
-    const int ntc = min(T, nt - t0);
+  for (int t0 = blockIdx.y * a.tc; t0 < nt; t0 += gridDim.y * a.tc) {
+    const int ntc = min(a.tc, nt - t0);
     const int nsub = T / ntc;
     const bool act = tid < ntc * nsub;
     const int ti = tid % ntc, sub = tid / ntc;
@@ -948,6 +949,13 @@ Fmm::Fmm(std::shared_ptr<const Tree> t, double eps, double mu, int res, int shel
   }
   n_pleaves_ = (int)pl.size();
   n_tleaves_ = (int)tl.size();
+  max_leaf_tgt_ = 0;
+  for (int n : tl) max_leaf_tgt_ = std::max(max_leaf_tgt_, tgt_ptr[n + 1] - tgt_ptr[n]);
+  {
+    int dev = 0;
+    NNK_HIP(hipGetDevice(&dev));
+    NNK_HIP(hipDeviceGetAttribute(&ncu_, hipDeviceAttributeMultiprocessorCount, dev));
+  }
   pleaves_.upload(pl);
   tleaves_.upload(tl);
   chk_.alloc(sizeof(double) * (size_t)nc * K_);
@@ -1553,8 +1561,15 @@ void Fmm::launch_leaf(int far, double *d_u, hipStream_t s) {
   a.c = 1.0 / (8.0 * kPi * mu_);
   a.far = far;
   a.out = d_u;
-  auto go = [&](auto kern, int threads) { hipLaunchKernelGGL(kern, dim3(n_tleaves_), dim3(threads), 0, s, a); };
   const int T = far ? leaf_threads_ : 256;
+  // Few target leaves (small trees, large leaves) would leave most CUs idle
+  // with one workgroup per leaf: their targets are then split into chunks
+  // of tc over a second grid dimension (the lanes of a target split its
+  // sources T / tc ways), until there are about two workgroups per CU
+  a.tc = T;
+  while (a.tc > 32 && (int64_t)n_tleaves_ * ((max_leaf_tgt_ + a.tc - 1) / a.tc) < 2 * (int64_t)ncu_) a.tc /= 2;
+  const dim3 grid((unsigned)n_tleaves_, (unsigned)std::max(1, (max_leaf_tgt_ + a.tc - 1) / a.tc));
+  auto go = [&](auto kern, int threads) { hipLaunchKernelGGL(kern, grid, dim3(threads), 0, s, a); };
   if (fp32_math()) {
     if (T == 128) go(k_leaf<true, 128>, 128);
     else if (T == 512) go(k_leaf<true, 512>, 512);

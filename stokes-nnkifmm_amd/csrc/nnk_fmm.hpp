@@ -373,6 +373,7 @@ class Fmm {
   DBuf pleaves_, tleaves_;
   std::vector<int32_t> src_ptr_h_;  // host copy of src_ptr_ (launch order by work)
   int n_pleaves_ = 0, n_tleaves_ = 0;
+  int max_leaf_tgt_ = 0, ncu_ = 256;  // most targets in a leaf; compute units of the device
   // work buffers
   DBuf chk_, uppot_, downpot_, tmp_;
   std::vector<std::shared_ptr<const LevelOps>> ops_;
