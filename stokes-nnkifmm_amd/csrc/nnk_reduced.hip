@@ -77,108 +77,118 @@ __global__ __launch_bounds__(256) void k_red_form(RedArgs a) {
   }
 }
 
-// In-place LU with partial pivoting (first maximal |a_ik|), one workgroup
-// per leaf, blocked right-looking (the order of LAPACK's getrf): a 32-column
-// panel is factored column by column (row interchanges applied to whole
-// rows), the panel's rows of U right of it are solved with its unit lower
-// triangle, and the trailing matrix takes one rank-32 update in 64 x 64
-// tiles staged through LDS -- one pass over the trailing matrix per panel
-// instead of one per column.
+// In-place LU with partial pivoting (first maximal |a_ik|), blocked
+// right-looking (the order of LAPACK's getrf), one launch pair per 32-column
+// panel for every leaf at once.  k_red_lu_panel (one workgroup per leaf)
+// factors the panel column by column (row interchanges applied to whole
+// rows) and solves the panel's rows of U right of it with its unit lower
+// triangle; k_red_lu_update gives every 64 x 64 tile of every leaf's
+// trailing matrix a workgroup of its own for the rank-32 update (staged
+// through LDS), so that large leaves use the whole GPU instead of one CU.
 constexpr int kLuThreads = 512, kLuNB = 32, kLuT = 64;
-__global__ __launch_bounds__(kLuThreads) void k_red_lu(RedArgs a) {
+__global__ __launch_bounds__(kLuThreads) void k_red_lu_panel(RedArgs a, int k0) {
   const int li = blockIdx.x;
   const int n = a.leaves[li];
   const int n3 = 3 * (a.tgt_ptr[n + 1] - a.tgt_ptr[n]);
+  if (k0 >= n3) return;  // the whole workgroup
   double *A = a.blk + a.off[li];
   int32_t *piv = a.piv + a.poff[li];
   __shared__ double sv[kLuThreads];
   __shared__ int si[kLuThreads];
+  const int tid = threadIdx.x;
+  const int k1 = min(k0 + kLuNB, n3);
+  for (int k = k0; k < k1; k++) {
+    double best = -1.0;
+    int bi = n3;
+    for (int i = k + tid; i < n3; i += kLuThreads) {
+      const double v = fabs(A[(size_t)i * n3 + k]);
+      if (v > best) { best = v; bi = i; }
+    }
+    sv[tid] = best;
+    si[tid] = bi;
+    __syncthreads();
+    for (int st = kLuThreads / 2; st > 0; st >>= 1) {
+      if (tid < st) {
+        const double v2 = sv[tid + st];
+        const int i2 = si[tid + st];
+        if (v2 > sv[tid] || (v2 == sv[tid] && i2 < si[tid])) {
+          sv[tid] = v2;
+          si[tid] = i2;
+        }
+      }
+      __syncthreads();
+    }
+    const int p = si[0];
+    if (tid == 0) piv[k] = p;
+    if (p != k)
+      for (int jj = tid; jj < n3; jj += kLuThreads) {
+        const double t = A[(size_t)k * n3 + jj];
+        A[(size_t)k * n3 + jj] = A[(size_t)p * n3 + jj];
+        A[(size_t)p * n3 + jj] = t;
+      }
+    __syncthreads();
+    const double d = A[(size_t)k * n3 + k];
+    if (d != 0.0)
+      for (int i = k + 1 + tid; i < n3; i += kLuThreads) A[(size_t)i * n3 + k] /= d;
+    __syncthreads();
+    const int w = k1 - k - 1;  // panel columns right of k
+    if (w > 0) {
+      const int m = n3 - k - 1;
+      for (int idx = tid; idx < m * w; idx += kLuThreads) {
+        const int i = k + 1 + idx / w, jj = k + 1 + idx % w;
+        A[(size_t)i * n3 + jj] -= A[(size_t)i * n3 + k] * A[(size_t)k * n3 + jj];
+      }
+    }
+    __syncthreads();
+  }
+  if (k1 >= n3) return;
+  const int m = n3 - k1;  // trailing size
+  // U12 = L11^-1 A12 (rows k0..k1-1, columns k1..)
+  for (int k = k0; k < k1 - 1; k++) {
+    const int h = k1 - k - 1;
+    for (int idx = tid; idx < h * m; idx += kLuThreads) {
+      const int i = k + 1 + idx / m, jj = k1 + idx % m;
+      A[(size_t)i * n3 + jj] -= A[(size_t)i * n3 + k] * A[(size_t)k * n3 + jj];
+    }
+    __syncthreads();
+  }
+}
+
+// A22 -= L21 U12 after the panel at k0: workgroup (leaf, t) updates tile t
+// of the leaf's trailing matrix (row-major over its ceil(m / 64)^2 tiles)
+__global__ __launch_bounds__(kLuThreads) void k_red_lu_update(RedArgs a, int k0) {
+  const int li = blockIdx.x;
+  const int n = a.leaves[li];
+  const int n3 = 3 * (a.tgt_ptr[n + 1] - a.tgt_ptr[n]);
+  const int k1 = k0 + kLuNB;
+  if (k1 >= n3) return;  // the whole workgroup
+  const int m = n3 - k1, nt = (m + kLuT - 1) / kLuT;
+  const int t = blockIdx.y;
+  if (t >= nt * nt) return;
+  double *A = a.blk + a.off[li];
   __shared__ double Ls[kLuT][kLuNB + 1];
   __shared__ double Us[kLuNB][kLuT + 1];
   const int tid = threadIdx.x;
-  for (int k0 = 0; k0 < n3; k0 += kLuNB) {
-    const int k1 = min(k0 + kLuNB, n3);
-    // ---- panel ----
-    for (int k = k0; k < k1; k++) {
-      double best = -1.0;
-      int bi = n3;
-      for (int i = k + tid; i < n3; i += kLuThreads) {
-        const double v = fabs(A[(size_t)i * n3 + k]);
-        if (v > best) { best = v; bi = i; }
-      }
-      sv[tid] = best;
-      si[tid] = bi;
-      __syncthreads();
-      for (int st = kLuThreads / 2; st > 0; st >>= 1) {
-        if (tid < st) {
-          const double v2 = sv[tid + st];
-          const int i2 = si[tid + st];
-          if (v2 > sv[tid] || (v2 == sv[tid] && i2 < si[tid])) {
-            sv[tid] = v2;
-            si[tid] = i2;
-          }
-        }
-        __syncthreads();
-      }
-      const int p = si[0];
-      if (tid == 0) piv[k] = p;
-      if (p != k)
-        for (int jj = tid; jj < n3; jj += kLuThreads) {
-          const double t = A[(size_t)k * n3 + jj];
-          A[(size_t)k * n3 + jj] = A[(size_t)p * n3 + jj];
-          A[(size_t)p * n3 + jj] = t;
-        }
-      __syncthreads();
-      const double d = A[(size_t)k * n3 + k];
-      if (d != 0.0)
-        for (int i = k + 1 + tid; i < n3; i += kLuThreads) A[(size_t)i * n3 + k] /= d;
-      __syncthreads();
-      const int w = k1 - k - 1;   // panel columns right of k
-      if (w > 0) {
-        const int m = n3 - k - 1;
-        for (int idx = tid; idx < m * w; idx += kLuThreads) {
-          const int i = k + 1 + idx / w, jj = k + 1 + idx % w;
-          A[(size_t)i * n3 + jj] -= A[(size_t)i * n3 + k] * A[(size_t)k * n3 + jj];
-        }
-      }
-      __syncthreads();
-    }
-    if (k1 >= n3) break;
-    const int m = n3 - k1;   // trailing size
-    // ---- U12 = L11^-1 A12 (rows k0..k1-1, columns k1..) ----
-    for (int k = k0; k < k1 - 1; k++) {
-      const int h = k1 - k - 1;
-      for (int idx = tid; idx < h * m; idx += kLuThreads) {
-        const int i = k + 1 + idx / m, jj = k1 + idx % m;
-        A[(size_t)i * n3 + jj] -= A[(size_t)i * n3 + k] * A[(size_t)k * n3 + jj];
-      }
-      __syncthreads();
-    }
-    // ---- A22 -= L21 U12, 64 x 64 tiles ----
-    const int kb = k1 - k0, nt = (m + kLuT - 1) / kLuT;
-    for (int t = 0; t < nt * nt; t++) {
-      const int r0 = k1 + (t / nt) * kLuT, c0 = k1 + (t % nt) * kLuT;
-      for (int idx = tid; idx < kLuT * kLuNB; idx += kLuThreads) {
-        const int r = idx / kLuNB, kk = idx % kLuNB;
-        Ls[r][kk] = (r0 + r < n3 && kk < kb) ? A[(size_t)(r0 + r) * n3 + k0 + kk] : 0.0;
-        const int kk2 = idx / kLuT, c = idx % kLuT;
-        Us[kk2][c] = (c0 + c < n3 && kk2 < kb) ? A[(size_t)(k0 + kk2) * n3 + c0 + c] : 0.0;
-      }
-      __syncthreads();
-      const int r = tid >> 3, cb = (tid & 7) * 8;
-      if (r0 + r < n3) {
-        double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-        for (int kk = 0; kk < kb; kk++) {
-          const double l = Ls[r][kk];
+  const int r0 = k1 + (t / nt) * kLuT, c0 = k1 + (t % nt) * kLuT;
+  for (int idx = tid; idx < kLuT * kLuNB; idx += kLuThreads) {
+    const int r = idx / kLuNB, kk = idx % kLuNB;
+    Ls[r][kk] = r0 + r < n3 ? A[(size_t)(r0 + r) * n3 + k0 + kk] : 0.0;
+    const int kk2 = idx / kLuT, c = idx % kLuT;
+    Us[kk2][c] = c0 + c < n3 ? A[(size_t)(k0 + kk2) * n3 + c0 + c] : 0.0;
+  }
+  __syncthreads();
+  const int r = tid >> 3, cb = (tid & 7) * 8;
+  if (r0 + r < n3) {
+    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll 4
+    for (int kk = 0; kk < kLuNB; kk++) {
+      const double l = Ls[r][kk];
 #pragma unroll
-          for (int c = 0; c < 8; c++) acc[c] += l * Us[kk][cb + c];
-        }
-#pragma unroll
-        for (int c = 0; c < 8; c++)
-          if (c0 + cb + c < n3) A[(size_t)(r0 + r) * n3 + c0 + cb + c] -= acc[c];
-      }
-      __syncthreads();
+      for (int c = 0; c < 8; c++) acc[c] += l * Us[kk][cb + c];
     }
+#pragma unroll
+    for (int c = 0; c < 8; c++)
+      if (c0 + cb + c < n3) A[(size_t)(r0 + r) * n3 + c0 + cb + c] -= acc[c];
   }
 }
 
@@ -367,7 +377,14 @@ void Fmm::reduced_factor() {
   if (n_tleaves_) {
     hipLaunchKernelGGL(k_red_form, dim3(n_tleaves_), dim3(256), 0, 0, a);
     NNK_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_red_lu, dim3(n_tleaves_), dim3(kLuThreads), 0, 0, a);
+    for (int k0 = 0; k0 < red_maxn_; k0 += kLuNB) {
+      hipLaunchKernelGGL(k_red_lu_panel, dim3(n_tleaves_), dim3(kLuThreads), 0, 0, a, k0);
+      const int m = red_maxn_ - k0 - kLuNB;
+      if (m > 0) {
+        const int nt = (m + kLuT - 1) / kLuT;
+        hipLaunchKernelGGL(k_red_lu_update, dim3(n_tleaves_, nt * nt), dim3(kLuThreads), 0, 0, a, k0);
+      }
+    }
     NNK_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_red_transpose, dim3(n_tleaves_), dim3(256), 0, 0, a);
     hipLaunchKernelGGL(k_red_perm, dim3((n_tleaves_ + 63) / 64), dim3(64), 0, 0, a, n_tleaves_);
