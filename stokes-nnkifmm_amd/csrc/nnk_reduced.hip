@@ -229,89 +229,97 @@ __global__ void k_red_perm(RedArgs a, int nleaves) {
 }
 
 // The inverse of each leaf block from its LU factors (set-up only), in the
-// row-major working block W (W[i n3 + j]).  A workgroup of one wavefront
-// owns 64 consecutive columns j of A^-1 = U^-1 L^-1 P (lane = column) and
-// walks the rows in blocks of kRedInvB: each lane keeps its column's entries
-// of the current row block in registers, subtracts the rows solved before
-// (each kRedInvB x kRedInvB factor tile staged in LDS and read as
-// broadcasts; the solved entries of the lane's own column read from W,
-// coalesced across the lanes), then solves the diagonal tile in registers.
-// Forward with the unit lower triangle (row blocks ascending), then back
-// with the upper one (descending).  Each solved entry is read from W once
-// per row block instead of once per row.
-constexpr int kRedInvCols = 64, kRedInvB = 32;
-__global__ __launch_bounds__(kRedInvCols) void k_red_inv(RedArgs a, double *__restrict__ W) {
-  constexpr int B = kRedInvB;
+// row-major working block W (W[i n3 + j]).  A workgroup owns 64 consecutive
+// columns j of A^-1 = U^-1 L^-1 P (lane = column) and walks the rows in
+// blocks of kRedInvB; each of its kRedInvWaves wavefronts keeps the
+// column's entries of a quarter of the current row block in registers and
+// subtracts the rows solved before (each kRedInvB x kRedInvB factor tile
+// staged in LDS and read as broadcasts; the solved entries read from W,
+// coalesced across the lanes).  The first wavefront then solves the
+// diagonal tile in LDS, and every wavefront stores its rows.  Forward with
+// the unit lower triangle (row blocks ascending), then back with the upper
+// one (descending).  Each solved entry is read once per row block and
+// wavefront instead of once per row.
+constexpr int kRedInvCols = 64, kRedInvB = 32, kRedInvWaves = 4;
+__global__ __launch_bounds__(kRedInvCols * kRedInvWaves) void k_red_inv(RedArgs a, double *__restrict__ W) {
+  constexpr int B = kRedInvB, NT = kRedInvCols * kRedInvWaves, RW = B / kRedInvWaves;
   __shared__ double F[B][B + 1];           // F[kk][r]: factor entry (row i0 + r, column k0 + kk)
   __shared__ double Xs[B][kRedInvCols];    // the row block of the 64 columns (diagonal solve)
   const int li = blockIdx.x;
   const int n = a.leaves[li];
   const int n3 = 3 * (a.tgt_ptr[n + 1] - a.tgt_ptr[n]);
   if ((int)blockIdx.y * kRedInvCols >= n3) return;  // the whole workgroup
-  const int tid = threadIdx.x;
-  const int j = blockIdx.y * kRedInvCols + tid;
+  const int tid = threadIdx.x, lane = tid % kRedInvCols, r0 = (tid / kRedInvCols) * RW;
+  const int j = blockIdx.y * kRedInvCols + lane;
   const bool col = j < n3;
   const double *A = a.blk + a.off[li];  // column-major factors: A[k n3 + i] = LU[i][k]
   const int32_t *perm = a.perm + a.poff[li];
   double *X = W + a.off[li];
   const int nb = (n3 + B - 1) / B;
+  // (the barriers also order the global stores of W before later reads by
+  // the other wavefronts)
   auto stage = [&](int i0, int k0) {
     __syncthreads();  // the previous tile has been read
-    for (int idx = tid; idx < B * B; idx += kRedInvCols) {
+    for (int idx = tid; idx < B * B; idx += NT) {
       const int kk = idx / B, r = idx % B, k = k0 + kk, i = i0 + r;
       F[kk][r] = (k < n3 && i < n3) ? A[(size_t)k * n3 + i] : 0.0;
     }
     __syncthreads();
   };
-  // subtract the already solved rows k0 .. k0+B-1 of this column times the
-  // tile, then hand the row block to the diagonal solve
-  double acc[B];
+  // this wavefront's rows r0 .. r0+RW-1 of the block, minus the already
+  // solved rows k0 .. k0+B-1 of the column times the tile
+  double acc[RW];
   auto update = [&](int k0) {
     for (int kk = 0; kk < B; kk++) {
       const double xk = (col && k0 + kk < n3) ? X[(size_t)(k0 + kk) * n3 + j] : 0.0;
 #pragma unroll
-      for (int r = 0; r < B; r++) acc[r] -= F[kk][r] * xk;
+      for (int r = 0; r < RW; r++) acc[r] -= F[kk][r0 + r] * xk;
     }
   };
-  auto store = [&](int i0) {
+  // hand the rows to the diagonal solve of the first wavefront (solve),
+  // then store every wavefront's rows
+  auto diag = [&](int i0, auto solve) {
+#pragma unroll
+    for (int r = 0; r < RW; r++) Xs[r0 + r][lane] = acc[r];
+    __syncthreads();
+    if (tid < kRedInvCols) solve();
+    __syncthreads();
     if (col)
-      for (int r = 0; r < B && i0 + r < n3; r++) X[(size_t)(i0 + r) * n3 + j] = Xs[r][tid];
+      for (int r = r0; r < r0 + RW && i0 + r < n3; r++) X[(size_t)(i0 + r) * n3 + j] = Xs[r][lane];
   };
   for (int I = 0; I < nb; I++) {  // L y = P e_j
     const int i0 = I * B;
 #pragma unroll
-    for (int r = 0; r < B; r++) acc[r] = (col && i0 + r < n3 && perm[i0 + r] == j) ? 1.0 : 0.0;
+    for (int r = 0; r < RW; r++) acc[r] = (col && i0 + r0 + r < n3 && perm[i0 + r0 + r] == j) ? 1.0 : 0.0;
     for (int K = 0; K < I; K++) {
       stage(i0, K * B);
       update(K * B);
     }
     stage(i0, i0);
-#pragma unroll
-    for (int r = 0; r < B; r++) Xs[r][tid] = acc[r];
-    for (int r = 1; r < B; r++) {  // unit lower diagonal tile (each lane its own column)
-      double t = Xs[r][tid];
-      for (int q = 0; q < r; q++) t -= F[q][r] * Xs[q][tid];
-      Xs[r][tid] = t;
-    }
-    store(i0);
+    diag(i0, [&]() {
+      for (int r = 1; r < B; r++) {  // unit lower diagonal tile
+        double t = Xs[r][lane];
+        for (int q = 0; q < r; q++) t -= F[q][r] * Xs[q][lane];
+        Xs[r][lane] = t;
+      }
+    });
   }
   for (int I = nb - 1; I >= 0; I--) {  // U x = y
     const int i0 = I * B;
 #pragma unroll
-    for (int r = 0; r < B; r++) acc[r] = (col && i0 + r < n3) ? X[(size_t)(i0 + r) * n3 + j] : 0.0;
+    for (int r = 0; r < RW; r++) acc[r] = (col && i0 + r0 + r < n3) ? X[(size_t)(i0 + r0 + r) * n3 + j] : 0.0;
     for (int K = I + 1; K < nb; K++) {
       stage(i0, K * B);
       update(K * B);
     }
     stage(i0, i0);
-#pragma unroll
-    for (int r = 0; r < B; r++) Xs[r][tid] = acc[r];
-    for (int r = B - 1; r >= 0; r--) {  // upper diagonal tile
-      double t = Xs[r][tid];
-      for (int q = r + 1; q < B; q++) t -= F[q][r] * Xs[q][tid];
-      Xs[r][tid] = i0 + r < n3 ? t / F[r][r] : 0.0;
-    }
-    store(i0);
+    diag(i0, [&]() {
+      for (int r = B - 1; r >= 0; r--) {  // upper diagonal tile
+        double t = Xs[r][lane];
+        for (int q = r + 1; q < B; q++) t -= F[q][r] * Xs[q][lane];
+        Xs[r][lane] = i0 + r < n3 ? t / F[r][r] : 0.0;
+      }
+    });
   }
 }
 
@@ -393,8 +401,8 @@ void Fmm::reduced_factor() {
     // pass over them at HBM bandwidth instead of a chain of substitutions
     DBuf W;
     W.alloc(sizeof(double) * std::max<int64_t>(off.back(), 1));
-    hipLaunchKernelGGL(k_red_inv, dim3(n_tleaves_, (red_maxn_ + kRedInvCols - 1) / kRedInvCols), dim3(kRedInvCols),
-                       0, 0, a, W.as<double>());
+    hipLaunchKernelGGL(k_red_inv, dim3(n_tleaves_, (red_maxn_ + kRedInvCols - 1) / kRedInvCols),
+                       dim3(kRedInvCols * kRedInvWaves), 0, 0, a, W.as<double>());
     NNK_HIP(hipGetLastError());
     RedArgs w = a;
     w.blk = W.as<double>();
